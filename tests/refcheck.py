@@ -1,0 +1,177 @@
+"""ctypes bindings to the two CPU checkers used by the tests.
+
+  Oracle     oracle/liboracle.so       clean-room C restatement (orc.h)
+  Reference  oracle/_ref/libtopazref.so  the reference's own sources, compiled
+             by oracle/Makefile (present only where it was built)
+
+Both are test infrastructure: the product never imports this module.
+"""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libtopazref.so"
+
+_mats = json.loads((ROOT / "topaz_amd" / "data" / "matrices.json").read_text())
+MATRIX_ORDER = _mats["order"]
+
+
+def matrix(name="BLOSUM62"):
+    return np.array(_mats["matrices"][name], dtype=np.int8)
+
+
+def _ptr8(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int8))
+
+
+class OrcEnd(C.Structure):
+    _fields_ = [("score", C.c_int32), ("ref", C.c_int32), ("read", C.c_int32)]
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        L = C.CDLL(str(path))
+        P8 = C.POINTER(C.c_int8)
+        I32 = C.c_int32
+        PI = C.POINTER(C.c_int32)
+        L.orc_sw_scan.restype = OrcEnd
+        L.orc_sw_scan.argtypes = [P8, I32, P8, I32, C.c_int, P8, I32, C.c_int,
+                                  I32, I32, I32, I32]
+        L.orc_ssw_align.restype = C.c_int
+        L.orc_ssw_align.argtypes = [P8, I32, P8, I32, P8, I32, I32, I32,
+                                    C.POINTER(OrcEnd)]
+        L.orc_ssw_align_stats.restype = C.c_int
+        L.orc_ssw_align_stats.argtypes = [P8, I32, P8, I32, P8, I32, I32, I32,
+                                          I32, I32, I32, PI, PI, PI, PI, PI,
+                                          C.POINTER(C.c_double)]
+        L.orc_banded_sw.restype = C.c_int
+        L.orc_banded_sw.argtypes = [P8, P8, I32, I32, I32, I32, I32, I32, P8,
+                                    I32, PI, PI, PI, C.POINTER(C.c_double)]
+        L.orc_sa_hsp.restype = C.c_int
+        L.orc_sa_hsp.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_char_p,
+                                 I32, I32, P8, PI]
+        L.orc_sa_gapless.restype = C.c_int
+        L.orc_sa_gapless.argtypes = L.orc_sa_hsp.argtypes
+        L.orc_evalue.restype = None
+        L.orc_evalue.argtypes = [I32, I32, C.c_uint64, C.c_uint64, C.c_double,
+                                 C.c_double, C.c_double, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), PI]
+        L.orc_aa_table2.restype = C.c_int8
+        L.orc_aa_table2.argtypes = [C.c_ubyte]
+        self.L = L
+
+    def sw_scan(self, q, s, P, go=11, ge=1, mat=None, dir=0, terminate=-1):
+        mat = matrix() if mat is None else mat
+        bias = -min(0, int(mat.min()))
+        r = self.L.orc_sw_scan(_ptr8(q), len(q), _ptr8(s), len(s), dir,
+                               _ptr8(mat), 25, P, go, ge, bias, terminate)
+        return r.score, r.ref, r.read
+
+    def ssw_align(self, q, s, go=11, ge=1, mat=None):
+        """(score, ref_end, read_end) or None where the reference gives NULL."""
+        mat = matrix() if mat is None else mat
+        out = OrcEnd()
+        if self.L.orc_ssw_align(_ptr8(q), len(q), _ptr8(s), len(s), _ptr8(mat),
+                                25, go, ge, C.byref(out)):
+            return None
+        return out.score, out.ref, out.read
+
+    def ssw_align_stats(self, q, s, read_end, ref_end, score, go=11, ge=1,
+                        mat=None):
+        mat = matrix() if mat is None else mat
+        v = [C.c_int32() for _ in range(5)]
+        ident = C.c_double()
+        rc = self.L.orc_ssw_align_stats(
+            _ptr8(q), len(q), _ptr8(s), len(s), _ptr8(mat), 25, go, ge,
+            read_end, ref_end, score, *[C.byref(x) for x in v], C.byref(ident))
+        return (rc,) + tuple(x.value for x in v) + (ident.value,)
+
+    def sa_hsp(self, T, idx, Q, o, mat=None):
+        mat = matrix() if mat is None else mat
+        sc = C.c_int32(0)
+        ln = self.L.orc_sa_hsp(T, len(T), idx, Q, len(Q), o, _ptr8(mat),
+                               C.byref(sc))
+        return (sc.value if ln != -1 else None), ln
+
+    def evalue(self, raw, qlen, db_size, db_proteins, lam=0.267, k=0.041,
+               H=0.140):
+        b, e, el = C.c_double(), C.c_double(), C.c_int32()
+        self.L.orc_evalue(raw, qlen, db_size, db_proteins, lam, k, H,
+                          C.byref(b), C.byref(e), C.byref(el))
+        return b.value, e.value, el.value
+
+
+class SAlign(C.Structure):
+    _fields_ = [("score1", C.c_uint16), ("score2", C.c_uint16),
+                ("ref_begin1", C.c_int32), ("ref_end1", C.c_int32),
+                ("read_begin1", C.c_int32), ("read_end1", C.c_int32),
+                ("ref_end2", C.c_int32), ("cigar", C.c_void_p),
+                ("cigarLen", C.c_int32)]
+
+
+class SProfile(C.Structure):
+    _fields_ = [("profile_byte", C.c_void_p), ("profile_word", C.c_void_p),
+                ("read", C.c_void_p), ("mat", C.c_void_p),
+                ("readLen", C.c_int32), ("n", C.c_int32), ("bias", C.c_uint8)]
+
+
+class Reference:
+    """The reference's compiled ssw.c / search.c, driven through its own API."""
+
+    def __init__(self, path=REF_SO):
+        L = C.CDLL(str(path))
+        P8 = C.POINTER(C.c_int8)
+        L.ssw_init.restype = C.POINTER(SProfile)
+        L.ssw_init.argtypes = [P8, C.c_int32, P8, C.c_int32, C.c_int8]
+        L.ssw_align.restype = C.POINTER(SAlign)
+        L.ssw_align.argtypes = [C.POINTER(SProfile), P8, C.c_int32, C.c_uint8,
+                                C.c_uint8, C.c_uint8, C.c_uint16, C.c_int32,
+                                C.c_int32]
+        L.align_destroy.argtypes = [C.POINTER(SAlign)]
+        L.init_destroy.argtypes = [C.POINTER(SProfile)]
+        PI = C.POINTER(C.c_int32)
+        L.ssw_align_stats.restype = C.c_int
+        L.ssw_align_stats.argtypes = [C.POINTER(SProfile), P8, C.c_int32,
+                                      C.c_uint8, C.c_uint8, C.c_int32,
+                                      C.c_int32, C.c_int32, C.c_uint16, PI, PI,
+                                      PI, PI, PI, C.POINTER(C.c_double)]
+        self.L = L
+
+    def profile(self, q, mat=None):
+        mat = matrix() if mat is None else mat
+        # keep buffers alive for the profile's lifetime
+        qb = np.ascontiguousarray(q, dtype=np.int8)
+        mb = np.ascontiguousarray(mat, dtype=np.int8)
+        p = self.L.ssw_init(_ptr8(qb), len(qb), _ptr8(mb), 25, 2)
+        return p, (qb, mb)
+
+    def ssw_align(self, q, s, go=11, ge=1, mat=None, prof=None):
+        own = prof is None
+        if own:
+            prof = self.profile(q, mat)
+        sb = np.ascontiguousarray(s, dtype=np.int8)
+        r = self.L.ssw_align(prof[0], _ptr8(sb), len(sb), go, ge, 0, 0, 0, 15)
+        out = None
+        if r:
+            a = r.contents
+            out = (a.score1, a.ref_end1, a.read_end1)
+            self.L.align_destroy(r)
+        if own:
+            self.L.init_destroy(prof[0])
+        return out
+
+    def ssw_align_stats(self, q, s, read_end, ref_end, score, go=11, ge=1,
+                        mat=None):
+        prof = self.profile(q, mat)
+        sb = np.ascontiguousarray(s, dtype=np.int8)
+        v = [C.c_int32(0) for _ in range(5)]
+        ident = C.c_double()
+        rc = self.L.ssw_align_stats(prof[0], _ptr8(sb), len(sb), go, ge, 15,
+                                    read_end, ref_end, score,
+                                    *[C.byref(x) for x in v], C.byref(ident))
+        self.L.init_destroy(prof[0])
+        return (rc,) + tuple(x.value for x in v) + (ident.value,)
