@@ -1,0 +1,80 @@
+/*
+ * topaz_amd.h -- C ABI of the MI355X implementation of TOPAZ's
+ * candidate-alignment hot path (gapless HSP extension + striped
+ * Smith-Waterman), built as libtopaz_amd.so.
+ *
+ * Plain C types only.  Every call returns 0 on success and a nonzero code on
+ * failure; tpz_last_error() gives the message.  There is no CPU fallback: the
+ * library needs a HIP device and fails if none is usable.
+ *
+ * Residues are substitution-matrix indices 0..24 (the residue order
+ * ARNDCQEGHILKMFPSTWYVBJZX* of substitution.h); the matrix is 25 x 25 int8,
+ * W[s * 25 + q] with s the subject (database) residue and q the query residue,
+ * as ssw.c:112 and search.c:134 index it.
+ */
+#ifndef TOPAZ_AMD_H
+#define TOPAZ_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Select the HIP device used by this process (default 0) and create its
+ * stream.  Called implicitly by every entry point with device 0. */
+int tpz_init(int device);
+const char* tpz_last_error(void);
+/* Library version string. */
+const char* tpz_version(void);
+
+/* ---- batched forward Smith-Waterman (score + end positions) -------------- */
+
+/* A device-resident batch of (query, subject) pairs.  Replaces the per-pair
+ * loop of query_perform_alignments (query.c:199-228) that calls
+ * align_using_ssw_profile -> ssw_align (ssw_wrapper.c:209, ssw.c:835-897). */
+typedef struct tpz_sw_batch tpz_sw_batch;
+
+/* Copy queries, subjects and the pair list to the device.
+ *   q_res/q_off/q_len   queries as residues; query k = q_res[q_off[k] ..
+ *                       q_off[k] + q_len[k])
+ *   s_res/s_off/s_len   subjects, same layout
+ *   pair_query/pair_subject  n_pairs pair definitions
+ *   matrix              25 x 25 scores; gap_open > gap_extend >= 0, both <= 255
+ * Per pair the result is what ssw_align (ssw.c:835-897, score_size 2) returns:
+ * the byte (16-lane) result if its score + bias < 255, else the word (8-lane)
+ * result, else "overflow" (the reference returns NULL). */
+int tpz_sw_batch_create(tpz_sw_batch** out,
+                        int32_t n_queries, const int8_t* q_res,
+                        const int64_t* q_off, const int32_t* q_len,
+                        int32_t n_subjects, const int8_t* s_res,
+                        const int64_t* s_off, const int32_t* s_len,
+                        int32_t n_pairs, const int32_t* pair_query,
+                        const int32_t* pair_subject, const int8_t* matrix,
+                        int32_t gap_open, int32_t gap_extend);
+
+/* Launch schedules of tpz_sw_batch_run. */
+#define TPZ_SCHED_PLAIN      0   /* one launch per query                 */
+#define TPZ_SCHED_PERSISTENT 1   /* one persistent launch, length-binned */
+
+/* Run the batch on the device (results stay on the device). */
+int tpz_sw_batch_run(tpz_sw_batch* b, int32_t schedule);
+
+/* Copy results to the host.  score[i] is the ssw_align score1 of pair i, or
+ * 32767 where the reference returns NULL (ssw.c:866-870); ref_end/read_end
+ * are ref_end1/read_end1 (ssw.c:886-887).  Any pointer may be null. */
+int tpz_sw_batch_results(const tpz_sw_batch* b, int32_t* score,
+                         int32_t* ref_end, int32_t* read_end);
+
+/* Device time of the kernels of the last run, in milliseconds (HIP events on
+ * the library stream). */
+int tpz_sw_batch_kernel_ms(const tpz_sw_batch* b, double* ms);
+
+void tpz_sw_batch_destroy(tpz_sw_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -503,3 +503,55 @@ void orc_evalue(int32_t rawscore, int32_t qlen, uint64_t db_size,
     *bitscore = nats / M_LN2;
     *effective_length = (int32_t)(qlen - correction);
 }
+
+/* ------------------------------------------------------------------ */
+/* Textbook Gotoh local alignment with the same column bookkeeping     */
+/* ------------------------------------------------------------------ */
+/* H = max(0, H(i-1,j-1) + W, E, F), E and F from H, gap of length k
+ * costing gapO + (k-1) gapE.  Same end rules as orc_sw_scan (first column
+ * reaching the max, lowest row of that column; byte/word overflow caps).
+ * Used by the tests to show that for gapO > gapE the reference's lane
+ * recurrence gives the same H in every cell. */
+orc_end_t orc_sw_textbook(const int8_t* read, int32_t readLen,
+                          const int8_t* ref, int32_t refLen,
+                          const int8_t* mat, int32_t nmat, int P,
+                          int32_t gapO, int32_t gapE, int32_t bias) {
+    orc_end_t r;
+    const int32_t m = readLen, n = refLen;
+    const int32_t cap = (P == 16) ? 255 - bias : 32767;
+    const int32_t ovf = (P == 16) ? 255 : 32767;
+    const size_t cells = (size_t)(m > 0 ? m : 1);
+    int32_t* hp = (int32_t*)calloc(cells, sizeof(int32_t));
+    int32_t* hc = (int32_t*)calloc(cells, sizeof(int32_t));
+    int32_t* e = (int32_t*)calloc(cells, sizeof(int32_t));
+    int32_t* kept = (int32_t*)calloc(cells, sizeof(int32_t));
+    int32_t best = 0, end_ref = (P == 16) ? -1 : 0, end_read, i, j;
+    for (i = 0; i < n; ++i) {
+        const int8_t* wrow = mat + (size_t)ref[i] * nmat;
+        int32_t f = 0, colmax = 0, *t;
+        for (j = 0; j < m; ++j) {
+            int32_t d = j ? hp[j - 1] : 0;
+            int32_t h = imax(imax(d + wrow[read[j]], 0), imax(e[j], f));
+            int32_t hgo = imax(h - gapO, 0);
+            e[j] = imax(imax(e[j] - gapE, 0), hgo);
+            f = imax(imax(f - gapE, 0), hgo);
+            hc[j] = h;
+            colmax = imax(colmax, h);
+        }
+        t = hp; hp = hc; hc = t;
+        if (colmax >= cap) colmax = cap;
+        if (colmax > best) {
+            best = colmax;
+            if (best >= cap) break;
+            end_ref = i;
+            memcpy(kept, hp, (size_t)m * sizeof(int32_t));
+        }
+    }
+    end_read = m - 1;
+    for (j = 0; j < m; ++j) if (kept[j] == best) { end_read = j; break; }
+    r.score = best >= cap ? ovf : best;
+    r.ref = end_ref;
+    r.read = m > 0 ? end_read : -1;
+    free(hp); free(hc); free(e); free(kept);
+    return r;
+}

@@ -63,6 +63,12 @@ orc_end_t orc_sw_scan_s1(const int8_t* read, int32_t readLen,
                          int32_t gapO, int32_t gapE, int32_t bias,
                          int32_t terminate);
 
+/* Textbook Gotoh (E and F from H) with the same bookkeeping and caps. */
+orc_end_t orc_sw_textbook(const int8_t* read, int32_t readLen,
+                          const int8_t* ref, int32_t refLen,
+                          const int8_t* mat, int32_t nmat, int P,
+                          int32_t gapO, int32_t gapE, int32_t bias);
+
 /* ssw_align (ssw.c:835-897) with score_size 2: byte, then word on 255.
  * Returns 0 and fills *out, or 1 where the reference returns NULL
  * (word score 32767). */

@@ -1,0 +1,239 @@
+"""The oracle, pinned: against the reference's own outputs (golden fixtures
+written by tests/golden/gen_golden.py from the compiled reference), against
+brute force on tiny inputs, and against closed forms.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.refcheck import REF_SO, Oracle, Reference, matrix
+from topaz_amd.matrices import NAMES, aa_table2
+
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def _seq(res, off, lens, k):
+    return res[off[k]:off[k] + lens[k]]
+
+
+def test_alphabet_map_matches_table():
+    o = Oracle()
+    t = aa_table2()
+    for c in range(128):
+        assert o.L.orc_aa_table2(c) == t[c], chr(c)
+    # spot checks of lexicographical.h:26-32: 'A'->0, 'X'->24 ('*'), 'V'->23 ('X')
+    assert t[ord("A")] == 0 and t[ord("X")] == 24 and t[ord("V")] == 23 and t[ord("Z")] == 23
+
+
+def test_sw_golden_scores_and_ends(orc):
+    d = np.load(GOLD / "sw_pairs.npz")
+    bad = []
+    for k in range(len(d["q_len"])):
+        q = _seq(d["q_res"], d["q_off"], d["q_len"], k)
+        s = _seq(d["s_res"], d["s_off"], d["s_len"], k)
+        mat = matrix(NAMES[d["matrix"][k]])
+        r = orc.ssw_align(q, s, int(d["gap_open"][k]), int(d["gap_extend"][k]), mat)
+        if d["is_null"][k]:
+            ok = r is None
+        else:
+            ok = r == (d["score"][k], d["ref_end"][k], d["read_end"][k])
+        if not ok:
+            bad.append((k, r))
+    assert not bad, bad[:5]
+    assert d["is_null"].sum() >= 1 and (d["score"] > 251).sum() > 100
+
+
+def test_sw_golden_traceback_stats(orc):
+    d = np.load(GOLD / "sw_pairs.npz")
+    bad = []
+    for k in np.nonzero(d["has_stats"])[0]:
+        q = _seq(d["q_res"], d["q_off"], d["q_len"], k)
+        s = _seq(d["s_res"], d["s_off"], d["s_len"], k)
+        mat = matrix(NAMES[d["matrix"][k]])
+        st = orc.ssw_align_stats(q, s, int(d["read_end"][k]), int(d["ref_end"][k]),
+                                 int(d["score"][k]), int(d["gap_open"][k]),
+                                 int(d["gap_extend"][k]), mat)
+        exp = (d["stats_rc"][k], d["read_begin"][k], d["ref_begin"][k], d["length"][k],
+               d["gapopen"][k], d["mismatch"][k])
+        same_ident = st[6] == d["identity"][k] or (math.isnan(st[6]) and math.isnan(d["identity"][k]))
+        if tuple(st[:6]) != tuple(int(x) for x in exp) or not same_ident:
+            bad.append((k, st, exp, d["identity"][k]))
+    assert not bad, bad[:3]
+
+
+def test_gapless_golden(orc):
+    d = np.load(GOLD / "gapless.npz")
+    T = d["T"].tobytes()
+    bad = []
+    for k in range(len(d["seed_pos"])):
+        qi = d["seed_query"][k]
+        Q = d["q"][d["q_off"][qi]:d["q_off"][qi] + d["q_len"][qi]].tobytes()
+        sc, ln = orc.sa_hsp(T, int(d["seed_pos"][k]), Q, int(d["seed_offset"][k]))
+        exp_sc = None if d["length"][k] == -1 else int(d["score"][k])
+        if (sc, ln) != (exp_sc, int(d["length"][k])):
+            bad.append((k, (sc, ln), (exp_sc, int(d["length"][k]))))
+    assert not bad, bad[:5]
+    assert (d["length"] != -1).sum() > 500
+
+
+def test_evalue_golden(orc):
+    d = np.load(GOLD / "evalue.npz")
+    n_aligned = 0
+    for k in range(len(d["qlen"])):
+        if d["gapless"][k] < 40:            # ssw_wrapper.c:171-176: no alignment
+            assert d["bitscore"][k] == 0 and d["evalue"][k] == 1000
+            continue
+        n_aligned += 1
+        raw = int(d["rawscore"][k])
+        if raw == 32767:
+            continue
+        b, e, el = orc.evalue(raw, int(d["qlen"][k]), int(d["db_size"][k]),
+                              int(d["db_proteins"][k]))
+        assert b == d["bitscore"][k] and e == d["evalue"][k] and el == d["effective_length"][k], k
+        q = _seq(d["q_res"], d["q_off"], d["q_len"], k)
+        s = _seq(d["s_res"], d["s_off"], d["s_len"], k)
+        assert orc.ssw_align(q, s) == (raw, d["send"][k], d["qend"][k])
+    assert n_aligned > 50
+
+
+def test_evalue_closed_form(orc):
+    # bits = (lambda*S - ln K)/ln 2 ; E = m' n' e^(-(lambda*S - ln K))
+    lam, K, H = 0.267, 0.041, 0.140
+    for raw, m, D, N in [(50, 300, 35_000_000, 100_000), (500, 1000, 10 ** 9, 10 ** 6),
+                         (12, 10, 1000, 10)]:
+        b, e, el = orc.evalue(raw, m, D, N)
+        corr = math.log(K * m * D) / H
+        cq, cdb = max(m - corr, 1.0), max(D - corr * N, 1.0)
+        nats = lam * raw - math.log(K)
+        assert b == pytest.approx(nats / math.log(2), rel=1e-15)
+        ev = cq * cdb * math.exp(-nats)
+        assert e == pytest.approx(ev if ev > 1e-180 else 0.0, rel=1e-12)
+
+
+def gotoh_textbook(q, s, mat, go, ge):
+    """Plain O(mn) local alignment, gap of length k costing go + (k-1) ge."""
+    m, n = len(q), len(s)
+    W = mat.reshape(25, 25)
+    H = np.zeros((n + 1, m + 1), np.int64)
+    E = np.zeros((n + 1, m + 1), np.int64)
+    F = np.zeros((n + 1, m + 1), np.int64)
+    best = 0
+    for i in range(1, n + 1):
+        for j in range(1, m + 1):
+            E[i, j] = max(E[i - 1, j] - ge, H[i - 1, j] - go)
+            F[i, j] = max(F[i, j - 1] - ge, H[i, j - 1] - go)
+            H[i, j] = max(0, H[i - 1, j - 1] + W[s[i - 1], q[j - 1]], E[i, j], F[i, j])
+            best = max(best, H[i, j])
+    return best
+
+
+def test_sw_against_textbook_gotoh(orc):
+    """The lane recurrence never exceeds textbook Gotoh and differs from it
+    only through a lost deletion->insertion corner, so on the pairs below
+    (few gaps) it equals it, and it is never more than a few points lower."""
+    rng = np.random.default_rng(11)
+    mat = matrix()
+    diffs = []
+    for _ in range(120):
+        m, n = int(rng.integers(1, 40)), int(rng.integers(1, 40))
+        q = rng.integers(0, 25, m).astype(np.int8)
+        s = rng.integers(0, 25, n).astype(np.int8)
+        r = orc.ssw_align(q, s)
+        t = gotoh_textbook(q, s, mat, 11, 1)
+        assert r[0] <= t
+        diffs.append(t - r[0])
+    assert max(diffs) <= 2 and np.mean(diffs) < 0.2
+    # ungapped pairs: equality (no gap corner can matter)
+    for _ in range(40):
+        m = int(rng.integers(1, 60))
+        q = rng.integers(0, 25, m).astype(np.int8)
+        assert orc.ssw_align(q, q.copy())[0] == gotoh_textbook(q, q, mat, 11, 1)
+
+
+def test_corner_loss_planted():
+    """A planted deletion->insertion corner across a stripe boundary is lost
+    by the lane recurrence (the E value is computed before lazy-F), so the
+    score is below textbook Gotoh -- a dropped-corner mistake would fail here.
+    """
+    o = Oracle()
+    mat = matrix()
+    rng = np.random.default_rng(12)
+    found = 0
+    for _ in range(4000):
+        m = int(rng.integers(2, 17))
+        q = rng.integers(0, 25, m).astype(np.int8)
+        s = rng.integers(0, 25, int(rng.integers(2, 30))).astype(np.int8)
+        r = o.ssw_align(q, s, 3, 1)
+        t = gotoh_textbook(q, s, mat, 3, 1)
+        if r[0] < t:
+            found += 1
+    assert found > 0
+
+
+def test_closed_form_equals_lane_emulation_when_gapO_gt_gapE(orc):
+    """The closed form the kernels implement equals the reference's lane
+    emulation whenever gapO > gapE, for both lane counts."""
+    rng = np.random.default_rng(13)
+    for _ in range(300):
+        m, n = int(rng.integers(1, 200)), int(rng.integers(1, 200))
+        q = rng.integers(0, 25, m).astype(np.int8)
+        s = q[: n].copy() if rng.random() < 0.5 else rng.integers(0, 25, n).astype(np.int8)
+        go = int(rng.integers(2, 16))
+        ge = int(rng.integers(0, go))
+        for P in (16, 8):
+            assert orc.sw_scan(q, s, P, go, ge) == orc.sw_scan(q, s, P, go, ge, closed_form=True)
+
+
+def test_gapless_brute_force(orc):
+    """sa_hsp against a direct prefix-sum formulation on random text."""
+    rng = np.random.default_rng(14)
+    mat = matrix()
+    t2 = aa_table2()
+    letters = np.frombuffer(b"ABCDEFGHIJKLMNOPQRSTUVWXY", np.uint8)
+    for _ in range(300):
+        Tb = bytearray(letters[rng.integers(0, 25, int(rng.integers(1, 300)))].tobytes())
+        for k in rng.integers(0, len(Tb), 3):
+            Tb[k] = 10                     # protein delimiters
+        Qb = bytearray(letters[rng.integers(0, 25, int(rng.integers(1, 200)))].tobytes())
+        T, Q = bytes(Tb), bytes(Qb)
+        idx = int(rng.integers(0, len(T)))
+        o = int(rng.integers(0, len(Q)))
+        if rng.random() < 0.7:
+            Qb[o] = T[idx] if T[idx] != 10 else Qb[o]
+            Q = bytes(Qb)
+        sc, ln = orc.sa_hsp(T, idx, Q, o)
+        if T[idx] != Q[o]:
+            assert ln == -1
+            continue
+
+        def side(step, start, limit):
+            best, arg, s = 0, 0, 0
+            for i in range(start, limit):
+                c = T[idx + step * i]
+                if c == 10:
+                    break
+                s += int(mat[t2[c] * 25 + t2[Q[o + step * i]]])
+                if s >= best:
+                    best, arg = s, i
+            return best, arg
+
+        br = side(1, 0, min(len(T) - idx, len(Q) - o, 128))
+        bl = side(-1, 1, min(idx, o, 128))
+        assert (sc, ln) == (br[0] + bl[0], br[1] + bl[1])
+
+
+@pytest.mark.skipif(not REF_SO.exists(), reason="reference checker not built here")
+def test_oracle_vs_compiled_reference_random():
+    o, r = Oracle(), Reference()
+    rng = np.random.default_rng(99)
+    for _ in range(300):
+        m, n = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+        q = rng.integers(0, 25, m).astype(np.int8)
+        s = rng.integers(0, 25, n).astype(np.int8)
+        go, ge = [(11, 1), (2, 2), (5, 3), (10, 2)][int(rng.integers(0, 4))]
+        assert o.ssw_align(q, s, go, ge) == r.ssw_align(q, s, go, ge)

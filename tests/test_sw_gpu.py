@@ -1,0 +1,108 @@
+"""Batched Smith-Waterman on the GPU against the oracle, bit-exact.
+
+The oracle (oracle/liboracle.so) is the checker: for every pair the GPU's
+(score, ref_end, read_end) must equal what ssw_align returns (ssw.c:835-897),
+including the 255 -> word-path switch and the 32767 overflow.
+"""
+import numpy as np
+import pytest
+
+from topaz_amd import matrix, synth
+from topaz_amd.sw import OVERFLOW, SCHED_PERSISTENT, SCHED_PLAIN, SwBatch
+from tests.refcheck import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_results(o, w, mat, go, ge):
+    q, s = w["q"], w["s"]
+    out = []
+    for qi, si in zip(w["pair_query"], w["pair_subject"]):
+        qq = q[0][q[1][qi]:q[1][qi] + q[2][qi]]
+        ss = s[0][s[1][si]:s[1][si] + s[2][si]]
+        r = o.ssw_align(qq, ss, go, ge, mat)
+        out.append((OVERFLOW, None, None) if r is None else r)
+    return out
+
+
+def compare(w, mat=None, go=11, ge=1, schedule=SCHED_PERSISTENT):
+    mat = matrix() if mat is None else mat
+    o = Oracle()
+    b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], mat, go, ge)
+    sc, re_, rd = b.run(schedule).results()
+    exp = oracle_results(o, w, mat, go, ge)
+    bad = []
+    for p, (es, er, ed) in enumerate(exp):
+        if es == OVERFLOW:
+            ok = sc[p] == OVERFLOW
+        else:
+            ok = (sc[p], re_[p], rd[p]) == (es, er, ed)
+        if not ok:
+            bad.append((p, (int(sc[p]), int(re_[p]), int(rd[p])), (es, er, ed)))
+    b.close()
+    assert not bad, "%d/%d mismatches, first: %s" % (len(bad), len(exp), bad[:5])
+    return sc
+
+
+def small_workload(seed, nq=12, per_q=40, mean=150, sd=100, hi=600, lo=1):
+    rng = np.random.default_rng(seed)
+    return synth.pair_workload(rng, nq, per_q, mean=mean, sd=sd, lo=lo, hi=hi)
+
+
+@pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
+def test_random_and_homolog_pairs(schedule):
+    sc = compare(small_workload(1), schedule=schedule)
+    assert (sc == 255).sum() == 0           # word results replaced the byte ones
+
+
+def test_long_and_tiny_queries():
+    rng = np.random.default_rng(2)
+    w = synth.pair_workload(rng, 6, 30, mean=400, sd=600, lo=1, hi=1500)
+    compare(w)
+
+
+def test_tiny_sequences_all_corners():
+    # m <= 16: every row is a stripe start, all F->E corners are lost
+    rng = np.random.default_rng(3)
+    qs = [rng.integers(0, 25, int(rng.integers(1, 20))).astype(np.int8) for _ in range(20)]
+    ss = [rng.integers(0, 25, int(rng.integers(1, 40))).astype(np.int8) for _ in range(40)]
+    pq = np.repeat(np.arange(20), 40).astype(np.int32)
+    ps = np.tile(np.arange(40), 20).astype(np.int32)
+    from topaz_amd.sw import _concat
+    w = dict(q=_concat(qs), s=_concat(ss), pair_query=pq, pair_subject=ps)
+    compare(w)
+
+
+@pytest.mark.parametrize("name,go,ge", [("BLOSUM45", 15, 2), ("PAM30", 9, 1),
+                                         ("BLOSUM80", 10, 1), ("PAM250", 5, 3)])
+def test_other_matrices_and_gaps(name, go, ge):
+    compare(small_workload(4, nq=6, per_q=30), mat=matrix(name), go=go, ge=ge)
+
+
+def test_overflow_to_word_and_drop():
+    # W-rich long identical pairs: byte overflows, word may reach 32767
+    rng = np.random.default_rng(5)
+    q1 = np.full(3100, 17, np.int8)          # W x 3100 -> self score 34100
+    q2 = rng.integers(0, 25, 800).astype(np.int8)
+    qs = [q1, q2]
+    ss = [q1.copy(), q1[:2000].copy(), q2.copy(), q2[100:700].copy(),
+          rng.integers(0, 25, 300).astype(np.int8)]
+    from topaz_amd.sw import _concat
+    pq = np.array([0, 0, 1, 1, 1, 0], np.int32)
+    ps = np.array([0, 1, 2, 3, 4, 4], np.int32)
+    w = dict(q=_concat(qs), s=_concat(ss), pair_query=pq, pair_subject=ps)
+    sc = compare(w)
+    assert sc[0] == OVERFLOW and sc[1] == 22000
+
+
+def test_empty_batch_and_empty_subjects():
+    from topaz_amd.sw import _concat
+    qs = [np.array([1, 2, 3], np.int8)]
+    ss = [np.zeros(0, np.int8), np.array([1, 2, 3], np.int8)]
+    w = dict(q=_concat(qs), s=_concat(ss), pair_query=np.array([0, 0], np.int32),
+             pair_subject=np.array([0, 1], np.int32))
+    compare(w)
+    b = SwBatch(_concat(qs), _concat(ss), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                matrix())
+    assert len(b.run().results()[0]) == 0
+    b.close()

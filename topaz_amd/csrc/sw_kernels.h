@@ -1,0 +1,43 @@
+// Kernel-side argument blocks of the batched Smith-Waterman kernels.
+#ifndef TOPAZ_AMD_SW_KERNELS_H
+#define TOPAZ_AMD_SW_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tpz {
+
+// Device pointers of one forward pass over a batch of (query, subject) pairs.
+// Residues are substitution-matrix indices 0..24; subjects start on 4-byte
+// boundaries of s_res.  A task is a range [task_begin[t], task_begin[t+1])
+// of at most 128 entries of pair_index (or of the pair numbers themselves
+// when pair_index is null), all with the same query.
+struct SwKernelArgs {
+    const int8_t* q_res;
+    const int64_t* q_off;
+    const int32_t* q_len;
+    const int8_t* s_res;
+    const int64_t* s_off;
+    const int32_t* s_len;
+    const int32_t* pair_query;
+    const int32_t* pair_subject;
+    const int32_t* pair_index;     // may be null
+    const int32_t* task_begin;     // n_tasks + 1 entries
+    int32_t n_tasks;
+    int32_t* task_counter;         // null: one workgroup per task
+    const int8_t* matrix;          // 25 x 25, W[s * 25 + q]
+    int32_t gap_open;
+    int32_t gap_extend;
+    uint32_t* workspace;           // ws_words_per_slot words per workgroup
+    int64_t ws_words_per_slot;
+    int32_t* out_score;
+    int32_t* out_ref;
+    int32_t* out_read;
+};
+
+}  // namespace tpz
+
+extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid,
+                                            hipStream_t stream);
+
+#endif

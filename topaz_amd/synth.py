@@ -1,0 +1,158 @@
+"""Synthetic protein workloads for tests and benchmarks.
+
+Residues are drawn i.i.d. from the Robinson & Robinson (1991, PNAS 88:8880)
+background amino-acid frequencies; lengths are lognormal (mean 350, sd 250 by
+default) clipped to a range; homologs are derived from a source sequence by
+BLOSUM62-conditional substitution at a target identity plus insertions and
+deletions at a per-residue rate with geometric lengths.  Everything is
+vectorised numpy and seeded, so a (seed, sizes) pair names a workload.
+"""
+import numpy as np
+
+from .matrices import MATRIX_ORDER, matrix
+
+# Robinson & Robinson background frequencies, per mille.
+RR_PER_MILLE = {
+    "A": 78.05, "C": 19.25, "D": 53.64, "E": 62.95, "F": 38.56, "G": 73.77,
+    "H": 21.99, "I": 51.42, "K": 57.44, "L": 90.19, "M": 22.43, "N": 44.87,
+    "P": 52.03, "Q": 42.64, "R": 51.29, "S": 71.20, "T": 58.41, "V": 64.41,
+    "W": 13.30, "Y": 32.16,
+}
+
+
+def background():
+    """Probabilities over the 25 matrix indices (0 for B, J, Z, X, *)."""
+    p = np.zeros(25)
+    for aa, f in RR_PER_MILLE.items():
+        p[MATRIX_ORDER.index(aa)] = f
+    return p / p.sum()
+
+
+def lognormal_lengths(rng, n, mean=350.0, sd=250.0, lo=30, hi=5000):
+    sigma2 = np.log1p((sd / mean) ** 2)
+    mu = np.log(mean) - sigma2 / 2
+    x = rng.lognormal(mu, np.sqrt(sigma2), n)
+    return np.clip(np.rint(x), lo, hi).astype(np.int32)
+
+
+def random_residues(rng, total):
+    return rng.choice(25, size=int(total), p=background()).astype(np.int8)
+
+
+def concat_random(rng, lengths):
+    """(flat residues, offsets, lengths) for i.i.d. background sequences."""
+    lengths = np.asarray(lengths, dtype=np.int32)
+    off = np.zeros(len(lengths), np.int64)
+    off[1:] = np.cumsum(lengths[:-1], dtype=np.int64)
+    return random_residues(rng, lengths.sum()), off, lengths
+
+
+def _substitution_cdf():
+    """Row a: cumulative P(b | a, b != a) ~ p_b exp(lambda_u s62(a, b)) with
+    lambda_u = ln 2 / 2, over standard residues."""
+    p = background()
+    s = matrix("BLOSUM62").reshape(25, 25).astype(np.float64)
+    w = p[None, :] * np.exp(np.log(2) / 2 * s)
+    np.fill_diagonal(w, 0.0)
+    w[:, p == 0] = 0.0
+    w /= np.maximum(w.sum(axis=1, keepdims=True), 1e-300)
+    return np.cumsum(w, axis=1)
+
+
+_CDF = None
+
+
+def mutate(rng, res, off, lens, identity, indel_rate=0.02, ext_p=0.5):
+    """Homologs of the given sequences.
+
+    res/off/lens: flat source sequences; identity: per-sequence target identity.
+    Returns (flat residues, offsets, lengths) of the homologs.
+    """
+    global _CDF
+    if _CDF is None:
+        _CDF = _substitution_cdf()
+    lens = np.asarray(lens, np.int64)
+    n = len(lens)
+    seg = np.repeat(np.arange(n), lens)
+    src = np.concatenate([res[o:o + l] for o, l in zip(off, lens)]) if n else \
+        np.zeros(0, np.int8)
+    src = src.astype(np.int64)
+    ident = np.asarray(identity, np.float64)[seg]
+    # substitutions
+    sub = rng.random(len(src)) >= ident
+    u = rng.random(int(sub.sum()))
+    a = src[sub]
+    cdf = _CDF[a]
+    b = (cdf < u[:, None]).sum(axis=1)
+    b = np.minimum(b, 24)
+    out = src.copy()
+    out[sub] = b
+    # indel events: half deletions, half insertions, geometric lengths
+    ev = rng.random(len(out)) < indel_rate
+    is_del = ev & (rng.random(len(out)) < 0.5)
+    is_ins = ev & ~is_del
+    starts = np.zeros(n, np.int64)
+    starts[1:] = np.cumsum(lens)[:-1]
+    ends = starts + lens
+    # deletion runs, clipped at the end of their sequence (difference array)
+    dpos = np.nonzero(is_del)[0]
+    dend = np.minimum(dpos + rng.geometric(ext_p, len(dpos)), ends[seg[dpos]])
+    diff = np.zeros(len(out) + 1, np.int64)
+    np.add.at(diff, dpos, 1)
+    np.add.at(diff, dend, -1)
+    keep = np.cumsum(diff[:-1]) == 0
+    keep[starts[lens > 0]] = True       # never delete a whole sequence
+    ins_len = np.zeros(len(out), np.int64)
+    ins_len[is_ins] = rng.geometric(ext_p, int(is_ins.sum()))
+    rep = keep.astype(np.int64) + ins_len
+    total = int(rep.sum())
+    new = np.empty(total, np.int64)
+    first = np.zeros(len(out), np.int64)
+    first[1:] = np.cumsum(rep)[:-1]
+    # each kept source residue is followed by its insertion run
+    new[:] = -1
+    kept_idx = np.nonzero(keep)[0]
+    new[first[kept_idx]] = out[kept_idx]
+    fill = new < 0
+    new[fill] = random_residues(rng, int(fill.sum()))
+    new_seg = np.repeat(seg, rep)
+    new_lens = np.bincount(new_seg, minlength=n).astype(np.int32)
+    new_off = np.zeros(n, np.int64)
+    new_off[1:] = np.cumsum(new_lens[:-1], dtype=np.int64)
+    return new.astype(np.int8), new_off, new_lens
+
+
+def pair_workload(rng, n_queries, subjects_per_query, homolog_frac=0.5,
+                  ident_range=(0.25, 0.95), mean=350.0, sd=250.0, lo=30, hi=5000):
+    """Queries, subjects and pairs for the batched-SW kernel benchmark:
+    every query is paired with subjects_per_query subjects of its own, a
+    fraction homolog_frac of them homologs of the query.
+
+    Returns dict with q=(res, off, len), s=(res, off, len), pair_query,
+    pair_subject, is_homolog.
+    """
+    ql = lognormal_lengths(rng, n_queries, mean, sd, lo, hi)
+    q = concat_random(rng, ql)
+    n_hom = int(round(subjects_per_query * homolog_frac))
+    n_rand = subjects_per_query - n_hom
+    # homologs of every query
+    rep_q = np.repeat(np.arange(n_queries), n_hom)
+    ident = rng.uniform(ident_range[0], ident_range[1], len(rep_q))
+    h_res, h_off, h_len = mutate(rng, q[0], q[1][rep_q], q[2][rep_q], ident)
+    h_len_c = np.clip(h_len, 1, None)
+    # random subjects
+    rl = lognormal_lengths(rng, n_queries * n_rand, mean, sd, lo, hi)
+    r_res, r_off, r_len = concat_random(rng, rl)
+    # subjects: homologs then randoms
+    s_res = np.concatenate([h_res, r_res])
+    s_off = np.concatenate([h_off, r_off + len(h_res)])
+    s_len = np.concatenate([h_len, r_len]).astype(np.int32)
+    # clip homolog lengths into range by truncation (and pad the rare empty one)
+    too_long = s_len > hi
+    s_len[too_long] = hi
+    pair_query = np.concatenate([rep_q, np.repeat(np.arange(n_queries), n_rand)]).astype(np.int32)
+    pair_subject = np.arange(len(s_len), dtype=np.int32)
+    is_hom = np.concatenate([np.ones(len(rep_q), bool), np.zeros(len(rl), bool)])
+    del h_len_c
+    return dict(q=q, s=(s_res, s_off, s_len), pair_query=pair_query,
+                pair_subject=pair_subject, is_homolog=is_hom)
