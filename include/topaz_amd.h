@@ -74,6 +74,73 @@ int tpz_sw_batch_kernel_ms(const tpz_sw_batch* b, double* ms);
 
 void tpz_sw_batch_destroy(tpz_sw_batch* b);
 
+/* ---- batched traceback statistics of reported hits ---------------------- */
+
+/* ssw_align_stats (ssw.c:900-978) for many hits: reverse pass for the start
+ * positions, then banded global alignment with traceback.  Replaces the
+ * per-hit stats_using_ssw_profile calls of query_perform_statistics
+ * (query.c:276-291).  Inputs per hit: the forward result (read_end1,
+ * ref_end1, score1 < 32767, from tpz_sw_batch_results).  Same residue and
+ * matrix conventions as tpz_sw_batch_create. */
+typedef struct tpz_stats_batch tpz_stats_batch;
+int tpz_stats_batch_create(tpz_stats_batch** out,
+                           int32_t n_queries, const int8_t* q_res,
+                           const int64_t* q_off, const int32_t* q_len,
+                           int32_t n_subjects, const int8_t* s_res,
+                           const int64_t* s_off, const int32_t* s_len,
+                           int32_t n_hits, const int32_t* hit_query,
+                           const int32_t* hit_subject, const int32_t* read_end1,
+                           const int32_t* ref_end1, const int32_t* score1,
+                           const int8_t* matrix, int32_t gap_open,
+                           int32_t gap_extend);
+int tpz_stats_batch_run(tpz_stats_batch* b);
+/* Per hit, the outputs of ssw_align_stats: its return value (1 ok, 0 for
+ * the reference's guarded failures), read_begin1, ref_begin1, length,
+ * gapopen, mismatch, identity.  Any pointer may be null. */
+int tpz_stats_batch_results(const tpz_stats_batch* b, int32_t* rc,
+                            int32_t* read_begin1, int32_t* ref_begin1,
+                            int32_t* length, int32_t* gapopen,
+                            int32_t* mismatch, double* identity);
+void tpz_stats_batch_destroy(tpz_stats_batch* b);
+
+/* ---- batched gapless HSP extension (stage G) ---------------------------- */
+
+/* Device-resident database text: the '\n'-separated internal-alphabet
+ * protein text of the .psq file (db->T, db->Tsize, db.h:28-29) and,
+ * optionally, its suffix array (db->SA, t_size int64 entries). */
+typedef struct tpz_text tpz_text;
+int tpz_text_create(tpz_text** out, const char* text, uint64_t t_size,
+                    const int64_t* sa);
+void tpz_text_destroy(tpz_text* t);
+
+/* One seed of sa_hsp (search.c:99): query number, query offset, and a text
+ * position -- or a suffix-array slot p (text position SA[p]) when the text
+ * was created with a suffix array.  16 bytes. */
+typedef struct {
+    int32_t query;
+    int32_t offset;
+    int64_t position;
+} tpz_seed;
+
+/* A device-resident batch of seeds against one text.  Replaces the sa_hsp
+ * calls of query_find_suffixes (query.c:117,125) and suffix_increment
+ * (suffix.c:47) in bulk.  Queries are internal-alphabet characters as in
+ * seq_ptr(q) (seq.c:123). */
+typedef struct tpz_gapless_batch tpz_gapless_batch;
+int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
+                             int32_t n_queries, const char* q_chars,
+                             const int64_t* q_off, const int32_t* q_len,
+                             int64_t n_seeds, const tpz_seed* seeds,
+                             const int8_t* matrix);
+int tpz_gapless_batch_run(tpz_gapless_batch* b);
+/* Per seed: length as sa_hsp returns it in *gapless_length (-1: first
+ * residues differ, or the position lies outside the text), and score as in
+ * *gapless_score (0 where length is -1: sa_hsp leaves it unwritten). */
+int tpz_gapless_batch_results(const tpz_gapless_batch* b, int32_t* score,
+                              int32_t* length);
+int tpz_gapless_batch_kernel_ms(const tpz_gapless_batch* b, double* ms);
+void tpz_gapless_batch_destroy(tpz_gapless_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

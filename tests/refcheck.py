@@ -43,6 +43,8 @@ class Oracle:
                                   I32, I32, I32, I32]
         L.orc_sw_scan_s1.restype = OrcEnd
         L.orc_sw_scan_s1.argtypes = L.orc_sw_scan.argtypes
+        L.orc_sw_textbook.restype = OrcEnd
+        L.orc_sw_textbook.argtypes = [P8, I32, P8, I32, P8, I32, C.c_int, I32, I32, I32]
         L.orc_ssw_align.restype = C.c_int
         L.orc_ssw_align.argtypes = [P8, I32, P8, I32, P8, I32, I32, I32,
                                     C.POINTER(OrcEnd)]
@@ -73,6 +75,13 @@ class Oracle:
         f = self.L.orc_sw_scan_s1 if closed_form else self.L.orc_sw_scan
         r = f(_ptr8(q), len(q), _ptr8(s), len(s), dir, _ptr8(mat), 25, P, go, ge,
               bias, terminate)
+        return r.score, r.ref, r.read
+
+    def sw_textbook(self, q, s, P, go=11, ge=1, mat=None):
+        mat = matrix() if mat is None else mat
+        bias = -min(0, int(mat.min()))
+        r = self.L.orc_sw_textbook(_ptr8(q), len(q), _ptr8(s), len(s), _ptr8(mat), 25, P,
+                                   go, ge, bias)
         return r.score, r.ref, r.read
 
     def ssw_align(self, q, s, go=11, ge=1, mat=None):

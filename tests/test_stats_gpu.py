@@ -1,0 +1,67 @@
+"""Traceback statistics on the GPU against the reference's ssw_align_stats
+(golden fixture) and the oracle, bit-exact (identity as a double)."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.refcheck import Oracle
+from topaz_amd import matrix, synth
+from topaz_amd.matrices import NAMES
+from topaz_amd.stats import align_stats
+from topaz_amd.sw import SwBatch
+
+pytestmark = pytest.mark.gpu
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+def _same(a, b):
+    return a == b or (math.isnan(a) and math.isnan(b))
+
+
+def test_golden_fixture():
+    d = np.load(GOLD / "sw_pairs.npz")
+    checked = 0
+    # one call per (matrix, gaps) setting; the kernels need gapO > gapE
+    keys = set(zip(d["matrix"], d["gap_open"], d["gap_extend"]))
+    for mi, go, ge in sorted(keys):
+        if go <= ge:
+            continue
+        sel = np.nonzero(d["has_stats"] & (d["matrix"] == mi) & (d["gap_open"] == go) &
+                         (d["gap_extend"] == ge))[0]
+        if len(sel) == 0:
+            continue
+        q = (d["q_res"], d["q_off"], d["q_len"])
+        s = (d["s_res"], d["s_off"], d["s_len"])
+        out = align_stats(q, s, sel, sel, d["read_end"][sel], d["ref_end"][sel], d["score"][sel],
+                          matrix(NAMES[mi]), int(go), int(ge))
+        rc, qb, rb, ln, gop, mis, ident = out
+        assert np.array_equal(rc, d["stats_rc"][sel])
+        assert np.array_equal(qb, d["read_begin"][sel])
+        assert np.array_equal(rb, d["ref_begin"][sel])
+        assert np.array_equal(ln, d["length"][sel])
+        assert np.array_equal(gop, d["gapopen"][sel])
+        assert np.array_equal(mis, d["mismatch"][sel])
+        assert all(_same(a, b) for a, b in zip(ident, d["identity"][sel]))
+        checked += len(sel)
+    assert checked > 800
+
+
+def test_forward_then_stats_against_oracle():
+    rng = np.random.default_rng(41)
+    w = synth.pair_workload(rng, 8, 40, mean=250, sd=200, lo=10, hi=1200)
+    b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], matrix())
+    sc, fe, re_ = b.run().results()
+    b.close()
+    keep = np.nonzero((sc > 0) & (sc < 32767))[0]
+    out = align_stats(w["q"], w["s"], w["pair_query"][keep], w["pair_subject"][keep],
+                      re_[keep], fe[keep], sc[keep], matrix())
+    o = Oracle()
+    q, s = w["q"], w["s"]
+    for k, p in enumerate(keep):
+        qi, si = w["pair_query"][p], w["pair_subject"][p]
+        qq = q[0][q[1][qi]:q[1][qi] + q[2][qi]]
+        ss = s[0][s[1][si]:s[1][si] + s[2][si]]
+        exp = o.ssw_align_stats(qq, ss, int(re_[p]), int(fe[p]), int(sc[p]))
+        got = tuple(int(x[k]) for x in out[:6]) + (float(out[6][k]),)
+        assert got[:6] == tuple(exp[:6]) and _same(got[6], exp[6]), (p, got, exp)

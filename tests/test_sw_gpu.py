@@ -52,7 +52,8 @@ def small_workload(seed, nq=12, per_q=40, mean=150, sd=100, hi=600, lo=1):
 @pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
 def test_random_and_homolog_pairs(schedule):
     sc = compare(small_workload(1), schedule=schedule)
-    assert (sc == 255).sum() == 0           # word results replaced the byte ones
+    # the workload reaches both sides of the byte/word switch (255 - bias)
+    assert (sc < 251).sum() > 100 and ((sc >= 251) & (sc < 32767)).sum() > 20
 
 
 def test_long_and_tiny_queries():

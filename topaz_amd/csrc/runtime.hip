@@ -13,6 +13,8 @@
 
 #include "../../include/topaz_amd.h"
 #include "runtime.h"
+#include "gapless_kernels.h"
+#include "traceback_kernels.h"
 #include "sw_kernels.h"
 
 namespace tpz {
@@ -360,5 +362,360 @@ void tpz_sw_batch_destroy(tpz_sw_batch* b) {
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
     delete b;
 }
+
+
+// ---------------------------------------------------------------------------
+// gapless extension
+// ---------------------------------------------------------------------------
+struct tpz_text {
+    uint64_t t_size = 0;
+    bool has_sa = false;
+    DeviceBuffer d_text, d_sa;
+};
+
+int tpz_text_create(tpz_text** out, const char* text, uint64_t t_size, const int64_t* sa) {
+    if (!out) return fail("tpz_text_create: null output pointer");
+    *out = nullptr;
+    if (ensure_init()) return -1;
+    if (!text && t_size) return fail("tpz_text_create: null text");
+    auto t = new tpz_text();
+    t->t_size = t_size;
+    t->has_sa = sa != nullptr;
+    if (t->d_text.upload(text, t_size + 1) ||
+        (sa && t->d_sa.upload(sa, t_size * 8)) ||
+        check(hipStreamSynchronize(stream()), "text upload")) {
+        delete t;
+        return -1;
+    }
+    *out = t;
+    return 0;
+}
+
+void tpz_text_destroy(tpz_text* t) { delete t; }
+
+struct tpz_gapless_batch {
+    const tpz_text* text = nullptr;
+    int64_t n_seeds = 0;
+    DeviceBuffer d_q, d_q_off, d_q_len, d_seeds, d_tmap, d_qmap, d_matrix, d_score, d_len;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double ms = 0;
+};
+
+// lexicographical.h:19-33 (aa_table2) and :60-74 (upper_aa_only): the
+// internal alphabet stores amino acid k of "ACMLJIVTSKRQZEDBNHYFWXP*G" as
+// 'A'+k (either case); every other byte maps to matrix row 23.
+static void alphabet_maps(int8_t* tmap, int8_t* qmap) {
+    static const char internal_order[] = "ACMLJIVTSKRQZEDBNHYFWXP*G";
+    static const char matrix_order[] = "ARNDCQEGHILKMFPSTWYVBJZX*";
+    for (int c = 0; c < 128; ++c) tmap[c] = 23;
+    for (int k = 0; k < 25; ++k) {
+        const int8_t idx = (int8_t)(strchr(matrix_order, internal_order[k]) - matrix_order);
+        tmap['A' + k] = idx;
+        tmap['a' + k] = idx;
+    }
+    for (int c = 0; c < 128; ++c) {
+        const bool keep = (c >= 'A' && c <= 'Y') || c == '*' || c == '\n';
+        qmap[c] = keep ? tmap[c] : tmap['V'];
+    }
+}
+
+int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
+                             int32_t n_queries, const char* q_chars, const int64_t* q_off,
+                             const int32_t* q_len, int64_t n_seeds, const tpz_seed* seeds,
+                             const int8_t* matrix) {
+    if (!out) return fail("tpz_gapless_batch_create: null output pointer");
+    *out = nullptr;
+    if (ensure_init()) return -1;
+    if (!text) return fail("null text");
+    if (n_queries < 0 || n_seeds < 0) return fail("negative counts");
+    if (!matrix) return fail("null matrix");
+    std::vector<int64_t> qo(n_queries);
+    int64_t qtot = 0;
+    for (int32_t k = 0; k < n_queries; ++k) {
+        if (q_len[k] < 0) return fail("query %d has negative length", k);
+        qo[k] = qtot;
+        qtot += q_len[k];
+    }
+    std::vector<uint8_t> qc((size_t)qtot + 16);
+    for (int32_t k = 0; k < n_queries; ++k)
+        memcpy(qc.data() + qo[k], q_chars + q_off[k], (size_t)q_len[k]);
+    for (int64_t i = 0; i < n_seeds; ++i) {
+        if (seeds[i].query < 0 || seeds[i].query >= n_queries)
+            return fail("seed %lld names query %d out of range", (long long)i, seeds[i].query);
+        if (text->has_sa && (seeds[i].position < 0 || (uint64_t)seeds[i].position >= text->t_size))
+            return fail("seed %lld names suffix-array slot %lld outside [0, %llu)", (long long)i,
+                        (long long)seeds[i].position, (unsigned long long)text->t_size);
+    }
+    int8_t tmap[128], qmap[128];
+    alphabet_maps(tmap, qmap);
+    auto b = new tpz_gapless_batch();
+    b->text = text;
+    b->n_seeds = n_seeds;
+    int rc = 0;
+    rc |= b->d_q.upload(qc.data(), qc.size());
+    rc |= b->d_q_off.upload(qo.data(), qo.size() * 8);
+    rc |= b->d_q_len.upload(q_len, (size_t)n_queries * 4);
+    rc |= b->d_seeds.upload(seeds, (size_t)n_seeds * sizeof(tpz_seed));
+    rc |= b->d_tmap.upload(tmap, 128);
+    rc |= b->d_qmap.upload(qmap, 128);
+    rc |= b->d_matrix.upload(matrix, 625);
+    rc |= b->d_score.alloc((size_t)n_seeds * 4 + 4);
+    rc |= b->d_len.alloc((size_t)n_seeds * 4 + 4);
+    if (!rc) rc |= check(hipMemsetAsync(b->d_score.ptr, 0, b->d_score.bytes, stream()), "memset");
+    for (auto& e : b->ev) rc |= check(hipEventCreate(&e), "hipEventCreate");
+    rc |= check(hipStreamSynchronize(stream()), "gapless upload");
+    if (rc) { tpz_gapless_batch_destroy(b); return -1; }
+    *out = b;
+    return 0;
+}
+
+int tpz_gapless_batch_run(tpz_gapless_batch* b) {
+    if (!b) return fail("null batch");
+    GaplessArgs a{};
+    a.text = (const uint8_t*)b->text->d_text.ptr;
+    a.t_size = (int64_t)b->text->t_size;
+    a.sa = b->text->has_sa ? (const int64_t*)b->text->d_sa.ptr : nullptr;
+    a.q_chars = (const uint8_t*)b->d_q.ptr;
+    a.q_off = (const int64_t*)b->d_q_off.ptr;
+    a.q_len = (const int32_t*)b->d_q_len.ptr;
+    a.seeds = (const SeedRec*)b->d_seeds.ptr;
+    a.n_seeds = b->n_seeds;
+    a.text_map = (const int8_t*)b->d_tmap.ptr;
+    a.query_map = (const int8_t*)b->d_qmap.ptr;
+    a.matrix = (const int8_t*)b->d_matrix.ptr;
+    a.out_score = (int32_t*)b->d_score.ptr;
+    a.out_length = (int32_t*)b->d_len.ptr;
+    const int64_t blocks_needed = (b->n_seeds + 255) / 256;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)num_cu() * 32));
+    if (check(hipEventRecord(b->ev[0], stream()), "event")) return -1;
+    if (b->n_seeds > 0 && check(tpz_launch_gapless(&a, grid, stream()), "gapless")) return -1;
+    if (check(hipEventRecord(b->ev[1], stream()), "event")) return -1;
+    if (check(hipStreamSynchronize(stream()), "gapless")) return -1;
+    float ms = 0;
+    if (check(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "elapsed")) return -1;
+    b->ms = ms;
+    return 0;
+}
+
+int tpz_gapless_batch_results(const tpz_gapless_batch* b, int32_t* score, int32_t* length) {
+    if (!b) return fail("null batch");
+    const size_t n = (size_t)b->n_seeds * 4;
+    if (score && n && check(hipMemcpy(score, b->d_score.ptr, n, hipMemcpyDeviceToHost), "results")) return -1;
+    if (length && n && check(hipMemcpy(length, b->d_len.ptr, n, hipMemcpyDeviceToHost), "results")) return -1;
+    return 0;
+}
+
+int tpz_gapless_batch_kernel_ms(const tpz_gapless_batch* b, double* ms) {
+    if (!b) return fail("null batch");
+    if (ms) *ms = b->ms;
+    return 0;
+}
+
+void tpz_gapless_batch_destroy(tpz_gapless_batch* b) {
+    if (!b) return;
+    for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
+    delete b;
+}
+
+
+// ---------------------------------------------------------------------------
+// traceback statistics
+// ---------------------------------------------------------------------------
+struct tpz_stats_batch {
+    int32_t n = 0;
+    int32_t gap_open = 11, gap_extend = 1, bias = 0;
+    std::vector<int32_t> read_end, ref_end, score;
+    DeviceBuffer d_q_res, d_q_off, d_s_res, d_s_off, d_hq, d_hs, d_read_end, d_ref_end,
+        d_score, d_matrix;
+    DeviceBuffer d_read_begin, d_ref_begin, d_band, d_rc, d_len, d_gapo, d_mis, d_ident;
+    DeviceBuffer d_ws, d_ws_off, d_dirs, d_dir_off;
+    std::vector<int32_t> h_rc, h_read_begin, h_ref_begin, h_len, h_gapo, h_mis, h_ident;
+};
+
+int tpz_stats_batch_create(tpz_stats_batch** out, int32_t n_queries, const int8_t* q_res,
+                           const int64_t* q_off, const int32_t* q_len, int32_t n_subjects,
+                           const int8_t* s_res, const int64_t* s_off, const int32_t* s_len,
+                           int32_t n_hits, const int32_t* hit_query, const int32_t* hit_subject,
+                           const int32_t* read_end1, const int32_t* ref_end1,
+                           const int32_t* score1, const int8_t* matrix, int32_t gap_open,
+                           int32_t gap_extend) {
+    if (!out) return fail("tpz_stats_batch_create: null output pointer");
+    *out = nullptr;
+    if (ensure_init()) return -1;
+    if (n_queries < 0 || n_subjects < 0 || n_hits < 0) return fail("negative counts");
+    if (!(gap_extend >= 0 && gap_open > gap_extend && gap_open <= 255))
+        return fail("gap penalties must satisfy 0 <= gap_extend < gap_open <= 255");
+    if (!matrix) return fail("null matrix");
+    int32_t minw = 0;
+    for (int i = 0; i < 625; ++i) minw = std::min<int32_t>(minw, matrix[i]);
+    for (int32_t h = 0; h < n_hits; ++h) {
+        const int32_t qi = hit_query[h], si = hit_subject[h];
+        if (qi < 0 || qi >= n_queries || si < 0 || si >= n_subjects)
+            return fail("hit %d references query %d / subject %d out of range", h, qi, si);
+        if (read_end1[h] < 0 || read_end1[h] >= q_len[qi] || ref_end1[h] < 0 ||
+            ref_end1[h] >= s_len[si] || score1[h] <= 0 || score1[h] >= 32767)
+            return fail("hit %d: end positions (%d, %d) / score %d outside the query, "
+                        "subject or (0, 32767)", h, read_end1[h], ref_end1[h], score1[h]);
+    }
+    for (int32_t k = 0; k < n_queries; ++k)
+        for (int32_t j = 0; j < q_len[k]; ++j)
+            if (q_res[q_off[k] + j] < 0 || q_res[q_off[k] + j] > 24)
+                return fail("query %d residue out of range", k);
+    for (int32_t k = 0; k < n_subjects; ++k)
+        for (int32_t j = 0; j < s_len[k]; ++j)
+            if (s_res[s_off[k] + j] < 0 || s_res[s_off[k] + j] > 24)
+                return fail("subject %d residue out of range", k);
+    auto b = new tpz_stats_batch();
+    b->n = n_hits;
+    b->gap_open = gap_open; b->gap_extend = gap_extend; b->bias = -minw;
+    b->read_end.assign(read_end1, read_end1 + n_hits);
+    b->ref_end.assign(ref_end1, ref_end1 + n_hits);
+    b->score.assign(score1, score1 + n_hits);
+    int64_t qtot = 0, stot = 0;
+    for (int32_t k = 0; k < n_queries; ++k) qtot = std::max<int64_t>(qtot, q_off[k] + q_len[k]);
+    for (int32_t k = 0; k < n_subjects; ++k) stot = std::max<int64_t>(stot, s_off[k] + s_len[k]);
+    // reverse-pass workspace: H and E over the lane-padded reversed prefix
+    std::vector<int64_t> ws_off(n_hits);
+    int64_t ws = 0;
+    for (int32_t h = 0; h < n_hits; ++h) {
+        ws_off[h] = ws;
+        ws += 2 * 16 * ((int64_t)(read_end1[h] + 1 + 7) / 8);
+    }
+    int rc = 0;
+    rc |= b->d_q_res.upload(q_res, (size_t)qtot + 16);
+    rc |= b->d_q_off.upload(q_off, (size_t)n_queries * 8);
+    rc |= b->d_s_res.upload(s_res, (size_t)stot + 16);
+    rc |= b->d_s_off.upload(s_off, (size_t)n_subjects * 8);
+    rc |= b->d_hq.upload(hit_query, (size_t)n_hits * 4);
+    rc |= b->d_hs.upload(hit_subject, (size_t)n_hits * 4);
+    rc |= b->d_read_end.upload(read_end1, (size_t)n_hits * 4);
+    rc |= b->d_ref_end.upload(ref_end1, (size_t)n_hits * 4);
+    rc |= b->d_score.upload(score1, (size_t)n_hits * 4);
+    rc |= b->d_matrix.upload(matrix, 625);
+    rc |= b->d_ws.alloc((size_t)ws * 4 + 16);
+    rc |= b->d_ws_off.upload(ws_off.data(), ws_off.size() * 8);
+    for (DeviceBuffer* d : {&b->d_read_begin, &b->d_ref_begin, &b->d_band, &b->d_rc, &b->d_len,
+                            &b->d_gapo, &b->d_mis, &b->d_ident})
+        rc |= d->alloc((size_t)n_hits * 4 + 4);
+    rc |= check(hipStreamSynchronize(stream()), "stats upload");
+    if (rc) { delete b; return -1; }
+    *out = b;
+    return 0;
+}
+
+int tpz_stats_batch_run(tpz_stats_batch* b) {
+    if (!b) return fail("null batch");
+    const int32_t n = b->n;
+    if (n == 0) return 0;
+    ReverseArgs r{};
+    r.q_res = (const int8_t*)b->d_q_res.ptr;
+    r.q_off = (const int64_t*)b->d_q_off.ptr;
+    r.s_res = (const int8_t*)b->d_s_res.ptr;
+    r.s_off = (const int64_t*)b->d_s_off.ptr;
+    r.pair_query = (const int32_t*)b->d_hq.ptr;
+    r.pair_subject = (const int32_t*)b->d_hs.ptr;
+    r.read_end = (const int32_t*)b->d_read_end.ptr;
+    r.ref_end = (const int32_t*)b->d_ref_end.ptr;
+    r.score = (const int32_t*)b->d_score.ptr;
+    r.n_pairs = n;
+    r.matrix = (const int8_t*)b->d_matrix.ptr;
+    r.gap_open = b->gap_open; r.gap_extend = b->gap_extend; r.bias = b->bias;
+    r.workspace = (int32_t*)b->d_ws.ptr;
+    r.ws_off = (const int64_t*)b->d_ws_off.ptr;
+    r.out_read_begin = (int32_t*)b->d_read_begin.ptr;
+    r.out_ref_begin = (int32_t*)b->d_ref_begin.ptr;
+    if (check(tpz_launch_sw_reverse(&r, stream()), "sw_reverse")) return -1;
+    std::vector<int32_t> rb(n), qb(n);
+    if (check(hipMemcpyAsync(rb.data(), b->d_ref_begin.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipMemcpyAsync(qb.data(), b->d_read_begin.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipStreamSynchronize(stream()), "sw_reverse"))
+        return -1;
+    for (int32_t h = 0; h < n; ++h)
+        if (rb[h] < 0 || qb[h] < 0 || qb[h] > b->read_end[h])
+            return fail("hit %d: reverse pass found no start (score1 %d not reached)", h, b->score[h]);
+    // band rows: the band half-width stays below refLen + readLen before the
+    // search stops, so 4 (refLen + readLen) + 8 words per row suffice
+    std::vector<int64_t> bo(n);
+    int64_t bw = 0;
+    int32_t band_words = 8;
+    for (int32_t h = 0; h < n; ++h) {
+        const int32_t sum = (b->ref_end[h] - rb[h] + 1) + (b->read_end[h] - qb[h] + 1);
+        band_words = std::max(band_words, 4 * sum + 8);
+    }
+    for (int32_t h = 0; h < n; ++h) { bo[h] = bw; bw += 3 * (int64_t)band_words; }
+    DeviceBuffer d_bo, d_bws;
+    if (d_bo.upload(bo.data(), bo.size() * 8) || d_bws.alloc((size_t)bw * 4 + 16)) return -1;
+    BandArgs a{};
+    a.q_res = r.q_res; a.q_off = r.q_off; a.s_res = r.s_res; a.s_off = r.s_off;
+    a.pair_query = r.pair_query; a.pair_subject = r.pair_subject;
+    a.read_begin = (const int32_t*)b->d_read_begin.ptr;
+    a.read_end = r.read_end;
+    a.ref_begin = (const int32_t*)b->d_ref_begin.ptr;
+    a.ref_end = r.ref_end;
+    a.score = r.score;
+    a.n_pairs = n;
+    a.matrix = r.matrix;
+    a.gap_open = b->gap_open; a.gap_extend = b->gap_extend;
+    a.workspace = (int32_t*)d_bws.ptr;
+    a.ws_off = (const int64_t*)d_bo.ptr;
+    a.ws_band_words = band_words;
+    a.out_band = (int32_t*)b->d_band.ptr;
+    if (check(tpz_launch_band_search(&a, stream()), "band_search")) return -1;
+    std::vector<int32_t> band(n);
+    if (check(hipMemcpyAsync(band.data(), b->d_band.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipStreamSynchronize(stream()), "band_search"))
+        return -1;
+    std::vector<int64_t> dofs(n);
+    int64_t dtot = 0;
+    for (int32_t h = 0; h < n; ++h) {
+        dofs[h] = dtot;
+        if (band[h] >= 0) dtot += (int64_t)(2 * band[h] + 1) * (b->read_end[h] - qb[h] + 1) * 3;
+    }
+    DeviceBuffer d_dirs, d_doff;
+    if (d_dirs.alloc((size_t)dtot + 16) || d_doff.upload(dofs.data(), dofs.size() * 8)) return -1;
+    a.dirs = (int8_t*)d_dirs.ptr;
+    a.dir_off = (const int64_t*)d_doff.ptr;
+    a.out_rc = (int32_t*)b->d_rc.ptr;
+    a.out_length = (int32_t*)b->d_len.ptr;
+    a.out_gapopen = (int32_t*)b->d_gapo.ptr;
+    a.out_mismatch = (int32_t*)b->d_mis.ptr;
+    a.out_identical = (int32_t*)b->d_ident.ptr;
+    if (check(tpz_launch_band_trace(&a, stream()), "band_trace")) return -1;
+    b->h_rc.resize(n); b->h_read_begin = qb; b->h_ref_begin = rb;
+    b->h_len.resize(n); b->h_gapo.resize(n); b->h_mis.resize(n); b->h_ident.resize(n);
+    if (check(hipMemcpyAsync(b->h_rc.data(), b->d_rc.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipMemcpyAsync(b->h_len.data(), b->d_len.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipMemcpyAsync(b->h_gapo.data(), b->d_gapo.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipMemcpyAsync(b->h_mis.data(), b->d_mis.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipMemcpyAsync(b->h_ident.data(), b->d_ident.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
+        check(hipStreamSynchronize(stream()), "band_trace"))
+        return -1;
+    return 0;
+}
+
+int tpz_stats_batch_results(const tpz_stats_batch* b, int32_t* rc, int32_t* read_begin1,
+                            int32_t* ref_begin1, int32_t* length, int32_t* gapopen,
+                            int32_t* mismatch, double* identity) {
+    if (!b) return fail("null batch");
+    if ((int32_t)b->h_rc.size() != b->n) return fail("tpz_stats_batch_results before run");
+    for (int32_t h = 0; h < b->n; ++h) {
+        if (rc) rc[h] = b->h_rc[h];
+        if (read_begin1) read_begin1[h] = b->h_read_begin[h];
+        if (ref_begin1) ref_begin1[h] = b->h_ref_begin[h];
+        if (length) length[h] = b->h_len[h];
+        if (gapopen) gapopen[h] = b->h_gapo[h];
+        if (mismatch) mismatch[h] = b->h_mis[h];
+        if (identity) {
+            // ssw.c:741: identical / (double) length; -1.0 / 0.0 for the
+            // size guard / trace failure (identity is never assigned there)
+            const int32_t ident = b->h_ident[h];
+            identity[h] = ident == -1 ? -1.0 : ident == -2 ? 0.0
+                                                           : ident / (double)b->h_len[h];
+        }
+    }
+    return 0;
+}
+
+void tpz_stats_batch_destroy(tpz_stats_batch* b) { delete b; }
 
 }  // extern "C"

@@ -1,0 +1,59 @@
+// Kernel-side argument blocks of the traceback-statistics kernels.
+#ifndef TOPAZ_AMD_TRACEBACK_KERNELS_H
+#define TOPAZ_AMD_TRACEBACK_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tpz {
+
+struct ReverseArgs {
+    const int8_t* q_res;
+    const int64_t* q_off;
+    const int8_t* s_res;
+    const int64_t* s_off;
+    const int32_t* pair_query;
+    const int32_t* pair_subject;
+    const int32_t* read_end;       // read_end1 of each hit
+    const int32_t* ref_end;        // ref_end1
+    const int32_t* score;          // score1
+    int32_t n_pairs;
+    const int8_t* matrix;
+    int32_t gap_open, gap_extend, bias;
+    int32_t* workspace;            // 2 * 16 * ceil((read_end1 + 1) / 8) words per hit
+    const int64_t* ws_off;
+    int32_t* out_read_begin;
+    int32_t* out_ref_begin;
+};
+
+struct BandArgs {
+    const int8_t* q_res;
+    const int64_t* q_off;
+    const int8_t* s_res;
+    const int64_t* s_off;
+    const int32_t* pair_query;
+    const int32_t* pair_subject;
+    const int32_t* read_begin;
+    const int32_t* read_end;
+    const int32_t* ref_begin;
+    const int32_t* ref_end;
+    const int32_t* score;
+    int32_t n_pairs;
+    const int8_t* matrix;
+    int32_t gap_open, gap_extend;
+    int32_t* workspace;            // three band rows per hit
+    const int64_t* ws_off;
+    int32_t ws_band_words;         // words per band row available per hit
+    int32_t* out_band;             // final band half-width, -1: size guard
+    int8_t* dirs;                  // direction matrices
+    const int64_t* dir_off;
+    int32_t* out_rc, *out_length, *out_gapopen, *out_mismatch, *out_identical;
+};
+
+}  // namespace tpz
+
+extern "C" hipError_t tpz_launch_sw_reverse(const tpz::ReverseArgs* a, hipStream_t stream);
+extern "C" hipError_t tpz_launch_band_search(const tpz::BandArgs* a, hipStream_t stream);
+extern "C" hipError_t tpz_launch_band_trace(const tpz::BandArgs* a, hipStream_t stream);
+
+#endif

@@ -35,8 +35,21 @@ def lognormal_lengths(rng, n, mean=350.0, sd=250.0, lo=30, hi=5000):
     return np.clip(np.rint(x), lo, hi).astype(np.int32)
 
 
+def _quantile_table(cdf):
+    """Map a 16-bit uniform draw to a residue: entry u is the first index
+    whose cumulative probability exceeds (u + 0.5) / 65536."""
+    u = (np.arange(65536) + 0.5) / 65536
+    return np.minimum(np.searchsorted(cdf, u, side="right"), len(cdf) - 1).astype(np.int8)
+
+
+_BG_TABLE = None
+
+
 def random_residues(rng, total):
-    return rng.choice(25, size=int(total), p=background()).astype(np.int8)
+    global _BG_TABLE
+    if _BG_TABLE is None:
+        _BG_TABLE = _quantile_table(np.cumsum(background()))
+    return _BG_TABLE[rng.integers(0, 65536, int(total), dtype=np.uint16)]
 
 
 def concat_random(rng, lengths):
@@ -60,6 +73,7 @@ def _substitution_cdf():
 
 
 _CDF = None
+_CDF_FLAT = None
 
 
 def mutate(rng, res, off, lens, identity, indel_rate=0.02, ext_p=0.5):
@@ -68,23 +82,25 @@ def mutate(rng, res, off, lens, identity, indel_rate=0.02, ext_p=0.5):
     res/off/lens: flat source sequences; identity: per-sequence target identity.
     Returns (flat residues, offsets, lengths) of the homologs.
     """
-    global _CDF
+    global _CDF, _CDF_FLAT
     if _CDF is None:
         _CDF = _substitution_cdf()
+        _CDF_FLAT = np.concatenate([_quantile_table(row) for row in _CDF])
     lens = np.asarray(lens, np.int64)
     n = len(lens)
     seg = np.repeat(np.arange(n), lens)
-    src = np.concatenate([res[o:o + l] for o, l in zip(off, lens)]) if n else \
-        np.zeros(0, np.int8)
-    src = src.astype(np.int64)
+    total = int(lens.sum())
+    starts_out = np.zeros(n, np.int64)
+    if n:
+        starts_out[1:] = np.cumsum(lens)[:-1]
+    gather = np.repeat(np.asarray(off, np.int64) - starts_out, lens) + np.arange(total)
+    src = res[gather].astype(np.int64)
     ident = np.asarray(identity, np.float64)[seg]
     # substitutions
     sub = rng.random(len(src)) >= ident
-    u = rng.random(int(sub.sum()))
+    u = rng.integers(0, 65536, int(sub.sum()), dtype=np.uint16)
     a = src[sub]
-    cdf = _CDF[a]
-    b = (cdf < u[:, None]).sum(axis=1)
-    b = np.minimum(b, 24)
+    b = _CDF_FLAT[a * 65536 + u]
     out = src.copy()
     out[sub] = b
     # indel events: half deletions, half insertions, geometric lengths
@@ -97,9 +113,8 @@ def mutate(rng, res, off, lens, identity, indel_rate=0.02, ext_p=0.5):
     # deletion runs, clipped at the end of their sequence (difference array)
     dpos = np.nonzero(is_del)[0]
     dend = np.minimum(dpos + rng.geometric(ext_p, len(dpos)), ends[seg[dpos]])
-    diff = np.zeros(len(out) + 1, np.int64)
-    np.add.at(diff, dpos, 1)
-    np.add.at(diff, dend, -1)
+    diff = (np.bincount(dpos, minlength=len(out) + 1) -
+            np.bincount(dend, minlength=len(out) + 1))
     keep = np.cumsum(diff[:-1]) == 0
     keep[starts[lens > 0]] = True       # never delete a whole sequence
     ins_len = np.zeros(len(out), np.int64)
