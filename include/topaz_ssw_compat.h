@@ -1,0 +1,168 @@
+/*
+ * topaz_ssw_compat.h -- drop-in ABI for TOPAZ's alignment boundary.
+ *
+ * libtopaz_amd.so exports these functions with the reference's names,
+ * signatures and struct layouts, so TOPAZ's own query.c / suffix.c / engine.c
+ * link against it in place of ssw.o, ssw_wrapper.o and search.c's sa_hsp.
+ * Every call computes on the GPU.  Errors end the process with a message,
+ * as the reference's own fatal paths do (query.c:311-314): these calls have
+ * no error return.
+ *
+ * The struct definitions below restate the reference's layouts field for
+ * field (same order and types), because callers compiled against the
+ * reference headers pass them in.  A program that includes the reference
+ * headers should not include this file.
+ */
+#ifndef TOPAZ_SSW_COMPAT_H
+#define TOPAZ_SSW_COMPAT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ssw.h:31-41.  The profile arrays are built on the device per call; the
+ * two pointers are non-null markers.  readLen is read by callers
+ * (ssw_wrapper.c:179, 245). */
+typedef struct {
+    void* profile_byte;
+    void* profile_word;
+    int8_t* read;
+    const int8_t* mat;
+    int32_t readLen;
+    int32_t n;
+    uint8_t bias;
+} s_profile;
+
+/* ssw.h:57-67 */
+typedef struct {
+    uint16_t score1;
+    uint16_t score2;
+    int32_t ref_begin1;
+    int32_t ref_end1;
+    int32_t read_begin1;
+    int32_t read_end1;
+    int32_t ref_end2;
+    uint32_t* cigar;
+    int32_t cigarLen;
+} s_align;
+
+/* seq.h:12-31 */
+typedef struct {
+    char* buf;
+    int len;
+} buffer_t;
+
+typedef struct {
+    buffer_t id_buf;
+    buffer_t seq_buf;
+    int id_len;
+    int seq_len;
+} seq_t;
+
+/* options.h:26-53 */
+typedef struct {
+    int num_threads;
+    int num_hits;
+    double evalue;
+    int smithwaterman;
+    int gap_open;
+    int gap_extend;
+    char* sub_name;
+    int8_t (*substitution)[625];
+    uint64_t db_size;
+    uint64_t db_proteins;
+    double lambda;
+    double k;
+    double H;
+    int min_hit_length;
+    int min_hit_score;
+    int number_of_alignments;
+    int number_of_seeds;
+    int seg_enabled;
+    int superfast;
+} options_t;
+
+/* hit.h:74-107 (priority_node is an rbnode_t* in the reference) */
+typedef struct {
+    uint64_t db_index;
+    int is_aligned;
+    int alignment_rank;
+    int rawscore;
+    int mismatch;
+    int length;
+    int gapopen;
+    double identity;
+    double bitscore;
+    double evalue;
+    int qstart;
+    int qend;
+    int sstart;
+    int send;
+    int effective_length;
+    int gapless_score;
+    int gapless_length;
+    int suffix_counter;
+    int suffix_offset;
+    int suffix_multiple;
+    double time_stats;
+    double time_align;
+    void* priority_node;
+} hit_t;
+
+/* db.h:10-39 (built without USE_LCP) */
+typedef struct {
+    uint64_t header_start;
+    uint64_t protein_start;
+    int header_len;
+    int protein_len;
+} position_t;
+
+typedef struct {
+    char* prefix;
+    char* headers;
+    uint64_t Hsize;
+    position_t* index;
+    uint64_t Psize;
+    char* T;
+    uint64_t Tsize;
+    int64_t* SA;
+    uint64_t SAsize;
+} db_t;
+
+/* ssw.h:87-210 -- ssw_init (ssw.c:799), init_destroy / init_destroy2
+ * (ssw.c:822-833), ssw_align (ssw.c:835), align_destroy (ssw.c:980),
+ * ssw_align_stats (ssw.c:900).  score_size must be 2 (what TOPAZ passes,
+ * ssw_wrapper.c:127); gap penalties must satisfy gapE < gapO. */
+s_profile* ssw_init(const int8_t* read, const int32_t readLen, const int8_t* mat,
+                    const int32_t n, const int8_t score_size);
+void init_destroy(s_profile* p);
+void init_destroy2(s_profile* p);
+s_align* ssw_align(const s_profile* prof, const int8_t* ref, int32_t refLen,
+                   const uint8_t weight_gapO, const uint8_t weight_gapE,
+                   const uint8_t flag, const uint16_t filters,
+                   const int32_t filterd, const int32_t maskLen);
+void align_destroy(s_align* a);
+int ssw_align_stats(const s_profile* prof, const int8_t* ref, int32_t refLen,
+                    const uint8_t weight_gapO, const uint8_t weight_gapE,
+                    const int32_t maskLen, int32_t read_end1, int32_t ref_end1,
+                    uint16_t score1, int32_t* read_begin1, int32_t* ref_begin1,
+                    int32_t* length, int32_t* gapopen, int32_t* mismatch,
+                    double* identity);
+
+/* ssw_wrapper.h:11-14 -- ssw_wrapper.c:105-130, 157-317, 132-155, 319-435 */
+s_profile* get_ssw_profile(seq_t* s1, options_t* opt);
+void align_using_ssw_profile(s_profile* profile, seq_t* s2, hit_t* hit, options_t* opt);
+void stats_using_ssw_profile(s_profile* profile, seq_t* s2, hit_t* hit, options_t* opt);
+void align_using_ssw(seq_t* s1, seq_t* s2, hit_t* hit, options_t* opt);
+
+/* search.h:14 -- search.c:99-190.  db->T and db->SA are uploaded once per
+ * db_t and stay resident on the device. */
+int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
+           int* gapless_score, int* gapless_length);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -188,3 +188,45 @@ class Reference:
                                     *[C.byref(x) for x in v], C.byref(ident))
         self.L.init_destroy(prof[0])
         return (rc,) + tuple(x.value for x in v) + (ident.value,)
+
+
+# ---- the reference's structs (seq.h, db.h, options.h, hit.h) -------------
+class BufferT(C.Structure):
+    _fields_ = [("buf", C.c_char_p), ("len", C.c_int)]
+
+
+class SeqT(C.Structure):
+    _fields_ = [("id_buf", BufferT), ("seq_buf", BufferT), ("id_len", C.c_int),
+                ("seq_len", C.c_int)]
+
+
+class DbT(C.Structure):
+    _fields_ = [("prefix", C.c_char_p), ("headers", C.c_char_p), ("Hsize", C.c_uint64),
+                ("index", C.c_void_p), ("Psize", C.c_uint64), ("T", C.c_char_p),
+                ("Tsize", C.c_uint64), ("SA", C.POINTER(C.c_int64)), ("SAsize", C.c_uint64)]
+
+
+class OptionsT(C.Structure):
+    _fields_ = [("num_threads", C.c_int), ("num_hits", C.c_int), ("evalue", C.c_double),
+                ("smithwaterman", C.c_int), ("gap_open", C.c_int), ("gap_extend", C.c_int),
+                ("sub_name", C.c_char_p), ("substitution", C.c_void_p),
+                ("db_size", C.c_uint64), ("db_proteins", C.c_uint64),
+                ("lambda_", C.c_double), ("k", C.c_double), ("H", C.c_double),
+                ("min_hit_length", C.c_int), ("min_hit_score", C.c_int),
+                ("number_of_alignments", C.c_int), ("number_of_seeds", C.c_int),
+                ("seg_enabled", C.c_int), ("superfast", C.c_int)]
+
+
+class HitT(C.Structure):
+    _fields_ = [("db_index", C.c_uint64), ("is_aligned", C.c_int),
+                ("alignment_rank", C.c_int), ("rawscore", C.c_int), ("mismatch", C.c_int),
+                ("length", C.c_int), ("gapopen", C.c_int), ("identity", C.c_double),
+                ("bitscore", C.c_double), ("evalue", C.c_double), ("qstart", C.c_int),
+                ("qend", C.c_int), ("sstart", C.c_int), ("send", C.c_int),
+                ("effective_length", C.c_int), ("gapless_score", C.c_int),
+                ("gapless_length", C.c_int), ("suffix_counter", C.c_int),
+                ("suffix_offset", C.c_int), ("suffix_multiple", C.c_int),
+                ("time_stats", C.c_double), ("time_align", C.c_double),
+                ("priority_node", C.c_void_p)]
+
+

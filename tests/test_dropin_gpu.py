@@ -1,0 +1,64 @@
+"""End-to-end drop-in check: TOPAZ's own search driver linked against
+libtopaz_amd.so (in place of ssw.o, ssw_wrapper.o and search.c's sa_hsp)
+prints exactly the hit lists of the stock reference binary.
+
+Both binaries come from the reference sources compiled by oracle/Makefile
+into oracle/_ref/ (the checker); only the alignment boundary differs.
+"""
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from topaz_amd import _lib, synth
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+REF = ROOT / "oracle" / "_ref"
+OBJ = REF / "obj"
+
+
+def _link_dropin(tmp):
+    objs = sorted(str(p) for p in OBJ.rglob("*.o"))
+    keep = [o for o in objs if Path(o).name not in ("ssw.o", "ssw_wrapper.o", "search.o")]
+    search_local = str(tmp / "search_local.o")
+    # keep the reference's sa_search & co., hide its sa_hsp
+    subprocess.run(["objcopy", "-L", "sa_hsp", str(OBJ / "search.o"), search_local], check=True)
+    exe = tmp / "topaz_dropin"
+    libdir = str(_lib.LIB_PATH.parent)
+    subprocess.run(["g++", "-o", str(exe)] + keep + [search_local, "-L" + libdir,
+                    "-ltopaz_amd", "-Wl,-rpath," + libdir, "-lm", "-lpthread"], check=True)
+    return exe
+
+
+@pytest.fixture(scope="module")
+def workspace(tmp_path_factory):
+    if not (REF / "topaz").exists():
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    tmp = tmp_path_factory.mktemp("dropin")
+    rng = np.random.default_rng(2024)
+    db, queries = synth.search_database(rng, 1500, 24, hom_frac=0.5, max_hom=4)
+    synth.write_fasta(tmp / "db.fa", db, "s")
+    synth.write_fasta(tmp / "q.fa", queries, "q")
+    subprocess.run([str(REF / "topaz"), "index", "-f", str(tmp / "db.fa"), "-p", str(tmp / "DB")],
+                   check=True, capture_output=True)
+    return tmp, _link_dropin(tmp)
+
+
+def _search(exe, tmp, *flags):
+    r = subprocess.run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB"),
+                        "-T", "1"] + list(flags), capture_output=True, text=True, timeout=600,
+                       cwd=str(tmp))
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"]])
+def test_hit_lists_identical(workspace, flags):
+    tmp, dropin = workspace
+    ref = _search(REF / "topaz", tmp, *flags)
+    ours = _search(dropin, tmp, *flags)
+    assert ref.count("\n") > 20
+    assert ours == ref

@@ -107,3 +107,13 @@ def test_empty_batch_and_empty_subjects():
                 matrix())
     assert len(b.run().results()[0]) == 0
     b.close()
+
+
+@pytest.mark.parametrize("big_fraction", ["0", "1e30"])
+def test_group_and_wave_paths(big_fraction, monkeypatch):
+    """All tasks worked by whole workgroups (chunks interleaved over four
+    wavefronts), then all tasks by single wavefronts: same results."""
+    monkeypatch.setenv("TOPAZ_AMD_BIG_FRACTION", big_fraction)
+    rng = np.random.default_rng(7)
+    w = synth.pair_workload(rng, 6, 150, mean=400, sd=350, lo=1, hi=2500)
+    compare(w)

@@ -24,6 +24,7 @@
 #include <mutex>
 #include <vector>
 
+#include "../../include/topaz_amd.h"
 #include "../../include/topaz_ssw_compat.h"
 #include "gapless_kernels.h"
 #include "runtime.h"
@@ -92,7 +93,7 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     const size_t m_pad = (size_t)m + 16, n_pad = (((size_t)n + 3) & ~(size_t)3) + 16;
     // layout of the input block
     const size_t o_q = 0, o_s = o_q + m_pad, o_mat = o_s + n_pad, o_meta = (o_mat + 625 + 15) & ~(size_t)15;
-    const size_t bytes = o_meta + 64;
+    const size_t bytes = o_meta + 128;
     c.host.assign(bytes, 0);
     char* h = c.host.data();
     memcpy(h + o_q, read, (size_t)m);
@@ -109,9 +110,14 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     meta32[3] = 0;                 // pair_subject
     meta32[4] = 0;                 // task_begin[0]
     meta32[5] = 1;                 // task_begin[1]
+    meta32[6] = 0;                 // item 0: task 0 on wavefront 0
+    meta32[7] = -1;
+    meta32[8] = -1;
+    meta32[9] = -1;
+    meta32[10] = 0;                // error flag
     char* d = (char*)c.in.get(bytes);
     int32_t* dout = (int32_t*)c.out.get(64);
-    const size_t ws_words = (size_t)2 * (n + 2) * 64;
+    const size_t ws_words = (size_t)4 * 2 * (n + 2) * 64;     // four wavefront buffers
     uint32_t* dws = (uint32_t*)c.ws.get(ws_words * 4);
     hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
     SwKernelArgs a{};
@@ -125,8 +131,10 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.pair_subject = (const int32_t*)(d + o_meta + 28);
     a.pair_index = nullptr;
     a.task_begin = (const int32_t*)(d + o_meta + 32);
-    a.n_tasks = 1;
+    a.item_task = (const int32_t*)(d + o_meta + 40);
+    a.n_items = 1;
     a.task_counter = nullptr;
+    a.error = (int32_t*)(d + o_meta + 56);
     a.matrix = (const int8_t*)(d + o_mat);
     a.gap_open = go;
     a.gap_extend = ge;
@@ -374,45 +382,101 @@ void align_using_ssw(seq_t* s1, seq_t* s2, hit_t* hit, options_t* opt) {
 // search.h: sa_hsp on a device-resident copy of db->T / db->SA
 // ---------------------------------------------------------------------------
 namespace {
+// db->T (and db->SA) uploaded once per db_t and kept on the device.
 struct ResidentText {
     const char* host_T = nullptr;
     uint64_t t_size = 0;
     const int64_t* host_SA = nullptr;
-    tpz_text* text = nullptr;
+    void* d_text = nullptr;
+    const int64_t* text_sa = nullptr;
 };
 std::mutex g_text_mu;
-std::vector<ResidentText> g_texts;
+std::vector<ResidentText*> g_texts;
 
-tpz_text* resident_text(const db_t* db) {
+struct Maps {
+    int8_t tmap[128], qmap[128];
+    Maps() {
+        // lexicographical.h:19-33 (aa_table2) and :60-74 (upper_aa_only)
+        const Alphabet& ab = alphabet();
+        for (int c = 0; c < 128; ++c) {
+            tmap[c] = ab.table2[c];
+            const bool keep = (c >= 'A' && c <= 'Y') || c == '*' || c == '\n';
+            qmap[c] = keep ? ab.table2[c] : ab.table2['V'];
+        }
+    }
+} const g_maps;
+
+const ResidentText& resident_text(const db_t* db) {
     std::lock_guard<std::mutex> lk(g_text_mu);
-    for (auto& t : g_texts)
-        if (t.host_T == db->T && t.t_size == db->Tsize && t.host_SA == db->SA) return t.text;
-    ResidentText r;
-    r.host_T = db->T;
-    r.t_size = db->Tsize;
-    r.host_SA = db->SA;
-    if (tpz_text_create(&r.text, db->T, db->Tsize, db->SA)) die("sa_hsp: text upload");
+    for (auto* t : g_texts)
+        if (t->host_T == db->T && t->t_size == db->Tsize && t->host_SA == db->SA) return *t;
+    auto* r = new ResidentText();
+    r->host_T = db->T;
+    r->t_size = db->Tsize;
+    r->host_SA = db->SA;
+    hip_or_die(hipMalloc(&r->d_text, db->Tsize + 16), "hipMalloc text");
+    hip_or_die(hipMemcpy(r->d_text, db->T, db->Tsize, hipMemcpyHostToDevice), "upload text");
+    if (db->SA) {
+        void* sa = nullptr;
+        hip_or_die(hipMalloc(&sa, db->Tsize * 8 + 16), "hipMalloc SA");
+        hip_or_die(hipMemcpy(sa, db->SA, db->Tsize * 8, hipMemcpyHostToDevice), "upload SA");
+        r->text_sa = (const int64_t*)sa;
+    }
     g_texts.push_back(r);
-    return r.text;
+    return *r;
 }
 }  // namespace
 
 int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
            int* gapless_score, int* gapless_length) {
-    tpz_text* text = resident_text(db);
-    const int64_t qo = 0;
-    const int32_t ql = q->seq_len;
-    tpz_seed seed{0, qoffset, position};
-    tpz_gapless_batch* b = nullptr;
-    if (tpz_gapless_batch_create(&b, text, 1, q->seq_buf.buf, &qo, &ql, 1, &seed,
-                                 (const int8_t*)*opt->substitution))
-        die("sa_hsp");
-    if (tpz_gapless_batch_run(b)) die("sa_hsp");
-    int32_t sc = 0, ln = 0;
-    if (tpz_gapless_batch_results(b, &sc, &ln)) die("sa_hsp");
-    tpz_gapless_batch_destroy(b);
-    *gapless_length = ln;
-    if (ln != -1) *gapless_score = sc;                           // search.c:113-116
+    const ResidentText& rt = resident_text(db);
+    if (rt.text_sa && (position < 0 || (uint64_t)position >= db->Tsize)) {
+        fprintf(stderr, "topaz_amd: sa_hsp: suffix-array slot %lld outside the text\n",
+                (long long)position);
+        _exit(EXIT_FAILURE);
+    }
+    ThreadCtx& c = ctx();
+    // one seed against the resident text: query bytes, seed, maps, matrix
+    const size_t m = (size_t)q->seq_len;
+    const size_t o_q = 0, o_seed = (m + 16 + 15) & ~(size_t)15, o_maps = o_seed + 32,
+                 o_mat = o_maps + 256, o_meta = (o_mat + 625 + 15) & ~(size_t)15;
+    const size_t bytes = o_meta + 32;
+    c.host.assign(bytes, 0);
+    char* h = c.host.data();
+    memcpy(h + o_q, q->seq_buf.buf, m);
+    SeedRec seed{0, qoffset, position};
+    memcpy(h + o_seed, &seed, sizeof seed);
+    memcpy(h + o_maps, g_maps.tmap, 128);
+    memcpy(h + o_maps + 128, g_maps.qmap, 128);
+    memcpy(h + o_mat, (const int8_t*)*opt->substitution, 625);
+    const int64_t zero64 = 0;
+    const int32_t qlen = (int32_t)m;
+    memcpy(h + o_meta, &zero64, 8);
+    memcpy(h + o_meta + 8, &qlen, 4);
+    char* d = (char*)c.in.get(bytes);
+    int32_t* dout = (int32_t*)c.out.get(64);
+    hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
+    hip_or_die(hipMemsetAsync(dout, 0, 8, c.st), "memset");
+    GaplessArgs a{};
+    a.text = (const uint8_t*)rt.d_text;
+    a.t_size = (int64_t)db->Tsize;
+    a.sa = rt.text_sa;
+    a.q_chars = (const uint8_t*)(d + o_q);
+    a.q_off = (const int64_t*)(d + o_meta);
+    a.q_len = (const int32_t*)(d + o_meta + 8);
+    a.seeds = (const SeedRec*)(d + o_seed);
+    a.n_seeds = 1;
+    a.text_map = (const int8_t*)(d + o_maps);
+    a.query_map = (const int8_t*)(d + o_maps + 128);
+    a.matrix = (const int8_t*)(d + o_mat);
+    a.out_score = dout;
+    a.out_length = dout + 1;
+    hip_or_die(tpz_launch_gapless(&a, 1, c.st), "gapless");
+    int32_t r[2];
+    hip_or_die(hipMemcpyAsync(r, dout, 8, hipMemcpyDeviceToHost, c.st), "readback");
+    hip_or_die(hipStreamSynchronize(c.st), "gapless");
+    *gapless_length = r[1];
+    if (r[1] != -1) *gapless_score = r[0];                       // search.c:113-116
     return 0;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -27,7 +28,8 @@ static std::mutex g_init_mu;
 static int g_device = -1;
 static hipStream_t g_stream = nullptr;
 static int g_num_cu = 0;
-constexpr int kWavesPerCu = 16;   // resident wavefronts of sw_forward per CU
+constexpr int kGroupsPerCu = 2;   // resident sw_forward workgroups (4 wavefronts) per CU
+constexpr int kGroupWaves = 4;
 
 int fail(const char* fmt, ...) {
     char buf[512];
@@ -122,15 +124,57 @@ struct tpz_sw_batch {
     DeviceBuffer d_q_res, d_q_off, d_q_len, d_s_res, d_s_off, d_s_len;
     DeviceBuffer d_pair_query, d_pair_subject, d_matrix;
     DeviceBuffer d_score, d_ref, d_read;            // results
-    DeviceBuffer d_workspace, d_counter;
-    // tasks grouped per query (plain schedule) and by cost (persistent)
-    Tasks tasks_plain, tasks_cost;
-    DeviceBuffer d_plain_index, d_plain_begin, d_cost_index, d_cost_begin;
+    DeviceBuffer d_workspace, d_counter, d_error;
+    Tasks tasks;                                    // grouped per query
+    DeviceBuffer d_task_index, d_task_begin;
+    // work items (4 task slots each): per query for the plain schedule, by
+    // decreasing cost for the persistent one
+    std::vector<int32_t> items_plain, items_persistent, plain_query_items;
+    DeviceBuffer d_items_plain, d_items_persistent;
+    int32_t big_tasks = 0;
+    double big_fraction = 0.5;
     size_t ws_slots = 0;
     int64_t ws_words_per_slot = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     double ms = 0;
 };
+
+namespace tpz {
+// Work items of the persistent schedule.  A task whose single-wavefront
+// time would exceed big_fraction of one wavefront's fair share of the batch
+// is worked by a whole workgroup (chunks interleaved over 4 wavefronts);
+// the others go four to an item, one per wavefront, neighbours in cost
+// order.  Items run most expensive first.
+static void build_items(const Tasks& t, int n_slots, double big_fraction,
+                        std::vector<int32_t>& items, int32_t& n_big) {
+    const size_t n = t.cost.size();
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return t.cost[x] > t.cost[y]; });
+    double total = 0;
+    for (int64_t c : t.cost) total += (double)c;
+    const double share = total / std::max(1, n_slots * kGroupWaves);
+    std::vector<std::pair<double, std::array<int32_t, 4>>> list;
+    n_big = 0;
+    size_t k = 0;
+    while (k < n && t.cost[order[k]] > big_fraction * share && t.cost[order[k]] > 0) {
+        list.push_back({t.cost[order[k]] / (double)kGroupWaves, {order[k], -2, -1, -1}});
+        ++n_big;
+        ++k;
+    }
+    while (k < n) {
+        std::array<int32_t, 4> it{-1, -1, -1, -1};
+        const double c0 = (double)t.cost[order[k]];
+        for (int w = 0; w < 4 && k < n; ++w) it[w] = order[k++];
+        list.push_back({c0, it});
+    }
+    std::stable_sort(list.begin(), list.end(),
+                     [](const auto& x, const auto& y) { return x.first > y.first; });
+    items.clear();
+    for (auto& e : list) items.insert(items.end(), e.second.begin(), e.second.end());
+}
+}  // namespace tpz
 
 extern "C" {
 
@@ -233,43 +277,45 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     rc |= b->d_counter.alloc(64);
     if (rc) { delete b; return -1; }
 
-    // tasks: per query (plain) and the same tasks most expensive first
+    rc |= b->d_error.alloc(64);
+    if (!rc) rc |= check(hipMemsetAsync(b->d_error.ptr, 0, 64, stream()), "memset");
+
     std::vector<int32_t> all(n_pairs);
     std::iota(all.begin(), all.end(), 0);
-    build_tasks(all, b->pair_query, b->pair_subject, b->q_len, b->s_len, b->tasks_plain);
-    {
-        const Tasks& tp = b->tasks_plain;
-        std::vector<int32_t> perm(tp.cost.size());
-        std::iota(perm.begin(), perm.end(), 0);
-        std::stable_sort(perm.begin(), perm.end(),
-                         [&](int32_t x, int32_t y) { return tp.cost[x] > tp.cost[y]; });
-        Tasks& tc = b->tasks_cost;
-        for (int32_t p : perm) {
-            tc.begin.push_back((int32_t)tc.index.size());
-            tc.query.push_back(tp.query[p]);
-            tc.cost.push_back(tp.cost[p]);
-            for (int32_t k = tp.begin[p]; k < tp.begin[p + 1]; ++k) tc.index.push_back(tp.index[k]);
-        }
-        tc.begin.push_back((int32_t)tc.index.size());
-    }
-    rc |= b->d_plain_index.upload(b->tasks_plain.index.data(), b->tasks_plain.index.size() * 4);
-    rc |= b->d_plain_begin.upload(b->tasks_plain.begin.data(), b->tasks_plain.begin.size() * 4);
-    rc |= b->d_cost_index.upload(b->tasks_cost.index.data(), b->tasks_cost.index.size() * 4);
-    rc |= b->d_cost_begin.upload(b->tasks_cost.begin.data(), b->tasks_cost.begin.size() * 4);
+    build_tasks(all, b->pair_query, b->pair_subject, b->q_len, b->s_len, b->tasks);
+    rc |= b->d_task_index.upload(b->tasks.index.data(), b->tasks.index.size() * 4);
+    rc |= b->d_task_begin.upload(b->tasks.begin.data(), b->tasks.begin.size() * 4);
 
-    // workspace: one slot (bottom H and F of a chunk, 2 words per column
-    // and lane) per concurrently running wavefront
-    b->ws_words_per_slot = (int64_t)2 * (b->max_subject + 2) * 64;
-    size_t max_tasks_per_query = 0;
+    // plain: per query, its tasks four to an item
+    size_t max_items_per_query = 0;
     {
-        std::vector<int32_t> per_q(n_queries, 0);
-        for (size_t t = 0; t + 1 < b->tasks_plain.begin.size(); ++t)
-            max_tasks_per_query = std::max<size_t>(max_tasks_per_query, ++per_q[b->tasks_plain.query[t]]);
+        const Tasks& t = b->tasks;
+        const int n_tasks = (int)t.begin.size() - 1;
+        int k = 0;
+        while (k < n_tasks) {
+            int e = k;
+            while (e < n_tasks && t.query[e] == t.query[k]) ++e;
+            b->plain_query_items.push_back((int32_t)(b->items_plain.size() / 4));
+            for (int j = k; j < e; j += 4)
+                for (int w = 0; w < 4; ++w) b->items_plain.push_back(j + w < e ? j + w : -1);
+            max_items_per_query = std::max<size_t>(max_items_per_query, (size_t)(e - k + 3) / 4);
+            k = e;
+        }
+        b->plain_query_items.push_back((int32_t)(b->items_plain.size() / 4));
     }
-    const size_t n_tasks_all = b->tasks_plain.begin.size() - 1;
-    const size_t persistent_slots =
-        std::min<size_t>((size_t)std::max(num_cu(), 1) * kWavesPerCu, n_tasks_all);
-    b->ws_slots = std::max<size_t>(std::max(persistent_slots, max_tasks_per_query), 1);
+    const size_t persistent_slots = (size_t)std::max(num_cu(), 1) * kGroupsPerCu;
+    if (const char* env = getenv("TOPAZ_AMD_BIG_FRACTION")) b->big_fraction = atof(env);
+    build_items(b->tasks, (int)persistent_slots, b->big_fraction, b->items_persistent,
+                b->big_tasks);
+    rc |= b->d_items_plain.upload(b->items_plain.data(), b->items_plain.size() * 4);
+    rc |= b->d_items_persistent.upload(b->items_persistent.data(), b->items_persistent.size() * 4);
+
+    // workspace: per workgroup, one boundary buffer (bottom H and F of a
+    // chunk: 2 words per column and lane) per wavefront
+    b->ws_words_per_slot = (int64_t)kGroupWaves * 2 * (b->max_subject + 2) * 64;
+    const size_t n_items_p = b->items_persistent.size() / 4;
+    b->ws_slots = std::max<size_t>(std::max(std::min(persistent_slots, n_items_p),
+                                            max_items_per_query), 1);
     rc |= b->d_workspace.alloc(b->ws_slots * (size_t)b->ws_words_per_slot * 4);
     for (auto& e : b->ev) rc |= check(hipEventCreate(&e), "hipEventCreate");
     rc |= check(hipStreamSynchronize(stream()), "upload sync");
@@ -304,33 +350,28 @@ int tpz_sw_batch_run(tpz_sw_batch* b, int32_t schedule) {
     if (schedule != TPZ_SCHED_PLAIN && schedule != TPZ_SCHED_PERSISTENT)
         return fail("unknown schedule %d", schedule);
     SwKernelArgs a = make_args(b);
+    a.pair_index = (const int32_t*)b->d_task_index.ptr;
+    a.task_begin = (const int32_t*)b->d_task_begin.ptr;
+    a.error = (int32_t*)b->d_error.ptr;
     if (check(hipEventRecord(b->ev[0], stream()), "event")) return -1;
     if (schedule == TPZ_SCHED_PERSISTENT) {
-        const Tasks& t = b->tasks_cost;
-        const int n_tasks = (int)t.begin.size() - 1;
-        if (n_tasks > 0) {
-            a.pair_index = (const int32_t*)b->d_cost_index.ptr;
-            a.task_begin = (const int32_t*)b->d_cost_begin.ptr;
-            a.n_tasks = n_tasks;
+        const int n_items = (int)(b->items_persistent.size() / 4);
+        if (n_items > 0) {
+            a.item_task = (const int32_t*)b->d_items_persistent.ptr;
+            a.n_items = n_items;
             a.task_counter = (int32_t*)b->d_counter.ptr;
             if (check(hipMemsetAsync(b->d_counter.ptr, 0, 4, stream()), "memset counter")) return -1;
-            const int grid = (int)std::min<size_t>((size_t)n_tasks, b->ws_slots);
+            const int grid = (int)std::min<size_t>((size_t)n_items, b->ws_slots);
             if (check(tpz_launch_sw_forward(&a, grid, stream()), "sw_forward (persistent)")) return -1;
         }
     } else {
-        // one launch per query, one workgroup per task of that query
-        const Tasks& t = b->tasks_plain;
-        const int n_tasks = (int)t.begin.size() - 1;
-        a.pair_index = (const int32_t*)b->d_plain_index.ptr;
+        // one launch per query, one workgroup per item of that query
         a.task_counter = nullptr;
-        int k = 0;
-        while (k < n_tasks) {
-            int e = k;
-            while (e < n_tasks && t.query[e] == t.query[k]) ++e;
-            a.task_begin = (const int32_t*)b->d_plain_begin.ptr + k;
-            a.n_tasks = e - k;
-            if (check(tpz_launch_sw_forward(&a, e - k, stream()), "sw_forward (plain)")) return -1;
-            k = e;
+        for (size_t qk = 0; qk + 1 < b->plain_query_items.size(); ++qk) {
+            const int i0 = b->plain_query_items[qk], i1 = b->plain_query_items[qk + 1];
+            a.item_task = (const int32_t*)b->d_items_plain.ptr + 4 * (size_t)i0;
+            a.n_items = i1 - i0;
+            if (check(tpz_launch_sw_forward(&a, i1 - i0, stream()), "sw_forward (plain)")) return -1;
         }
     }
     if (check(hipEventRecord(b->ev[1], stream()), "event")) return -1;
@@ -338,6 +379,9 @@ int tpz_sw_batch_run(tpz_sw_batch* b, int32_t schedule) {
     float ms = 0;
     if (check(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "elapsed")) return -1;
     b->ms = ms;
+    int32_t err = 0;
+    if (check(hipMemcpy(&err, b->d_error.ptr, 4, hipMemcpyDeviceToHost), "error flag")) return -1;
+    if (err) return fail("sw_forward: a cross-wavefront wait timed out");
     return 0;
 }
 

@@ -9,9 +9,10 @@ namespace tpz {
 
 // Device pointers of one forward pass over a batch of (query, subject) pairs.
 // Residues are substitution-matrix indices 0..24; subjects start on 4-byte
-// boundaries of s_res.  A task is a range [task_begin[t], task_begin[t+1])
-// of at most 128 entries of pair_index (or of the pair numbers themselves
-// when pair_index is null), all with the same query.
+// boundaries of s_res and are padded with residue 25.  A task is a range of
+// at most 128 entries of pair_index (or of the pair numbers themselves when
+// pair_index is null), all with the same query.  A work item is either one
+// task worked by a whole workgroup, or up to four tasks, one per wavefront.
 struct SwKernelArgs {
     const int8_t* q_res;
     const int64_t* q_off;
@@ -22,9 +23,11 @@ struct SwKernelArgs {
     const int32_t* pair_query;
     const int32_t* pair_subject;
     const int32_t* pair_index;     // may be null
-    const int32_t* task_begin;     // n_tasks + 1 entries
-    int32_t n_tasks;
-    int32_t* task_counter;         // null: one workgroup per task
+    const int32_t* task_begin;     // task t = entries [task_begin[t], task_begin[t+1])
+    const int32_t* item_task;      // 4 entries per work item (see sw_forward_kernel)
+    int32_t n_items;
+    int32_t* task_counter;         // null: workgroup b runs item b
+    int32_t* error;                // set when a cross-wavefront wait gave up
     const int8_t* matrix;          // 25 x 25, W[s * 25 + q]
     int32_t gap_open;
     int32_t gap_extend;

@@ -65,11 +65,13 @@ struct WaveLds {
     uint16_t saved[2][kRows][64];          // H column of the chunk best, per half
 };
 
-// Best of the chunks finished so far, per half of the lane.
+// Best of the chunks finished so far, per half of the lane; chunk is the
+// chunk index of that best (ties on score and column go to the earlier one).
 struct LaneBest {
     int score[2];
     int ref[2];
     int read[2];
+    int chunk[2];
 };
 
 __device__ __forceinline__ uint32_t load4(const int8_t* s, int i, int n) {
@@ -77,12 +79,57 @@ __device__ __forceinline__ uint32_t load4(const int8_t* s, int i, int n) {
     return i < n ? *reinterpret_cast<const uint32_t*>(s + i) : 0x19191919u;
 }
 
+constexpr int kWaves = 4;          // wavefronts per workgroup
+constexpr int kSyncCols = 8;       // progress is published every 8 columns
+constexpr int kAhead = 16;         // a consumer runs at least 16 columns behind
+constexpr int kSpinLimit = 1 << 24;
+
+// Where a chunk reads its top boundary and writes its bottom one, and how it
+// synchronises with the wavefront that produces its top (another wavefront
+// of the same workgroup working on the chunk above).
+struct ChunkIO {
+    const uint32_t* top;      // null: first chunk (zero boundary)
+    uint32_t* bottom;         // null: last chunk
+    int* wait_ctr;            // producer progress in LDS, or null
+    int wait_base;            // producer progress at column 0 of this chunk's top
+    int* pub_ctr;             // own progress in LDS, or null
+    int pub_base;
+    int* error;               // LDS flag: a wait timed out
+    int* g_error;             // global flag for the host
+};
+
+__device__ __forceinline__ bool wait_for(const ChunkIO& io, int need) {
+    int spins = 0;
+    for (;;) {
+        const int v = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(io.wait_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (v >= need) break;
+        const int e = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(io.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (e) return false;
+        if (++spins > kSpinLimit) {
+            __hip_atomic_store(io.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (threadIdx.x % 64 == 0) atomicExch(io.g_error, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
+}
+
+__device__ __forceinline__ void publish(const ChunkIO& io, int value) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (threadIdx.x % 64 == 0)
+        __hip_atomic_store(io.pub_ctr, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Event bookkeeping for one finished column: if a half's column max rises
 // strictly above the chunk best, remember the column and save that half of
 // the column's H values.
 __device__ __forceinline__ void column_event(WaveLds& lds, const s2 (&Hc)[kRows], s2 cm,
                                              s2& bc, int col, int& ci_lo, int& ci_hi) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % 64;
     if (cm.x > bc.x) {
         ci_lo = col;
 #pragma unroll
@@ -98,14 +145,13 @@ __device__ __forceinline__ void column_event(WaveLds& lds, const s2 (&Hc)[kRows]
 
 // One chunk of kRows rows [j0, j0 + kRows) over all subject columns, two
 // columns at a time: the second column runs one row behind the first, so
-// every step advances two independent dependency chains.
-__device__ __forceinline__ void run_chunk(const SwKernelArgs& a, WaveLds& lds,
-                                          uint32_t* ws, int ncols,
-                                          const int8_t* s_lo, int n_lo,
-                                          const int8_t* s_hi, int n_hi,
-                                          bool first, bool store_bottom, int j0,
+// every step advances two independent dependency chains.  Returns false if
+// a wait on the producer of the top boundary gave up.
+__device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, const ChunkIO& io,
+                                          int ncols, const int8_t* s_lo, int n_lo,
+                                          const int8_t* s_hi, int n_hi, int j0, int chunk,
                                           LaneBest& best) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % 64;
     const s2 go = s2{(short)a.gap_open, (short)a.gap_open};
     const s2 ge = s2{(short)a.gap_extend, (short)a.gap_extend};
     const s2 zero = s2{0, 0};
@@ -116,24 +162,36 @@ __device__ __forceinline__ void run_chunk(const SwKernelArgs& a, WaveLds& lds,
     int ci_lo = -1, ci_hi = -1;
     s2 hdiag = zero;              // H(i-1, j0-1)
 
-    uint32_t* wsH = ws;                         // [col][lane]
-    uint32_t* wsF = ws + (size_t)ncols * 64;
+    const uint32_t* tH = io.top;                         // [col][lane]
+    const uint32_t* tF = io.top ? io.top + (size_t)ncols * 64 : nullptr;
+    uint32_t* bH = io.bottom;
+    uint32_t* bF = io.bottom ? io.bottom + (size_t)ncols * 64 : nullptr;
 
+    int waited = ncols < kAhead ? ncols : kAhead;       // columns known final
+    if (io.wait_ctr && !wait_for(io, io.wait_base + waited)) return false;
     uint32_t res_lo = load4(s_lo, 0, n_lo), res_hi = load4(s_hi, 0, n_hi);
     uint32_t ta_h = 0, ta_f = 0, tb_h = 0, tb_f = 0;
-    if (!first) {
-        ta_h = wsH[lane]; ta_f = wsF[lane];
-        tb_h = wsH[64 + lane]; tb_f = wsF[64 + lane];
+    if (tH) {
+        ta_h = tH[lane]; ta_f = tF[lane];
+        tb_h = tH[64 + lane]; tb_f = tF[64 + lane];
     }
 
     // ncols is even (the host pads subjects with residue 25 columns)
     for (int i = 0; i < ncols; i += 2) {
+        if (io.wait_ctr && (i % kSyncCols) == 0 && waited < ncols) {
+            // columns up to i + kAhead must be final before they are prefetched
+            const int need = i + kAhead < ncols ? i + kAhead : ncols;
+            if (need > waited) {
+                if (!wait_for(io, io.wait_base + need)) return false;
+                waited = need;
+            }
+        }
         uint32_t na_h = 0, na_f = 0, nb_h = 0, nb_f = 0;
-        if (!first && i + 2 < ncols) {
-            na_h = wsH[(size_t)(i + 2) * 64 + lane];
-            na_f = wsF[(size_t)(i + 2) * 64 + lane];
-            nb_h = wsH[(size_t)(i + 3) * 64 + lane];
-            nb_f = wsF[(size_t)(i + 3) * 64 + lane];
+        if (tH && i + 2 < ncols) {
+            na_h = tH[(size_t)(i + 2) * 64 + lane];
+            na_f = tF[(size_t)(i + 2) * 64 + lane];
+            nb_h = tH[(size_t)(i + 3) * 64 + lane];
+            nb_f = tF[(size_t)(i + 3) * 64 + lane];
         }
         const int sh = (i & 3) * 8;
         const int ra_lo = (res_lo >> sh) & 0xff, ra_hi = (res_hi >> sh) & 0xff;
@@ -201,15 +259,17 @@ __device__ __forceinline__ void run_chunk(const SwKernelArgs& a, WaveLds& lds,
             E[q] = vmax(usub(E[q], ge), hg);
             fb = vmax(usub(fb, ge), hg);
         }
-        if (store_bottom) {
-            wsH[(size_t)i * 64 + lane] = bits(Ha[kRows - 1]);
-            wsF[(size_t)i * 64 + lane] = bits(fa);
-            wsH[(size_t)(i + 1) * 64 + lane] = bits(H[kRows - 1]);
-            wsF[(size_t)(i + 1) * 64 + lane] = bits(fb);
+        if (bH) {
+            bH[(size_t)i * 64 + lane] = bits(Ha[kRows - 1]);
+            bF[(size_t)i * 64 + lane] = bits(fa);
+            bH[(size_t)(i + 1) * 64 + lane] = bits(H[kRows - 1]);
+            bF[(size_t)(i + 1) * 64 + lane] = bits(fb);
         }
         column_event(lds, Ha, cma, bc, i, ci_lo, ci_hi);
         column_event(lds, H, cmb, bc, i + 1, ci_lo, ci_hi);
         ta_h = na_h; ta_f = na_f; tb_h = nb_h; tb_f = nb_f;
+        if (io.pub_ctr && (((i + 2) % kSyncCols) == 0 || i + 2 == ncols))
+            publish(io, io.pub_base + i + 2);
     }
 
     // Lowest row of the saved column equal to the chunk best, then merge:
@@ -228,15 +288,19 @@ __device__ __forceinline__ void run_chunk(const SwKernelArgs& a, WaveLds& lds,
             best.score[half] = b;
             best.ref[half] = ci;
             best.read[half] = j0 + row;
+            best.chunk[half] = chunk;
         }
     }
+    return true;
 }
 
 // Substitution scores of rows j0 .. j0+31 for all 26 residues; rows past
-// the query end score kPadScore.
+// the query end score kPadScore.  Written and read by one wavefront only.
 __device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds& lds,
                                               const int8_t* q, int m, int j0) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % 64;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     for (int e = lane; e < kRows * kNRes; e += 64) {
         const int r = e / kNRes, s = e - r * kNRes;
         const int j = j0 + r;
@@ -244,74 +308,188 @@ __device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds& ld
         if (s < kPadResidue && j < m) w = a.matrix[s * 25 + q[j]];
         lds.prof[e] = (int16_t)w;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One task = one query and up to 128 of its pairs.  With a task counter the
-// grid is persistent: each workgroup (one wavefront) pulls tasks until none
-// are left and owns workspace slot blockIdx.x; without one, workgroup b runs
-// task b.
-__global__ void __launch_bounds__(64)
-sw_forward_kernel(SwKernelArgs a) {
-    __shared__ WaveLds lds;
-    const int lane = threadIdx.x;
-    uint32_t* ws = a.workspace + (size_t)blockIdx.x * a.ws_words_per_slot;
-    for (;;) {
-        int task;
-        if (a.task_counter) {
-            int t0 = 0;
-            if (lane == 0) t0 = atomicAdd(a.task_counter, 1);
-            task = __builtin_amdgcn_readfirstlane(t0);
-        } else {
-            task = blockIdx.x;
-        }
-        if (task >= a.n_tasks) break;
-        const int p0 = a.task_begin[task], p1 = a.task_begin[task + 1];
-        const int np = p1 - p0;
-        const bool has_lo = lane < np, has_hi = lane + 64 < np;
-        const int k_lo = p0 + (has_lo ? lane : 0);
-        const int k_hi = p0 + (has_hi ? lane + 64 : 0);
-        const int pair_lo = a.pair_index ? a.pair_index[k_lo] : k_lo;
-        const int pair_hi = a.pair_index ? a.pair_index[k_hi] : k_hi;
-        const int qid = a.pair_query[pair_lo];
-        const int8_t* q = a.q_res + a.q_off[qid];
-        const int m = a.q_len[qid];
-        const int sid_lo = a.pair_subject[pair_lo], sid_hi = a.pair_subject[pair_hi];
-        const int8_t* s_lo = a.s_res + a.s_off[sid_lo];
-        const int8_t* s_hi = a.s_res + a.s_off[sid_hi];
-        const int n_lo = has_lo ? a.s_len[sid_lo] : 0;
-        const int n_hi = has_hi ? a.s_len[sid_hi] : 0;
-        int ncols = n_lo > n_hi ? n_lo : n_hi;
-        for (int off = 32; off >= 1; off >>= 1) {
-            const int o = __shfl_xor(ncols, off);
-            ncols = o > ncols ? o : ncols;
-        }
-        ncols = __builtin_amdgcn_readfirstlane(ncols);
-        ncols += ncols & 1;            // columns go in pairs; the pad column scores kPadScore
+// The pairs of one task as seen by one lane.
+struct TaskView {
+    int pair_lo, pair_hi;
+    bool has_lo, has_hi;
+    const int8_t* q;
+    int m;
+    const int8_t* s_lo;
+    const int8_t* s_hi;
+    int n_lo, n_hi;
+    int ncols;           // even, wave-uniform
+};
 
-        // score 0: ref_end1 -1 and read_end1 0, or -1 for an empty query
-        // (ssw.c:148-149, 288-297)
-        LaneBest best;
+__device__ __forceinline__ TaskView task_view(const SwKernelArgs& a, int task) {
+    const int lane = threadIdx.x % 64;
+    TaskView t;
+    const int p0 = a.task_begin[task], p1 = a.task_begin[task + 1];
+    const int np = p1 - p0;
+    t.has_lo = lane < np;
+    t.has_hi = lane + 64 < np;
+    const int k_lo = p0 + (t.has_lo ? lane : 0);
+    const int k_hi = p0 + (t.has_hi ? lane + 64 : 0);
+    t.pair_lo = a.pair_index ? a.pair_index[k_lo] : k_lo;
+    t.pair_hi = a.pair_index ? a.pair_index[k_hi] : k_hi;
+    const int qid = a.pair_query[t.pair_lo];
+    t.q = a.q_res + a.q_off[qid];
+    t.m = a.q_len[qid];
+    const int sid_lo = a.pair_subject[t.pair_lo], sid_hi = a.pair_subject[t.pair_hi];
+    t.s_lo = a.s_res + a.s_off[sid_lo];
+    t.s_hi = a.s_res + a.s_off[sid_hi];
+    t.n_lo = t.has_lo ? a.s_len[sid_lo] : 0;
+    t.n_hi = t.has_hi ? a.s_len[sid_hi] : 0;
+    int ncols = t.n_lo > t.n_hi ? t.n_lo : t.n_hi;
+    for (int off = 32; off >= 1; off >>= 1) {
+        const int o = __shfl_xor(ncols, off);
+        ncols = o > ncols ? o : ncols;
+    }
+    ncols = __builtin_amdgcn_readfirstlane(ncols);
+    t.ncols = ncols + (ncols & 1);     // columns go in pairs; the pad column scores kPadScore
+    return t;
+}
+
+__device__ __forceinline__ void init_best(LaneBest& best, int m) {
+    // score 0: ref_end1 -1 and read_end1 0, or -1 for an empty query
+    // (ssw.c:148-149, 288-297)
+    for (int h = 0; h < 2; ++h) {
+        best.score[h] = 0; best.ref[h] = -1; best.read[h] = m > 0 ? 0 : -1; best.chunk[h] = 0;
+    }
+}
+
+__device__ __forceinline__ void write_best(const SwKernelArgs& a, const TaskView& t,
+                                           const LaneBest& best) {
+    if (t.has_lo) {
+        a.out_score[t.pair_lo] = best.score[0] >= 32767 ? 32767 : best.score[0];
+        a.out_ref[t.pair_lo] = best.ref[0];
+        a.out_read[t.pair_lo] = best.read[0];
+    }
+    if (t.has_hi) {
+        a.out_score[t.pair_hi] = best.score[1] >= 32767 ? 32767 : best.score[1];
+        a.out_ref[t.pair_hi] = best.ref[1];
+        a.out_read[t.pair_hi] = best.read[1];
+    }
+}
+
+// LDS of one workgroup.
+struct GroupLds {
+    WaveLds wave[kWaves];
+    int item;
+    int error;
+    int progress[kWaves];
+    int bscore[kWaves][2][64], bref[kWaves][2][64], bread[kWaves][2][64], bchunk[kWaves][2][64];
+};
+
+// A task worked by one wavefront: its chunks in order, boundary in place.
+__device__ void run_task_wave(const SwKernelArgs& a, WaveLds& lds, uint32_t* buf, int task,
+                              GroupLds& g) {
+    const TaskView t = task_view(a, task);
+    LaneBest best;
+    init_best(best, t.m);
+    int chunk = 0;
+    for (int j0 = 0; j0 < t.m; j0 += kRows, ++chunk) {
+        build_profile(a, lds, t.q, t.m, j0);
+        ChunkIO io{};
+        io.top = j0 == 0 ? nullptr : buf;
+        io.bottom = j0 + kRows < t.m ? buf : nullptr;
+        io.error = &g.error;
+        io.g_error = a.error;
+        run_chunk(a, lds, io, t.ncols, t.s_lo, t.n_lo, t.s_hi, t.n_hi, j0, chunk, best);
+    }
+    write_best(a, t, best);
+}
+
+// A task worked by all wavefronts of the workgroup: chunk c by wavefront
+// c % kWaves, each one trailing the wavefront of the chunk above by at least
+// kAhead columns; chunk c's bottom boundary lives in buffer c % kWaves.
+__device__ void run_task_group(const SwKernelArgs& a, GroupLds& g, uint32_t* bufs,
+                               size_t buf_words, int task) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const TaskView t = task_view(a, task);
+    LaneBest best;
+    init_best(best, t.m);
+    const int n_chunks = (t.m + kRows - 1) / kRows;
+    const int nc = t.ncols;
+    for (int c = wave; c < n_chunks; c += kWaves) {
+        const int j0 = c * kRows;
+        build_profile(a, g.wave[wave], t.q, t.m, j0);
+        ChunkIO io{};
+        io.top = c == 0 ? nullptr : bufs + (size_t)((c - 1) % kWaves) * buf_words;
+        io.bottom = c + 1 < n_chunks ? bufs + (size_t)(c % kWaves) * buf_words : nullptr;
+        if (c > 0) {
+            io.wait_ctr = &g.progress[(c - 1) % kWaves];
+            io.wait_base = ((c - 1) / kWaves) * nc;
+        }
+        io.pub_ctr = &g.progress[wave];
+        io.pub_base = (c / kWaves) * nc;
+        io.error = &g.error;
+        io.g_error = a.error;
+        if (!run_chunk(a, g.wave[wave], io, nc, t.s_lo, t.n_lo, t.s_hi, t.n_hi, j0, c, best))
+            break;
+        if (nc == 0) publish(io, io.pub_base);
+    }
+    // merge the wavefronts' bests: larger score, then smaller column, then
+    // the earlier chunk
+    for (int h = 0; h < 2; ++h) {
+        g.bscore[wave][h][lane] = best.score[h];
+        g.bref[wave][h][lane] = best.ref[h];
+        g.bread[wave][h][lane] = best.read[h];
+        g.bchunk[wave][h][lane] = best.chunk[h];
+    }
+    __syncthreads();
+    if (wave == 0) {
         for (int h = 0; h < 2; ++h) {
-            best.score[h] = 0; best.ref[h] = -1; best.read[h] = m > 0 ? 0 : -1;
+            for (int w = 1; w < kWaves; ++w) {
+                const int sc = g.bscore[w][h][lane], rf = g.bref[w][h][lane];
+                const int ck = g.bchunk[w][h][lane];
+                if (sc <= 0) continue;
+                if (sc > best.score[h] ||
+                    (sc == best.score[h] &&
+                     (rf < best.ref[h] || (rf == best.ref[h] && ck < best.chunk[h])))) {
+                    best.score[h] = sc;
+                    best.ref[h] = rf;
+                    best.read[h] = g.bread[w][h][lane];
+                    best.chunk[h] = ck;
+                }
+            }
         }
-        for (int j0 = 0; j0 < m; j0 += kRows) {
-            __syncthreads();
-            build_profile(a, lds, q, m, j0);
-            __syncthreads();
-            run_chunk(a, lds, ws, ncols, s_lo, n_lo, s_hi, n_hi, j0 == 0,
-                      j0 + kRows < m, j0, best);
-        }
+        if (!__hip_atomic_load(&g.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            write_best(a, t, best);
+    }
+}
 
-        if (has_lo) {
-            a.out_score[pair_lo] = best.score[0] >= 32767 ? 32767 : best.score[0];
-            a.out_ref[pair_lo] = best.ref[0];
-            a.out_read[pair_lo] = best.read[0];
+// Work items: item k is either one large task worked by the whole workgroup
+// (item_task[4k] >= 0, item_task[4k+1] == -2) or up to four small tasks,
+// one per wavefront (-1 = none).  With a counter the grid is persistent and
+// workgroup b owns workspace slot b; without one, workgroup b runs item b.
+__global__ void __launch_bounds__(kWaves * 64)
+sw_forward_kernel(SwKernelArgs a) {
+    __shared__ GroupLds g;
+    const int wave = threadIdx.x / 64;
+    const size_t buf_words = (size_t)a.ws_words_per_slot / kWaves;
+    uint32_t* bufs = a.workspace + (size_t)blockIdx.x * a.ws_words_per_slot;
+    if (threadIdx.x == 0) g.error = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            g.item = a.task_counter ? atomicAdd(a.task_counter, 1) : (int)blockIdx.x;
+            for (int w = 0; w < kWaves; ++w) g.progress[w] = 0;
         }
-        if (has_hi) {
-            a.out_score[pair_hi] = best.score[1] >= 32767 ? 32767 : best.score[1];
-            a.out_ref[pair_hi] = best.ref[1];
-            a.out_read[pair_hi] = best.read[1];
+        __syncthreads();
+        const int item = g.item;
+        if (item >= a.n_items || g.error) break;
+        const int t0 = a.item_task[4 * item];
+        if (a.item_task[4 * item + 1] == -2) {
+            run_task_group(a, g, bufs, buf_words, t0);
+        } else {
+            const int task = a.item_task[4 * item + wave];
+            if (task >= 0) run_task_wave(a, g.wave[wave], bufs + (size_t)wave * buf_words, task, g);
         }
+        __syncthreads();
         if (!a.task_counter) break;
     }
 }
@@ -320,6 +498,7 @@ sw_forward_kernel(SwKernelArgs a) {
 
 extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid,
                                             hipStream_t stream) {
-    hipLaunchKernelGGL(tpz::sw_forward_kernel, dim3(grid), dim3(64), 0, stream, *args);
+    hipLaunchKernelGGL(tpz::sw_forward_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream,
+                       *args);
     return hipGetLastError();
 }

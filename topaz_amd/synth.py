@@ -171,3 +171,126 @@ def pair_workload(rng, n_queries, subjects_per_query, homolog_frac=0.5,
     del h_len_c
     return dict(q=q, s=(s_res, s_off, s_len), pair_query=pair_query,
                 pair_subject=pair_subject, is_homolog=is_hom)
+
+
+def to_letters(res):
+    """Matrix indices -> amino-acid letters (substitution.h order)."""
+    table = np.frombuffer(MATRIX_ORDER.encode(), np.uint8)
+    return table[np.asarray(res, np.int64)].tobytes().decode()
+
+
+def write_fasta(path, seqs, prefix="seq"):
+    """seqs: iterable of matrix-index arrays; 60 letters per line."""
+    with open(path, "w") as f:
+        for k, s in enumerate(seqs):
+            f.write(">%s%d\n" % (prefix, k))
+            letters = to_letters(s)
+            for i in range(0, len(letters), 60):
+                f.write(letters[i:i + 60] + "\n")
+
+
+def search_database(rng, n_db, n_queries, hom_frac=0.2, max_hom=5, ident=(0.3, 0.9),
+                    mean=350.0, sd=250.0, lo=30, hi=2000):
+    """A database with planted homologs of a fraction of the queries (the
+    end-to-end search workload): returns (db_seqs, query_seqs)."""
+    ql = lognormal_lengths(rng, n_queries, mean, sd, lo, hi)
+    qr, qo, qn = concat_random(rng, ql)
+    queries = [qr[o:o + n] for o, n in zip(qo, qn)]
+    homs = []
+    for k in range(n_queries):
+        if rng.random() < hom_frac:
+            for _ in range(int(rng.integers(1, max_hom + 1))):
+                r_, o_, l_ = mutate(rng, queries[k], np.array([0]), np.array([len(queries[k])]),
+                                    np.array([rng.uniform(*ident)]))
+                homs.append(r_[:l_[0]])
+    n_rand = max(0, n_db - len(homs))
+    dl = lognormal_lengths(rng, n_rand, mean, sd, lo, hi)
+    dr, do, dn = concat_random(rng, dl)
+    db = [dr[o:o + n] for o, n in zip(do, dn)] + homs
+    order = rng.permutation(len(db))
+    return [db[i] for i in order], queries
+
+
+def internal_text_and_seeds(rng, t_residues, n_queries, n_seeds, homolog_frac=0.05,
+                            mean=350.0, sd=250.0, lo=30, hi=5000):
+    """The gapless-extension workload: a newline-separated internal-alphabet
+    text of about t_residues residues (lognormal protein lengths), n_queries
+    queries, and n_seeds seeds (query, offset, text position) drawn uniformly
+    over query x text positions, with a fraction homolog_frac placed on the
+    diagonals of homolog copies of the queries planted in the text.
+
+    Returns (text bytes as uint8 array, list of query byte strings, seeds as a
+    structured array with fields query/offset/position, newline positions).
+    """
+    internal = np.zeros(25, np.uint8)
+    from .matrices import INTERNAL_ORDER
+    for k, aa in enumerate(INTERNAL_ORDER):
+        internal[MATRIX_ORDER.index(aa)] = ord("A") + k
+    ql = lognormal_lengths(rng, n_queries, mean, sd, lo, hi)
+    qr, qo, qn = concat_random(rng, ql)
+    queries = [internal[qr[o:o + n]].tobytes() for o, n in zip(qo, qn)]
+    # planted homologs: one mutated copy of each query (first n_plant)
+    n_plant = min(n_queries, 2000)
+    hr, ho, hn = mutate(rng, qr, qo[:n_plant], qn[:n_plant], rng.uniform(0.3, 0.9, n_plant))
+    n_prot = int(t_residues / mean)
+    lens = lognormal_lengths(rng, n_prot, mean, sd, lo, hi)
+    # text: proteins with '\n' after each; the planted copies go first
+    all_lens = np.concatenate([hn, lens]).astype(np.int64)
+    total = int(all_lens.sum() + len(all_lens))
+    text = np.empty(total, np.uint8)
+    starts = np.zeros(len(all_lens), np.int64)
+    starts[1:] = np.cumsum(all_lens[:-1] + 1)
+    nl = starts + all_lens
+    body = internal[random_residues(rng, total)]
+    text[:] = body
+    text[nl] = ord("\n")
+    # copy the planted homologs in (they occupy the first n_plant proteins)
+    first = np.concatenate([[0], np.cumsum(hn[:-1])]).astype(np.int64)
+    pos_in_text = np.repeat(starts[:n_plant] - first, hn) + np.arange(int(hn.sum()))
+    text[pos_in_text] = internal[hr]
+    seeds = np.zeros(n_seeds, dtype=[("query", "<i4"), ("offset", "<i4"), ("position", "<i8")])
+    n_hom = int(n_seeds * homolog_frac)
+    q = rng.integers(0, n_queries, n_seeds)
+    seeds["query"] = q
+    seeds["offset"] = (rng.random(n_seeds) * ql[q]).astype(np.int32)
+    seeds["position"] = rng.integers(0, total, n_seeds)
+    # homolog seeds: a planted copy's start plus the offset (its diagonal)
+    hq = rng.integers(0, n_plant, n_hom)
+    off = (rng.random(n_hom) * np.minimum(ql[hq], hn[hq])).astype(np.int64)
+    seeds["query"][:n_hom] = hq
+    seeds["offset"][:n_hom] = off
+    seeds["position"][:n_hom] = starts[hq] + off
+    return text, queries, seeds, nl
+
+
+def match_first_residue(rng, seeds, queries, text):
+    """Move every seed to a text position holding its query residue (as real
+    suffix-array seeds do), keeping query and offset: the position is drawn
+    uniformly among the occurrences of that letter."""
+    qc = np.array([queries[q][o] for q, o in zip(seeds["query"], seeds["offset"])], np.uint8)
+    out = seeds.copy()
+    for c in np.unique(qc):
+        occ = np.nonzero(text == c)[0]
+        sel = np.nonzero(qc == c)[0]
+        if len(occ):
+            out["position"][sel] = occ[rng.integers(0, len(occ), len(sel))]
+    return out
+
+
+def gapless_scan_bytes(seeds, q_len, text, nl):
+    """Text bytes the reference rule reads per seed (search.c:99-190): the
+    first residue when it differs from the query's, else the right window up
+    to and including a '\\n' (at most min(Tsize-idx, m-o, 128)) and the left
+    window i = 1 .. min(idx, o, 128)-1 up to and including a '\\n'."""
+    idx = seeds["position"].astype(np.int64)
+    o = seeds["offset"].astype(np.int64)
+    m = np.asarray(q_len, np.int64)[seeds["query"]]
+    tsize = len(text)
+    k = np.searchsorted(nl, idx)                 # first newline at or after idx
+    nxt = np.where(k < len(nl), nl[np.minimum(k, len(nl) - 1)], tsize)
+    prv = np.where(k > 0, nl[np.maximum(k - 1, 0)], -1)
+    R = np.minimum(np.minimum(tsize - idx, m - o), 128)
+    right = np.minimum(R, nxt - idx + 1)
+    Lw = np.minimum(np.minimum(idx, o), 128)
+    left = np.maximum(np.minimum(Lw - 1, idx - prv), 0)
+    return right, left
