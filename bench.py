@@ -1,16 +1,24 @@
-"""Benchmark of the batched Smith-Waterman hot path on one MI355X.
+"""Benchmark of the candidate-alignment hot path on one MI355X.
 
-Workload: 2,000,000 query-subject pairs -- 2,000 queries x 1,000 subjects
-each, both sides lognormal length (mean 350, sd 250) clipped to [30, 5000],
-residues from Robinson-Robinson background frequencies, half of every query's
-subjects planted homologs at 25-95 % identity with 2 % indels; BLOSUM62,
-gap 11/1.  Inputs are uploaded once; a step is one full pass of the batched
-kernel over all pairs (scores and end positions of every pair).
+Stage S (the headline): 2,000,000 query-subject pairs -- 2,000 queries x
+1,000 subjects each, both sides lognormal length (mean 350, sd 250) clipped
+to [30, 5000], residues from Robinson-Robinson background frequencies, half
+of every query's subjects planted homologs at 25-95 % identity with 2 %
+indels; BLOSUM62, gap 11/1.  Inputs are uploaded once; a step is one full
+pass of the batched kernel over all pairs (scores and end positions of every
+pair).  The plain one-launch-per-query schedule is timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs-scale F]
+Stage G (the "gapless" object): a 1e9-residue newline-separated text
+resident in HBM, 10,000 queries and 50,000,000 seeds (query, offset, text
+position), twice: drawn uniformly over query x text positions (5 % on the
+diagonals of planted homologs), and drawn with agreeing first residues, as
+suffix-array seeds are.  A step is one pass of the gapless kernel over all
+seeds; the rate is the bytes the reference rule reads per seed plus the
+seed record and the result, over the kernel time.
 
-Prints one JSON line.  `value` is the persistent (length-binned) schedule's
-GCUPS; the plain one-launch-per-query schedule is timed beside it.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--leg all|sw|gapless]
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -108,22 +116,103 @@ def cpu_baseline(w, seconds=12.0):
                          dt, cores)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pairs-scale", type=float, default=1.0,
-                    help="fraction of the 2,000-query workload (tests only)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(queries would be sharded across GPUs with a replicated index); "
-              "run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+SEED_BYTES, RESULT_BYTES = 16, 8
 
+
+def gapless_cpu_baseline(text_bytes, queries, seeds, per_seed_bytes, seconds=8.0):
+    """The oracle's sa_hsp loop (a scalar port of search.c:99-190) on all host
+    cores over a bounded prefix of the seeds."""
+    from tests.refcheck import Oracle
+    orc = Oracle()
+    cores = len(os.sched_getaffinity(0))
+    chunk = 200_000
+    lock = threading.Lock()
+    state = {"next": 0, "seeds": 0, "bytes": 0}
+    deadline = time.perf_counter() + seconds
+
+    def worker():
+        while time.perf_counter() < deadline:
+            with lock:
+                k = state["next"]
+                state["next"] += chunk
+            if k >= len(seeds):
+                return
+            part = seeds[k:k + chunk]
+            orc.sa_hsp_batch(text_bytes, queries, part)
+            with lock:
+                state["seeds"] += len(part)
+                state["bytes"] += int(per_seed_bytes[k:k + chunk].sum())
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    return {"value": round(state["bytes"] / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
+            "kind": "port",
+            "sample": "first %d seeds of the first-residue-matched set, %.1f s, oracle "
+                      "sa_hsp loop, %d threads" % (state["seeds"], dt, cores)}
+
+
+def gapless_leg(args):
+    from topaz_amd import matrix, synth
+    from topaz_amd.gapless import GaplessBatch, Text
+    rng = np.random.default_rng(2025)
+    t_res = int(1e9 * args.gapless_scale)
+    n_seeds = int(5e7 * args.gapless_scale)
+    t0 = time.perf_counter()
+    text, queries, uniform, nl = synth.internal_text_and_seeds(rng, t_res, 10000, n_seeds,
+                                                               homolog_frac=0.05)
+    matched = synth.matching_seeds(rng, queries, text, n_seeds, uniform[:int(0.05 * n_seeds)])
+    ql = np.array([len(q) for q in queries], np.int64)
+    print("[bench] gapless workload: %d text bytes, %d seeds x 2, generated in %.1f s"
+          % (len(text), n_seeds, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    text_bytes = text.tobytes()
+    dev_text = Text(text_bytes)
+    rank = synth.newline_rank(text)
+    out = {}
+    for name, seeds in (("uniform", uniform), ("matched", matched)):
+        b = GaplessBatch(dev_text, queries, seeds, matrix())
+        for _ in range(args.warmup):
+            b.run()
+        ref = b.results()
+        times, kms = [], []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            b.run()
+            times.append(time.perf_counter() - t1)
+            kms.append(b.kernel_ms())
+        sc, ln = b.results()
+        assert np.array_equal(sc, ref[0]) and np.array_equal(ln, ref[1])
+        b.close()
+        right, left = synth.gapless_scan_bytes(seeds, ql, text, nl, rank)
+        per_seed = np.where(ln == -1, 1, right + left) + SEED_BYTES + RESULT_BYTES
+        total = int(per_seed.sum())
+        kernel_s = float(np.mean(kms)) / 1e3
+        gbs = total / kernel_s / 1e9
+        out[name] = {"seeds": int(n_seeds), "passing": int((ln != -1).sum()),
+                     "hsp_ge_5": int((ln >= 5).sum()),
+                     "bytes_per_launch": total, "kernel_ms": round(kernel_s * 1e3, 3),
+                     "ms_per_step": round(float(np.mean(times)) * 1e3, 3),
+                     "seeds_per_s": round(n_seeds / kernel_s, 1),
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}}
+        print("[bench] gapless %s: kernel %.3f ms, %.1f GB/s"
+              % (name, kernel_s * 1e3, gbs), file=sys.stderr, flush=True)
+        if name == "matched" and not args.no_cpu:
+            out["cpu_baseline"] = gapless_cpu_baseline(text_bytes, queries, seeds, per_seed)
+    dev_text.close()
+    out["workload"] = ("gapless HSP extension: %d-byte text (lognormal proteins, mean 350, "
+                       "clipped [30,5000]), 10000 queries, %d seeds per set; bytes = text "
+                       "bytes the rule reads + 16 B seed + 8 B result" % (len(text), n_seeds))
+    return out
+
+
+def sw_leg(args):
     from topaz_amd import matrix
     from topaz_amd.sw import SCHED_PERSISTENT, SCHED_PLAIN, SwBatch
 
@@ -196,6 +285,43 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs-scale", type=float, default=1.0,
+                    help="fraction of the 2,000-query workload (tests only)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--leg", choices=["all", "sw", "gapless"], default="all")
+    ap.add_argument("--gapless-scale", type=float, default=1.0,
+                    help="fraction of the 1e9-residue, 5e7-seed gapless workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(queries would be sharded across GPUs with a replicated index); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    if args.leg in ("all", "sw"):
+        out = sw_leg(args)
+        if args.leg == "all":
+            out["gapless"] = gapless_leg(args)
+    else:
+        g = gapless_leg(args)
+        m = g["matched"]
+        out = {"metric": "gapless HSP extension, achieved HBM GB/s (first-residue-matched seeds)",
+               "value": m["roofline"]["achieved"], "unit": "GB/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+               "data": "synthetic", "config": {"workload": g["workload"]},
+               "roofline": m["roofline"], "gapless": g}
+        if "cpu_baseline" in g:
+            out["cpu_baseline"] = g["cpu_baseline"]
     print(json.dumps(out), flush=True)
 
 

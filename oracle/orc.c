@@ -555,3 +555,14 @@ orc_end_t orc_sw_textbook(const int8_t* read, int32_t readLen,
     free(hp); free(hc); free(e); free(kept);
     return r;
 }
+
+void orc_sa_hsp_batch(const char* T, int64_t Tsize, const char* Qcat,
+                      const int64_t* q_off, const int32_t* q_len,
+                      const orc_seed* seeds, int64_t n, const int8_t* mat,
+                      int32_t* score, int32_t* len) {
+    for (int64_t k = 0; k < n; ++k) {
+        const orc_seed* s = &seeds[k];
+        len[k] = orc_sa_hsp(T, Tsize, s->position, Qcat + q_off[s->query],
+                            q_len[s->query], s->offset, mat, &score[k]);
+    }
+}

@@ -104,6 +104,17 @@ int orc_sa_hsp(const char* T, int64_t Tsize, int64_t idx,
                const char* Q, int32_t m, int32_t o,
                const int8_t* mat, int32_t* score);
 
+/* orc_sa_hsp over many seeds {query, offset, idx} (16-byte records, the
+ * layout of the GPU seed batch) against one text: the loop of sa_hsp calls
+ * query_find_suffixes and suffix_increment make (query.c:117,125,
+ * suffix.c:47).  Qcat holds the queries back to back at q_off[], q_len[].
+ * score[k] is written only when len[k] != -1. */
+typedef struct { int32_t query; int32_t offset; int64_t position; } orc_seed;
+void orc_sa_hsp_batch(const char* T, int64_t Tsize, const char* Qcat,
+                      const int64_t* q_off, const int32_t* q_len,
+                      const orc_seed* seeds, int64_t n, const int8_t* mat,
+                      int32_t* score, int32_t* len);
+
 /* sa_gapless (search.c:48-97), dead in the reference; kept for parity. */
 int orc_sa_gapless(const char* T, int64_t Tsize, int64_t idx,
                    const char* Q, int32_t m, int32_t o,

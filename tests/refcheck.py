@@ -60,6 +60,10 @@ class Oracle:
                                  I32, I32, P8, PI]
         L.orc_sa_gapless.restype = C.c_int
         L.orc_sa_gapless.argtypes = L.orc_sa_hsp.argtypes
+        L.orc_sa_hsp_batch.restype = None
+        L.orc_sa_hsp_batch.argtypes = [C.c_char_p, C.c_int64, C.c_char_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]
         L.orc_evalue.restype = None
         L.orc_evalue.argtypes = [I32, I32, C.c_uint64, C.c_uint64, C.c_double,
                                  C.c_double, C.c_double, C.POINTER(C.c_double),
@@ -109,6 +113,21 @@ class Oracle:
         ln = self.L.orc_sa_hsp(T, len(T), idx, Q, len(Q), o, _ptr8(mat),
                                C.byref(sc))
         return (sc.value if ln != -1 else None), ln
+
+    def sa_hsp_batch(self, T, queries, seeds, mat=None):
+        """(score, length) of every seed record (query, offset, position);
+        score is 0 where length is -1.  T: bytes; queries: list of bytes."""
+        mat = matrix() if mat is None else mat
+        ql = np.array([len(q) for q in queries], np.int32)
+        qo = np.zeros(len(ql), np.int64)
+        qo[1:] = np.cumsum(ql[:-1])
+        seeds = np.ascontiguousarray(seeds)
+        sc = np.zeros(len(seeds), np.int32)
+        ln = np.zeros(len(seeds), np.int32)
+        self.L.orc_sa_hsp_batch(T, len(T), b"".join(queries), qo.ctypes.data, ql.ctypes.data,
+                                seeds.ctypes.data, len(seeds), _ptr8(mat), sc.ctypes.data,
+                                ln.ctypes.data)
+        return sc, ln
 
     def evalue(self, raw, qlen, db_size, db_proteins, lam=0.267, k=0.041,
                H=0.140):

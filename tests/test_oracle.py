@@ -252,3 +252,38 @@ def test_oracle_vs_compiled_reference_random():
         s = rng.integers(0, 25, n).astype(np.int8)
         go, ge = [(11, 1), (2, 2), (5, 3), (10, 2)][int(rng.integers(0, 4))]
         assert o.ssw_align(q, s, go, ge) == r.ssw_align(q, s, go, ge)
+
+
+def test_gapless_batch_and_scan_bytes(orc):
+    """The batch loop equals per-seed calls, and the scan-byte accounting the
+    bench uses equals the bytes the rule reads, counted by direct walking."""
+    from topaz_amd import synth
+    rng = np.random.default_rng(21)
+    text, queries, seeds, nl = synth.internal_text_and_seeds(rng, 60_000, 40, 3000,
+                                                             homolog_frac=0.1)
+    seeds = np.concatenate([seeds, synth.matching_seeds(rng, queries, text, 3000, seeds[:300])])
+    T = text.tobytes()
+    sc, ln = orc.sa_hsp_batch(T, queries, seeds)
+    ql = np.array([len(q) for q in queries])
+    right, left = synth.gapless_scan_bytes(seeds, ql, text, nl)
+    r2, l2 = synth.gapless_scan_bytes(seeds, ql, text, nl, synth.newline_rank(text))
+    assert np.array_equal(right, r2) and np.array_equal(left, l2)
+    assert (ln != -1).sum() > 3000
+    for k in range(len(seeds)):
+        q, o, idx = int(seeds["query"][k]), int(seeds["offset"][k]), int(seeds["position"][k])
+        s1, l1 = orc.sa_hsp(T, idx, queries[q], o)
+        assert (l1, s1 if l1 != -1 else 0) == (ln[k], sc[k])
+        if l1 == -1:
+            continue
+        m = len(queries[q])
+        r = 0
+        for i in range(min(len(T) - idx, m - o, 128)):
+            r += 1
+            if T[idx + i] == 10:
+                break
+        lft = 0
+        for i in range(1, min(idx, o, 128)):
+            lft += 1
+            if T[idx - i] == 10:
+                break
+        assert (right[k], left[k]) == (r, lft), k

@@ -263,21 +263,52 @@ def internal_text_and_seeds(rng, t_residues, n_queries, n_seeds, homolog_frac=0.
     return text, queries, seeds, nl
 
 
-def match_first_residue(rng, seeds, queries, text):
-    """Move every seed to a text position holding its query residue (as real
-    suffix-array seeds do), keeping query and offset: the position is drawn
-    uniformly among the occurrences of that letter."""
-    qc = np.array([queries[q][o] for q, o in zip(seeds["query"], seeds["offset"])], np.uint8)
-    out = seeds.copy()
-    for c in np.unique(qc):
-        occ = np.nonzero(text == c)[0]
-        sel = np.nonzero(qc == c)[0]
-        if len(occ):
-            out["position"][sel] = occ[rng.integers(0, len(occ), len(sel))]
-    return out
+def matching_seeds(rng, queries, text, n_seeds, homolog_seeds=None):
+    """Seeds whose first residues agree, as suffix-array seeds always do: a
+    text position uniform over the residues of the text, then a query
+    position uniform over the query residues equal to it.  homolog_seeds
+    (structured seeds, e.g. the planted-diagonal ones of
+    internal_text_and_seeds) fill the front of the array where their first
+    residues agree."""
+    ql = np.array([len(q) for q in queries], np.int64)
+    qoff = np.zeros(len(ql), np.int64)
+    qoff[1:] = np.cumsum(ql[:-1])
+    flat = np.frombuffer(b"".join(queries), np.uint8)
+    order = np.argsort(flat, kind="stable")
+    counts = np.bincount(flat, minlength=256)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    qid = np.repeat(np.arange(len(ql)), ql)
+    seeds = np.zeros(n_seeds, dtype=[("query", "<i4"), ("offset", "<i4"), ("position", "<i8")])
+    k0 = 0
+    if homolog_seeds is not None and len(homolog_seeds):
+        h = homolog_seeds
+        ok = text[h["position"]] == flat[qoff[h["query"]] + h["offset"]]
+        h = h[ok][:n_seeds]
+        seeds[:len(h)] = h
+        k0 = len(h)
+    pos = rng.integers(0, len(text), n_seeds - k0)
+    while True:
+        bad = np.nonzero(counts[text[pos]] == 0)[0]     # '\n' or a letter no query has
+        if not len(bad):
+            break
+        pos[bad] = rng.integers(0, len(text), len(bad))
+    c = text[pos]
+    k = first[c] + (rng.random(len(pos)) * counts[c]).astype(np.int64)
+    j = order[k]
+    seeds["query"][k0:] = qid[j]
+    seeds["offset"][k0:] = j - qoff[qid[j]]
+    seeds["position"][k0:] = pos
+    return seeds
 
 
-def gapless_scan_bytes(seeds, q_len, text, nl):
+def newline_rank(text):
+    """rank[i] = number of '\n' in text[0..i] (int32): with it the first
+    newline at or after a position is found by one gather instead of a
+    binary search."""
+    return np.cumsum(text == 10, dtype=np.int32)
+
+
+def gapless_scan_bytes(seeds, q_len, text, nl, rank=None):
     """Text bytes the reference rule reads per seed (search.c:99-190): the
     first residue when it differs from the query's, else the right window up
     to and including a '\\n' (at most min(Tsize-idx, m-o, 128)) and the left
@@ -286,7 +317,10 @@ def gapless_scan_bytes(seeds, q_len, text, nl):
     o = seeds["offset"].astype(np.int64)
     m = np.asarray(q_len, np.int64)[seeds["query"]]
     tsize = len(text)
-    k = np.searchsorted(nl, idx)                 # first newline at or after idx
+    if rank is None:
+        k = np.searchsorted(nl, idx)             # first newline at or after idx
+    else:
+        k = rank[idx].astype(np.int64) - (text[idx] == 10)
     nxt = np.where(k < len(nl), nl[np.minimum(k, len(nl) - 1)], tsize)
     prv = np.where(k > 0, nl[np.maximum(k - 1, 0)], -1)
     R = np.minimum(np.minimum(tsize - idx, m - o), 128)
