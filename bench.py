@@ -178,13 +178,15 @@ def gapless_leg(args):
         b = GaplessBatch(dev_text, queries, seeds, matrix())
         for _ in range(args.warmup):
             b.run()
-        ref = b.results()
+        ref = b.results() if args.warmup else None
         times, kms = [], []
         for _ in range(args.steps):
             t1 = time.perf_counter()
             b.run()
             times.append(time.perf_counter() - t1)
             kms.append(b.kernel_ms())
+            if ref is None:
+                ref = b.results()
         sc, ln = b.results()
         assert np.array_equal(sc, ref[0]) and np.array_equal(ln, ref[1])
         b.close()
@@ -228,13 +230,16 @@ def sw_leg(args):
     print("[bench] uploaded", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         b.run(SCHED_PERSISTENT)
-    ref_sc, ref_re, ref_rd = b.results()
+    ref = b.results() if args.warmup else None
     times, kms = [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
         b.run(SCHED_PERSISTENT)
         times.append(time.perf_counter() - t0)
         kms.append(b.kernel_ms())
+        if ref is None:
+            ref = b.results()
+    ref_sc, ref_re, ref_rd = ref
     sc, re_, rd = b.results()
     assert np.array_equal(sc, ref_sc) and np.array_equal(re_, ref_re) and np.array_equal(rd, ref_rd)
     t_step = float(np.sum(times)) / args.steps
@@ -288,6 +293,25 @@ def sw_leg(args):
     return out
 
 
+def copy_leg(args):
+    """A plain device-to-device copy of 1 GiB, three times: the calibration
+    run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch of
+    a copy of known size)."""
+    import torch
+    n = 1 << 30
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    return {"metric": "device copy, GB/s (read + write)", "value": round(2 * n / dt / 1e9, 1),
+            "unit": "GB/s", "bytes_copied": n, "ms_per_copy": round(dt * 1e3, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,7 +321,7 @@ def main():
                     help="fraction of the 2,000-query workload (tests only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--leg", choices=["all", "sw", "gapless"], default="all")
+    ap.add_argument("--leg", choices=["all", "sw", "gapless", "copy"], default="all")
     ap.add_argument("--gapless-scale", type=float, default=1.0,
                     help="fraction of the 1e9-residue, 5e7-seed gapless workload")
     args = ap.parse_args()
@@ -307,6 +331,9 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.leg == "copy":
+        print(json.dumps(copy_leg(args)), flush=True)
+        return
     if args.leg in ("all", "sw"):
         out = sw_leg(args)
         if args.leg == "all":

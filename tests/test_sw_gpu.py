@@ -117,3 +117,25 @@ def test_group_and_wave_paths(big_fraction, monkeypatch):
     rng = np.random.default_rng(7)
     w = synth.pair_workload(rng, 6, 150, mean=400, sd=350, lo=1, hi=2500)
     compare(w)
+
+
+@pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
+def test_packed_tasks_mix_queries(schedule):
+    """Wavefronts holding pairs of up to four queries: queries of equal and
+    unequal chunk counts (lengths 1..100, some empty), uneven pair counts,
+    subjects from empty to long, subjects shared between queries."""
+    rng = np.random.default_rng(31)
+    nq = 48
+    ql = rng.integers(1, 100, nq)
+    ql[rng.integers(0, nq, 3)] = 0
+    qs = [rng.integers(0, 25, int(n)).astype(np.int8) for n in ql]
+    ns = 300
+    sl = np.minimum(rng.lognormal(4.5, 1.0, ns).astype(int), 1200)
+    sl[:4] = 0
+    ss = [rng.integers(0, 25, int(n)).astype(np.int8) for n in sl]
+    counts = rng.integers(1, 200, nq)
+    pq = np.repeat(np.arange(nq), counts).astype(np.int32)
+    ps = rng.integers(0, ns, len(pq)).astype(np.int32)
+    from topaz_amd.sw import _concat
+    w = dict(q=_concat(qs), s=_concat(ss), pair_query=pq, pair_subject=ps)
+    compare(w, schedule=schedule)

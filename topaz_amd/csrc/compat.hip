@@ -115,6 +115,10 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     meta32[8] = -1;
     meta32[9] = -1;
     meta32[10] = 0;                // error flag
+    meta32[11] = 0;                // task_query of task 0: query 0 only
+    meta32[12] = -1;
+    meta32[13] = -1;
+    meta32[14] = -1;
     char* d = (char*)c.in.get(bytes);
     int32_t* dout = (int32_t*)c.out.get(64);
     const size_t ws_words = (size_t)4 * 2 * (n + 2) * 64;     // four wavefront buffers
@@ -131,6 +135,7 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.pair_subject = (const int32_t*)(d + o_meta + 28);
     a.pair_index = nullptr;
     a.task_begin = (const int32_t*)(d + o_meta + 32);
+    a.task_query = (const int32_t*)(d + o_meta + 60);
     a.item_task = (const int32_t*)(d + o_meta + 40);
     a.n_items = 1;
     a.task_counter = nullptr;
@@ -406,7 +411,7 @@ struct Maps {
     }
 } const g_maps;
 
-const ResidentText& resident_text(const db_t* db) {
+static const ResidentText& resident_text(const db_t* db) {
     std::lock_guard<std::mutex> lk(g_text_mu);
     for (auto* t : g_texts)
         if (t->host_T == db->T && t->t_size == db->Tsize && t->host_SA == db->SA) return *t;

@@ -77,40 +77,134 @@ int DeviceBuffer::upload(const void* src, size_t n) {
 // ---------------------------------------------------------------------------
 // batched forward Smith-Waterman
 // ---------------------------------------------------------------------------
+constexpr int kTaskPairs = 128;     // two pairs per lane of a wavefront
+constexpr int kTaskQueries = 4;     // queries whose profiles a wavefront holds
+constexpr int kChunkRows = 32;      // query rows per chunk of the kernel
+
 struct Tasks {
     std::vector<int32_t> index;   // pair numbers, grouped per task
     std::vector<int32_t> begin;   // task boundaries into index
-    std::vector<int32_t> query;   // query of each task
-    std::vector<int64_t> cost;    // rows x widest subject
+    std::vector<int32_t> query;   // kTaskQueries query numbers per task, -1 = none
+    std::vector<int64_t> cost;    // rows (whole chunks) x widest subject
+    void clear() { index.clear(); begin.clear(); query.clear(); cost.clear(); }
+    void open(const int32_t* qs, int nq, int64_t c) {
+        begin.push_back((int32_t)index.size());
+        for (int k = 0; k < kTaskQueries; ++k) query.push_back(k < nq ? qs[k] : -1);
+        cost.push_back(c);
+    }
+    int32_t count() const { return (int32_t)cost.size(); }
 };
 
-// Group pairs by query, sort each query's pairs by subject length (longest
-// first) and cut them into tasks of at most 128 pairs: the two halves of a
-// lane and the lanes of a wavefront then walk subjects of similar length.
-static void build_tasks(const std::vector<int32_t>& pairs,
-                        const std::vector<int32_t>& pair_query,
-                        const std::vector<int32_t>& pair_subject,
-                        const std::vector<int32_t>& q_len,
-                        const std::vector<int32_t>& s_len, Tasks& t) {
-    std::vector<int32_t> order(pairs);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-        if (pair_query[a] != pair_query[b]) return pair_query[a] < pair_query[b];
-        return s_len[pair_subject[a]] > s_len[pair_subject[b]];
-    });
-    t.index.clear(); t.begin.clear(); t.query.clear(); t.cost.clear();
-    size_t i = 0;
-    while (i < order.size()) {
-        const int32_t q = pair_query[order[i]];
-        size_t j = i;
-        while (j < order.size() && pair_query[order[j]] == q && j - i < 128) ++j;
-        t.begin.push_back((int32_t)t.index.size());
-        t.query.push_back(q);
-        t.cost.push_back((int64_t)q_len[q] * s_len[pair_subject[order[i]]]);
-        for (size_t k = i; k < j; ++k) t.index.push_back(order[k]);
-        i = j;
+static int64_t chunk_rows(int32_t m) {
+    return (int64_t)(m + kChunkRows - 1) / kChunkRows * kChunkRows;
+}
+
+// Pairs of each query, longest subject first (ties: lower pair number).
+static std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pair_query,
+                                                        const std::vector<int32_t>& pair_subject,
+                                                        const std::vector<int32_t>& s_len,
+                                                        int32_t n_queries) {
+    std::vector<std::vector<int32_t>> per(n_queries);
+    for (int32_t p = 0; p < (int32_t)pair_query.size(); ++p) per[pair_query[p]].push_back(p);
+    for (auto& v : per)
+        std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
+            return s_len[pair_subject[x]] > s_len[pair_subject[y]];
+        });
+    return per;
+}
+
+// One query per task (the plain schedule): a query's pairs, longest subject
+// first, cut into tasks of 128, so the two halves of a lane and the lanes of
+// a wavefront walk subjects of similar length.
+static void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
+                              const std::vector<int32_t>& pair_subject,
+                              const std::vector<int32_t>& q_len,
+                              const std::vector<int32_t>& s_len, Tasks& t) {
+    t.clear();
+    for (int32_t q = 0; q < (int32_t)per.size(); ++q) {
+        const auto& v = per[q];
+        for (size_t i = 0; i < v.size(); i += kTaskPairs) {
+            t.open(&q, 1, chunk_rows(q_len[q]) * s_len[pair_subject[v[i]]]);
+            for (size_t k = i; k < v.size() && k < i + kTaskPairs; ++k) t.index.push_back(v[k]);
+        }
     }
     t.begin.push_back((int32_t)t.index.size());
 }
+
+// Tasks of up to four queries (the persistent schedule).  Queries with the
+// same number of 32-row chunks are packed together: repeatedly the four
+// queries whose longest remaining subjects are longest give their longest
+// remaining pairs, longest first, until the task holds 128.  A query's long
+// subjects then share a wavefront with other queries' long subjects instead
+// of its own short ones, which cuts the columns a wavefront walks past the
+// end of a lane's subject.
+static void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
+                               const std::vector<int32_t>& pair_subject,
+                               const std::vector<int32_t>& q_len,
+                               const std::vector<int32_t>& s_len, Tasks& t) {
+    t.clear();
+    const int32_t nq = (int32_t)per.size();
+    std::vector<int32_t> order;
+    for (int32_t q = 0; q < nq; ++q)
+        if (!per[q].empty()) order.push_back(q);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return chunk_rows(q_len[x]) < chunk_rows(q_len[y]);
+    });
+    std::vector<size_t> head(nq, 0);
+    auto head_len = [&](int32_t q) { return s_len[pair_subject[per[q][head[q]]]]; };
+    // max-heap on (longest remaining subject, then lower query number)
+    auto below = [&](int32_t x, int32_t y) {
+        const int32_t lx = head_len(x), ly = head_len(y);
+        return lx != ly ? lx < ly : x > y;
+    };
+    size_t g0 = 0;
+    while (g0 < order.size()) {
+        const int64_t rows = chunk_rows(q_len[order[g0]]);
+        size_t g1 = g0;
+        while (g1 < order.size() && chunk_rows(q_len[order[g1]]) == rows) ++g1;
+        std::vector<int32_t> heap(order.begin() + g0, order.begin() + g1);
+        std::make_heap(heap.begin(), heap.end(), below);
+        while (!heap.empty()) {
+            int32_t sel[kTaskQueries];
+            int ns = 0;
+            while (ns < kTaskQueries && !heap.empty()) {
+                std::pop_heap(heap.begin(), heap.end(), below);
+                sel[ns++] = heap.back();
+                heap.pop_back();
+            }
+            t.open(sel, ns, rows * head_len(sel[0]));
+            for (int taken = 0; taken < kTaskPairs; ++taken) {
+                int best = -1;
+                for (int k = 0; k < ns; ++k) {
+                    const int32_t q = sel[k];
+                    if (head[q] < per[q].size() && (best < 0 || below(sel[best], q))) best = k;
+                }
+                if (best < 0) break;
+                const int32_t q = sel[best];
+                t.index.push_back(per[q][head[q]++]);
+            }
+            for (int k = 0; k < ns; ++k) {
+                if (head[sel[k]] < per[sel[k]].size()) {
+                    heap.push_back(sel[k]);
+                    std::push_heap(heap.begin(), heap.end(), below);
+                }
+            }
+        }
+        g0 = g1;
+    }
+    t.begin.push_back((int32_t)t.index.size());
+}
+
+// A task set on the device.
+struct DeviceTasks {
+    Tasks host;
+    DeviceBuffer d_index, d_begin, d_query;
+    int upload() {
+        return d_index.upload(host.index.data(), host.index.size() * 4) |
+               d_begin.upload(host.begin.data(), host.begin.size() * 4) |
+               d_query.upload(host.query.data(), host.query.size() * 4);
+    }
+};
 
 }  // namespace tpz
 
@@ -125,8 +219,8 @@ struct tpz_sw_batch {
     DeviceBuffer d_pair_query, d_pair_subject, d_matrix;
     DeviceBuffer d_score, d_ref, d_read;            // results
     DeviceBuffer d_workspace, d_counter, d_error;
-    Tasks tasks;                                    // grouped per query
-    DeviceBuffer d_task_index, d_task_begin;
+    DeviceTasks plain_tasks;                        // one query per task
+    DeviceTasks packed_tasks;                       // up to four queries per task
     // work items (4 task slots each): per query for the plain schedule, by
     // decreasing cost for the persistent one
     std::vector<int32_t> items_plain, items_persistent, plain_query_items;
@@ -280,21 +374,23 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     rc |= b->d_error.alloc(64);
     if (!rc) rc |= check(hipMemsetAsync(b->d_error.ptr, 0, 64, stream()), "memset");
 
-    std::vector<int32_t> all(n_pairs);
-    std::iota(all.begin(), all.end(), 0);
-    build_tasks(all, b->pair_query, b->pair_subject, b->q_len, b->s_len, b->tasks);
-    rc |= b->d_task_index.upload(b->tasks.index.data(), b->tasks.index.size() * 4);
-    rc |= b->d_task_begin.upload(b->tasks.begin.data(), b->tasks.begin.size() * 4);
+    {
+        const auto per = pairs_by_query(b->pair_query, b->pair_subject, b->s_len, n_queries);
+        build_query_tasks(per, b->pair_subject, b->q_len, b->s_len, b->plain_tasks.host);
+        build_packed_tasks(per, b->pair_subject, b->q_len, b->s_len, b->packed_tasks.host);
+    }
+    rc |= b->plain_tasks.upload();
+    rc |= b->packed_tasks.upload();
 
     // plain: per query, its tasks four to an item
     size_t max_items_per_query = 0;
     {
-        const Tasks& t = b->tasks;
-        const int n_tasks = (int)t.begin.size() - 1;
+        const Tasks& t = b->plain_tasks.host;
+        const int n_tasks = t.count();
         int k = 0;
         while (k < n_tasks) {
             int e = k;
-            while (e < n_tasks && t.query[e] == t.query[k]) ++e;
+            while (e < n_tasks && t.query[kTaskQueries * e] == t.query[kTaskQueries * k]) ++e;
             b->plain_query_items.push_back((int32_t)(b->items_plain.size() / 4));
             for (int j = k; j < e; j += 4)
                 for (int w = 0; w < 4; ++w) b->items_plain.push_back(j + w < e ? j + w : -1);
@@ -305,7 +401,7 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     }
     const size_t persistent_slots = (size_t)std::max(num_cu(), 1) * kGroupsPerCu;
     if (const char* env = getenv("TOPAZ_AMD_BIG_FRACTION")) b->big_fraction = atof(env);
-    build_items(b->tasks, (int)persistent_slots, b->big_fraction, b->items_persistent,
+    build_items(b->packed_tasks.host, (int)persistent_slots, b->big_fraction, b->items_persistent,
                 b->big_tasks);
     rc |= b->d_items_plain.upload(b->items_plain.data(), b->items_plain.size() * 4);
     rc |= b->d_items_persistent.upload(b->items_persistent.data(), b->items_persistent.size() * 4);
@@ -350,8 +446,10 @@ int tpz_sw_batch_run(tpz_sw_batch* b, int32_t schedule) {
     if (schedule != TPZ_SCHED_PLAIN && schedule != TPZ_SCHED_PERSISTENT)
         return fail("unknown schedule %d", schedule);
     SwKernelArgs a = make_args(b);
-    a.pair_index = (const int32_t*)b->d_task_index.ptr;
-    a.task_begin = (const int32_t*)b->d_task_begin.ptr;
+    const DeviceTasks& dt = schedule == TPZ_SCHED_PERSISTENT ? b->packed_tasks : b->plain_tasks;
+    a.pair_index = (const int32_t*)dt.d_index.ptr;
+    a.task_begin = (const int32_t*)dt.d_begin.ptr;
+    a.task_query = (const int32_t*)dt.d_query.ptr;
     a.error = (int32_t*)b->d_error.ptr;
     if (check(hipEventRecord(b->ev[0], stream()), "event")) return -1;
     if (schedule == TPZ_SCHED_PERSISTENT) {

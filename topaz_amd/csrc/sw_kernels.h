@@ -11,7 +11,8 @@ namespace tpz {
 // Residues are substitution-matrix indices 0..24; subjects start on 4-byte
 // boundaries of s_res and are padded with residue 25.  A task is a range of
 // at most 128 entries of pair_index (or of the pair numbers themselves when
-// pair_index is null), all with the same query.  A work item is either one
+// pair_index is null) whose pairs belong to at most four queries, listed in
+// task_query (4 per task, -1 past the last).  A work item is either one
 // task worked by a whole workgroup, or up to four tasks, one per wavefront.
 struct SwKernelArgs {
     const int8_t* q_res;
@@ -24,6 +25,7 @@ struct SwKernelArgs {
     const int32_t* pair_subject;
     const int32_t* pair_index;     // may be null
     const int32_t* task_begin;     // task t = entries [task_begin[t], task_begin[t+1])
+    const int32_t* task_query;     // 4 query numbers per task, -1 = none
     const int32_t* item_task;      // 4 entries per work item (see sw_forward_kernel)
     int32_t n_items;
     int32_t* task_counter;         // null: workgroup b runs item b

@@ -21,14 +21,17 @@
 // gives the byte result when the score is below 255 - bias, the word result
 // below 32767, and the reference's NULL at 32767.
 //
-// Mapping (inter-task): one wavefront = one query and up to 128 subjects;
-// lane l holds two pairs, one in each 16-bit half of every VGPR, so each
-// VOP3P instruction advances two DP cells.  Query rows are processed in
+// Mapping (inter-task): one wavefront = up to 128 pairs of up to four
+// queries; lane l holds two pairs, one in each 16-bit half of every VGPR, so
+// each VOP3P instruction advances two DP cells.  Query rows are processed in
 // chunks of 32 consecutive rows held in registers (H, E) while the lane walks
 // the subject; a chunk's bottom row (H, F) is written per column to a
-// per-wave workspace that the next chunk reads back in place.  The last
-// chunk is padded with rows scoring -8192, which never reach a column's
-// running maximum.
+// per-wave workspace that the next chunk reads back in place.  The profile
+// of the chunk (32 rows x 26 residues) of each of the task's queries sits in
+// LDS and every lane half reads the one of its own query, so packing pairs of
+// several queries into a wavefront costs nothing in the inner loop.  Rows past
+// a query's end score -8192: a value there stays below a cell of the same
+// chunk already counted in the running maximum, so it is never a maximum.
 //
 // End positions: within a chunk, a column whose max rises strictly above the
 // chunk's best so far is an "event"; on an event the lane's 32 H values of
@@ -47,6 +50,9 @@ typedef unsigned short u2 __attribute__((ext_vector_type(2)));
 
 constexpr int kRows = 32;          // query rows per chunk, held in registers
 constexpr int kNRes = 26;          // 25 matrix residues + 1 padding residue
+constexpr int kQ = 4;              // queries per task
+constexpr int kProf = kRows * kNRes;
+constexpr int kRowStride = kQ * kNRes;   // profile row r of query k at r * 104 + k * 26
 constexpr int kPadResidue = 25;
 constexpr int kPadScore = -8192;   // score of padding rows / columns
 
@@ -61,7 +67,10 @@ __device__ __forceinline__ s2 from_bits(uint32_t b) { return __builtin_bit_cast(
 
 // LDS of one wavefront.
 struct WaveLds {
-    int16_t prof[kRows * kNRes];           // W(s, q[j0 + r]) at [r * 26 + s]
+    // W(s, q_k[j0 + r]) at [r * 104 + k * 26 + s]: the four queries' entries
+    // of a row are adjacent, so the lanes of one read spread over 52
+    // consecutive dwords of LDS
+    int16_t prof[kQ * kProf];
     uint16_t saved[2][kRows][64];          // H column of the chunk best, per half
 };
 
@@ -149,8 +158,8 @@ __device__ __forceinline__ void column_event(WaveLds& lds, const s2 (&Hc)[kRows]
 // a wait on the producer of the top boundary gave up.
 __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, const ChunkIO& io,
                                           int ncols, const int8_t* s_lo, int n_lo,
-                                          const int8_t* s_hi, int n_hi, int j0, int chunk,
-                                          LaneBest& best) {
+                                          const int8_t* s_hi, int n_hi, int off_lo, int off_hi,
+                                          int j0, int chunk, LaneBest& best) {
     const int lane = threadIdx.x % 64;
     const s2 go = s2{(short)a.gap_open, (short)a.gap_open};
     const s2 ge = s2{(short)a.gap_extend, (short)a.gap_extend};
@@ -200,10 +209,10 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
             res_lo = load4(s_lo, i + 2, n_lo);
             res_hi = load4(s_hi, i + 2, n_hi);
         }
-        const int16_t* pa_lo = lds.prof + ra_lo;
-        const int16_t* pa_hi = lds.prof + ra_hi;
-        const int16_t* pb_lo = lds.prof + rb_lo;
-        const int16_t* pb_hi = lds.prof + rb_hi;
+        const int16_t* pa_lo = lds.prof + off_lo + ra_lo;
+        const int16_t* pa_hi = lds.prof + off_hi + ra_hi;
+        const int16_t* pb_lo = lds.prof + off_lo + rb_lo;
+        const int16_t* pb_hi = lds.prof + off_hi + rb_hi;
 
         s2 fa = from_bits(ta_f), fb = from_bits(tb_f);
         s2 da = hdiag;                  // H(i-1, j0-1)
@@ -226,7 +235,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
         for (int r = 1; r < kRows; ++r) {
             // column a, row r
             {
-                s2 w = s2{pa_lo[r * kNRes], pa_hi[r * kNRes]};
+                s2 w = s2{pa_lo[r * kRowStride], pa_hi[r * kRowStride]};
                 s2 h = vmax(vmax(add_sat(da, w), E[r]), fa);
                 da = H[r];
                 Ha[r] = h;
@@ -238,7 +247,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
             // column b, row r - 1
             {
                 const int q = r - 1;
-                s2 w = s2{pb_lo[q * kNRes], pb_hi[q * kNRes]};
+                s2 w = s2{pb_lo[q * kRowStride], pb_hi[q * kRowStride]};
                 s2 h = vmax(vmax(add_sat(db, w), E[q]), fb);
                 db = Ha[q];
                 H[q] = h;
@@ -251,7 +260,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
         // column b, last row
         {
             const int q = kRows - 1;
-            s2 w = s2{pb_lo[q * kNRes], pb_hi[q * kNRes]};
+            s2 w = s2{pb_lo[q * kRowStride], pb_hi[q * kRowStride]};
             s2 h = vmax(vmax(add_sat(db, w), E[q]), fb);
             H[q] = h;
             cmb = vmax(cmb, h);
@@ -294,39 +303,50 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
     return true;
 }
 
-// Substitution scores of rows j0 .. j0+31 for all 26 residues; rows past
-// the query end score kPadScore.  Written and read by one wavefront only.
+// The pairs of one task as seen by one lane.
+struct TaskView {
+    int pair_lo, pair_hi;
+    bool has_lo, has_hi;
+    const int8_t* qs[kQ];    // the task's queries (wave-uniform); null past the last
+    int ms[kQ];
+    int m_max;
+    int off_lo, off_hi;      // profile offset of each half's query
+    int m_lo, m_hi;
+    const int8_t* s_lo;
+    const int8_t* s_hi;
+    int n_lo, n_hi;
+    int ncols;               // even, wave-uniform
+};
+
+// Substitution scores of rows j0 .. j0+31 of every query of the task for all
+// 26 residues; rows past a query's end score kPadScore.  Written and read by
+// one wavefront only.
 __device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds& lds,
-                                              const int8_t* q, int m, int j0) {
+                                              const TaskView& t, int j0) {
     const int lane = threadIdx.x % 64;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (int e = lane; e < kRows * kNRes; e += 64) {
-        const int r = e / kNRes, s = e - r * kNRes;
-        const int j = j0 + r;
-        int w = kPadScore;
-        if (s < kPadResidue && j < m) w = a.matrix[s * 25 + q[j]];
-        lds.prof[e] = (int16_t)w;
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+        const int8_t* q = t.qs[k];
+        if (!q) break;
+        const int m = t.ms[k];
+        for (int e = lane; e < kProf; e += 64) {
+            const int r = e / kNRes, s = e - r * kNRes;
+            const int j = j0 + r;
+            int w = kPadScore;
+            if (s < kPadResidue && j < m) w = a.matrix[s * 25 + q[j]];
+            lds.prof[r * kRowStride + k * kNRes + s] = (int16_t)w;
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The pairs of one task as seen by one lane.
-struct TaskView {
-    int pair_lo, pair_hi;
-    bool has_lo, has_hi;
-    const int8_t* q;
-    int m;
-    const int8_t* s_lo;
-    const int8_t* s_hi;
-    int n_lo, n_hi;
-    int ncols;           // even, wave-uniform
-};
-
 __device__ __forceinline__ TaskView task_view(const SwKernelArgs& a, int task) {
     const int lane = threadIdx.x % 64;
+    task = __builtin_amdgcn_readfirstlane(task);
     TaskView t;
     const int p0 = a.task_begin[task], p1 = a.task_begin[task + 1];
     const int np = p1 - p0;
@@ -336,9 +356,26 @@ __device__ __forceinline__ TaskView task_view(const SwKernelArgs& a, int task) {
     const int k_hi = p0 + (t.has_hi ? lane + 64 : 0);
     t.pair_lo = a.pair_index ? a.pair_index[k_lo] : k_lo;
     t.pair_hi = a.pair_index ? a.pair_index[k_hi] : k_hi;
-    const int qid = a.pair_query[t.pair_lo];
-    t.q = a.q_res + a.q_off[qid];
-    t.m = a.q_len[qid];
+    int tq[kQ];
+    t.m_max = 0;
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+        tq[k] = a.task_query[kQ * task + k];
+        t.qs[k] = tq[k] >= 0 ? a.q_res + a.q_off[tq[k]] : nullptr;
+        t.ms[k] = tq[k] >= 0 ? a.q_len[tq[k]] : 0;
+        t.m_max = t.ms[k] > t.m_max ? t.ms[k] : t.m_max;
+    }
+    const int qid_lo = a.pair_query[t.pair_lo], qid_hi = a.pair_query[t.pair_hi];
+    int slot_lo = 0, slot_hi = 0;
+#pragma unroll
+    for (int k = 1; k < kQ; ++k) {
+        if (qid_lo == tq[k]) slot_lo = k;
+        if (qid_hi == tq[k]) slot_hi = k;
+    }
+    t.off_lo = slot_lo * kNRes;
+    t.off_hi = slot_hi * kNRes;
+    t.m_lo = t.has_lo ? a.q_len[qid_lo] : 0;
+    t.m_hi = t.has_hi ? a.q_len[qid_hi] : 0;
     const int sid_lo = a.pair_subject[t.pair_lo], sid_hi = a.pair_subject[t.pair_hi];
     t.s_lo = a.s_res + a.s_off[sid_lo];
     t.s_hi = a.s_res + a.s_off[sid_hi];
@@ -354,11 +391,12 @@ __device__ __forceinline__ TaskView task_view(const SwKernelArgs& a, int task) {
     return t;
 }
 
-__device__ __forceinline__ void init_best(LaneBest& best, int m) {
+__device__ __forceinline__ void init_best(LaneBest& best, const TaskView& t) {
     // score 0: ref_end1 -1 and read_end1 0, or -1 for an empty query
     // (ssw.c:148-149, 288-297)
     for (int h = 0; h < 2; ++h) {
-        best.score[h] = 0; best.ref[h] = -1; best.read[h] = m > 0 ? 0 : -1; best.chunk[h] = 0;
+        best.score[h] = 0; best.ref[h] = -1; best.chunk[h] = 0;
+        best.read[h] = (h ? t.m_hi : t.m_lo) > 0 ? 0 : -1;
     }
 }
 
@@ -390,16 +428,17 @@ __device__ void run_task_wave(const SwKernelArgs& a, WaveLds& lds, uint32_t* buf
                               GroupLds& g) {
     const TaskView t = task_view(a, task);
     LaneBest best;
-    init_best(best, t.m);
+    init_best(best, t);
     int chunk = 0;
-    for (int j0 = 0; j0 < t.m; j0 += kRows, ++chunk) {
-        build_profile(a, lds, t.q, t.m, j0);
+    for (int j0 = 0; j0 < t.m_max; j0 += kRows, ++chunk) {
+        build_profile(a, lds, t, j0);
         ChunkIO io{};
         io.top = j0 == 0 ? nullptr : buf;
-        io.bottom = j0 + kRows < t.m ? buf : nullptr;
+        io.bottom = j0 + kRows < t.m_max ? buf : nullptr;
         io.error = &g.error;
         io.g_error = a.error;
-        run_chunk(a, lds, io, t.ncols, t.s_lo, t.n_lo, t.s_hi, t.n_hi, j0, chunk, best);
+        run_chunk(a, lds, io, t.ncols, t.s_lo, t.n_lo, t.s_hi, t.n_hi, t.off_lo, t.off_hi, j0,
+                  chunk, best);
     }
     write_best(a, t, best);
 }
@@ -412,12 +451,12 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds& g, uint32_t* buf
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const TaskView t = task_view(a, task);
     LaneBest best;
-    init_best(best, t.m);
-    const int n_chunks = (t.m + kRows - 1) / kRows;
+    init_best(best, t);
+    const int n_chunks = (t.m_max + kRows - 1) / kRows;
     const int nc = t.ncols;
     for (int c = wave; c < n_chunks; c += kWaves) {
         const int j0 = c * kRows;
-        build_profile(a, g.wave[wave], t.q, t.m, j0);
+        build_profile(a, g.wave[wave], t, j0);
         ChunkIO io{};
         io.top = c == 0 ? nullptr : bufs + (size_t)((c - 1) % kWaves) * buf_words;
         io.bottom = c + 1 < n_chunks ? bufs + (size_t)(c % kWaves) * buf_words : nullptr;
@@ -429,7 +468,8 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds& g, uint32_t* buf
         io.pub_base = (c / kWaves) * nc;
         io.error = &g.error;
         io.g_error = a.error;
-        if (!run_chunk(a, g.wave[wave], io, nc, t.s_lo, t.n_lo, t.s_hi, t.n_hi, j0, c, best))
+        if (!run_chunk(a, g.wave[wave], io, nc, t.s_lo, t.n_lo, t.s_hi, t.n_hi, t.off_lo,
+                       t.off_hi, j0, c, best))
             break;
         if (nc == 0) publish(io, io.pub_base);
     }
