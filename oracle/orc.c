@@ -287,6 +287,31 @@ static int32_t band_dir(int32_t w, int32_t i, int32_t j, int32_t p) {
     return (j - imax(i - w, 0)) * 3 + p;
 }
 
+/* Next power of two above x (kroundup32 after ++, ssw.c:65-66, 572-574,
+ * 593-595). */
+static int64_t grow_pow2(int64_t x) {
+    int64_t v = x + 1;
+    --v;
+    v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+    return v + 1;
+}
+
+/* Grow a buffer keeping its contents; the new part reads as zero (what a
+ * fresh allocation holds; the reference's realloc leaves it unspecified). */
+static void* grow_zero(void* p, size_t old_bytes, size_t new_bytes) {
+    char* q = (char*)realloc(p, new_bytes);
+    memset(q + old_bytes, 0, new_bytes - old_bytes);
+    return q;
+}
+
+/* The reference keeps one direction buffer and three band rows for all
+ * passes of the doubling loop (ssw.c:559-563, 575-597): each pass writes
+ * the cells of its own band layout, E is never cleared between passes and
+ * H only on slots 1..width-2, and the traceback indexes the direction
+ * buffer as one flat array, so a step outside the final band's row reads
+ * what the row above or below (or an earlier pass) left there.  This
+ * restatement does the same with buffers that start zeroed; reads outside
+ * the direction buffer end the traceback as an error. */
 int orc_banded_sw(const int8_t* ref, const int8_t* read,
                   int32_t refLen, int32_t readLen, int32_t score,
                   int32_t gapO, int32_t gapE, int32_t band_width,
@@ -294,11 +319,14 @@ int orc_banded_sw(const int8_t* ref, const int8_t* read,
                   int32_t* length, int32_t* gapopen, int32_t* mismatch,
                   double* identity) {
     int32_t w = band_width, best = 0, width = 0, width_d = 0;
-    int32_t *hb = NULL, *eb = NULL, *hc = NULL;
-    int8_t* dirs = NULL;
+    int64_t s1 = 8, s2 = 1024;
+    int32_t* hb = (int32_t*)calloc((size_t)s1, sizeof(int32_t));
+    int32_t* eb = (int32_t*)calloc((size_t)s1, sizeof(int32_t));
+    int32_t* hc = (int32_t*)calloc((size_t)s1, sizeof(int32_t));
+    int8_t* dirs = (int8_t*)calloc((size_t)s2, 1);
     int32_t i, j, state, identical = 0;
     char op, prev_op;
-    const int8_t* line;
+    int64_t line;
 
     *length = 0; *gapopen = 0; *mismatch = 0; *identity = 0.0;
 
@@ -306,20 +334,29 @@ int orc_banded_sw(const int8_t* ref, const int8_t* read,
         int64_t ncell;
         width = 2 * w + 3;
         width_d = 2 * w + 1;
+        if (width >= s1) {
+            int64_t ns = s1;
+            while (width >= ns) ns = grow_pow2(ns + 1);
+            hb = (int32_t*)grow_zero(hb, (size_t)s1 * 4, (size_t)ns * 4);
+            eb = (int32_t*)grow_zero(eb, (size_t)s1 * 4, (size_t)ns * 4);
+            hc = (int32_t*)grow_zero(hc, (size_t)s1 * 4, (size_t)ns * 4);
+            s1 = ns;
+        }
+        /* ssw.c:580-614: the direction buffer must stay below INT_MAX */
         ncell = (int64_t)width_d * readLen * 3;
-        /* ssw.c:580-614: the direction matrix must stay below 2^30 bytes
-         * (the reference's power-of-two growth stops at INT_MAX). */
         if (ncell >= ((int64_t)1 << 30)) {
             *length = *gapopen = *mismatch = -1;
             *identity = -1.0;
             free(hb); free(eb); free(hc); free(dirs);
             return 0;
         }
-        free(hb); free(eb); free(hc); free(dirs);
-        hb = (int32_t*)calloc((size_t)width + 1, sizeof(int32_t));
-        eb = (int32_t*)calloc((size_t)width + 1, sizeof(int32_t));
-        hc = (int32_t*)calloc((size_t)width + 1, sizeof(int32_t));
-        dirs = (int8_t*)calloc((size_t)(ncell > 0 ? ncell : 1), 1);
+        if (ncell >= s2) {
+            int64_t ns = s2;
+            while (ncell >= ns) ns = grow_pow2(ns + 1);
+            dirs = (int8_t*)grow_zero(dirs, (size_t)s2, (size_t)ns);
+            s2 = ns;
+        }
+        for (j = 1; j < width - 1; ++j) hb[j] = 0;
 
         for (i = 0; i < readLen; ++i) {
             const int32_t beg = imax(0, i - w);
@@ -368,10 +405,10 @@ int orc_banded_sw(const int8_t* ref, const int8_t* read,
     j = refLen - 1;
     state = 2;
     op = prev_op = 'M';
-    line = dirs + (size_t)width_d * (readLen > 0 ? readLen - 1 : 0) * 3;
+    line = (int64_t)width_d * (readLen > 0 ? readLen - 1 : 0) * 3;
     while (i > 0) {
-        const int32_t c = band_dir(w, i, j, state);
-        const int8_t code = (c >= 0 && c < width_d * 3) ? line[c] : 0;
+        const int64_t c = line + band_dir(w, i, j, state);
+        const int8_t code = (c >= 0 && c < s2) ? dirs[c] : 0;
         switch (code) {
         case 1: --i; --j; state = 2; line -= width_d * 3; op = 'M'; break;
         case 2: --i; state = 0; line -= width_d * 3; op = 'I'; break;

@@ -55,10 +55,26 @@ def _search(exe, tmp, *flags):
     return r.stdout
 
 
+def _per_query(text):
+    """The hit list as {query id: its lines in printed order}, plus the
+    lines before the first hit.  The reference prints each query's block
+    when that query finishes, and the order of the blocks varies between
+    runs of the stock binary itself (the same input printed q0's block
+    second or eighth, 3 runs in 30), so blocks are compared per query."""
+    head, blocks = [], {}
+    for line in text.splitlines():
+        fields = line.split("\t")
+        if len(fields) < 2:
+            head.append(line)
+        else:
+            blocks.setdefault(fields[0], []).append(line)
+    return head, blocks
+
+
 @pytest.mark.parametrize("flags", [[], ["-B"], ["-F"]])
 def test_hit_lists_identical(workspace, flags):
     tmp, dropin = workspace
     ref = _search(REF / "topaz", tmp, *flags)
     ours = _search(dropin, tmp, *flags)
     assert ref.count("\n") > 20
-    assert ours == ref
+    assert _per_query(ours) == _per_query(ref)

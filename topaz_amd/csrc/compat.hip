@@ -387,29 +387,16 @@ void align_using_ssw(seq_t* s1, seq_t* s2, hit_t* hit, options_t* opt) {
 // search.h: sa_hsp on a device-resident copy of db->T / db->SA
 // ---------------------------------------------------------------------------
 namespace {
-// db->T (and db->SA) uploaded once per db_t and kept on the device.
+// db->T (and db->SA) uploaded once per db_t and kept on the device as the
+// gapless kernel's text.
 struct ResidentText {
     const char* host_T = nullptr;
     uint64_t t_size = 0;
     const int64_t* host_SA = nullptr;
-    void* d_text = nullptr;
-    const int64_t* text_sa = nullptr;
+    tpz_text* text = nullptr;
 };
 std::mutex g_text_mu;
 std::vector<ResidentText*> g_texts;
-
-struct Maps {
-    int8_t tmap[128], qmap[128];
-    Maps() {
-        // lexicographical.h:19-33 (aa_table2) and :60-74 (upper_aa_only)
-        const Alphabet& ab = alphabet();
-        for (int c = 0; c < 128; ++c) {
-            tmap[c] = ab.table2[c];
-            const bool keep = (c >= 'A' && c <= 'Y') || c == '*' || c == '\n';
-            qmap[c] = keep ? ab.table2[c] : ab.table2['V'];
-        }
-    }
-} const g_maps;
 
 static const ResidentText& resident_text(const db_t* db) {
     std::lock_guard<std::mutex> lk(g_text_mu);
@@ -419,13 +406,9 @@ static const ResidentText& resident_text(const db_t* db) {
     r->host_T = db->T;
     r->t_size = db->Tsize;
     r->host_SA = db->SA;
-    hip_or_die(hipMalloc(&r->d_text, db->Tsize + 16), "hipMalloc text");
-    hip_or_die(hipMemcpy(r->d_text, db->T, db->Tsize, hipMemcpyHostToDevice), "upload text");
-    if (db->SA) {
-        void* sa = nullptr;
-        hip_or_die(hipMalloc(&sa, db->Tsize * 8 + 16), "hipMalloc SA");
-        hip_or_die(hipMemcpy(sa, db->SA, db->Tsize * 8, hipMemcpyHostToDevice), "upload SA");
-        r->text_sa = (const int64_t*)sa;
+    if (tpz_text_create(&r->text, db->T, db->Tsize, db->SA)) {
+        fprintf(stderr, "topaz_amd: sa_hsp: %s\n", tpz_last_error());
+        _exit(EXIT_FAILURE);
     }
     g_texts.push_back(r);
     return *r;
@@ -435,45 +418,51 @@ static const ResidentText& resident_text(const db_t* db) {
 int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
            int* gapless_score, int* gapless_length) {
     const ResidentText& rt = resident_text(db);
-    if (rt.text_sa && (position < 0 || (uint64_t)position >= db->Tsize)) {
+    if (db->SA && (position < 0 || (uint64_t)position >= db->Tsize)) {
         fprintf(stderr, "topaz_amd: sa_hsp: suffix-array slot %lld outside the text\n",
                 (long long)position);
         _exit(EXIT_FAILURE);
     }
     ThreadCtx& c = ctx();
-    // one seed against the resident text: query bytes, seed, maps, matrix
-    const size_t m = (size_t)q->seq_len;
-    const size_t o_q = 0, o_seed = (m + 16 + 15) & ~(size_t)15, o_maps = o_seed + 32,
-                 o_mat = o_maps + 256, o_meta = (o_mat + 625 + 15) & ~(size_t)15;
+    // one seed against the resident text: query copies, seed, table, lengths
+    QueryCodes qc;
+    const int64_t zero64 = 0;
+    const int32_t qlen = q->seq_len;
+    if (gapless_query_codes(rt.text->codes, 1, q->seq_buf.buf, &zero64, &qlen, qc)) {
+        fprintf(stderr, "topaz_amd: sa_hsp: %s\n", tpz_last_error());
+        _exit(EXIT_FAILURE);
+    }
+    const size_t o_qf = 0, o_qr = (qc.fwd.size() + 15) & ~(size_t)15,
+                 o_tab = (o_qr + qc.rev.size() + 15) & ~(size_t)15,
+                 o_seed = o_tab + kTableRows * kTableCols * 2, o_meta = o_seed + 32;
     const size_t bytes = o_meta + 32;
     c.host.assign(bytes, 0);
     char* h = c.host.data();
-    memcpy(h + o_q, q->seq_buf.buf, m);
+    memcpy(h + o_qf, qc.fwd.data(), qc.fwd.size());
+    memcpy(h + o_qr, qc.rev.data(), qc.rev.size());
+    gapless_table(rt.text->codes, (const int8_t*)*opt->substitution, (int16_t*)(h + o_tab));
     SeedRec seed{0, qoffset, position};
     memcpy(h + o_seed, &seed, sizeof seed);
-    memcpy(h + o_maps, g_maps.tmap, 128);
-    memcpy(h + o_maps + 128, g_maps.qmap, 128);
-    memcpy(h + o_mat, (const int8_t*)*opt->substitution, 625);
-    const int64_t zero64 = 0;
-    const int32_t qlen = (int32_t)m;
-    memcpy(h + o_meta, &zero64, 8);
-    memcpy(h + o_meta + 8, &qlen, 4);
+    memcpy(h + o_meta, &zero64, 8);            // forward and reversed offsets
+    memcpy(h + o_meta + 8, &zero64, 8);
+    memcpy(h + o_meta + 16, &qlen, 4);
     char* d = (char*)c.in.get(bytes);
     int32_t* dout = (int32_t*)c.out.get(64);
     hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
     hip_or_die(hipMemsetAsync(dout, 0, 8, c.st), "memset");
     GaplessArgs a{};
-    a.text = (const uint8_t*)rt.d_text;
+    a.t_fwd = (const uint8_t*)rt.text->d_fwd.ptr;
+    a.t_rev = (const uint8_t*)rt.text->d_rev.ptr;
     a.t_size = (int64_t)db->Tsize;
-    a.sa = rt.text_sa;
-    a.q_chars = (const uint8_t*)(d + o_q);
-    a.q_off = (const int64_t*)(d + o_meta);
-    a.q_len = (const int32_t*)(d + o_meta + 8);
+    a.sa = rt.text->has_sa ? (const int64_t*)rt.text->d_sa.ptr : nullptr;
+    a.q_fwd = (const uint8_t*)(d + o_qf);
+    a.q_rev = (const uint8_t*)(d + o_qr);
+    a.qf_off = (const int64_t*)(d + o_meta);
+    a.qr_off = (const int64_t*)(d + o_meta + 8);
+    a.q_len = (const int32_t*)(d + o_meta + 16);
     a.seeds = (const SeedRec*)(d + o_seed);
     a.n_seeds = 1;
-    a.text_map = (const int8_t*)(d + o_maps);
-    a.query_map = (const int8_t*)(d + o_maps + 128);
-    a.matrix = (const int8_t*)(d + o_mat);
+    a.table = (const int16_t*)(d + o_tab);
     a.out_score = dout;
     a.out_length = dout + 1;
     hip_or_die(tpz_launch_gapless(&a, 1, c.st), "gapless");

@@ -15,18 +15,35 @@ struct SeedRec {
     int64_t position;
 };
 
+// Letter codes.  Every byte value present in the text (other than '\n') has
+// a text code 0..30 and '\n' has kTextTerm.  A query byte that occurs in the
+// text has the same code, any other query byte a code 32 + its matrix row,
+// so two codes are equal exactly when the two bytes are.  kQueryTerm ends a
+// query.  Query codes are stored doubled: the 16-bit pair (text code << 8 |
+// 2 * query code) is then the byte address of W(text code, query code) in a
+// 32 x 128 table of int16.
+constexpr int kTextTerm = 31;
+constexpr int kQueryTerm = 63;
+constexpr int kForeignCode = 32;
+constexpr int kTableRows = 32;
+constexpr int kTableCols = 128;
+constexpr int kTermScore = -30000;   // past a terminator no prefix sum is >= 0 again
+constexpr int kWindow = 128;         // sa_hsp's window on each side (search.c:99-190)
+constexpr int kTextPad = 256;        // terminator bytes after each code array
+
 struct GaplessArgs {
-    const uint8_t* text;        // t_size bytes, proteins separated by '\n'
+    const uint8_t* t_fwd;       // text codes, t_size bytes then kTextPad x kTextTerm
+    const uint8_t* t_rev;       // t_rev[j] = code of T[t_size-1-j], with T[0]'s slot kTextTerm
     int64_t t_size;
     const int64_t* sa;          // may be null: positions are text positions
-    const uint8_t* q_chars;     // queries, internal alphabet characters
-    const int64_t* q_off;
+    const uint8_t* q_fwd;       // per query: doubled codes of q[0..m-1], then 2 * kQueryTerm
+    const int64_t* qf_off;
+    const uint8_t* q_rev;       // per query: doubled codes of q[m-1..1], then two terminators
+    const int64_t* qr_off;
     const int32_t* q_len;
     const SeedRec* seeds;
     int64_t n_seeds;
-    const int8_t* text_map;     // 128 entries: text byte -> matrix index
-    const int8_t* query_map;    // 128 entries: query byte -> matrix index
-    const int8_t* matrix;       // 25 x 25
+    const int16_t* table;       // kTableRows x kTableCols, W(text code, query code)
     int32_t* out_score;
     int32_t* out_length;
 };

@@ -11,16 +11,28 @@
 //   * left: the same over i = 1 .. min(idx, o, 128)-1 on T[idx-i], Q[o-i];
 //   * score = best_left + best_right, length = len_left + len_right.
 // Text residues map to matrix rows through aa_table2; query residues through
-// aa_table2 of upper_aa_only (lexicographical.h:19-74), so any byte outside
-// 'A'..'Y', '*', '\n' in the query scores as 'V' (row 23).
+// aa_table2 of upper_aa_only (lexicographical.h:19-74).
 //
-// Mapping: one thread per seed loads the seed (16 B, coalesced) and tests the
-// first residue; about one seed in seventeen passes on uniform seeds.  The
-// wavefront then takes its passing seeds four at a time: a group of 16 lanes
-// per seed, each lane scoring 8 consecutive positions of the right window and
-// 8 of the left, so the group reads its 128-byte text windows as whole
-// segments.  Prefix sums, the first '\n' and the last argmax are 16-lane
-// scans and reductions.
+// Both sides become the same forward scan over 128 positions.  The host keeps
+// the text as letter codes twice, forward and reversed, and the queries the
+// same way; the left side of a seed is the forward scan of the reversed
+// copies from T[idx-1] and Q[o-1].  Every bound of the rule is a terminator
+// in the data: '\n' and the text's ends in the text copies, the query's end
+// in the forward query copy, and Q[0] and T[0] in the reversed copies (the
+// left scan never reaches index 0: i < min(idx, o)).  A terminator scores
+// -30000, after which no prefix sum is >= 0 again, so a scan simply runs its
+// 128 positions; only the left side's 128th position (i = 128, outside
+// i < 128) is dropped explicitly.
+//
+// Mapping: phase 1 gives one lane per seed: it loads the seed (16 B,
+// coalesced) and tests the first residue; about one uniform seed in
+// seventeen passes.  The passing seeds of the wavefront are compacted in LDS
+// and phase 2 serves them 32 at a time, two lanes per seed: the even lane
+// scans the right side, the odd lane the left.  A lane loads its 132-byte
+// text and query windows with dword loads, aligns them with v_alignbyte, and
+// per position pair forms (text code << 8 | doubled query code) with one
+// v_perm: each 16-bit half is the byte address of W in the LDS table.  The
+// running maximum and its last position are one max over (sum << 8 | i).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -28,96 +40,75 @@
 
 namespace tpz {
 
-constexpr int kWindow = 128;
-constexpr int kGroup = 16;          // lanes per seed
-constexpr int kPerLane = kWindow / kGroup;
+constexpr int kWaves = 4;
+constexpr int kWindowDwords = kWindow / 4;
 
-// Inclusive prefix sum over the 16 lanes of a group.
-__device__ __forceinline__ int group_scan_add(int v, int k) {
+struct Slot {
+    int64_t idx;
+    int64_t sid;
+    int32_t offset;
+    int32_t query;
+};
+
+struct GaplessLds {
+    int16_t table[kTableRows * kTableCols];
+    Slot slot[kWaves][64];
+};
+
+// 33 dwords starting at the dword holding p, aligned so that w[k] holds the
+// bytes p + 4k .. p + 4k + 3.
+__device__ __forceinline__ void load_window(const uint8_t* p, uint32_t (&w)[kWindowDwords]) {
+    typedef const __attribute__((address_space(1))) uint32_t* global_u32;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    const global_u32 d = (global_u32)(a & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(a & 3);
+    uint32_t x[kWindowDwords + 1];
 #pragma unroll
-    for (int d = 1; d < kGroup; d <<= 1) {
-        const int o = __shfl_up(v, d, kGroup);
-        if (k >= d) v += o;
-    }
-    return v;
-}
-__device__ __forceinline__ int group_max(int v) {
+    for (int k = 0; k <= kWindowDwords; ++k) x[k] = d[k];
 #pragma unroll
-    for (int d = kGroup / 2; d >= 1; d >>= 1) {
-        const int o = __shfl_xor(v, d, kGroup);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-__device__ __forceinline__ int group_min(int v) {
-#pragma unroll
-    for (int d = kGroup / 2; d >= 1; d >>= 1) {
-        const int o = __shfl_xor(v, d, kGroup);
-        v = o < v ? o : v;
-    }
-    return v;
+    for (int k = 0; k < kWindowDwords; ++k) w[k] = __builtin_amdgcn_alignbyte(x[k + 1], x[k], sh);
 }
 
-// One side of the extension for the seed of this lane's group.  Positions
-// i = first .. limit-1 (limit <= 128), lane k scoring i = 8k .. 8k+7; the
-// text index of position i is idx + dir*i, the query index o + dir*i.
-// Returns best << 8 | argmax (best >= 0), or 0 when no prefix is >= 0.
-__device__ __forceinline__ int extend_side(const uint8_t* T, int64_t idx, const uint8_t* Q,
-                                           int o, int dir, int first, int limit, int k,
-                                           const int8_t* tmap, const int8_t* qmap,
-                                           const int8_t* mat) {
-    int sc[kPerLane];
-    int nl = kWindow;                 // first '\n' position scored by this lane
-    int sum = 0;
+// One side: the last maximum of the prefix sums over positions 0..127 as
+// (sum << 8 | position), or -1 when no prefix sum is >= 0.  last_ok: whether
+// position 127 counts.
+__device__ __forceinline__ int scan_side(const uint8_t* t, const uint8_t* q, bool last_ok,
+                                         const GaplessLds& lds) {
+    uint32_t tw[kWindowDwords], qw[kWindowDwords];
+    load_window(t, tw);
+    load_window(q, qw);
+    const char* tab = reinterpret_cast<const char*>(lds.table);
+    int sum = 0, key = -1;
 #pragma unroll
-    for (int t = 0; t < kPerLane; ++t) {
-        const int i = k * kPerLane + t;
-        sc[t] = 0;
-        if (i >= first && i < limit) {
-            const uint8_t c = T[idx + (int64_t)dir * i];
-            if (c == '\n') {
-                nl = nl < i ? nl : i;
-            } else {
-                const uint8_t qc = Q[o + dir * i];
-                sc[t] = mat[tmap[c & 127] * 25 + qmap[qc & 127]];
-            }
+    for (int k = 0; k < kWindowDwords; ++k) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // bytes: q(2h), t(2h), q(2h+1), t(2h+1) -> two (t << 8 | 2q) addresses
+            const uint32_t x = __builtin_amdgcn_perm(tw[k], qw[k], h ? 0x07030602u : 0x05010400u);
+            const int p = 4 * k + 2 * h;
+            int w0 = *reinterpret_cast<const int16_t*>(tab + (x & 0xffffu));
+            int w1 = *reinterpret_cast<const int16_t*>(tab + (x >> 16));
+            if (p + 1 == kWindow - 1 && !last_ok) w1 = kTermScore;
+            sum += w0;
+            key = max(key, (int)(((uint32_t)sum << 8) | (uint32_t)p));
+            sum += w1;
+            key = max(key, (int)(((uint32_t)sum << 8) | (uint32_t)(p + 1)));
         }
     }
-    const int cut = group_min(nl);    // the scan stops at the first '\n'
-#pragma unroll
-    for (int t = 0; t < kPerLane; ++t) {
-        const int i = k * kPerLane + t;
-        if (i >= cut) sc[t] = 0;
-        sum += sc[t];
-    }
-    const int before = group_scan_add(sum, k) - sum;   // exclusive prefix of lanes
-    int key = 0;
-    int run = before;
-#pragma unroll
-    for (int t = 0; t < kPerLane; ++t) {
-        const int i = k * kPerLane + t;
-        run += sc[t];
-        if (i >= first && i < limit && i < cut && run >= 0) {
-            const int kk = (run << 8) | i;
-            key = kk > key ? kk : key;
-        }
-    }
-    return group_max(key);
+    return key;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kWaves * 64)
 gapless_kernel(GaplessArgs a) {
-    __shared__ int8_t s_tmap[128], s_qmap[128], s_mat[625];
-    for (int e = threadIdx.x; e < 128; e += blockDim.x) {
-        s_tmap[e] = a.text_map[e];
-        s_qmap[e] = a.query_map[e];
-    }
-    for (int e = threadIdx.x; e < 625; e += blockDim.x) s_mat[e] = a.matrix[e];
+    __shared__ GaplessLds lds;
+    for (int e = threadIdx.x; e < kTableRows * kTableCols; e += blockDim.x) lds.table[e] = a.table[e];
     __syncthreads();
 
-    const int lane = threadIdx.x & 63;
-    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
-    for (int64_t w0 = base; w0 < a.n_seeds; w0 += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    Slot* slots = lds.slot[wave];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + wave * 64; w0 < a.n_seeds; w0 += stride) {
+        // phase 1: one lane per seed
         const int64_t sid = w0 + lane;
         bool pass = false;
         int64_t idx = 0;
@@ -127,49 +118,49 @@ gapless_kernel(GaplessArgs a) {
             qid = s.query;
             o = s.offset;
             idx = a.sa ? a.sa[s.position] : s.position;
-            const int m = a.q_len[qid];
-            // positions outside the text, or query offsets outside the query,
-            // are not seeds (the reference reads out of bounds there)
-            if (idx >= 0 && idx < a.t_size && o >= 0 && o < m) {
-                pass = a.text[idx] == a.q_chars[a.q_off[qid] + o];
-            }
+            // positions outside the text, or query offsets outside the
+            // query, are not seeds (the reference reads out of bounds there)
+            if (idx >= 0 && idx < a.t_size && o >= 0 && o < a.q_len[qid])
+                pass = 2 * (int)a.t_fwd[idx] == (int)a.q_fwd[a.qf_off[qid] + o];
             if (!pass) a.out_length[sid] = -1;
         }
-        uint64_t todo = __ballot(pass);
-        const int g = lane / kGroup, k = lane % kGroup;
-        while (todo) {
-            // the g-th remaining passing lane is served by group g
-            uint64_t rest = todo;
-            int src = -1;
-            for (int j = 0; j < 4 && rest; ++j) {
-                const int b = __builtin_ctzll(rest);
-                if (j == g) src = b;
-                rest &= rest - 1;
-            }
-            const int64_t s_idx = __shfl(idx, src < 0 ? 0 : src);
-            const int s_o = __shfl(o, src < 0 ? 0 : src);
-            const int s_q = __shfl(qid, src < 0 ? 0 : src);
-            if (src >= 0) {
-                const uint8_t* Q = a.q_chars + a.q_off[s_q];
-                const int m = a.q_len[s_q];
-                int64_t rlim = a.t_size - s_idx;
-                if ((int64_t)(m - s_o) < rlim) rlim = m - s_o;
-                if (rlim > kWindow) rlim = kWindow;
-                int64_t llim = s_idx < s_o ? s_idx : s_o;
-                if (llim > kWindow) llim = kWindow;
-                const int kr = extend_side(a.text, s_idx, Q, s_o, 1, 0, (int)rlim, k,
-                                           s_tmap, s_qmap, s_mat);
-                const int kl = extend_side(a.text, s_idx, Q, s_o, -1, 1, (int)llim, k,
-                                           s_tmap, s_qmap, s_mat);
-                if (k == 0) {
-                    const int64_t out = w0 + src;
-                    a.out_score[out] = (kr >> 8) + (kl >> 8);
-                    a.out_length[out] = (kr & 255) + (kl & 255);
-                }
-            }
-            // drop the (up to) four served lanes
-            for (int j = 0; j < 4 && todo; ++j) todo &= todo - 1;
+        const uint64_t mask = __ballot(pass);
+        const int n_pass = __popcll(mask);
+        if (pass) {
+            const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+            slots[rank] = Slot{idx, sid, o, qid};
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // phase 2: two lanes per passing seed, even = right side, odd = left
+        for (int r = 0; r < n_pass; r += 32) {
+            const int j = r + lane / 2;
+            const bool active = j < n_pass;
+            const bool left = lane & 1;
+            int key = -1;
+            Slot s{};
+            if (active) {
+                s = slots[j];
+                const int m = a.q_len[s.query];
+                const uint8_t* t = left ? a.t_rev + (a.t_size - s.idx) : a.t_fwd + s.idx;
+                const uint8_t* q = left ? a.q_rev + a.qr_off[s.query] + (m - s.offset)
+                                        : a.q_fwd + a.qf_off[s.query] + s.offset;
+                key = scan_side(t, q, !left, lds);
+            }
+            const int other = __shfl_xor(key, 1);
+            if (active && !left) {
+                // right: i = position; left: i = position + 1; -1: (0, 0)
+                const int sr = key < 0 ? 0 : key >> 8, lr = key < 0 ? 0 : key & 255;
+                const int sl = other < 0 ? 0 : other >> 8, ll = other < 0 ? 0 : (other & 255) + 1;
+                a.out_score[s.sid] = sr + sl;
+                a.out_length[s.sid] = lr + ll;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -177,6 +168,6 @@ gapless_kernel(GaplessArgs a) {
 
 extern "C" hipError_t tpz_launch_gapless(const tpz::GaplessArgs* args, int grid,
                                          hipStream_t stream) {
-    hipLaunchKernelGGL(tpz::gapless_kernel, dim3(grid), dim3(256), 0, stream, *args);
+    hipLaunchKernelGGL(tpz::gapless_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream, *args);
     return hipGetLastError();
 }

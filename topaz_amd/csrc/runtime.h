@@ -4,6 +4,9 @@
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
 
 namespace tpz {
 
@@ -26,6 +29,32 @@ struct DeviceBuffer {
     int upload(const void* src, size_t n);
 };
 
+// Letter codes of a text for the gapless kernel (see gapless_kernels.h).
+struct TextCodes {
+    uint8_t code[256];        // text byte -> code; 0xff: the byte is not in the text
+    uint8_t byte_of[32];      // code -> byte
+    int n_codes = 0;          // codes 0 .. n_codes-1 (kTextTerm is '\n')
+};
+
+// Query copies of a gapless batch in the kernel's layout: forward and
+// reversed doubled codes with terminators, and their offsets.
+struct QueryCodes {
+    std::vector<uint8_t> fwd, rev;
+    std::vector<int64_t> fwd_off, rev_off;
+};
+int gapless_query_codes(const TextCodes& tc, int32_t n_queries, const char* q_chars,
+                        const int64_t* q_off, const int32_t* q_len, QueryCodes& out);
+// W(text code, query code) for the kernel, kTableRows x kTableCols int16.
+void gapless_table(const TextCodes& tc, const int8_t* matrix, int16_t* table);
+
 }  // namespace tpz
+
+// Device-resident text of the gapless kernel.
+struct tpz_text {
+    uint64_t t_size = 0;
+    bool has_sa = false;
+    tpz::TextCodes codes;
+    tpz::DeviceBuffer d_fwd, d_rev, d_sa;
+};
 
 #endif

@@ -509,43 +509,14 @@ void tpz_sw_batch_destroy(tpz_sw_batch* b) {
 // ---------------------------------------------------------------------------
 // gapless extension
 // ---------------------------------------------------------------------------
-struct tpz_text {
-    uint64_t t_size = 0;
-    bool has_sa = false;
-    DeviceBuffer d_text, d_sa;
-};
+}  // extern "C"
 
-int tpz_text_create(tpz_text** out, const char* text, uint64_t t_size, const int64_t* sa) {
-    if (!out) return fail("tpz_text_create: null output pointer");
-    *out = nullptr;
-    if (ensure_init()) return -1;
-    if (!text && t_size) return fail("tpz_text_create: null text");
-    auto t = new tpz_text();
-    t->t_size = t_size;
-    t->has_sa = sa != nullptr;
-    if (t->d_text.upload(text, t_size + 1) ||
-        (sa && t->d_sa.upload(sa, t_size * 8)) ||
-        check(hipStreamSynchronize(stream()), "text upload")) {
-        delete t;
-        return -1;
-    }
-    *out = t;
-    return 0;
-}
-
-void tpz_text_destroy(tpz_text* t) { delete t; }
-
-struct tpz_gapless_batch {
-    const tpz_text* text = nullptr;
-    int64_t n_seeds = 0;
-    DeviceBuffer d_q, d_q_off, d_q_len, d_seeds, d_tmap, d_qmap, d_matrix, d_score, d_len;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    double ms = 0;
-};
+namespace tpz {
 
 // lexicographical.h:19-33 (aa_table2) and :60-74 (upper_aa_only): the
 // internal alphabet stores amino acid k of "ACMLJIVTSKRQZEDBNHYFWXP*G" as
-// 'A'+k (either case); every other byte maps to matrix row 23.
+// 'A'+k (either case); every other byte maps to matrix row 23.  Query bytes
+// other than 'A'..'Y', '*' and '\n' score as 'V'.
 static void alphabet_maps(int8_t* tmap, int8_t* qmap) {
     static const char internal_order[] = "ACMLJIVTSKRQZEDBNHYFWXP*G";
     static const char matrix_order[] = "ARNDCQEGHILKMFPSTWYVBJZX*";
@@ -561,6 +532,122 @@ static void alphabet_maps(int8_t* tmap, int8_t* qmap) {
     }
 }
 
+// Matrix rows of a text byte and of a query byte (bytes >= 128 use their low
+// seven bits, as the reference's 128-entry tables would need).
+struct AlphabetMaps {
+    int8_t tmap[128], qmap[128];
+    AlphabetMaps() { alphabet_maps(tmap, qmap); }
+};
+static const AlphabetMaps& maps() {
+    static const AlphabetMaps m;
+    return m;
+}
+static int text_row(uint8_t c) { return maps().tmap[c & 127]; }
+static int query_row(uint8_t c) { return maps().qmap[c & 127]; }
+
+static int text_codes(const uint8_t* text, uint64_t n, TextCodes& tc) {
+    bool seen[256] = {false};
+    for (uint64_t i = 0; i < n; ++i) seen[text[i]] = true;
+    memset(tc.code, 0xff, sizeof tc.code);
+    tc.n_codes = 0;
+    for (int c = 0; c < 256; ++c) {
+        if (!seen[c] || c == '\n') continue;
+        if (tc.n_codes >= kTextTerm)
+            return fail("the text holds more than %d distinct letters", kTextTerm);
+        tc.byte_of[tc.n_codes] = (uint8_t)c;
+        tc.code[c] = (uint8_t)tc.n_codes++;
+    }
+    tc.code['\n'] = kTextTerm;
+    tc.byte_of[kTextTerm] = '\n';
+    return 0;
+}
+
+static uint8_t query_code(const TextCodes& tc, uint8_t c) {
+    return tc.code[c] != 0xff ? tc.code[c] : (uint8_t)(kForeignCode + query_row(c));
+}
+
+int gapless_query_codes(const TextCodes& tc, int32_t n_queries, const char* q_chars,
+                        const int64_t* q_off, const int32_t* q_len, QueryCodes& out) {
+    out.fwd.clear(); out.rev.clear();
+    out.fwd_off.assign(n_queries, 0);
+    out.rev_off.assign(n_queries, 0);
+    for (int32_t k = 0; k < n_queries; ++k) {
+        if (q_len[k] < 0) return fail("query %d has negative length", k);
+        const uint8_t* q = reinterpret_cast<const uint8_t*>(q_chars + q_off[k]);
+        const int32_t m = q_len[k];
+        out.fwd_off[k] = (int64_t)out.fwd.size();
+        for (int32_t j = 0; j < m; ++j) out.fwd.push_back((uint8_t)(2 * query_code(tc, q[j])));
+        out.fwd.push_back(2 * kQueryTerm);
+        // reversed: q[m-1] .. q[1], then q[0]'s slot and one more terminator
+        out.rev_off[k] = (int64_t)out.rev.size();
+        for (int32_t j = m - 1; j >= 1; --j) out.rev.push_back((uint8_t)(2 * query_code(tc, q[j])));
+        out.rev.push_back(2 * kQueryTerm);
+        out.rev.push_back(2 * kQueryTerm);
+    }
+    out.fwd.insert(out.fwd.end(), kTextPad, 2 * kQueryTerm);
+    out.rev.insert(out.rev.end(), kTextPad, 2 * kQueryTerm);
+    return 0;
+}
+
+void gapless_table(const TextCodes& tc, const int8_t* matrix, int16_t* table) {
+    for (int t = 0; t < kTableRows; ++t) {
+        for (int q = 0; q < kTableCols; ++q) {
+            int w = kTermScore;
+            if (t < tc.n_codes && q != kQueryTerm) {
+                const int tr = text_row(tc.byte_of[t]);
+                int qr = -1;
+                if (q < tc.n_codes || q == kTextTerm) qr = query_row(tc.byte_of[q]);
+                else if (q >= kForeignCode && q < kForeignCode + 25) qr = q - kForeignCode;
+                if (qr >= 0) w = matrix[tr * 25 + qr];
+            }
+            table[t * kTableCols + q] = (int16_t)w;
+        }
+    }
+}
+
+}  // namespace tpz
+
+extern "C" {
+
+int tpz_text_create(tpz_text** out, const char* text, uint64_t t_size, const int64_t* sa) {
+    if (!out) return fail("tpz_text_create: null output pointer");
+    *out = nullptr;
+    if (ensure_init()) return -1;
+    if (!text && t_size) return fail("tpz_text_create: null text");
+    auto t = new tpz_text();
+    t->t_size = t_size;
+    t->has_sa = sa != nullptr;
+    const uint8_t* T = reinterpret_cast<const uint8_t*>(text);
+    if (text_codes(T, t_size, t->codes)) { delete t; return -1; }
+    // forward and reversed code copies, each followed by terminators; the
+    // reversed copy ends with T[0]'s slot set to the terminator
+    std::vector<uint8_t> buf(t_size + kTextPad);
+    const uint8_t* code = t->codes.code;
+    int rc = 0;
+    for (uint64_t i = 0; i < t_size; ++i) buf[i] = code[T[i]];
+    memset(buf.data() + t_size, kTextTerm, kTextPad);
+    rc |= t->d_fwd.upload(buf.data(), buf.size());
+    rc |= check(hipStreamSynchronize(stream()), "text upload");
+    for (uint64_t j = 0; j + 1 < t_size; ++j) buf[j] = code[T[t_size - 1 - j]];
+    if (t_size) buf[t_size - 1] = kTextTerm;
+    if (!rc) rc |= t->d_rev.upload(buf.data(), buf.size());
+    if (!rc && sa) rc |= t->d_sa.upload(sa, t_size * 8);
+    if (!rc) rc |= check(hipStreamSynchronize(stream()), "text upload");
+    if (rc) { delete t; return -1; }
+    *out = t;
+    return 0;
+}
+
+void tpz_text_destroy(tpz_text* t) { delete t; }
+
+struct tpz_gapless_batch {
+    const tpz_text* text = nullptr;
+    int64_t n_seeds = 0;
+    DeviceBuffer d_qf, d_qf_off, d_qr, d_qr_off, d_q_len, d_seeds, d_table, d_score, d_len;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double ms = 0;
+};
+
 int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
                              int32_t n_queries, const char* q_chars, const int64_t* q_off,
                              const int32_t* q_len, int64_t n_seeds, const tpz_seed* seeds,
@@ -571,16 +658,8 @@ int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
     if (!text) return fail("null text");
     if (n_queries < 0 || n_seeds < 0) return fail("negative counts");
     if (!matrix) return fail("null matrix");
-    std::vector<int64_t> qo(n_queries);
-    int64_t qtot = 0;
-    for (int32_t k = 0; k < n_queries; ++k) {
-        if (q_len[k] < 0) return fail("query %d has negative length", k);
-        qo[k] = qtot;
-        qtot += q_len[k];
-    }
-    std::vector<uint8_t> qc((size_t)qtot + 16);
-    for (int32_t k = 0; k < n_queries; ++k)
-        memcpy(qc.data() + qo[k], q_chars + q_off[k], (size_t)q_len[k]);
+    QueryCodes qc;
+    if (gapless_query_codes(text->codes, n_queries, q_chars, q_off, q_len, qc)) return -1;
     for (int64_t i = 0; i < n_seeds; ++i) {
         if (seeds[i].query < 0 || seeds[i].query >= n_queries)
             return fail("seed %lld names query %d out of range", (long long)i, seeds[i].query);
@@ -588,19 +667,19 @@ int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
             return fail("seed %lld names suffix-array slot %lld outside [0, %llu)", (long long)i,
                         (long long)seeds[i].position, (unsigned long long)text->t_size);
     }
-    int8_t tmap[128], qmap[128];
-    alphabet_maps(tmap, qmap);
+    std::vector<int16_t> table(kTableRows * kTableCols);
+    gapless_table(text->codes, matrix, table.data());
     auto b = new tpz_gapless_batch();
     b->text = text;
     b->n_seeds = n_seeds;
     int rc = 0;
-    rc |= b->d_q.upload(qc.data(), qc.size());
-    rc |= b->d_q_off.upload(qo.data(), qo.size() * 8);
+    rc |= b->d_qf.upload(qc.fwd.data(), qc.fwd.size());
+    rc |= b->d_qr.upload(qc.rev.data(), qc.rev.size());
+    rc |= b->d_qf_off.upload(qc.fwd_off.data(), qc.fwd_off.size() * 8);
+    rc |= b->d_qr_off.upload(qc.rev_off.data(), qc.rev_off.size() * 8);
     rc |= b->d_q_len.upload(q_len, (size_t)n_queries * 4);
     rc |= b->d_seeds.upload(seeds, (size_t)n_seeds * sizeof(tpz_seed));
-    rc |= b->d_tmap.upload(tmap, 128);
-    rc |= b->d_qmap.upload(qmap, 128);
-    rc |= b->d_matrix.upload(matrix, 625);
+    rc |= b->d_table.upload(table.data(), table.size() * 2);
     rc |= b->d_score.alloc((size_t)n_seeds * 4 + 4);
     rc |= b->d_len.alloc((size_t)n_seeds * 4 + 4);
     if (!rc) rc |= check(hipMemsetAsync(b->d_score.ptr, 0, b->d_score.bytes, stream()), "memset");
@@ -614,17 +693,18 @@ int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
 int tpz_gapless_batch_run(tpz_gapless_batch* b) {
     if (!b) return fail("null batch");
     GaplessArgs a{};
-    a.text = (const uint8_t*)b->text->d_text.ptr;
+    a.t_fwd = (const uint8_t*)b->text->d_fwd.ptr;
+    a.t_rev = (const uint8_t*)b->text->d_rev.ptr;
     a.t_size = (int64_t)b->text->t_size;
     a.sa = b->text->has_sa ? (const int64_t*)b->text->d_sa.ptr : nullptr;
-    a.q_chars = (const uint8_t*)b->d_q.ptr;
-    a.q_off = (const int64_t*)b->d_q_off.ptr;
+    a.q_fwd = (const uint8_t*)b->d_qf.ptr;
+    a.qf_off = (const int64_t*)b->d_qf_off.ptr;
+    a.q_rev = (const uint8_t*)b->d_qr.ptr;
+    a.qr_off = (const int64_t*)b->d_qr_off.ptr;
     a.q_len = (const int32_t*)b->d_q_len.ptr;
     a.seeds = (const SeedRec*)b->d_seeds.ptr;
     a.n_seeds = b->n_seeds;
-    a.text_map = (const int8_t*)b->d_tmap.ptr;
-    a.query_map = (const int8_t*)b->d_qmap.ptr;
-    a.matrix = (const int8_t*)b->d_matrix.ptr;
+    a.table = (const int16_t*)b->d_table.ptr;
     a.out_score = (int32_t*)b->d_score.ptr;
     a.out_length = (int32_t*)b->d_len.ptr;
     const int64_t blocks_needed = (b->n_seeds + 255) / 256;
@@ -658,7 +738,6 @@ void tpz_gapless_batch_destroy(tpz_gapless_batch* b) {
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
     delete b;
 }
-
 
 // ---------------------------------------------------------------------------
 // traceback statistics
@@ -807,16 +886,32 @@ int tpz_stats_batch_run(tpz_stats_batch* b) {
     if (check(hipMemcpyAsync(band.data(), b->d_band.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
         check(hipStreamSynchronize(stream()), "band_search"))
         return -1;
-    std::vector<int64_t> dofs(n);
+    // direction buffers: the reference's size, 1024 bytes grown to the next
+    // power of two above each pass's (2w+1) * readLen * 3 (ssw.c:555, 591-597)
+    std::vector<int64_t> dofs(n), dsize(n);
     int64_t dtot = 0;
     for (int32_t h = 0; h < n; ++h) {
         dofs[h] = dtot;
-        if (band[h] >= 0) dtot += (int64_t)(2 * band[h] + 1) * (b->read_end[h] - qb[h] + 1) * 3;
+        int64_t s2 = 1024;
+        if (band[h] >= 0) {
+            const int64_t ncell = (int64_t)(2 * band[h] + 1) * (b->read_end[h] - qb[h] + 1) * 3;
+            while (ncell >= s2) {
+                int64_t v = s2;           // ++s2 then round up to a power of two
+                v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+                s2 = v + 1;
+            }
+        }
+        dsize[h] = band[h] >= 0 ? s2 : 0;
+        dtot += dsize[h];
     }
-    DeviceBuffer d_dirs, d_doff;
-    if (d_dirs.alloc((size_t)dtot + 16) || d_doff.upload(dofs.data(), dofs.size() * 8)) return -1;
+    DeviceBuffer d_dirs, d_doff, d_dsize;
+    if (d_dirs.alloc((size_t)dtot + 16) || d_doff.upload(dofs.data(), dofs.size() * 8) ||
+        d_dsize.upload(dsize.data(), dsize.size() * 8) ||
+        check(hipMemsetAsync(d_dirs.ptr, 0, (size_t)dtot + 16, stream()), "memset"))
+        return -1;
     a.dirs = (int8_t*)d_dirs.ptr;
     a.dir_off = (const int64_t*)d_doff.ptr;
+    a.dir_size = (const int64_t*)d_dsize.ptr;
     a.out_rc = (int32_t*)b->d_rc.ptr;
     a.out_length = (int32_t*)b->d_len.ptr;
     a.out_gapopen = (int32_t*)b->d_gapo.ptr;

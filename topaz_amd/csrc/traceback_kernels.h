@@ -41,12 +41,13 @@ struct BandArgs {
     int32_t n_pairs;
     const int8_t* matrix;
     int32_t gap_open, gap_extend;
-    int32_t* workspace;            // three band rows per hit
+    int32_t* workspace;            // three band rows (H, E, H of the row) per hit
     const int64_t* ws_off;
     int32_t ws_band_words;         // words per band row available per hit
     int32_t* out_band;             // final band half-width, -1: size guard
-    int8_t* dirs;                  // direction matrices
+    int8_t* dirs;                  // direction buffers, zeroed
     const int64_t* dir_off;
+    const int64_t* dir_size;       // bytes of each hit's buffer
     int32_t* out_rc, *out_length, *out_gapopen, *out_mismatch, *out_identical;
 };
 

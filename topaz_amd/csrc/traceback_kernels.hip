@@ -17,9 +17,9 @@
 // real row there equal to the max.
 //
 // banded_sw: band half-width w starts at |refLen - readLen| + 1 and doubles
-// until the band's best cell reaches score1 (search kernel); the final band
-// is then recomputed with its direction matrix and traced back from the
-// bottom-right corner while the row is > 0 (trace kernel).
+// until the band's best cell reaches score1 (search kernel); the passes are
+// then replayed with their direction cells and the final band traced back
+// from the bottom-right corner while the row is > 0 (trace kernel).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -85,13 +85,18 @@ __device__ __forceinline__ int band_dir(int w, int i, int j, int d) {
     return (j - imax(i - w, 0)) * 3 + d;
 }
 
-// One band DP.  When dirs is null only the best cell is tracked.
-__device__ int band_dp(const int8_t* ref, const int8_t* read, int refLen, int readLen, int w,
-                       const int8_t* mat, int go, int ge, int32_t* hb, int32_t* eb,
-                       int32_t* hc, int8_t* dirs) {
+// One pass of the band DP with half-width w; returns the pass's best cell.
+// As in the reference (ssw.c:559-660) the three band rows live across all
+// passes: H is cleared on slots 1..width-2 at the start of a pass, E never,
+// and slots a row does not write keep what an earlier row or pass left.
+// When dirs is set the pass writes its direction cells there, row i at
+// byte (2w+1) * 3 * i.
+__device__ int band_pass(const int8_t* ref, const int8_t* read, int refLen, int readLen, int w,
+                         const int8_t* mat, int go, int ge, int32_t* hb, int32_t* eb,
+                         int32_t* hc, int8_t* dirs) {
     const int width = 2 * w + 3, width_d = 2 * w + 1;
     int best = 0;
-    for (int j = 0; j <= width; ++j) { hb[j] = 0; eb[j] = 0; hc[j] = 0; }
+    for (int j = 1; j < width - 1; ++j) hb[j] = 0;
     for (int i = 0; i < readLen; ++i) {
         const int beg = imax(0, i - w);
         const int end = imin(refLen - 1, i + w);
@@ -128,63 +133,82 @@ __device__ int band_dp(const int8_t* ref, const int8_t* read, int refLen, int re
     return best;
 }
 
+struct BandView {
+    const int8_t* ref;
+    const int8_t* read;
+    int refLen, readLen, w0;
+    int32_t *hb, *eb, *hc;
+};
+
+__device__ __forceinline__ BandView band_view(const BandArgs& a, int p) {
+    BandView v;
+    const int rb = a.ref_begin[p], qb = a.read_begin[p];
+    v.refLen = a.ref_end[p] - rb + 1;
+    v.readLen = a.read_end[p] - qb + 1;
+    v.ref = a.s_res + a.s_off[a.pair_subject[p]] + rb;
+    v.read = a.q_res + a.q_off[a.pair_query[p]] + qb;
+    v.w0 = v.refLen > v.readLen ? v.refLen - v.readLen + 1 : v.readLen - v.refLen + 1;
+    v.hb = a.workspace + (size_t)a.ws_off[p];
+    v.eb = v.hb + a.ws_band_words;
+    v.hc = v.eb + a.ws_band_words;
+    // the reference's band rows start as fresh allocations: zero here
+    for (int j = 0; j < a.ws_band_words; ++j) { v.hb[j] = 0; v.eb[j] = 0; v.hc[j] = 0; }
+    return v;
+}
+
 __global__ void __launch_bounds__(64)
 banded_search_kernel(BandArgs a) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n_pairs) return;
-    const int rb = a.ref_begin[p], qb = a.read_begin[p];
-    const int refLen = a.ref_end[p] - rb + 1, readLen = a.read_end[p] - qb + 1;
-    const int8_t* ref = a.s_res + a.s_off[a.pair_subject[p]] + rb;
-    const int8_t* read = a.q_res + a.q_off[a.pair_query[p]] + qb;
-    int32_t* hb = a.workspace + (size_t)a.ws_off[p];
-    int w = refLen > readLen ? refLen - readLen + 1 : readLen - refLen + 1;
-    int best = 0;
-    int status = 0;
+    const BandView v = band_view(a, p);
+    int w = v.w0, best = 0, status = 0;
     for (;;) {
-        const int64_t cells = (int64_t)(2 * w + 1) * readLen * 3;
-        // the reference's direction matrix stops growing at 2^30 bytes
+        const int64_t cells = (int64_t)(2 * w + 1) * v.readLen * 3;
+        // the reference's direction buffer stops growing at 2^31 bytes
         if (cells >= ((int64_t)1 << 30) || 2 * w + 4 > a.ws_band_words) { status = -1; break; }
-        const int width = 2 * w + 3;
-        best = imax(best, band_dp(ref, read, refLen, readLen, w, a.matrix, a.gap_open,
-                                  a.gap_extend, hb, hb + width + 1, hb + 2 * (width + 1),
-                                  nullptr));
+        best = imax(best, band_pass(v.ref, v.read, v.refLen, v.readLen, w, a.matrix, a.gap_open,
+                                    a.gap_extend, v.hb, v.eb, v.hc, nullptr));
         if (best >= a.score[p]) break;
         // once the band covers the whole rectangle, wider bands give the same
         // best cell and the reference doubles until its size check fails
-        if (w >= refLen + readLen) { status = -1; break; }
+        if (w >= v.refLen + v.readLen) { status = -1; break; }
         w *= 2;
     }
     a.out_band[p] = status < 0 ? -1 : w;
 }
 
+// Replays every pass up to the final band into the hit's zeroed direction
+// buffer (dir_size[p] bytes, the reference's power-of-two size), then
+// traces back as ssw.c:663-739 does, indexing the buffer flat: a step that
+// leaves the final band's row reads what the neighbouring row or an earlier
+// pass wrote there.  A read outside the buffer ends as the reference's
+// "trace back error".
 __global__ void __launch_bounds__(64)
 banded_trace_kernel(BandArgs a) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n_pairs) return;
-    const int w = a.out_band[p];
-    if (w < 0) {
+    const int wf = a.out_band[p];
+    if (wf < 0) {
         a.out_rc[p] = 0; a.out_length[p] = -1; a.out_gapopen[p] = -1;
         a.out_mismatch[p] = -1; a.out_identical[p] = -1;
         return;
     }
-    const int rb = a.ref_begin[p], qb = a.read_begin[p];
-    const int refLen = a.ref_end[p] - rb + 1, readLen = a.read_end[p] - qb + 1;
-    const int8_t* ref = a.s_res + a.s_off[a.pair_subject[p]] + rb;
-    const int8_t* read = a.q_res + a.q_off[a.pair_query[p]] + qb;
-    int32_t* hb = a.workspace + (size_t)a.ws_off[p];
+    const BandView v = band_view(a, p);
     int8_t* dirs = a.dirs + a.dir_off[p];
-    const int width = 2 * w + 3, width_d = 2 * w + 1;
-    band_dp(ref, read, refLen, readLen, w, a.matrix, a.gap_open, a.gap_extend, hb,
-            hb + width + 1, hb + 2 * (width + 1), dirs);
-    int i = readLen - 1, j = refLen - 1, state = 2;
+    const int64_t size = a.dir_size[p];
+    for (int w = v.w0;; w *= 2) {
+        band_pass(v.ref, v.read, v.refLen, v.readLen, w, a.matrix, a.gap_open, a.gap_extend,
+                  v.hb, v.eb, v.hc, dirs);
+        if (w >= wf) break;
+    }
+    const int width_d = 2 * wf + 1;
+    int i = v.readLen - 1, j = v.refLen - 1, state = 2;
     int length = 0, gapopen = 0, mismatch = 0, identical = 0, rc = 1;
     char prev = 'M';
-    const int8_t* line = dirs + (size_t)width_d * (readLen > 0 ? readLen - 1 : 0) * 3;
-    const int8_t* lo = dirs;
-    const int8_t* hi = dirs + (size_t)width_d * readLen * 3;
+    int64_t line = (int64_t)width_d * 3 * (v.readLen > 0 ? v.readLen - 1 : 0);
     while (i > 0) {
-        const int c = band_dir(w, i, j, state);
-        const int8_t code = (c >= 0 && c < width_d * 3 && line >= lo && line + c < hi) ? line[c] : 0;
+        const int64_t c = line + band_dir(wf, i, j, state);
+        const int8_t code = (c >= 0 && c < size) ? dirs[c] : 0;
         char op;
         if (code == 1) { --i; --j; state = 2; line -= width_d * 3; op = 'M'; }
         else if (code == 2) { --i; state = 0; line -= width_d * 3; op = 'I'; }
@@ -193,7 +217,7 @@ banded_trace_kernel(BandArgs a) {
         else if (code == 5) { --j; state = 2; op = 'D'; }
         else { rc = 0; break; }
         if (op == 'M') {
-            if (read[i] != ref[j]) ++mismatch;
+            if (v.read[i] != v.ref[j]) ++mismatch;
             else ++identical;
         }
         if ((op == 'I' && prev != 'I') || (op == 'D' && prev != 'D')) ++gapopen;
