@@ -228,6 +228,16 @@ def sw_leg(args):
 
     b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], matrix(), 11, 1)
     print("[bench] uploaded", file=sys.stderr, flush=True)
+    tuning = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        cfg, tuned_ms, n_tried = b.autotune()
+        tuning = {"config": cfg.as_dict(), "candidates": n_tried,
+                  "best_kernel_ms": round(tuned_ms, 3),
+                  "seconds": round(time.perf_counter() - t0, 1)}
+        print("[bench] autotune: %s (%d candidates, %.1f s)" % (cfg.as_dict(), n_tried,
+              tuning["seconds"]), file=sys.stderr, flush=True)
+    b.set_config(use_graph=1)
     for _ in range(args.warmup):
         b.run(SCHED_PERSISTENT)
     ref = b.results() if args.warmup else None
@@ -249,6 +259,7 @@ def sw_leg(args):
 
     plain = None
     if not args.no_plain:
+        b.set_config(use_graph=0)
         b.run(SCHED_PLAIN)
         t0 = time.perf_counter()
         b.run(SCHED_PLAIN)
@@ -280,7 +291,9 @@ def sw_leg(args):
                                "planted homologs at 25-95%% identity), BLOSUM62 11/1"
                                % (n_pairs, n_queries),
                    "pairs": n_pairs, "queries": n_queries, "cells": cells,
-                   "schedule": "persistent, length-binned"},
+                   "schedule": "persistent work queue, length-binned tasks of up to 4 "
+                               "queries, autotuned, replayed from a hipGraph"},
+        "tuning": tuning,
         "queries_per_s": round(n_queries / t_step, 2),
         "plain": plain,
         "speedup_vs_plain": round(plain["ms_per_step"] / (t_step * 1e3), 2) if plain else None,
@@ -321,6 +334,7 @@ def main():
                     help="fraction of the 2,000-query workload (tests only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
     ap.add_argument("--leg", choices=["all", "sw", "gapless", "copy"], default="all")
     ap.add_argument("--gapless-scale", type=float, default=1.0,
                     help="fraction of the 1e9-residue, 5e7-seed gapless workload")

@@ -72,6 +72,33 @@ int tpz_sw_batch_results(const tpz_sw_batch* b, int32_t* score,
  * the library stream). */
 int tpz_sw_batch_kernel_ms(const tpz_sw_batch* b, double* ms);
 
+/* Launch configuration of a batch.  No reference counterpart: these are the
+ * tile shape and schedule knobs of the GPU kernel.  Results never depend on
+ * them (bit-identical for every configuration). */
+typedef struct {
+    int32_t rows_per_chunk;    /* query rows a wavefront holds in registers: 16 or 32  */
+    int32_t groups_per_cu;     /* resident 4-wavefront workgroups per CU (persistent):
+                                  1..2 for 32 rows, 1..3 for 16 rows                   */
+    int32_t queries_per_task;  /* queries whose pairs may share a wavefront: 1, 2, 4   */
+    int32_t use_graph;         /* nonzero: replay the launches from a captured hipGraph */
+    double big_fraction;       /* a task costing more than this fraction of one
+                                  wavefront's share of the batch is split over the 4
+                                  wavefronts of a workgroup (chunk staircase); 0: all
+                                  tasks                                                 */
+} tpz_sw_config;
+
+/* The configuration the batch runs with (defaults: 32, 2, 4, 0, 0.5). */
+int tpz_sw_batch_get_config(const tpz_sw_batch* b, tpz_sw_config* cfg);
+/* Re-plan the batch for cfg: rebuilds tasks, work items and workspace. */
+int tpz_sw_batch_set_config(tpz_sw_batch* b, const tpz_sw_config* cfg);
+/* Time the persistent schedule under each candidate configuration on this
+ * batch (one warm-up run, then the faster of two timed runs each), check
+ * that every candidate gives bit-identical results, and keep the fastest.
+ * use_graph of the kept configuration is what the batch had.  best / best_ms
+ * may be null; n_tried receives the number of candidates timed. */
+int tpz_sw_batch_autotune(tpz_sw_batch* b, tpz_sw_config* best, double* best_ms,
+                          int32_t* n_tried);
+
 void tpz_sw_batch_destroy(tpz_sw_batch* b);
 
 /* ---- batched traceback statistics of reported hits ---------------------- */

@@ -25,10 +25,12 @@ def oracle_results(o, w, mat, go, ge):
     return out
 
 
-def compare(w, mat=None, go=11, ge=1, schedule=SCHED_PERSISTENT):
+def compare(w, mat=None, go=11, ge=1, schedule=SCHED_PERSISTENT, config=None):
     mat = matrix() if mat is None else mat
     o = Oracle()
     b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], mat, go, ge)
+    if config:
+        b.set_config(**config)
     sc, re_, rd = b.run(schedule).results()
     exp = oracle_results(o, w, mat, go, ge)
     bad = []
@@ -109,14 +111,13 @@ def test_empty_batch_and_empty_subjects():
     b.close()
 
 
-@pytest.mark.parametrize("big_fraction", ["0", "1e30"])
-def test_group_and_wave_paths(big_fraction, monkeypatch):
+@pytest.mark.parametrize("big_fraction", [0.0, 1e30])
+def test_group_and_wave_paths(big_fraction):
     """All tasks worked by whole workgroups (chunks interleaved over four
     wavefronts), then all tasks by single wavefronts: same results."""
-    monkeypatch.setenv("TOPAZ_AMD_BIG_FRACTION", big_fraction)
     rng = np.random.default_rng(7)
     w = synth.pair_workload(rng, 6, 150, mean=400, sd=350, lo=1, hi=2500)
-    compare(w)
+    compare(w, config=dict(big_fraction=big_fraction))
 
 
 @pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
@@ -139,3 +140,59 @@ def test_packed_tasks_mix_queries(schedule):
     from topaz_amd.sw import _concat
     w = dict(q=_concat(qs), s=_concat(ss), pair_query=pq, pair_subject=ps)
     compare(w, schedule=schedule)
+
+
+CONFIGS = [dict(rows_per_chunk=16, groups_per_cu=3, queries_per_task=4),
+           dict(rows_per_chunk=16, groups_per_cu=1, queries_per_task=1, big_fraction=1e30),
+           dict(rows_per_chunk=32, groups_per_cu=1, queries_per_task=2, big_fraction=0.01),
+           dict(rows_per_chunk=16, groups_per_cu=2, queries_per_task=2, big_fraction=0.01)]
+
+
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_configurations_match_oracle(cfg):
+    """Every tile shape / schedule knob gives the reference's results."""
+    w = small_workload(8, nq=20, per_q=50, mean=200, sd=200, hi=900)
+    mat = matrix()
+    b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], mat, 11, 1)
+    b.set_config(**cfg)
+    got = {s: b.run(s).results() for s in (SCHED_PERSISTENT, SCHED_PLAIN)}
+    b.close()
+    exp = oracle_results(Oracle(), w, mat, 11, 1)
+    for sched, (sc, re_, rd) in got.items():
+        bad = [p for p, (es, er, ed) in enumerate(exp)
+               if not (sc[p] == OVERFLOW if es == OVERFLOW else (sc[p], re_[p], rd[p]) == (es, er, ed))]
+        assert not bad, (cfg, sched, bad[:5])
+
+
+def test_graph_replays_match_eager():
+    """Captured launches (counter cleared by a kernel, no memset node) give
+    the eager results on every replay, with eager work in between."""
+    w = small_workload(9, nq=16, per_q=60, mean=180, sd=150, hi=800)
+    b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], matrix(), 11, 1)
+    eager = {s: b.run(s).results() for s in (SCHED_PERSISTENT, SCHED_PLAIN)}
+    b.set_config(use_graph=1)
+    other = SwBatch(w["q"], w["s"], w["pair_query"][::-1].copy(), w["pair_subject"][::-1].copy(),
+                    matrix(), 11, 1)
+    for rep in range(4):
+        for s in (SCHED_PERSISTENT, SCHED_PLAIN):
+            got = b.run(s).results()
+            for x, y in zip(got, eager[s]):
+                assert np.array_equal(x, y), (rep, s)
+            other.run(SCHED_PERSISTENT)          # eager work between replays
+    assert b.config().use_graph == 1
+    b.close()
+    other.close()
+
+
+def test_autotune_keeps_results():
+    w = small_workload(10, nq=12, per_q=64, mean=200, sd=150, hi=700)
+    b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], matrix(), 11, 1)
+    before = b.run(SCHED_PERSISTENT).results()
+    cfg, ms, n = b.autotune()
+    assert n >= 10 and ms > 0
+    assert cfg.rows_per_chunk in (16, 32) and 1 <= cfg.queries_per_task <= 4
+    assert b.config().as_dict() == cfg.as_dict()
+    after = b.run(SCHED_PERSISTENT).results()
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    b.close()

@@ -50,6 +50,13 @@ def _declare(L):
     L.tpz_sw_batch_kernel_ms.restype = C.c_int
     L.tpz_sw_batch_destroy.argtypes = [VP]
     L.tpz_sw_batch_destroy.restype = None
+    from .sw import SwConfig
+    L.tpz_sw_batch_get_config.argtypes = [VP, C.POINTER(SwConfig)]
+    L.tpz_sw_batch_get_config.restype = C.c_int
+    L.tpz_sw_batch_set_config.argtypes = [VP, C.POINTER(SwConfig)]
+    L.tpz_sw_batch_set_config.restype = C.c_int
+    L.tpz_sw_batch_autotune.argtypes = [VP, C.POINTER(SwConfig), PD, C.POINTER(C.c_int32)]
+    L.tpz_sw_batch_autotune.restype = C.c_int
 
 
 def check(rc, what="topaz_amd"):

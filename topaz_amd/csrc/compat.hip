@@ -148,7 +148,7 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.out_score = dout;
     a.out_ref = dout + 1;
     a.out_read = dout + 2;
-    hip_or_die(tpz_launch_sw_forward(&a, 1, c.st), "sw_forward");
+    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, c.st), "sw_forward");
     hip_or_die(hipMemcpyAsync(out, dout, 12, hipMemcpyDeviceToHost, c.st), "readback");
     hip_or_die(hipStreamSynchronize(c.st), "sw_forward");
 }

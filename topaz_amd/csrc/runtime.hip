@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -78,8 +79,7 @@ int DeviceBuffer::upload(const void* src, size_t n) {
 // batched forward Smith-Waterman
 // ---------------------------------------------------------------------------
 constexpr int kTaskPairs = 128;     // two pairs per lane of a wavefront
-constexpr int kTaskQueries = 4;     // queries whose profiles a wavefront holds
-constexpr int kChunkRows = 32;      // query rows per chunk of the kernel
+constexpr int kTaskQueries = 4;     // queries whose profiles a wavefront can hold
 
 struct Tasks {
     std::vector<int32_t> index;   // pair numbers, grouped per task
@@ -95,8 +95,8 @@ struct Tasks {
     int32_t count() const { return (int32_t)cost.size(); }
 };
 
-static int64_t chunk_rows(int32_t m) {
-    return (int64_t)(m + kChunkRows - 1) / kChunkRows * kChunkRows;
+static int64_t chunk_rows(int32_t m, int rows) {
+    return (int64_t)(m + rows - 1) / rows * rows;
 }
 
 // Pairs of each query, longest subject first (ties: lower pair number).
@@ -119,20 +119,20 @@ static std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_
 static void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
                               const std::vector<int32_t>& pair_subject,
                               const std::vector<int32_t>& q_len,
-                              const std::vector<int32_t>& s_len, Tasks& t) {
+                              const std::vector<int32_t>& s_len, int rows, Tasks& t) {
     t.clear();
     for (int32_t q = 0; q < (int32_t)per.size(); ++q) {
         const auto& v = per[q];
         for (size_t i = 0; i < v.size(); i += kTaskPairs) {
-            t.open(&q, 1, chunk_rows(q_len[q]) * s_len[pair_subject[v[i]]]);
+            t.open(&q, 1, chunk_rows(q_len[q], rows) * s_len[pair_subject[v[i]]]);
             for (size_t k = i; k < v.size() && k < i + kTaskPairs; ++k) t.index.push_back(v[k]);
         }
     }
     t.begin.push_back((int32_t)t.index.size());
 }
 
-// Tasks of up to four queries (the persistent schedule).  Queries with the
-// same number of 32-row chunks are packed together: repeatedly the four
+// Tasks of up to nq (<= 4) queries (the persistent schedule).  Queries with
+// the same number of row chunks are packed together: repeatedly the nq
 // queries whose longest remaining subjects are longest give their longest
 // remaining pairs, longest first, until the task holds 128.  A query's long
 // subjects then share a wavefront with other queries' long subjects instead
@@ -141,14 +141,15 @@ static void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
 static void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
                                const std::vector<int32_t>& pair_subject,
                                const std::vector<int32_t>& q_len,
-                               const std::vector<int32_t>& s_len, Tasks& t) {
+                               const std::vector<int32_t>& s_len, int rows, int nq_max,
+                               Tasks& t) {
     t.clear();
     const int32_t nq = (int32_t)per.size();
     std::vector<int32_t> order;
     for (int32_t q = 0; q < nq; ++q)
         if (!per[q].empty()) order.push_back(q);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        return chunk_rows(q_len[x]) < chunk_rows(q_len[y]);
+        return chunk_rows(q_len[x], rows) < chunk_rows(q_len[y], rows);
     });
     std::vector<size_t> head(nq, 0);
     auto head_len = [&](int32_t q) { return s_len[pair_subject[per[q][head[q]]]]; };
@@ -159,20 +160,20 @@ static void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
     };
     size_t g0 = 0;
     while (g0 < order.size()) {
-        const int64_t rows = chunk_rows(q_len[order[g0]]);
+        const int64_t task_rows = chunk_rows(q_len[order[g0]], rows);
         size_t g1 = g0;
-        while (g1 < order.size() && chunk_rows(q_len[order[g1]]) == rows) ++g1;
+        while (g1 < order.size() && chunk_rows(q_len[order[g1]], rows) == task_rows) ++g1;
         std::vector<int32_t> heap(order.begin() + g0, order.begin() + g1);
         std::make_heap(heap.begin(), heap.end(), below);
         while (!heap.empty()) {
             int32_t sel[kTaskQueries];
             int ns = 0;
-            while (ns < kTaskQueries && !heap.empty()) {
+            while (ns < nq_max && !heap.empty()) {
                 std::pop_heap(heap.begin(), heap.end(), below);
                 sel[ns++] = heap.back();
                 heap.pop_back();
             }
-            t.open(sel, ns, rows * head_len(sel[0]));
+            t.open(sel, ns, task_rows * head_len(sel[0]));
             for (int taken = 0; taken < kTaskPairs; ++taken) {
                 int best = -1;
                 for (int k = 0; k < ns; ++k) {
@@ -210,25 +211,44 @@ struct DeviceTasks {
 
 using namespace tpz;
 
-struct tpz_sw_batch {
-    int32_t n_queries = 0, n_subjects = 0, n_pairs = 0;
-    int32_t gap_open = 11, gap_extend = 1, bias = 0;
-    int32_t max_subject = 0;
-    std::vector<int32_t> q_len, s_len, pair_query, pair_subject;
-    DeviceBuffer d_q_res, d_q_off, d_q_len, d_s_res, d_s_off, d_s_len;
-    DeviceBuffer d_pair_query, d_pair_subject, d_matrix;
-    DeviceBuffer d_score, d_ref, d_read;            // results
-    DeviceBuffer d_workspace, d_counter, d_error;
+// Everything that depends on the batch's configuration: tasks, work items,
+// workspace and the captured launch graphs.
+struct SwPlan {
+    tpz_sw_config cfg{};
     DeviceTasks plain_tasks;                        // one query per task
-    DeviceTasks packed_tasks;                       // up to four queries per task
+    DeviceTasks packed_tasks;                       // up to queries_per_task per task
     // work items (4 task slots each): per query for the plain schedule, by
     // decreasing cost for the persistent one
     std::vector<int32_t> items_plain, items_persistent, plain_query_items;
     DeviceBuffer d_items_plain, d_items_persistent;
     int32_t big_tasks = 0;
-    double big_fraction = 0.5;
     size_t ws_slots = 0;
     int64_t ws_words_per_slot = 0;
+    DeviceBuffer d_workspace;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    SwPlan() = default;
+    SwPlan(const SwPlan&) = delete;
+    SwPlan& operator=(const SwPlan&) = delete;
+    ~SwPlan() {
+        for (int k = 0; k < 2; ++k) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        }
+    }
+};
+
+struct tpz_sw_batch {
+    int32_t n_queries = 0, n_subjects = 0, n_pairs = 0;
+    int32_t gap_open = 11, gap_extend = 1, bias = 0;
+    int32_t max_subject = 0;
+    std::vector<int32_t> q_len, s_len, pair_query, pair_subject;
+    std::vector<std::vector<int32_t>> per_query;    // pairs of each query, longest subject first
+    DeviceBuffer d_q_res, d_q_off, d_q_len, d_s_res, d_s_off, d_s_len;
+    DeviceBuffer d_pair_query, d_pair_subject, d_matrix;
+    DeviceBuffer d_score, d_ref, d_read;            // results
+    DeviceBuffer d_counter, d_error;
+    std::unique_ptr<SwPlan> plan;
     hipEvent_t ev[2] = {nullptr, nullptr};
     double ms = 0;
 };
@@ -267,6 +287,76 @@ static void build_items(const Tasks& t, int n_slots, double big_fraction,
                      [](const auto& x, const auto& y) { return x.first > y.first; });
     items.clear();
     for (auto& e : list) items.insert(items.end(), e.second.begin(), e.second.end());
+}
+
+static tpz_sw_config default_config() {
+    tpz_sw_config c{};
+    c.rows_per_chunk = 32;
+    c.groups_per_cu = kGroupsPerCu;
+    c.queries_per_task = kTaskQueries;
+    c.use_graph = 0;
+    c.big_fraction = 0.5;
+    return c;
+}
+
+static int check_config(const tpz_sw_config& c) {
+    if (c.rows_per_chunk != 16 && c.rows_per_chunk != 32)
+        return fail("rows_per_chunk must be 16 or 32 (got %d)", c.rows_per_chunk);
+    const int max_groups = c.rows_per_chunk == 16 ? 3 : 2;    // VGPR and LDS occupancy
+    if (c.groups_per_cu < 1 || c.groups_per_cu > max_groups)
+        return fail("groups_per_cu must be 1..%d for %d rows (got %d)", max_groups,
+                    c.rows_per_chunk, c.groups_per_cu);
+    if (c.queries_per_task != 1 && c.queries_per_task != 2 && c.queries_per_task != 4)
+        return fail("queries_per_task must be 1, 2 or 4 (got %d)", c.queries_per_task);
+    if (!(c.big_fraction >= 0)) return fail("big_fraction must be >= 0");
+    return 0;
+}
+
+// Tasks, work items and workspace of the batch for configuration c.
+static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
+    if (check_config(c)) return -1;
+    auto plan = std::make_unique<SwPlan>();
+    plan->cfg = c;
+    build_query_tasks(b->per_query, b->pair_subject, b->q_len, b->s_len, c.rows_per_chunk,
+                      plan->plain_tasks.host);
+    build_packed_tasks(b->per_query, b->pair_subject, b->q_len, b->s_len, c.rows_per_chunk,
+                       c.queries_per_task, plan->packed_tasks.host);
+    int rc = plan->plain_tasks.upload() | plan->packed_tasks.upload();
+    // plain: per query, its tasks four to an item
+    size_t max_items_per_query = 0;
+    {
+        const Tasks& t = plan->plain_tasks.host;
+        const int n_tasks = t.count();
+        int k = 0;
+        while (k < n_tasks) {
+            int e = k;
+            while (e < n_tasks && t.query[kTaskQueries * e] == t.query[kTaskQueries * k]) ++e;
+            plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / 4));
+            for (int j = k; j < e; j += 4)
+                for (int w = 0; w < 4; ++w) plan->items_plain.push_back(j + w < e ? j + w : -1);
+            max_items_per_query = std::max<size_t>(max_items_per_query, (size_t)(e - k + 3) / 4);
+            k = e;
+        }
+        plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / 4));
+    }
+    const size_t persistent_slots = (size_t)std::max(num_cu(), 1) * c.groups_per_cu;
+    build_items(plan->packed_tasks.host, (int)persistent_slots, c.big_fraction,
+                plan->items_persistent, plan->big_tasks);
+    rc |= plan->d_items_plain.upload(plan->items_plain.data(), plan->items_plain.size() * 4);
+    rc |= plan->d_items_persistent.upload(plan->items_persistent.data(),
+                                          plan->items_persistent.size() * 4);
+    // workspace: per workgroup, one boundary buffer (bottom H and F of a
+    // chunk: 2 words per column and lane) per wavefront
+    plan->ws_words_per_slot = (int64_t)kGroupWaves * 2 * (b->max_subject + 2) * 64;
+    const size_t n_items_p = plan->items_persistent.size() / 4;
+    plan->ws_slots = std::max<size_t>(std::max(std::min(persistent_slots, n_items_p),
+                                               max_items_per_query), 1);
+    rc |= plan->d_workspace.alloc(plan->ws_slots * (size_t)plan->ws_words_per_slot * 4);
+    if (rc) return -1;
+    // the old plan's buffers may still be read by queued work
+    if (check(hipStreamSynchronize(stream()), "plan")) return -1;
+    b->plan = std::move(plan);
+    return 0;
 }
 }  // namespace tpz
 
@@ -373,49 +463,12 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
 
     rc |= b->d_error.alloc(64);
     if (!rc) rc |= check(hipMemsetAsync(b->d_error.ptr, 0, 64, stream()), "memset");
-
-    {
-        const auto per = pairs_by_query(b->pair_query, b->pair_subject, b->s_len, n_queries);
-        build_query_tasks(per, b->pair_subject, b->q_len, b->s_len, b->plain_tasks.host);
-        build_packed_tasks(per, b->pair_subject, b->q_len, b->s_len, b->packed_tasks.host);
-    }
-    rc |= b->plain_tasks.upload();
-    rc |= b->packed_tasks.upload();
-
-    // plain: per query, its tasks four to an item
-    size_t max_items_per_query = 0;
-    {
-        const Tasks& t = b->plain_tasks.host;
-        const int n_tasks = t.count();
-        int k = 0;
-        while (k < n_tasks) {
-            int e = k;
-            while (e < n_tasks && t.query[kTaskQueries * e] == t.query[kTaskQueries * k]) ++e;
-            b->plain_query_items.push_back((int32_t)(b->items_plain.size() / 4));
-            for (int j = k; j < e; j += 4)
-                for (int w = 0; w < 4; ++w) b->items_plain.push_back(j + w < e ? j + w : -1);
-            max_items_per_query = std::max<size_t>(max_items_per_query, (size_t)(e - k + 3) / 4);
-            k = e;
-        }
-        b->plain_query_items.push_back((int32_t)(b->items_plain.size() / 4));
-    }
-    const size_t persistent_slots = (size_t)std::max(num_cu(), 1) * kGroupsPerCu;
-    if (const char* env = getenv("TOPAZ_AMD_BIG_FRACTION")) b->big_fraction = atof(env);
-    build_items(b->packed_tasks.host, (int)persistent_slots, b->big_fraction, b->items_persistent,
-                b->big_tasks);
-    rc |= b->d_items_plain.upload(b->items_plain.data(), b->items_plain.size() * 4);
-    rc |= b->d_items_persistent.upload(b->items_persistent.data(), b->items_persistent.size() * 4);
-
-    // workspace: per workgroup, one boundary buffer (bottom H and F of a
-    // chunk: 2 words per column and lane) per wavefront
-    b->ws_words_per_slot = (int64_t)kGroupWaves * 2 * (b->max_subject + 2) * 64;
-    const size_t n_items_p = b->items_persistent.size() / 4;
-    b->ws_slots = std::max<size_t>(std::max(std::min(persistent_slots, n_items_p),
-                                            max_items_per_query), 1);
-    rc |= b->d_workspace.alloc(b->ws_slots * (size_t)b->ws_words_per_slot * 4);
+    b->per_query = pairs_by_query(b->pair_query, b->pair_subject, b->s_len, n_queries);
     for (auto& e : b->ev) rc |= check(hipEventCreate(&e), "hipEventCreate");
+    tpz_sw_config cfg = default_config();
+    if (!rc) rc |= build_plan(b, cfg);
     rc |= check(hipStreamSynchronize(stream()), "upload sync");
-    if (rc) { delete b; return -1; }
+    if (rc) { tpz_sw_batch_destroy(b); return -1; }
     *out = b;
     return 0;
 }
@@ -433,44 +486,80 @@ static SwKernelArgs make_args(tpz_sw_batch* b) {
     a.matrix = (const int8_t*)b->d_matrix.ptr;
     a.gap_open = b->gap_open;
     a.gap_extend = b->gap_extend;
-    a.workspace = (uint32_t*)b->d_workspace.ptr;
-    a.ws_words_per_slot = b->ws_words_per_slot;
+    a.workspace = (uint32_t*)b->plan->d_workspace.ptr;
+    a.ws_words_per_slot = b->plan->ws_words_per_slot;
     a.out_score = (int32_t*)b->d_score.ptr;
     a.out_ref = (int32_t*)b->d_ref.ptr;
     a.out_read = (int32_t*)b->d_read.ptr;
+    a.error = (int32_t*)b->d_error.ptr;
     return a;
+}
+
+// Enqueue the launches of one schedule on the library stream (what a graph
+// captures).  The persistent counter is cleared by a kernel, not a memset.
+static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
+    const SwPlan& pl = *b->plan;
+    const int rows = pl.cfg.rows_per_chunk;
+    SwKernelArgs a = make_args(b);
+    const DeviceTasks& dt = schedule == TPZ_SCHED_PERSISTENT ? pl.packed_tasks : pl.plain_tasks;
+    a.pair_index = (const int32_t*)dt.d_index.ptr;
+    a.task_begin = (const int32_t*)dt.d_begin.ptr;
+    a.task_query = (const int32_t*)dt.d_query.ptr;
+    if (schedule == TPZ_SCHED_PERSISTENT) {
+        const int n_items = (int)(pl.items_persistent.size() / 4);
+        if (n_items > 0) {
+            a.item_task = (const int32_t*)pl.d_items_persistent.ptr;
+            a.n_items = n_items;
+            a.task_counter = (int32_t*)b->d_counter.ptr;
+            if (check(tpz_launch_fill_i32(a.task_counter, 0, 1, stream()), "clear counter")) return -1;
+            const int grid = (int)std::min<size_t>((size_t)n_items, pl.ws_slots);
+            if (check(tpz_launch_sw_forward(&a, grid, rows, stream()), "sw_forward (persistent)"))
+                return -1;
+        }
+    } else {
+        // one launch per query, one workgroup per item of that query
+        a.task_counter = nullptr;
+        for (size_t qk = 0; qk + 1 < pl.plain_query_items.size(); ++qk) {
+            const int i0 = pl.plain_query_items[qk], i1 = pl.plain_query_items[qk + 1];
+            a.item_task = (const int32_t*)pl.d_items_plain.ptr + 4 * (size_t)i0;
+            a.n_items = i1 - i0;
+            if (check(tpz_launch_sw_forward(&a, i1 - i0, rows, stream()), "sw_forward (plain)"))
+                return -1;
+        }
+    }
+    return 0;
+}
+
+static int capture_schedule(tpz_sw_batch* b, int32_t schedule) {
+    SwPlan& pl = *b->plan;
+    hipGraph_t g = nullptr;
+    if (check(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal), "begin capture"))
+        return -1;
+    const int rc = enqueue_schedule(b, schedule);
+    const hipError_t e = hipStreamEndCapture(stream(), &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return -1; }
+    if (check(e, "end capture")) return -1;
+    hipGraphExec_t x = nullptr;
+    if (check(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "instantiate graph")) {
+        (void)hipGraphDestroy(g);
+        return -1;
+    }
+    pl.graph[schedule] = g;
+    pl.exec[schedule] = x;
+    return 0;
 }
 
 int tpz_sw_batch_run(tpz_sw_batch* b, int32_t schedule) {
     if (!b) return fail("null batch");
     if (schedule != TPZ_SCHED_PLAIN && schedule != TPZ_SCHED_PERSISTENT)
         return fail("unknown schedule %d", schedule);
-    SwKernelArgs a = make_args(b);
-    const DeviceTasks& dt = schedule == TPZ_SCHED_PERSISTENT ? b->packed_tasks : b->plain_tasks;
-    a.pair_index = (const int32_t*)dt.d_index.ptr;
-    a.task_begin = (const int32_t*)dt.d_begin.ptr;
-    a.task_query = (const int32_t*)dt.d_query.ptr;
-    a.error = (int32_t*)b->d_error.ptr;
+    SwPlan& pl = *b->plan;
+    if (pl.cfg.use_graph && !pl.exec[schedule] && capture_schedule(b, schedule)) return -1;
     if (check(hipEventRecord(b->ev[0], stream()), "event")) return -1;
-    if (schedule == TPZ_SCHED_PERSISTENT) {
-        const int n_items = (int)(b->items_persistent.size() / 4);
-        if (n_items > 0) {
-            a.item_task = (const int32_t*)b->d_items_persistent.ptr;
-            a.n_items = n_items;
-            a.task_counter = (int32_t*)b->d_counter.ptr;
-            if (check(hipMemsetAsync(b->d_counter.ptr, 0, 4, stream()), "memset counter")) return -1;
-            const int grid = (int)std::min<size_t>((size_t)n_items, b->ws_slots);
-            if (check(tpz_launch_sw_forward(&a, grid, stream()), "sw_forward (persistent)")) return -1;
-        }
-    } else {
-        // one launch per query, one workgroup per item of that query
-        a.task_counter = nullptr;
-        for (size_t qk = 0; qk + 1 < b->plain_query_items.size(); ++qk) {
-            const int i0 = b->plain_query_items[qk], i1 = b->plain_query_items[qk + 1];
-            a.item_task = (const int32_t*)b->d_items_plain.ptr + 4 * (size_t)i0;
-            a.n_items = i1 - i0;
-            if (check(tpz_launch_sw_forward(&a, i1 - i0, stream()), "sw_forward (plain)")) return -1;
-        }
+    if (pl.cfg.use_graph) {
+        if (check(hipGraphLaunch(pl.exec[schedule], stream()), "graph launch")) return -1;
+    } else if (enqueue_schedule(b, schedule)) {
+        return -1;
     }
     if (check(hipEventRecord(b->ev[1], stream()), "event")) return -1;
     if (check(hipStreamSynchronize(stream()), "sw_forward")) return -1;
@@ -480,6 +569,60 @@ int tpz_sw_batch_run(tpz_sw_batch* b, int32_t schedule) {
     int32_t err = 0;
     if (check(hipMemcpy(&err, b->d_error.ptr, 4, hipMemcpyDeviceToHost), "error flag")) return -1;
     if (err) return fail("sw_forward: a cross-wavefront wait timed out");
+    return 0;
+}
+
+int tpz_sw_batch_get_config(const tpz_sw_batch* b, tpz_sw_config* cfg) {
+    if (!b || !cfg) return fail("null argument");
+    *cfg = b->plan->cfg;
+    return 0;
+}
+
+int tpz_sw_batch_set_config(tpz_sw_batch* b, const tpz_sw_config* cfg) {
+    if (!b || !cfg) return fail("null argument");
+    return build_plan(b, *cfg);
+}
+
+int tpz_sw_batch_autotune(tpz_sw_batch* b, tpz_sw_config* best, double* best_ms,
+                          int32_t* n_tried) {
+    if (!b) return fail("null batch");
+    const tpz_sw_config start = b->plan->cfg;
+    std::vector<tpz_sw_config> cands;
+    for (int rows : {32, 16})
+        for (int groups = 1; groups <= (rows == 16 ? 3 : 2); ++groups)
+            for (int nq : {4, 2, 1})
+                for (double bf : {0.25, 0.5, 1.0, 1e30}) {
+                    if ((nq != 4 || groups != (rows == 16 ? 3 : 2)) && bf != 0.5) continue;
+                    tpz_sw_config c = start;
+                    c.rows_per_chunk = rows; c.groups_per_cu = groups;
+                    c.queries_per_task = nq; c.big_fraction = bf; c.use_graph = 0;
+                    cands.push_back(c);
+                }
+    const size_t n = (size_t)b->n_pairs;
+    std::vector<int32_t> ref_sc(n), ref_re(n), ref_rd(n), sc(n), re(n), rd(n);
+    double best_t = 1e300;
+    tpz_sw_config best_c = start;
+    for (size_t k = 0; k < cands.size(); ++k) {
+        if (build_plan(b, cands[k]) || tpz_sw_batch_run(b, TPZ_SCHED_PERSISTENT)) return -1;
+        double t = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            if (tpz_sw_batch_run(b, TPZ_SCHED_PERSISTENT)) return -1;
+            t = std::min(t, b->ms);
+        }
+        if (tpz_sw_batch_results(b, k ? sc.data() : ref_sc.data(), k ? re.data() : ref_re.data(),
+                                 k ? rd.data() : ref_rd.data()))
+            return -1;
+        if (k && (sc != ref_sc || re != ref_re || rd != ref_rd))
+            return fail("autotune: configuration %zu (rows %d, groups %d, queries %d) changed "
+                        "the results", k, cands[k].rows_per_chunk, cands[k].groups_per_cu,
+                        cands[k].queries_per_task);
+        if (t < best_t) { best_t = t; best_c = cands[k]; }
+    }
+    best_c.use_graph = start.use_graph;
+    if (build_plan(b, best_c)) return -1;
+    if (best) *best = best_c;
+    if (best_ms) *best_ms = best_t;
+    if (n_tried) *n_tried = (int32_t)cands.size();
     return 0;
 }
 

@@ -42,7 +42,10 @@ struct SwKernelArgs {
 
 }  // namespace tpz
 
-extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid,
+extern "C" hipError_t tpz_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipStream_t stream);
+
+// rows: query rows per chunk, 16 or 32.
+extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid, int rows,
                                             hipStream_t stream);
 
 #endif

@@ -48,10 +48,8 @@ namespace tpz {
 typedef short s2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u2 __attribute__((ext_vector_type(2)));
 
-constexpr int kRows = 32;          // query rows per chunk, held in registers
 constexpr int kNRes = 26;          // 25 matrix residues + 1 padding residue
 constexpr int kQ = 4;              // queries per task
-constexpr int kProf = kRows * kNRes;
 constexpr int kRowStride = kQ * kNRes;   // profile row r of query k at r * 104 + k * 26
 constexpr int kPadResidue = 25;
 constexpr int kPadScore = -8192;   // score of padding rows / columns
@@ -66,12 +64,14 @@ __device__ __forceinline__ uint32_t bits(s2 v) { return __builtin_bit_cast(uint3
 __device__ __forceinline__ s2 from_bits(uint32_t b) { return __builtin_bit_cast(s2, b); }
 
 // LDS of one wavefront.
+// R (16 or 32) is the number of query rows per chunk, held in registers.
+template <int R>
 struct WaveLds {
     // W(s, q_k[j0 + r]) at [r * 104 + k * 26 + s]: the four queries' entries
     // of a row are adjacent, so the lanes of one read spread over 52
     // consecutive dwords of LDS
-    int16_t prof[kQ * kProf];
-    uint16_t saved[2][kRows][64];          // H column of the chunk best, per half
+    int16_t prof[kQ * R * kNRes];
+    uint16_t saved[2][R][64];              // H column of the chunk best, per half
 };
 
 // Best of the chunks finished so far, per half of the lane; chunk is the
@@ -136,27 +136,29 @@ __device__ __forceinline__ void publish(const ChunkIO& io, int value) {
 // Event bookkeeping for one finished column: if a half's column max rises
 // strictly above the chunk best, remember the column and save that half of
 // the column's H values.
-__device__ __forceinline__ void column_event(WaveLds& lds, const s2 (&Hc)[kRows], s2 cm,
+template <int R>
+__device__ __forceinline__ void column_event(WaveLds<R>& lds, const s2 (&Hc)[R], s2 cm,
                                              s2& bc, int col, int& ci_lo, int& ci_hi) {
     const int lane = threadIdx.x % 64;
     if (cm.x > bc.x) {
         ci_lo = col;
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) lds.saved[0][r][lane] = (uint16_t)Hc[r].x;
+        for (int r = 0; r < R; ++r) lds.saved[0][r][lane] = (uint16_t)Hc[r].x;
     }
     if (cm.y > bc.y) {
         ci_hi = col;
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) lds.saved[1][r][lane] = (uint16_t)Hc[r].y;
+        for (int r = 0; r < R; ++r) lds.saved[1][r][lane] = (uint16_t)Hc[r].y;
     }
     bc = vmax(bc, cm);
 }
 
-// One chunk of kRows rows [j0, j0 + kRows) over all subject columns, two
+// One chunk of R rows [j0, j0 + R) over all subject columns, two
 // columns at a time: the second column runs one row behind the first, so
 // every step advances two independent dependency chains.  Returns false if
 // a wait on the producer of the top boundary gave up.
-__device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, const ChunkIO& io,
+template <int R>
+__device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds, const ChunkIO& io,
                                           int ncols, const int8_t* s_lo, int n_lo,
                                           const int8_t* s_hi, int n_hi, int off_lo, int off_hi,
                                           int j0, int chunk, LaneBest& best) {
@@ -164,9 +166,9 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
     const s2 go = s2{(short)a.gap_open, (short)a.gap_open};
     const s2 ge = s2{(short)a.gap_extend, (short)a.gap_extend};
     const s2 zero = s2{0, 0};
-    s2 E[kRows], H[kRows], Ha[kRows];
+    s2 E[R], H[R], Ha[R];
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) { E[r] = zero; H[r] = zero; }
+    for (int r = 0; r < R; ++r) { E[r] = zero; H[r] = zero; }
     s2 bc = zero;                 // chunk best per half
     int ci_lo = -1, ci_hi = -1;
     s2 hdiag = zero;              // H(i-1, j0-1)
@@ -232,7 +234,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
             fa = vmax(usub(fa, ge), hg);
         }
 #pragma unroll
-        for (int r = 1; r < kRows; ++r) {
+        for (int r = 1; r < R; ++r) {
             // column a, row r
             {
                 s2 w = s2{pa_lo[r * kRowStride], pa_hi[r * kRowStride]};
@@ -259,7 +261,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
         }
         // column b, last row
         {
-            const int q = kRows - 1;
+            const int q = R - 1;
             s2 w = s2{pb_lo[q * kRowStride], pb_hi[q * kRowStride]};
             s2 h = vmax(vmax(add_sat(db, w), E[q]), fb);
             H[q] = h;
@@ -269,9 +271,9 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
             fb = vmax(usub(fb, ge), hg);
         }
         if (bH) {
-            bH[(size_t)i * 64 + lane] = bits(Ha[kRows - 1]);
+            bH[(size_t)i * 64 + lane] = bits(Ha[R - 1]);
             bF[(size_t)i * 64 + lane] = bits(fa);
-            bH[(size_t)(i + 1) * 64 + lane] = bits(H[kRows - 1]);
+            bH[(size_t)(i + 1) * 64 + lane] = bits(H[R - 1]);
             bF[(size_t)(i + 1) * 64 + lane] = bits(fb);
         }
         column_event(lds, Ha, cma, bc, i, ci_lo, ci_hi);
@@ -289,9 +291,9 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds& lds, c
         const int b = half ? bc.y : bc.x;
         const int ci = half ? ci_hi : ci_lo;
         if (b <= 0) continue;
-        int row = kRows;
+        int row = R;
 #pragma unroll
-        for (int r = kRows - 1; r >= 0; --r)
+        for (int r = R - 1; r >= 0; --r)
             if ((int)(int16_t)lds.saved[half][r][lane] == b) row = r;
         if (b > best.score[half] || (b == best.score[half] && ci < best.ref[half])) {
             best.score[half] = b;
@@ -321,7 +323,8 @@ struct TaskView {
 // Substitution scores of rows j0 .. j0+31 of every query of the task for all
 // 26 residues; rows past a query's end score kPadScore.  Written and read by
 // one wavefront only.
-__device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds& lds,
+template <int R>
+__device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds<R>& lds,
                                               const TaskView& t, int j0) {
     const int lane = threadIdx.x % 64;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -331,7 +334,7 @@ __device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds& ld
         const int8_t* q = t.qs[k];
         if (!q) break;
         const int m = t.ms[k];
-        for (int e = lane; e < kProf; e += 64) {
+        for (int e = lane; e < R * kNRes; e += 64) {
             const int r = e / kNRes, s = e - r * kNRes;
             const int j = j0 + r;
             int w = kPadScore;
@@ -415,8 +418,9 @@ __device__ __forceinline__ void write_best(const SwKernelArgs& a, const TaskView
 }
 
 // LDS of one workgroup.
+template <int R>
 struct GroupLds {
-    WaveLds wave[kWaves];
+    WaveLds<R> wave[kWaves];
     int item;
     int error;
     int progress[kWaves];
@@ -424,17 +428,18 @@ struct GroupLds {
 };
 
 // A task worked by one wavefront: its chunks in order, boundary in place.
-__device__ void run_task_wave(const SwKernelArgs& a, WaveLds& lds, uint32_t* buf, int task,
-                              GroupLds& g) {
+template <int R>
+__device__ void run_task_wave(const SwKernelArgs& a, WaveLds<R>& lds, uint32_t* buf, int task,
+                              GroupLds<R>& g) {
     const TaskView t = task_view(a, task);
     LaneBest best;
     init_best(best, t);
     int chunk = 0;
-    for (int j0 = 0; j0 < t.m_max; j0 += kRows, ++chunk) {
+    for (int j0 = 0; j0 < t.m_max; j0 += R, ++chunk) {
         build_profile(a, lds, t, j0);
         ChunkIO io{};
         io.top = j0 == 0 ? nullptr : buf;
-        io.bottom = j0 + kRows < t.m_max ? buf : nullptr;
+        io.bottom = j0 + R < t.m_max ? buf : nullptr;
         io.error = &g.error;
         io.g_error = a.error;
         run_chunk(a, lds, io, t.ncols, t.s_lo, t.n_lo, t.s_hi, t.n_hi, t.off_lo, t.off_hi, j0,
@@ -446,16 +451,17 @@ __device__ void run_task_wave(const SwKernelArgs& a, WaveLds& lds, uint32_t* buf
 // A task worked by all wavefronts of the workgroup: chunk c by wavefront
 // c % kWaves, each one trailing the wavefront of the chunk above by at least
 // kAhead columns; chunk c's bottom boundary lives in buffer c % kWaves.
-__device__ void run_task_group(const SwKernelArgs& a, GroupLds& g, uint32_t* bufs,
+template <int R>
+__device__ void run_task_group(const SwKernelArgs& a, GroupLds<R>& g, uint32_t* bufs,
                                size_t buf_words, int task) {
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const TaskView t = task_view(a, task);
     LaneBest best;
     init_best(best, t);
-    const int n_chunks = (t.m_max + kRows - 1) / kRows;
+    const int n_chunks = (t.m_max + R - 1) / R;
     const int nc = t.ncols;
     for (int c = wave; c < n_chunks; c += kWaves) {
-        const int j0 = c * kRows;
+        const int j0 = c * R;
         build_profile(a, g.wave[wave], t, j0);
         ChunkIO io{};
         io.top = c == 0 ? nullptr : bufs + (size_t)((c - 1) % kWaves) * buf_words;
@@ -507,9 +513,10 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds& g, uint32_t* buf
 // (item_task[4k] >= 0, item_task[4k+1] == -2) or up to four small tasks,
 // one per wavefront (-1 = none).  With a counter the grid is persistent and
 // workgroup b owns workspace slot b; without one, workgroup b runs item b.
+template <int R>
 __global__ void __launch_bounds__(kWaves * 64)
 sw_forward_kernel(SwKernelArgs a) {
-    __shared__ GroupLds g;
+    __shared__ GroupLds<R> g;
     const int wave = threadIdx.x / 64;
     const size_t buf_words = (size_t)a.ws_words_per_slot / kWaves;
     uint32_t* bufs = a.workspace + (size_t)blockIdx.x * a.ws_words_per_slot;
@@ -534,11 +541,29 @@ sw_forward_kernel(SwKernelArgs a) {
     }
 }
 
+// Sets n words to v (graph-capturable, unlike a memset node).
+__global__ void fill_i32_kernel(int32_t* p, int32_t v, int32_t n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 }  // namespace tpz
 
-extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid,
+extern "C" hipError_t tpz_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tpz::fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid, int rows,
                                             hipStream_t stream) {
-    hipLaunchKernelGGL(tpz::sw_forward_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream,
-                       *args);
+    if (rows == 16)
+        hipLaunchKernelGGL(tpz::sw_forward_kernel<16>, dim3(grid), dim3(tpz::kWaves * 64), 0,
+                           stream, *args);
+    else if (rows == 32)
+        hipLaunchKernelGGL(tpz::sw_forward_kernel<32>, dim3(grid), dim3(tpz::kWaves * 64), 0,
+                           stream, *args);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }

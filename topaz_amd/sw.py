@@ -16,6 +16,17 @@ SCHED_PERSISTENT = 1
 OVERFLOW = 32767      # ssw_align returns NULL; align_using_ssw_profile stores 32767
 
 
+class SwConfig(C.Structure):
+    """tpz_sw_config: launch configuration of a batch (results never depend
+    on it)."""
+    _fields_ = [("rows_per_chunk", C.c_int32), ("groups_per_cu", C.c_int32),
+                ("queries_per_task", C.c_int32), ("use_graph", C.c_int32),
+                ("big_fraction", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _concat(seqs):
     lens = np.array([len(s) for s in seqs], dtype=np.int32)
     off = np.zeros(len(seqs), dtype=np.int64)
@@ -77,6 +88,30 @@ class SwBatch:
         a = C.c_double()
         check(_lib.lib().tpz_sw_batch_kernel_ms(self._h, C.byref(a)))
         return a.value
+
+    def config(self):
+        c = SwConfig()
+        check(_lib.lib().tpz_sw_batch_get_config(self._h, C.byref(c)), "tpz_sw_batch_get_config")
+        return c
+
+    def set_config(self, **kw):
+        """Change fields of the configuration (rows_per_chunk, groups_per_cu,
+        queries_per_task, use_graph, big_fraction) and re-plan the batch."""
+        c = self.config()
+        for k, v in kw.items():
+            if k not in dict(SwConfig._fields_):
+                raise KeyError(k)
+            setattr(c, k, v)
+        check(_lib.lib().tpz_sw_batch_set_config(self._h, C.byref(c)), "tpz_sw_batch_set_config")
+        return self
+
+    def autotune(self):
+        """Time every candidate configuration on this batch and keep the
+        fastest; returns (config, kernel ms, candidates tried)."""
+        c, ms, n = SwConfig(), C.c_double(), C.c_int32()
+        check(_lib.lib().tpz_sw_batch_autotune(self._h, C.byref(c), C.byref(ms), C.byref(n)),
+              "tpz_sw_batch_autotune")
+        return c, ms.value, n.value
 
     def close(self):
         if getattr(self, "_h", None):
