@@ -306,6 +306,99 @@ def sw_leg(args):
     return out
 
 
+def serving_leg(args, rates=(10, 50, 100, 200), seconds=3.0):
+    """Requests arriving as a Poisson process against a resident database:
+    each request is 1-500 queries (uniform), each query with 200 candidate
+    subjects (half of the queries carry planted homologs among them).  For
+    each arrival rate: p50 / p99 latency (submit to results on the host) and
+    the sustained query rate; every served result is compared with the same
+    request run alone."""
+    import threading
+    from topaz_amd import matrix, synth
+    from topaz_amd.serve import Server
+    from topaz_amd.sw import _concat
+    rng = np.random.default_rng(2026)
+    t0 = time.perf_counter()
+    n_db = int(100000 * args.serve_scale)
+    dl = synth.lognormal_lengths(rng, n_db, 350.0, 250.0, 30, 5000)
+    dr, do, dn = synth.concat_random(rng, dl)
+    pool = []
+    for _ in range(24):
+        nq = int(rng.integers(1, 501))
+        ql = synth.lognormal_lengths(rng, nq, 350.0, 250.0, 30, 5000)
+        qr, qo, qn = synth.concat_random(rng, ql)
+        pq = np.repeat(np.arange(nq, dtype=np.int32), 200)
+        ps = rng.integers(0, n_db, len(pq)).astype(np.int32)
+        # planted homologs: mutated copies of the query written over a candidate
+        hom = np.nonzero(rng.random(nq) < 0.5)[0]
+        for k in hom:
+            j = int(ps[200 * k])
+            res, _, hn = synth.mutate(rng, qr, qo[k:k + 1], qn[k:k + 1], rng.uniform(0.25, 0.95, 1))
+            m = min(int(hn[0]), int(dn[j]))
+            dr[do[j]:do[j] + m] = res[:m]
+        pool.append(((qr.astype(np.int8), qo, qn), pq, ps))
+    db = (dr.astype(np.int8), do, dn)
+    print("[bench] serving workload: %d subjects, %d requests in the pool, generated in %.1f s"
+          % (n_db, len(pool), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    srv = Server(db, matrix(), 11, 1, memory_budget=1 << 30, max_inflight=2)
+    offline = []
+    for req in pool:
+        offline.append(srv.wait(srv.submit(*req))[:3])
+    rows = []
+    identical = True
+    for rate in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds) + 1))
+        arrivals = arrivals[arrivals < seconds]
+        picks = rng.integers(0, len(pool), len(arrivals))
+        tickets = []
+        done = []
+        st0 = srv.stats()
+
+        def waiter():
+            k = 0
+            while k < len(arrivals):
+                if k >= len(tickets):
+                    time.sleep(0.0005)
+                    continue
+                sc, re_, rd, lat = srv.wait(tickets[k])
+                done.append((picks[k], lat, (sc, re_, rd)))
+                k += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        for a, p in zip(arrivals, picks):
+            delay = start + a - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(*pool[p]))
+        th.join()
+        wall = time.perf_counter() - start
+        st = srv.stats()
+        lats = np.array([d[1] for d in done])
+        for p, _, res in done:
+            for x, y in zip(res, offline[p]):
+                if not np.array_equal(x, y):
+                    identical = False
+        n_q = int(sum(len(pool[p][0][2]) for p in picks))
+        launches = st["launches"] - st0["launches"]
+        rows.append({"offered_requests_per_s": rate, "requests": len(arrivals),
+                     "queries_per_s": round(n_q / wall, 1),
+                     "p50_ms": round(float(np.percentile(lats, 50)), 2),
+                     "p99_ms": round(float(np.percentile(lats, 99)), 2),
+                     "launches": launches,
+                     "requests_per_launch": round(len(arrivals) / max(launches, 1), 2)})
+        print("[bench] serving %d/s: %s" % (rate, rows[-1]), file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    return {"workload": "serving: %d resident subjects (lognormal mean 350 sd 250 clipped "
+                        "[30,5000]); requests of U{1..500} queries x 200 candidates, Poisson "
+                        "arrivals, %.0f s per rate, memory budget 1 GiB, 2 launches in flight"
+                        % (n_db, seconds),
+            "rates": rows, "identical_to_offline": identical,
+            "peak_admitted_bytes": st["peak_bytes"], "workspace_bytes": st["workspace_bytes"]}
+
+
 def copy_leg(args):
     """A plain device-to-device copy of 1 GiB, three times: the calibration
     run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch of
@@ -335,7 +428,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
-    ap.add_argument("--leg", choices=["all", "sw", "gapless", "copy"], default="all")
+    ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "copy"], default="all")
+    ap.add_argument("--serve-scale", type=float, default=1.0,
+                    help="fraction of the 100,000-subject serving database")
     ap.add_argument("--gapless-scale", type=float, default=1.0,
                     help="fraction of the 1e9-residue, 5e7-seed gapless workload")
     args = ap.parse_args()
@@ -348,10 +443,20 @@ def main():
     if args.leg == "copy":
         print(json.dumps(copy_leg(args)), flush=True)
         return
+    if args.leg == "serve":
+        sv = serving_leg(args)
+        print(json.dumps({"metric": "serving p50/p99 latency vs sustained queries/s",
+                          "serving": sv}), flush=True)
+        return
     if args.leg in ("all", "sw"):
         out = sw_leg(args)
         if args.leg == "all":
             out["gapless"] = gapless_leg(args)
+            out["serving"] = serving_leg(args)
+            top = [r for r in out["serving"]["rates"] if r["offered_requests_per_s"] == 100]
+            if top:
+                out["latency_p50_ms"] = top[0]["p50_ms"]
+                out["latency_p99_ms"] = top[0]["p99_ms"]
     else:
         g = gapless_leg(args)
         m = g["matched"]

@@ -101,6 +101,58 @@ int tpz_sw_batch_autotune(tpz_sw_batch* b, tpz_sw_config* best, double* best_ms,
 
 void tpz_sw_batch_destroy(tpz_sw_batch* b);
 
+/* ---- serving loop: continuous batching over a resident database -------- */
+
+/* A request loop for stage S.  The database subjects stay resident on the
+ * device; requests (a set of queries and their candidate pairs) arrive from
+ * any number of host threads.  A scheduler thread admits waiting requests in
+ * arrival order while their device memory fits the budget, merges everything
+ * admitted at that moment into one persistent launch on a free stream, and
+ * lets launches on different streams overlap on the device, so requests that
+ * arrive while others run join the next launch instead of queueing behind the
+ * whole batch.  Per-pair results are independent of which requests share a
+ * launch.  No reference counterpart: the reference runs one thread per query
+ * over a length-keyed work queue (work_queue.c:317-352, engine.c:423-457). */
+typedef struct tpz_server tpz_server;
+
+typedef struct {
+    uint64_t memory_budget;   /* bytes of admitted request data (queries, pairs, task
+                                 lists, results) on the device at once; 0: 1 GiB     */
+    int32_t max_inflight;     /* launches in flight (one stream each); 0: 2          */
+    int32_t max_launch_pairs; /* pairs merged into one launch at most; 0: 4,000,000  */
+} tpz_server_config;
+
+typedef struct {
+    uint64_t requests_done;
+    uint64_t launches;
+    uint64_t max_requests_per_launch;
+    uint64_t pairs_done;
+    uint64_t peak_bytes;      /* largest admitted request bytes in flight            */
+    uint64_t workspace_bytes; /* fixed device workspace of the launch slots          */
+} tpz_server_stats;
+
+/* Upload the database subjects (same layout and conventions as
+ * tpz_sw_batch_create) and start the scheduler thread.  cfg may be null. */
+int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
+                      const int64_t* s_off, const int32_t* s_len, const int8_t* matrix,
+                      int32_t gap_open, int32_t gap_extend, const tpz_server_config* cfg);
+/* Queue a request: n_queries queries (residues, offsets, lengths) and n_pairs
+ * pairs (query number within the request, subject number in the database).
+ * Returns at once with a ticket; fails if the request alone exceeds the
+ * memory budget or references anything out of range. */
+int tpz_server_submit(tpz_server* s, int32_t n_queries, const int8_t* q_res,
+                      const int64_t* q_off, const int32_t* q_len, int32_t n_pairs,
+                      const int32_t* pair_query, const int32_t* pair_subject,
+                      uint64_t* ticket);
+/* Block until the request's results are on the host, copy them (per pair as
+ * tpz_sw_batch_results; any pointer may be null) and give the latency from
+ * submit to results in milliseconds.  Each ticket is waited for once. */
+int tpz_server_wait(tpz_server* s, uint64_t ticket, int32_t* score, int32_t* ref_end,
+                    int32_t* read_end, double* latency_ms);
+int tpz_server_get_stats(tpz_server* s, tpz_server_stats* st);
+/* Finishes the launches in flight, then stops; requests still queued fail. */
+void tpz_server_destroy(tpz_server* s);
+
 /* ---- batched traceback statistics of reported hits ---------------------- */
 
 /* ssw_align_stats (ssw.c:900-978) for many hits: reverse pass for the start

@@ -11,7 +11,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libtopaz_amd.so"
-SOURCES = ["runtime.hip", "sw_kernels.hip", "gapless_kernels.hip", "traceback_kernels.hip", "compat.hip"]
+SOURCES = ["runtime.hip", "sw_kernels.hip", "gapless_kernels.hip", "traceback_kernels.hip", "compat.hip",
+           "server.hip"]
 ARCH = os.environ.get("TOPAZ_AMD_ARCH", "gfx950")
 
 

@@ -15,6 +15,7 @@ int check(hipError_t e, const char* what);
 int ensure_init();                       // tpz_init(0) on first use
 hipStream_t stream();
 int num_cu();
+int device_id();                         // the device tpz_init selected
 
 // Owning device allocation (hipMalloc / hipFree).
 struct DeviceBuffer {
@@ -28,6 +29,55 @@ struct DeviceBuffer {
     int alloc(size_t n);
     int upload(const void* src, size_t n);
 };
+
+// ---- batched Smith-Waterman planning (shared by batches and the server) ----
+constexpr int kGroupsPerCu = 2;     // default resident sw_forward workgroups per CU
+constexpr int kGroupWaves = 4;      // wavefronts per sw_forward workgroup
+constexpr int kTaskPairs = 128;     // two pairs per lane of a wavefront
+constexpr int kTaskQueries = 4;     // queries whose profiles a wavefront can hold
+
+// Tasks of the kernel: ranges of at most 128 pairs of up to 4 queries.
+struct Tasks {
+    std::vector<int32_t> index;   // pair numbers, grouped per task
+    std::vector<int32_t> begin;   // task boundaries into index
+    std::vector<int32_t> query;   // kTaskQueries query numbers per task, -1 = none
+    std::vector<int64_t> cost;    // rows (whole chunks) x widest subject
+    void clear() { index.clear(); begin.clear(); query.clear(); cost.clear(); }
+    void open(const int32_t* qs, int nq, int64_t c) {
+        begin.push_back((int32_t)index.size());
+        for (int k = 0; k < kTaskQueries; ++k) query.push_back(k < nq ? qs[k] : -1);
+        cost.push_back(c);
+    }
+    int32_t count() const { return (int32_t)cost.size(); }
+};
+
+// A task set on the device.
+struct DeviceTasks {
+    Tasks host;
+    DeviceBuffer d_index, d_begin, d_query;
+    int upload() {
+        return d_index.upload(host.index.data(), host.index.size() * 4) |
+               d_begin.upload(host.begin.data(), host.begin.size() * 4) |
+               d_query.upload(host.query.data(), host.query.size() * 4);
+    }
+};
+
+// Pairs of each query, longest subject first (ties: lower pair number).
+std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pair_query,
+                                                 const std::vector<int32_t>& pair_subject,
+                                                 const std::vector<int32_t>& s_len,
+                                                 int32_t n_queries);
+// One query per task (the plain schedule).
+void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
+                       const std::vector<int32_t>& pair_subject, const std::vector<int32_t>& q_len,
+                       const std::vector<int32_t>& s_len, int rows, Tasks& t);
+// Up to nq_max queries of equal chunk count per task (the persistent schedule).
+void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
+                        const std::vector<int32_t>& pair_subject, const std::vector<int32_t>& q_len,
+                        const std::vector<int32_t>& s_len, int rows, int nq_max, Tasks& t);
+// Work items of the persistent schedule (4 entries each), most expensive first.
+void build_items(const Tasks& t, int n_slots, double big_fraction, std::vector<int32_t>& items,
+                 int32_t& n_big);
 
 // Letter codes of a text for the gapless kernel (see gapless_kernels.h).
 struct TextCodes {

@@ -29,8 +29,6 @@ static std::mutex g_init_mu;
 static int g_device = -1;
 static hipStream_t g_stream = nullptr;
 static int g_num_cu = 0;
-constexpr int kGroupsPerCu = 2;   // resident sw_forward workgroups (4 wavefronts) per CU
-constexpr int kGroupWaves = 4;
 
 int fail(const char* fmt, ...) {
     char buf[512];
@@ -54,6 +52,7 @@ int ensure_init() {
 
 hipStream_t stream() { return g_stream; }
 int num_cu() { return g_num_cu; }
+int device_id() { return g_device; }
 
 DeviceBuffer::~DeviceBuffer() { reset(); }
 void DeviceBuffer::reset() {
@@ -78,29 +77,12 @@ int DeviceBuffer::upload(const void* src, size_t n) {
 // ---------------------------------------------------------------------------
 // batched forward Smith-Waterman
 // ---------------------------------------------------------------------------
-constexpr int kTaskPairs = 128;     // two pairs per lane of a wavefront
-constexpr int kTaskQueries = 4;     // queries whose profiles a wavefront can hold
-
-struct Tasks {
-    std::vector<int32_t> index;   // pair numbers, grouped per task
-    std::vector<int32_t> begin;   // task boundaries into index
-    std::vector<int32_t> query;   // kTaskQueries query numbers per task, -1 = none
-    std::vector<int64_t> cost;    // rows (whole chunks) x widest subject
-    void clear() { index.clear(); begin.clear(); query.clear(); cost.clear(); }
-    void open(const int32_t* qs, int nq, int64_t c) {
-        begin.push_back((int32_t)index.size());
-        for (int k = 0; k < kTaskQueries; ++k) query.push_back(k < nq ? qs[k] : -1);
-        cost.push_back(c);
-    }
-    int32_t count() const { return (int32_t)cost.size(); }
-};
-
 static int64_t chunk_rows(int32_t m, int rows) {
     return (int64_t)(m + rows - 1) / rows * rows;
 }
 
 // Pairs of each query, longest subject first (ties: lower pair number).
-static std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pair_query,
+std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pair_query,
                                                         const std::vector<int32_t>& pair_subject,
                                                         const std::vector<int32_t>& s_len,
                                                         int32_t n_queries) {
@@ -116,7 +98,7 @@ static std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_
 // One query per task (the plain schedule): a query's pairs, longest subject
 // first, cut into tasks of 128, so the two halves of a lane and the lanes of
 // a wavefront walk subjects of similar length.
-static void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
+void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
                               const std::vector<int32_t>& pair_subject,
                               const std::vector<int32_t>& q_len,
                               const std::vector<int32_t>& s_len, int rows, Tasks& t) {
@@ -138,7 +120,7 @@ static void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
 // subjects then share a wavefront with other queries' long subjects instead
 // of its own short ones, which cuts the columns a wavefront walks past the
 // end of a lane's subject.
-static void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
+void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
                                const std::vector<int32_t>& pair_subject,
                                const std::vector<int32_t>& q_len,
                                const std::vector<int32_t>& s_len, int rows, int nq_max,
@@ -196,16 +178,6 @@ static void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
     t.begin.push_back((int32_t)t.index.size());
 }
 
-// A task set on the device.
-struct DeviceTasks {
-    Tasks host;
-    DeviceBuffer d_index, d_begin, d_query;
-    int upload() {
-        return d_index.upload(host.index.data(), host.index.size() * 4) |
-               d_begin.upload(host.begin.data(), host.begin.size() * 4) |
-               d_query.upload(host.query.data(), host.query.size() * 4);
-    }
-};
 
 }  // namespace tpz
 
@@ -259,7 +231,7 @@ namespace tpz {
 // is worked by a whole workgroup (chunks interleaved over 4 wavefronts);
 // the others go four to an item, one per wavefront, neighbours in cost
 // order.  Items run most expensive first.
-static void build_items(const Tasks& t, int n_slots, double big_fraction,
+void build_items(const Tasks& t, int n_slots, double big_fraction,
                         std::vector<int32_t>& items, int32_t& n_big) {
     const size_t n = t.cost.size();
     std::vector<int32_t> order(n);

@@ -1,0 +1,524 @@
+// Serving loop of libtopaz_amd: continuous batching of Smith-Waterman
+// requests over a device-resident database (tpz_server_* in
+// include/topaz_amd.h).
+//
+// A request's task list (pairs of up to four of its queries per wavefront,
+// as the batch path builds it) is planned in the submitting thread.  Requests
+// then wait in arrival order.  A scheduler thread owns a few launch slots,
+// each with its own stream and grow-only device and pinned host buffers.
+// Whenever a slot is free it admits waiting requests from the head of the
+// queue while their bytes fit the memory budget, concatenates their task
+// lists, orders the work items by cost, uploads everything in one copy, and
+// launches the persistent kernel with the results copied back behind it on
+// the same stream.  While that launch runs, new
+// arrivals queue up and go into the next free slot, whose launch starts on the
+// device as the first one's workgroups drain.  A finished slot's results are
+// split back to their requests; the latency of a request is the time from its
+// submit to the moment its results are on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/topaz_amd.h"
+#include "runtime.h"
+#include "sw_kernels.h"
+
+namespace tpz {
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// Device allocation that only grows.
+struct GrowBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    GrowBuffer() = default;
+    GrowBuffer(const GrowBuffer&) = delete;
+    GrowBuffer& operator=(const GrowBuffer&) = delete;
+    ~GrowBuffer() { if (ptr) (void)hipFree(ptr); }
+    int reserve(size_t n) {
+        if (n <= bytes) return 0;
+        if (ptr) { (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
+        const size_t want = std::max(n, (size_t)1 << 20);
+        if (check(hipMalloc(&ptr, want), "hipMalloc")) return -1;
+        bytes = want;
+        return 0;
+    }
+};
+
+// Pinned host allocation that only grows.
+struct PinnedBuffer {
+    char* ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() { if (ptr) (void)hipHostFree(ptr); }
+    int reserve(size_t n) {
+        if (n <= bytes) return 0;
+        if (ptr) { (void)hipHostFree(ptr); ptr = nullptr; bytes = 0; }
+        const size_t want = std::max(n, (size_t)1 << 20);
+        if (check(hipHostMalloc((void**)&ptr, want, hipHostMallocDefault), "hipHostMalloc"))
+            return -1;
+        bytes = want;
+        return 0;
+    }
+};
+
+struct Request {
+    uint64_t id = 0;
+    std::vector<int8_t> q_res;
+    std::vector<int64_t> q_off;
+    std::vector<int32_t> q_len;
+    std::vector<int32_t> pq, ps;
+    Tasks tasks;                        // planned at submit
+    std::vector<int32_t> score, ref, read;
+    size_t bytes = 0;
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+    int rc = 0;
+    std::string err;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_done = nullptr;
+    bool busy = false;
+    std::vector<Request*> reqs;
+    std::vector<size_t> pair_base;      // first merged pair of each request
+    size_t n_pairs = 0;
+    size_t bytes = 0;                   // admitted request bytes
+    GrowBuffer d_in, d_out;
+    DeviceBuffer d_ws;
+    PinnedBuffer h_in, h_out;
+    int error_word = 0;                 // offset (words) of the error flag in d_out
+};
+
+// Bytes a request holds on the device while in flight: its queries and pairs,
+// its share of the task and item lists, and its results.
+size_t request_bytes(size_t q_residues, size_t n_queries, size_t n_pairs) {
+    return align16(q_residues) + 12 * n_queries + 40 * n_pairs + 256;
+}
+
+}  // namespace
+}  // namespace tpz
+
+using namespace tpz;
+
+struct tpz_server {
+    int32_t n_subjects = 0, max_subject = 0;
+    std::vector<int32_t> s_len;
+    DeviceBuffer d_s_res, d_s_off, d_s_len, d_matrix;
+    int32_t gap_open = 11, gap_extend = 1;
+    tpz_server_config cfg{};
+    int device = 0;
+    int grid = 0;                        // workgroups of one launch at most
+    int64_t ws_words_per_slot = 0;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, Request*> reqs;
+    std::vector<std::unique_ptr<Slot>> slots;
+    size_t mem_in_use = 0;
+    uint64_t next_id = 1;
+    bool stop = false;
+    tpz_server_stats stats{};
+    std::thread thread;
+};
+
+namespace tpz {
+namespace {
+
+// Merge the slot's requests, upload, launch, and queue the result copy.
+int launch_slot(tpz_server* s, Slot& sl) {
+    size_t nq = 0, nres = 0, np = 0;
+    for (Request* r : sl.reqs) { nq += r->q_len.size(); nres += r->q_res.size(); np += r->pq.size(); }
+    std::vector<int32_t> q_len(nq), pq(np), ps(np);
+    std::vector<int64_t> q_off(nq);
+    sl.pair_base.clear();
+    {
+        size_t qb = 0, pb = 0, rb = 0;
+        for (Request* r : sl.reqs) {
+            sl.pair_base.push_back(pb);
+            for (size_t k = 0; k < r->q_len.size(); ++k) {
+                q_len[qb + k] = r->q_len[k];
+                q_off[qb + k] = (int64_t)rb + r->q_off[k];
+            }
+            for (size_t p = 0; p < r->pq.size(); ++p) {
+                pq[pb + p] = (int32_t)qb + r->pq[p];
+                ps[pb + p] = r->ps[p];
+            }
+            qb += r->q_len.size();
+            pb += r->pq.size();
+            rb += r->q_res.size();
+        }
+    }
+    sl.n_pairs = np;
+    // concatenate the requests' task lists
+    Tasks t;
+    {
+        size_t qb = 0, pb = 0;
+        for (Request* r : sl.reqs) {
+            const Tasks& rt = r->tasks;
+            for (int32_t k = 0; k < rt.count(); ++k) {
+                int32_t qs[kTaskQueries];
+                int nq_k = 0;
+                for (int j = 0; j < kTaskQueries; ++j)
+                    if (rt.query[kTaskQueries * k + j] >= 0)
+                        qs[nq_k++] = (int32_t)qb + rt.query[kTaskQueries * k + j];
+                t.open(qs, nq_k, rt.cost[k]);
+                for (int32_t e = rt.begin[k]; e < rt.begin[k + 1]; ++e)
+                    t.index.push_back((int32_t)pb + rt.index[e]);
+            }
+            qb += r->q_len.size();
+            pb += r->pq.size();
+        }
+        t.begin.push_back((int32_t)t.index.size());
+    }
+    std::vector<int32_t> items;
+    int32_t n_big = 0;
+    build_items(t, s->grid, 1.0, items, n_big);
+    const int n_items = (int)(items.size() / 4);
+
+    // one host blob, one copy: residues | q_off | q_len | pq | ps | index | begin | tquery | items
+    const size_t o_res = 0;
+    const size_t o_qoff = align16(o_res + nres + 16);
+    const size_t o_qlen = align16(o_qoff + 8 * nq);
+    const size_t o_pq = align16(o_qlen + 4 * nq);
+    const size_t o_ps = align16(o_pq + 4 * np);
+    const size_t o_index = align16(o_ps + 4 * np);
+    const size_t o_begin = align16(o_index + 4 * t.index.size());
+    const size_t o_tq = align16(o_begin + 4 * t.begin.size());
+    const size_t o_items = align16(o_tq + 4 * t.query.size());
+    const size_t in_bytes = align16(o_items + 4 * items.size()) + 16;
+    const size_t out_words = 3 * np + 2;          // score | ref | read | error | counter
+    if (sl.h_in.reserve(in_bytes) || sl.d_in.reserve(in_bytes) ||
+        sl.h_out.reserve(out_words * 4) || sl.d_out.reserve(out_words * 4))
+        return -1;
+    char* h = sl.h_in.ptr;
+    size_t rb = 0;
+    for (Request* r : sl.reqs) {
+        memcpy(h + o_res + rb, r->q_res.data(), r->q_res.size());
+        rb += r->q_res.size();
+    }
+    memcpy(h + o_qoff, q_off.data(), 8 * nq);
+    memcpy(h + o_qlen, q_len.data(), 4 * nq);
+    memcpy(h + o_pq, pq.data(), 4 * np);
+    memcpy(h + o_ps, ps.data(), 4 * np);
+    memcpy(h + o_index, t.index.data(), 4 * t.index.size());
+    memcpy(h + o_begin, t.begin.data(), 4 * t.begin.size());
+    memcpy(h + o_tq, t.query.data(), 4 * t.query.size());
+    memcpy(h + o_items, items.data(), 4 * items.size());
+    char* d = (char*)sl.d_in.ptr;
+    int32_t* dout = (int32_t*)sl.d_out.ptr;
+    sl.error_word = (int)(3 * np);
+    if (check(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, sl.stream), "upload") ||
+        check(tpz_launch_fill_i32(dout + 3 * np, 0, 2, sl.stream), "clear flags"))
+        return -1;
+    if (n_items > 0) {
+        SwKernelArgs a{};
+        a.q_res = (const int8_t*)(d + o_res);
+        a.q_off = (const int64_t*)(d + o_qoff);
+        a.q_len = (const int32_t*)(d + o_qlen);
+        a.s_res = (const int8_t*)s->d_s_res.ptr;
+        a.s_off = (const int64_t*)s->d_s_off.ptr;
+        a.s_len = (const int32_t*)s->d_s_len.ptr;
+        a.pair_query = (const int32_t*)(d + o_pq);
+        a.pair_subject = (const int32_t*)(d + o_ps);
+        a.pair_index = (const int32_t*)(d + o_index);
+        a.task_begin = (const int32_t*)(d + o_begin);
+        a.task_query = (const int32_t*)(d + o_tq);
+        a.item_task = (const int32_t*)(d + o_items);
+        a.n_items = n_items;
+        a.task_counter = dout + 3 * np + 1;
+        a.error = dout + 3 * np;
+        a.matrix = (const int8_t*)s->d_matrix.ptr;
+        a.gap_open = s->gap_open;
+        a.gap_extend = s->gap_extend;
+        a.workspace = (uint32_t*)sl.d_ws.ptr;
+        a.ws_words_per_slot = s->ws_words_per_slot;
+        a.out_score = dout;
+        a.out_ref = dout + np;
+        a.out_read = dout + 2 * np;
+        const int grid = std::min(n_items, s->grid);
+        if (check(tpz_launch_sw_forward(&a, grid, 32, sl.stream), "sw_forward")) return -1;
+    }
+    if (check(hipMemcpyAsync(sl.h_out.ptr, dout, out_words * 4, hipMemcpyDeviceToHost, sl.stream),
+              "readback") ||
+        check(hipEventRecord(sl.ev_done, sl.stream), "event"))
+        return -1;
+    return 0;
+}
+
+// Results of a finished slot back to its requests.  Called with the lock held.
+void retire_slot(tpz_server* s, Slot& sl, int launch_rc, const std::string& launch_err) {
+    const int32_t* out = (const int32_t*)sl.h_out.ptr;
+    const bool timed_out = launch_rc == 0 && sl.n_pairs && out[sl.error_word] != 0;
+    const double t = now_ms();
+    for (size_t k = 0; k < sl.reqs.size(); ++k) {
+        Request* r = sl.reqs[k];
+        const size_t n = r->pq.size(), b = sl.pair_base[k];
+        if (launch_rc) {
+            r->rc = -1;
+            r->err = launch_err;
+        } else if (timed_out) {
+            r->rc = -1;
+            r->err = "sw_forward: a cross-wavefront wait timed out";
+        } else {
+            r->score.assign(out + b, out + b + n);
+            r->ref.assign(out + sl.n_pairs + b, out + sl.n_pairs + b + n);
+            r->read.assign(out + 2 * sl.n_pairs + b, out + 2 * sl.n_pairs + b + n);
+        }
+        r->t_done = t;
+        r->done = true;
+        s->stats.requests_done += 1;
+        s->stats.pairs_done += n;
+    }
+    s->mem_in_use -= sl.bytes;
+    sl.reqs.clear();
+    sl.bytes = 0;
+    sl.busy = false;
+    s->cv_done.notify_all();
+}
+
+void scheduler(tpz_server* s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        // retire finished launches
+        bool any_busy = false;
+        for (auto& p : s->slots) {
+            Slot& sl = *p;
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.ev_done);
+            if (q == hipSuccess) retire_slot(s, sl, 0, "");
+            else if (q != hipErrorNotReady) retire_slot(s, sl, -1, hipGetErrorString(q));
+            else any_busy = true;
+        }
+        if (s->stop && !any_busy) break;
+        // admit into a free slot
+        Slot* free_slot = nullptr;
+        for (auto& p : s->slots)
+            if (!p->busy) { free_slot = p.get(); break; }
+        if (!s->stop && free_slot && !s->pending.empty()) {
+            Slot& sl = *free_slot;
+            size_t pairs = 0;
+            while (!s->pending.empty()) {
+                Request* r = s->pending.front();
+                if (s->mem_in_use + r->bytes > s->cfg.memory_budget) break;
+                if (!sl.reqs.empty() && pairs + r->pq.size() > (size_t)s->cfg.max_launch_pairs) break;
+                s->pending.pop_front();
+                sl.reqs.push_back(r);
+                sl.bytes += r->bytes;
+                s->mem_in_use += r->bytes;
+                pairs += r->pq.size();
+            }
+            if (!sl.reqs.empty()) {
+                sl.busy = true;
+                s->stats.launches += 1;
+                s->stats.max_requests_per_launch =
+                    std::max<uint64_t>(s->stats.max_requests_per_launch, sl.reqs.size());
+                s->stats.peak_bytes = std::max<uint64_t>(s->stats.peak_bytes, s->mem_in_use);
+                lk.unlock();
+                const int rc = launch_slot(s, sl);
+                const std::string err = rc ? std::string(tpz_last_error()) : std::string();
+                lk.lock();
+                if (rc) {
+                    (void)hipStreamSynchronize(sl.stream);
+                    retire_slot(s, sl, rc, err);
+                }
+                continue;
+            }
+        }
+        if (any_busy)
+            s->cv_sched.wait_for(lk, std::chrono::microseconds(50));
+        else
+            s->cv_sched.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+    }
+    // requests that never got a launch
+    for (Request* r : s->pending) {
+        r->rc = -1;
+        r->err = "server stopped";
+        r->done = true;
+    }
+    s->pending.clear();
+    s->cv_done.notify_all();
+}
+
+}  // namespace
+}  // namespace tpz
+
+extern "C" {
+
+int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
+                      const int64_t* s_off, const int32_t* s_len, const int8_t* matrix,
+                      int32_t gap_open, int32_t gap_extend, const tpz_server_config* cfg) {
+    if (!out) return fail("tpz_server_create: null output pointer");
+    *out = nullptr;
+    if (ensure_init()) return -1;
+    if (n_subjects < 0) return fail("negative subject count");
+    if (!(gap_extend >= 0 && gap_open > gap_extend && gap_open <= 255))
+        return fail("gap penalties must satisfy 0 <= gap_extend < gap_open <= 255 "
+                    "(got %d/%d)", gap_open, gap_extend);
+    if (!matrix) return fail("null matrix");
+    auto s = std::make_unique<tpz_server>();
+    s->n_subjects = n_subjects;
+    s->gap_open = gap_open;
+    s->gap_extend = gap_extend;
+    s->device = device_id();
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.memory_budget == 0) s->cfg.memory_budget = (uint64_t)1 << 30;
+    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
+    if (s->cfg.max_launch_pairs <= 0) s->cfg.max_launch_pairs = 4000000;
+    s->s_len.assign(s_len, s_len + n_subjects);
+    // subjects: each starts on a 4-byte boundary, padded with residue 25
+    std::vector<int64_t> so(n_subjects);
+    int64_t stot = 0;
+    for (int32_t k = 0; k < n_subjects; ++k) {
+        if (s_len[k] < 0) return fail("subject %d has negative length", k);
+        so[k] = stot;
+        stot += (s_len[k] + 3) & ~3;
+        s->max_subject = std::max(s->max_subject, s_len[k]);
+    }
+    std::vector<int8_t> sr((size_t)stot + 64, 25);
+    for (int32_t k = 0; k < n_subjects; ++k)
+        for (int32_t j = 0; j < s_len[k]; ++j) {
+            const int8_t r = s_res[s_off[k] + j];
+            if (r < 0 || r > 24) return fail("subject %d residue %d out of range", k, r);
+            sr[so[k] + j] = r;
+        }
+    int rc = s->d_s_res.upload(sr.data(), sr.size()) | s->d_s_off.upload(so.data(), so.size() * 8) |
+             s->d_s_len.upload(s_len, (size_t)n_subjects * 4) | s->d_matrix.upload(matrix, 625);
+    s->grid = std::max(num_cu(), 1) * kGroupsPerCu;
+    s->ws_words_per_slot = (int64_t)kGroupWaves * 2 * (s->max_subject + 2) * 64;
+    for (int k = 0; k < s->cfg.max_inflight && !rc; ++k) {
+        auto sl = std::make_unique<Slot>();
+        rc |= check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");
+        rc |= check(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming), "hipEventCreate");
+        rc |= sl->d_ws.alloc((size_t)s->grid * s->ws_words_per_slot * 4);
+        s->stats.workspace_bytes += (size_t)s->grid * s->ws_words_per_slot * 4;
+        s->slots.push_back(std::move(sl));
+    }
+    rc |= check(hipStreamSynchronize(stream()), "upload sync");
+    if (rc) {
+        for (auto& sl : s->slots) {
+            if (sl->ev_done) (void)hipEventDestroy(sl->ev_done);
+            if (sl->stream) (void)hipStreamDestroy(sl->stream);
+        }
+        return -1;
+    }
+    tpz_server* raw = s.release();
+    raw->thread = std::thread(tpz::scheduler, raw);
+    *out = raw;
+    return 0;
+}
+
+int tpz_server_submit(tpz_server* s, int32_t n_queries, const int8_t* q_res,
+                      const int64_t* q_off, const int32_t* q_len, int32_t n_pairs,
+                      const int32_t* pair_query, const int32_t* pair_subject, uint64_t* ticket) {
+    if (!s || !ticket) return fail("null argument");
+    if (n_queries < 0 || n_pairs < 0) return fail("negative counts");
+    auto r = std::make_unique<Request>();
+    size_t tot = 0;
+    for (int32_t k = 0; k < n_queries; ++k) {
+        if (q_len[k] < 0) return fail("query %d has negative length", k);
+        tot += (size_t)q_len[k];
+    }
+    r->q_res.resize(tot);
+    r->q_off.resize(n_queries);
+    r->q_len.assign(q_len, q_len + n_queries);
+    size_t o = 0;
+    for (int32_t k = 0; k < n_queries; ++k) {
+        for (int32_t j = 0; j < q_len[k]; ++j) {
+            const int8_t v = q_res[q_off[k] + j];
+            if (v < 0 || v > 24) return fail("query %d residue %d out of range", k, v);
+            r->q_res[o + j] = v;
+        }
+        r->q_off[k] = (int64_t)o;
+        o += (size_t)q_len[k];
+    }
+    for (int32_t p = 0; p < n_pairs; ++p)
+        if (pair_query[p] < 0 || pair_query[p] >= n_queries || pair_subject[p] < 0 ||
+            pair_subject[p] >= s->n_subjects)
+            return fail("pair %d references query %d / subject %d out of range", p,
+                        pair_query[p], pair_subject[p]);
+    r->pq.assign(pair_query, pair_query + n_pairs);
+    r->ps.assign(pair_subject, pair_subject + n_pairs);
+    r->bytes = request_bytes(tot, (size_t)n_queries, (size_t)n_pairs);
+    if (r->bytes > s->cfg.memory_budget)
+        return fail("request needs %zu device bytes, more than the budget of %llu", r->bytes,
+                    (unsigned long long)s->cfg.memory_budget);
+    {
+        const auto per = pairs_by_query(r->pq, r->ps, s->s_len, n_queries);
+        build_packed_tasks(per, r->ps, r->q_len, s->s_len, 32, kTaskQueries, r->tasks);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return fail("server stopped");
+    r->id = s->next_id++;
+    r->t_submit = now_ms();
+    *ticket = r->id;
+    s->reqs[r->id] = r.get();
+    s->pending.push_back(r.release());
+    s->cv_sched.notify_one();
+    return 0;
+}
+
+int tpz_server_wait(tpz_server* s, uint64_t ticket, int32_t* score, int32_t* ref_end,
+                    int32_t* read_end, double* latency_ms) {
+    if (!s) return fail("null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail("unknown ticket %llu", (unsigned long long)ticket);
+    Request* r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->reqs.erase(it);
+    lk.unlock();
+    std::unique_ptr<Request> own(r);
+    if (r->rc) return fail("%s", r->err.c_str());
+    const size_t n = r->score.size();
+    if (score && n) memcpy(score, r->score.data(), n * 4);
+    if (ref_end && n) memcpy(ref_end, r->ref.data(), n * 4);
+    if (read_end && n) memcpy(read_end, r->read.data(), n * 4);
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    return 0;
+}
+
+int tpz_server_get_stats(tpz_server* s, tpz_server_stats* st) {
+    if (!s || !st) return fail("null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->stats;
+    return 0;
+}
+
+void tpz_server_destroy(tpz_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_sched.notify_all();
+    }
+    if (s->thread.joinable()) s->thread.join();
+    for (auto& kv : s->reqs) delete kv.second;
+    for (auto& sl : s->slots) {
+        if (sl->ev_done) (void)hipEventDestroy(sl->ev_done);
+        if (sl->stream) (void)hipStreamDestroy(sl->stream);
+    }
+    delete s;
+}
+
+}  // extern "C"
