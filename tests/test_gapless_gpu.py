@@ -93,3 +93,34 @@ def test_out_of_text_positions_are_not_seeds():
     b = GaplessBatch(Text(T), [b"ABCDE"], seeds, matrix()).run()
     sc, ln = b.results()
     assert list(ln[:3]) == [-1, -1, -1] and ln[3] != -1
+
+
+@pytest.mark.parametrize("text_len", [60, 5000])
+def test_seeds_at_the_text_start(text_len):
+    """Left scans that end at T[1] (i < idx) and Q[1] (i < o): every text
+    position below 140 against every query offset below 150, first residues
+    matching; a text shorter than one window too."""
+    rng = np.random.default_rng(55 + text_len)
+    body = _internal(synth.random_residues(rng, text_len))
+    T = bytearray(body)
+    for k in rng.integers(5, len(T) - 1, len(T) // 300 + 1):
+        T[k] = 10
+    T = bytes(T) + b"\n"
+    qs = [_internal(synth.random_residues(rng, int(L))) for L in (160, 200, 3, 1)]
+    recs = []
+    for idx in range(min(140, len(T))):
+        for q, Q in enumerate(qs):
+            for o in range(min(150, len(Q))):
+                if Q[o] == T[idx]:
+                    recs.append((q, o, idx))
+    seeds = np.zeros(len(recs), SEED_DTYPE)
+    seeds["query"] = [r[0] for r in recs]
+    seeds["offset"] = [r[1] for r in recs]
+    seeds["position"] = [r[2] for r in recs]
+    assert len(seeds) > 200
+    b = GaplessBatch(Text(T), qs, seeds, matrix()).run()
+    sc, ln = b.results()
+    o = Oracle()
+    for k, (q, off, idx) in enumerate(recs):
+        es, el = o.sa_hsp(T, idx, qs[q], off)
+        assert (int(ln[k]), int(sc[k]) if el != -1 else None) == (el, es), (q, off, idx)

@@ -452,7 +452,7 @@ int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
     hip_or_die(hipMemsetAsync(dout, 0, 8, c.st), "memset");
     GaplessArgs a{};
     a.t_fwd = (const uint8_t*)rt.text->d_fwd.ptr;
-    a.t_rev = (const uint8_t*)rt.text->d_rev.ptr;
+    a.t_head = (const uint8_t*)rt.text->d_head.ptr;
     a.t_size = (int64_t)db->Tsize;
     a.sa = rt.text->has_sa ? (const int64_t*)rt.text->d_sa.ptr : nullptr;
     a.q_fwd = (const uint8_t*)(d + o_qf);

@@ -30,10 +30,13 @@ constexpr int kTableCols = 128;
 constexpr int kTermScore = -30000;   // past a terminator no prefix sum is >= 0 again
 constexpr int kWindow = 128;         // sa_hsp's window on each side (search.c:99-190)
 constexpr int kTextPad = 256;        // terminator bytes after each code array
+constexpr int kHeadBytes = 2 * kWindow + kTextPad;
 
 struct GaplessArgs {
     const uint8_t* t_fwd;       // text codes, t_size bytes then kTextPad x kTextTerm
-    const uint8_t* t_rev;       // t_rev[j] = code of T[t_size-1-j], with T[0]'s slot kTextTerm
+    const uint8_t* t_head;      // kWindow x kTextTerm, then the codes of T[0 .. kWindow) with
+                                // T[0] as kTextTerm, then terminators: the left window of
+                                // idx < kWindow
     int64_t t_size;
     const int64_t* sa;          // may be null: positions are text positions
     const uint8_t* q_fwd;       // per query: doubled codes of q[0..m-1], then 2 * kQueryTerm

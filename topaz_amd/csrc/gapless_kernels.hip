@@ -13,26 +13,30 @@
 // Text residues map to matrix rows through aa_table2; query residues through
 // aa_table2 of upper_aa_only (lexicographical.h:19-74).
 //
-// Both sides become the same forward scan over 128 positions.  The host keeps
-// the text as letter codes twice, forward and reversed, and the queries the
-// same way; the left side of a seed is the forward scan of the reversed
-// copies from T[idx-1] and Q[o-1].  Every bound of the rule is a terminator
-// in the data: '\n' and the text's ends in the text copies, the query's end
-// in the forward query copy, and Q[0] and T[0] in the reversed copies (the
-// left scan never reaches index 0: i < min(idx, o)).  A terminator scores
-// -30000, after which no prefix sum is >= 0 again, so a scan simply runs its
-// 128 positions; only the left side's 128th position (i = 128, outside
-// i < 128) is dropped explicitly.
+// Both sides are the same scan over 128 positions.  The host keeps the text
+// as letter codes, and the queries as codes twice, forward and reversed.  The
+// right side reads T[idx..] and Q[o..]; the left side reads the same text
+// window backwards (T[idx-1], T[idx-2], ...) and the reversed query copy
+// forwards from Q[o-1].  Every bound of the rule is a terminator in the data:
+// '\n' and the text's end in the text, the query's end in the forward query
+// copy, Q[0] in the reversed copy, and T[0] in a small head window that
+// replaces the text for seeds with idx < 128 (the left scan never reaches
+// index 0: i < min(idx, o)).  A terminator scores -30000, after which no
+// prefix sum is >= 0 again, so a scan simply runs its 128 positions; only
+// the left side's 128th position (i = 128, outside i < 128) is dropped
+// explicitly.  The two lanes of a seed read adjacent text, 260 bytes in all.
 //
 // Mapping: phase 1 gives one lane per seed: it loads the seed (16 B,
 // coalesced) and tests the first residue; about one uniform seed in
 // seventeen passes.  The passing seeds of the wavefront are compacted in LDS
 // and phase 2 serves them 32 at a time, two lanes per seed: the even lane
 // scans the right side, the odd lane the left.  A lane loads its 132-byte
-// text and query windows with dword loads, aligns them with v_alignbyte, and
-// per position pair forms (text code << 8 | doubled query code) with one
-// v_perm: each 16-bit half is the byte address of W in the LDS table.  The
-// running maximum and its last position are one max over (sum << 8 | i).
+// text and query windows with dword loads and aligns them with v_alignbyte;
+// per position pair one v_perm forms (text code << 8 | doubled query code),
+// taking the text bytes in reverse order on the left lane (whose text dwords
+// are also taken last to first): each 16-bit half is the byte address of W in
+// the LDS table.  The running maximum and its last position are one max over
+// (sum << 8 | i).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -70,25 +74,30 @@ __device__ __forceinline__ void load_window(const uint8_t* p, uint32_t (&w)[kWin
 }
 
 // One side: the last maximum of the prefix sums over positions 0..127 as
-// (sum << 8 | position), or -1 when no prefix sum is >= 0.  last_ok: whether
-// position 127 counts.
-__device__ __forceinline__ int scan_side(const uint8_t* t, const uint8_t* q, bool last_ok,
+// (sum << 8 | position), or -1 when no prefix sum is >= 0.  t: the 128-byte
+// text window, read forwards (right side) or backwards from its end (left
+// side); q: the query codes in scan order.
+__device__ __forceinline__ int scan_side(const uint8_t* t, const uint8_t* q, bool left,
                                          const GaplessLds& lds) {
     uint32_t tw[kWindowDwords], qw[kWindowDwords];
     load_window(t, tw);
     load_window(q, qw);
     const char* tab = reinterpret_cast<const char*>(lds.table);
+    // perm selectors (bytes 0-3: query dword, 4-7: text dword) giving
+    // q(2h), t, q(2h+1), t with the text bytes ascending or descending
+    const uint32_t sel0 = left ? 0x06010700u : 0x05010400u;
+    const uint32_t sel1 = left ? 0x04030502u : 0x07030602u;
     int sum = 0, key = -1;
 #pragma unroll
     for (int k = 0; k < kWindowDwords; ++k) {
+        const uint32_t td = left ? tw[kWindowDwords - 1 - k] : tw[k];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            // bytes: q(2h), t(2h), q(2h+1), t(2h+1) -> two (t << 8 | 2q) addresses
-            const uint32_t x = __builtin_amdgcn_perm(tw[k], qw[k], h ? 0x07030602u : 0x05010400u);
+            const uint32_t x = __builtin_amdgcn_perm(td, qw[k], h ? sel1 : sel0);
             const int p = 4 * k + 2 * h;
             int w0 = *reinterpret_cast<const int16_t*>(tab + (x & 0xffffu));
             int w1 = *reinterpret_cast<const int16_t*>(tab + (x >> 16));
-            if (p + 1 == kWindow - 1 && !last_ok) w1 = kTermScore;
+            if (p + 1 == kWindow - 1 && left) w1 = kTermScore;
             sum += w0;
             key = max(key, (int)(((uint32_t)sum << 8) | (uint32_t)p));
             sum += w1;
@@ -144,10 +153,14 @@ gapless_kernel(GaplessArgs a) {
             if (active) {
                 s = slots[j];
                 const int m = a.q_len[s.query];
-                const uint8_t* t = left ? a.t_rev + (a.t_size - s.idx) : a.t_fwd + s.idx;
+                // left: the 128 text bytes before idx (the head window when
+                // idx < 128), scanned from the end
+                const uint8_t* t = !left ? a.t_fwd + s.idx
+                                  : s.idx >= kWindow ? a.t_fwd + (s.idx - kWindow)
+                                                     : a.t_head + s.idx;
                 const uint8_t* q = left ? a.q_rev + a.qr_off[s.query] + (m - s.offset)
                                         : a.q_fwd + a.qf_off[s.query] + s.offset;
-                key = scan_side(t, q, !left, lds);
+                key = scan_side(t, q, left, lds);
             }
             const int other = __shfl_xor(key, 1);
             if (active && !left) {

@@ -104,7 +104,7 @@ struct tpz_text {
     uint64_t t_size = 0;
     bool has_sa = false;
     tpz::TextCodes codes;
-    tpz::DeviceBuffer d_fwd, d_rev, d_sa;
+    tpz::DeviceBuffer d_fwd, d_head, d_sa;
 };
 
 #endif

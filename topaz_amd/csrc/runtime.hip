@@ -734,18 +734,17 @@ int tpz_text_create(tpz_text** out, const char* text, uint64_t t_size, const int
     t->has_sa = sa != nullptr;
     const uint8_t* T = reinterpret_cast<const uint8_t*>(text);
     if (text_codes(T, t_size, t->codes)) { delete t; return -1; }
-    // forward and reversed code copies, each followed by terminators; the
-    // reversed copy ends with T[0]'s slot set to the terminator
+    // the code copy followed by terminators, and the head window: the left
+    // scan stops before T[0] (i < idx), so there T[0] is a terminator
     std::vector<uint8_t> buf(t_size + kTextPad);
     const uint8_t* code = t->codes.code;
     int rc = 0;
     for (uint64_t i = 0; i < t_size; ++i) buf[i] = code[T[i]];
     memset(buf.data() + t_size, kTextTerm, kTextPad);
     rc |= t->d_fwd.upload(buf.data(), buf.size());
-    rc |= check(hipStreamSynchronize(stream()), "text upload");
-    for (uint64_t j = 0; j + 1 < t_size; ++j) buf[j] = code[T[t_size - 1 - j]];
-    if (t_size) buf[t_size - 1] = kTextTerm;
-    if (!rc) rc |= t->d_rev.upload(buf.data(), buf.size());
+    std::vector<uint8_t> head(kHeadBytes, kTextTerm);
+    for (uint64_t i = 1; i < t_size && i < (uint64_t)kWindow; ++i) head[kWindow + i] = buf[i];
+    rc |= t->d_head.upload(head.data(), head.size());
     if (!rc && sa) rc |= t->d_sa.upload(sa, t_size * 8);
     if (!rc) rc |= check(hipStreamSynchronize(stream()), "text upload");
     if (rc) { delete t; return -1; }
@@ -809,7 +808,7 @@ int tpz_gapless_batch_run(tpz_gapless_batch* b) {
     if (!b) return fail("null batch");
     GaplessArgs a{};
     a.t_fwd = (const uint8_t*)b->text->d_fwd.ptr;
-    a.t_rev = (const uint8_t*)b->text->d_rev.ptr;
+    a.t_head = (const uint8_t*)b->text->d_head.ptr;
     a.t_size = (int64_t)b->text->t_size;
     a.sa = b->text->has_sa ? (const int64_t*)b->text->d_sa.ptr : nullptr;
     a.q_fwd = (const uint8_t*)b->d_qf.ptr;
