@@ -38,6 +38,19 @@ VALU_PEAK_TOPS = 78.6      # 256 CU x 64 lanes x 2 (packed int16) x 2.4 GHz
 OPS_PER_CELL = 10          # integer ops per DP cell in the reference inner loop
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel at its bench workload, as measured
+    with rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy of
+    known size) and recorded in profiles/hbm_traffic.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block).get(key)
+    except (OSError, IndexError, ValueError):
+        return None
+
+
 def make_workload(scale, seed=2024):
     from topaz_amd import synth
     rng = np.random.default_rng(seed)
@@ -207,7 +220,9 @@ def gapless_leg(args):
                      "seeds_per_s": round(n_seeds / kernel_s, 1),
                      "roofline": {"bound": "hbm", "achieved": round(gbs, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}}
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                  "traffic": measured_traffic("gapless_%s_50M" % name)
+                                  if args.gapless_scale == 1.0 else None}}
         print("[bench] gapless %s: kernel %.3f ms, %.1f GB/s"
               % (name, kernel_s * 1e3, gbs), file=sys.stderr, flush=True)
         if name == "matched" and not args.no_cpu:
@@ -305,7 +320,8 @@ def sw_leg(args):
         "speedup_vs_plain": round(plain["ms_per_step"] / (t_step * 1e3), 2) if plain else None,
         "roofline": {"bound": "valu", "achieved": round(kernel_tops, 2), "peak": VALU_PEAK_TOPS,
                      "unit": "Tint16op/s", "frac": round(kernel_tops / VALU_PEAK_TOPS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic("sw_forward_2Mpairs")
+                     if args.pairs_scale == 1.0 else None},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)

@@ -17,8 +17,14 @@
  *        pairs (committed fixtures), plus brute-force textbook Gotoh bounds.
  *   orc_ssw_align_stats / orc_banded_sw -- against the compiled reference
  *        (fixtures) on reported-hit-like pairs.
+ *        The shared-buffer replay of banded_sw is also stress-checked against
+ *        the compiled reference (stats of 4,200 random and homologous pairs).
+ *   orc_sw_textbook -- brute-force Gotoh in numpy on small pairs.
  *   orc_sa_hsp -- against compiled sa_hsp (fixtures) and brute force.
+ *   orc_sa_hsp_batch -- equal to orc_sa_hsp seed by seed.
+ *   orc_sa_gapless -- brute force, and the compiled reference's sa_gapless.
  *   orc_evalue -- closed form, against compiled align_using_ssw_profile.
+ *   orc_aa_table2 / orc_upper_aa_only -- against the committed tables.
  */
 #ifndef TOPAZ_ORACLE_ORC_H
 #define TOPAZ_ORACLE_ORC_H

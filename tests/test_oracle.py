@@ -1,6 +1,7 @@
 """The oracle, pinned: against the reference's own outputs (golden fixtures
 written by tests/golden/gen_golden.py from the compiled reference), against
 brute force on tiny inputs, and against closed forms.  CPU only."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -287,3 +288,118 @@ def test_gapless_batch_and_scan_bytes(orc):
             if T[idx - i] == 10:
                 break
         assert (right[k], left[k]) == (r, lft), k
+
+
+def _gapless_cases(rng, n_text=60, n_q=30):
+    letters = np.frombuffer(b"ABCDEFGHIJKLMNOPQRSTUVWXY", np.uint8)
+    T = b"".join(letters[rng.integers(0, 25, int(L))].tobytes() + b"\n"
+                 for L in rng.integers(1, 200, n_text))
+    qs = [letters[rng.integers(0, 25, int(L))].tobytes() for L in rng.integers(1, 150, n_q)]
+    cases = []
+    Tn = np.frombuffer(T, np.uint8)
+    for _ in range(3000):
+        q = int(rng.integers(0, n_q))
+        o = int(rng.integers(0, len(qs[q])))
+        if rng.random() < 0.7:
+            cand = np.nonzero(Tn == qs[q][o])[0]
+            p = int(cand[int(rng.integers(0, len(cand)))]) if len(cand) else 0
+        else:
+            p = int(rng.integers(0, len(T)))
+        cases.append((q, o, p))
+    return T, qs, cases
+
+
+def test_sa_gapless_brute_force(orc):
+    """The dead variant sa_gapless (search.c:48-97): right-only prefix
+    maximum, at least 3 positions, first residues equal."""
+    rng = np.random.default_rng(61)
+    T, qs, cases = _gapless_cases(rng)
+    mat = matrix()
+    t2 = aa_table2()
+    seen = 0
+    for q, o, p in cases:
+        Q = qs[q]
+        sc = C.c_int32(0)
+        ln = orc.L.orc_sa_gapless(T, len(T), p, Q, len(Q), o,
+                                  mat.ctypes.data_as(C.POINTER(C.c_int8)), C.byref(sc))
+        lim = min(len(T) - p, len(Q) - o)
+        if lim < 3 or T[p] != Q[o]:
+            assert ln == -1
+            continue
+        best = arg = s = 0
+        for i in range(lim):
+            if T[p + i] == 10:
+                break
+            s += int(mat[int(t2[T[p + i]]) * 25 + int(t2[Q[o + i]])])
+            if s >= best:
+                best, arg = s, i
+        assert (ln, sc.value) == (arg, best)
+        seen += 1
+    assert seen > 1000
+
+
+@pytest.mark.skipif(not REF_SO.exists(), reason="reference checker not built here")
+def test_sa_gapless_vs_compiled_reference(orc):
+    from tests.refcheck import DbT, OptionsT, SeqT
+    lib = C.CDLL(str(REF_SO))
+    lib.sa_gapless.restype = C.c_int
+    lib.sa_gapless.argtypes = [C.POINTER(DbT), C.POINTER(SeqT), C.c_int, C.c_int64,
+                               C.POINTER(OptionsT), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    rng = np.random.default_rng(62)
+    T, qs, cases = _gapless_cases(rng)
+    SA = np.arange(len(T), dtype=np.int64)
+    db = DbT()
+    db.T = T
+    db.Tsize = len(T)
+    db.SA = SA.ctypes.data_as(C.POINTER(C.c_int64))
+    db.SAsize = len(T) * 8
+    mat = matrix()
+    mat_c = (C.c_int8 * 625)(*mat.tolist())
+    opt = OptionsT()
+    opt.substitution = C.cast(mat_c, C.c_void_p)
+    for q, o, p in cases:
+        seq = SeqT()
+        seq.seq_buf.buf = qs[q]
+        seq.seq_buf.len = len(qs[q])
+        seq.seq_len = len(qs[q])
+        rs, rl = C.c_int(-7), C.c_int(0)
+        lib.sa_gapless(C.byref(db), C.byref(seq), o, p, C.byref(opt), C.byref(rs), C.byref(rl))
+        sc = C.c_int32(-7)
+        ln = orc.L.orc_sa_gapless(T, len(T), p, qs[q], len(qs[q]), o,
+                                  mat.ctypes.data_as(C.POINTER(C.c_int8)), C.byref(sc))
+        assert ln == rl.value and (ln == -1 or sc.value == rs.value), (q, o, p)
+
+
+@pytest.mark.skipif(not REF_SO.exists(), reason="reference checker not built here")
+def test_stats_vs_compiled_reference_homologs(orc):
+    """ssw_align_stats (reverse pass + banded_sw with its buffers shared
+    across band passes) against the compiled reference on homologous pairs
+    with indels, embedded fragments and random pairs."""
+    from topaz_amd import synth
+    r = Reference()
+    rng = np.random.default_rng(63)
+    n = 0
+    for it in range(3000):
+        m = int(rng.integers(5, 300))
+        q = synth.random_residues(rng, m).astype(np.int8)
+        kind = it % 3
+        if kind < 2:
+            res, _, _ = synth.mutate(rng, q.astype(np.int64), np.array([0]), np.array([m]),
+                                     rng.uniform(0.2, 0.95, 1), indel_rate=0.06)
+            s = res.astype(np.int8)
+            if kind == 1:
+                s = np.concatenate([synth.random_residues(rng, int(rng.integers(0, 200))), s,
+                                    synth.random_residues(rng, int(rng.integers(0, 200)))]
+                                   ).astype(np.int8)
+        else:
+            s = synth.random_residues(rng, int(rng.integers(5, 300))).astype(np.int8)
+        go, ge = [(11, 1), (10, 2), (5, 3)][it % 3]
+        a = r.ssw_align(q, s, go, ge)
+        if a is None or a[0] <= 0:
+            continue
+        sc, re_, rd = a
+        x = orc.ssw_align_stats(q, s, rd, re_, sc, go, ge)
+        y = r.ssw_align_stats(q, s, rd, re_, sc, go, ge)
+        assert x[:6] == y[:6] and (x[6] == y[6] or (math.isnan(x[6]) and math.isnan(y[6]))), it
+        n += 1
+    assert n > 2500
