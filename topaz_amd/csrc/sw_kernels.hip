@@ -28,8 +28,10 @@
 // the subject; a chunk's bottom row (H, F) is written per column to a
 // per-wave workspace that the next chunk reads back in place.  The profile
 // of the chunk (32 rows x 26 residues) of each of the task's queries sits in
-// LDS and every lane half reads the one of its own query, so packing pairs of
-// several queries into a wavefront costs nothing in the inner loop.  Rows past
+// LDS, residue-major: per column a lane half reads the 32 scores of its
+// subject residue in its own query's profile with four 16-byte loads, and one
+// v_perm per row packs the two halves' scores.  Packing pairs of several
+// queries into a wavefront therefore costs nothing in the inner loop.  Rows past
 // a query's end score -8192: a value there stays below a cell of the same
 // chunk already counted in the running maximum, so it is never a maximum.
 //
@@ -50,7 +52,6 @@ typedef unsigned short u2 __attribute__((ext_vector_type(2)));
 
 constexpr int kNRes = 26;          // 25 matrix residues + 1 padding residue
 constexpr int kQ = 4;              // queries per task
-constexpr int kRowStride = kQ * kNRes;   // profile row r of query k at r * 104 + k * 26
 constexpr int kPadResidue = 25;
 constexpr int kPadScore = -8192;   // score of padding rows / columns
 
@@ -65,14 +66,40 @@ __device__ __forceinline__ s2 from_bits(uint32_t b) { return __builtin_bit_cast(
 
 // LDS of one wavefront.
 // R (16 or 32) is the number of query rows per chunk, held in registers.
+// The profile is residue-major: W(s, q_k[j0 + r]) at [(k * 26 + s) * (R + 8)
+// + r], so a lane reads the R scores of its subject residue with R / 8
+// 16-byte loads.  The 16-byte pad makes the row stride 20 dwords (R = 32) or
+// 12 (R = 16): sixteen different residues then start on sixteen different
+// groups of four banks.
 template <int R>
 struct WaveLds {
-    // W(s, q_k[j0 + r]) at [r * 104 + k * 26 + s]: the four queries' entries
-    // of a row are adjacent, so the lanes of one read spread over 52
-    // consecutive dwords of LDS
-    int16_t prof[kQ * R * kNRes];
+    static constexpr int kStride = R + 8;
+    alignas(16) int16_t prof[kQ * kNRes * kStride];
     uint16_t saved[2][R][64];              // H column of the chunk best, per half
 };
+
+// The R profile scores of one residue: rows 2i and 2i+1 in dword i.
+template <int R>
+struct ProfCol {
+    uint32_t d[R / 2];
+};
+
+template <int R>
+__device__ __forceinline__ void load_col(ProfCol<R>& c, const int16_t* p) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int i = 0; i < R / 8; ++i) {
+        const uint4 v = q[i];
+        c.d[4 * i] = v.x; c.d[4 * i + 1] = v.y; c.d[4 * i + 2] = v.z; c.d[4 * i + 3] = v.w;
+    }
+}
+
+// Row r of the low-half and high-half columns as one packed pair.
+template <int R>
+__device__ __forceinline__ s2 row_w(const ProfCol<R>& lo, const ProfCol<R>& hi, int r) {
+    return from_bits(__builtin_amdgcn_perm(hi.d[r / 2], lo.d[r / 2],
+                                           (r & 1) ? 0x07060302u : 0x05040100u));
+}
 
 // Best of the chunks finished so far, per half of the lane; chunk is the
 // chunk index of that best (ties on score and column go to the earlier one).
@@ -211,10 +238,12 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             res_lo = load4(s_lo, i + 2, n_lo);
             res_hi = load4(s_hi, i + 2, n_hi);
         }
-        const int16_t* pa_lo = lds.prof + off_lo + ra_lo;
-        const int16_t* pa_hi = lds.prof + off_hi + ra_hi;
-        const int16_t* pb_lo = lds.prof + off_lo + rb_lo;
-        const int16_t* pb_hi = lds.prof + off_hi + rb_hi;
+        constexpr int kS = WaveLds<R>::kStride;
+        ProfCol<R> pa_lo, pa_hi, pb_lo, pb_hi;
+        load_col<R>(pa_lo, lds.prof + (off_lo + ra_lo) * kS);
+        load_col<R>(pa_hi, lds.prof + (off_hi + ra_hi) * kS);
+        load_col<R>(pb_lo, lds.prof + (off_lo + rb_lo) * kS);
+        load_col<R>(pb_hi, lds.prof + (off_hi + rb_hi) * kS);
 
         s2 fa = from_bits(ta_f), fb = from_bits(tb_f);
         s2 da = hdiag;                  // H(i-1, j0-1)
@@ -224,7 +253,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
 
         // column a, row 0
         {
-            s2 w = s2{pa_lo[0], pa_hi[0]};
+            s2 w = row_w<R>(pa_lo, pa_hi, 0);
             s2 h = vmax(vmax(add_sat(da, w), E[0]), fa);
             da = H[0];
             Ha[0] = h;
@@ -237,7 +266,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
         for (int r = 1; r < R; ++r) {
             // column a, row r
             {
-                s2 w = s2{pa_lo[r * kRowStride], pa_hi[r * kRowStride]};
+                s2 w = row_w<R>(pa_lo, pa_hi, r);
                 s2 h = vmax(vmax(add_sat(da, w), E[r]), fa);
                 da = H[r];
                 Ha[r] = h;
@@ -249,7 +278,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             // column b, row r - 1
             {
                 const int q = r - 1;
-                s2 w = s2{pb_lo[q * kRowStride], pb_hi[q * kRowStride]};
+                s2 w = row_w<R>(pb_lo, pb_hi, q);
                 s2 h = vmax(vmax(add_sat(db, w), E[q]), fb);
                 db = Ha[q];
                 H[q] = h;
@@ -262,7 +291,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
         // column b, last row
         {
             const int q = R - 1;
-            s2 w = s2{pb_lo[q * kRowStride], pb_hi[q * kRowStride]};
+            s2 w = row_w<R>(pb_lo, pb_hi, q);
             s2 h = vmax(vmax(add_sat(db, w), E[q]), fb);
             H[q] = h;
             cmb = vmax(cmb, h);
@@ -335,11 +364,11 @@ __device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds<R>&
         if (!q) break;
         const int m = t.ms[k];
         for (int e = lane; e < R * kNRes; e += 64) {
-            const int r = e / kNRes, s = e - r * kNRes;
+            const int s = e / R, r = e - s * R;
             const int j = j0 + r;
             int w = kPadScore;
             if (s < kPadResidue && j < m) w = a.matrix[s * 25 + q[j]];
-            lds.prof[r * kRowStride + k * kNRes + s] = (int16_t)w;
+            lds.prof[(k * kNRes + s) * WaveLds<R>::kStride + r] = (int16_t)w;
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
