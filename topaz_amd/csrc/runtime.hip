@@ -274,7 +274,8 @@ static tpz_sw_config default_config() {
 static int check_config(const tpz_sw_config& c) {
     if (c.rows_per_chunk != 16 && c.rows_per_chunk != 32)
         return fail("rows_per_chunk must be 16 or 32 (got %d)", c.rows_per_chunk);
-    const int max_groups = c.rows_per_chunk == 16 ? 3 : 2;    // VGPR and LDS occupancy
+    // resident workgroups per CU allowed by the kernel's registers and LDS
+    const int max_groups = std::max(1, tpz_sw_forward_groups_per_cu(c.rows_per_chunk));
     if (c.groups_per_cu < 1 || c.groups_per_cu > max_groups)
         return fail("groups_per_cu must be 1..%d for %d rows (got %d)", max_groups,
                     c.rows_per_chunk, c.groups_per_cu);
@@ -560,16 +561,18 @@ int tpz_sw_batch_autotune(tpz_sw_batch* b, tpz_sw_config* best, double* best_ms,
     if (!b) return fail("null batch");
     const tpz_sw_config start = b->plan->cfg;
     std::vector<tpz_sw_config> cands;
-    for (int rows : {32, 16})
-        for (int groups = 1; groups <= (rows == 16 ? 3 : 2); ++groups)
+    for (int rows : {32, 16}) {
+        const int max_groups = std::max(1, tpz_sw_forward_groups_per_cu(rows));
+        for (int groups = 1; groups <= max_groups; ++groups)
             for (int nq : {4, 2, 1})
                 for (double bf : {0.25, 0.5, 1.0, 1e30}) {
-                    if ((nq != 4 || groups != (rows == 16 ? 3 : 2)) && bf != 0.5) continue;
+                    if ((nq != 4 || groups != max_groups) && bf != 0.5) continue;
                     tpz_sw_config c = start;
                     c.rows_per_chunk = rows; c.groups_per_cu = groups;
                     c.queries_per_task = nq; c.big_fraction = bf; c.use_graph = 0;
                     cands.push_back(c);
                 }
+    }
     const size_t n = (size_t)b->n_pairs;
     std::vector<int32_t> ref_sc(n), ref_re(n), ref_rd(n), sc(n), re(n), rd(n);
     double best_t = 1e300;

@@ -160,6 +160,18 @@ __device__ __forceinline__ void publish(const ChunkIO& io, int value) {
         __hip_atomic_store(io.pub_ctr, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+template <int R>
+__device__ __forceinline__ s2 tree_max(const s2 (&v)[R]) {
+    s2 t[R / 2];
+#pragma unroll
+    for (int r = 0; r < R / 2; ++r) t[r] = vmax(v[2 * r], v[2 * r + 1]);
+#pragma unroll
+    for (int n = R / 4; n >= 1; n /= 2)
+#pragma unroll
+        for (int r = 0; r < n; ++r) t[r] = vmax(t[2 * r], t[2 * r + 1]);
+    return t[0];
+}
+
 // Event bookkeeping for one finished column: if a half's column max rises
 // strictly above the chunk best, remember the column and save that half of
 // the column's H values.
@@ -249,7 +261,6 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
         s2 da = hdiag;                  // H(i-1, j0-1)
         s2 db = from_bits(ta_h);        // H(i, j0-1)
         hdiag = from_bits(tb_h);        // H(i+1, j0-1)
-        s2 cma = zero, cmb = zero;
 
         // column a, row 0
         {
@@ -257,7 +268,6 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             s2 h = vmax(vmax(add_sat(da, w), E[0]), fa);
             da = H[0];
             Ha[0] = h;
-            cma = h;
             s2 hg = usub(h, go);
             E[0] = vmax(usub(E[0], ge), hg);
             fa = vmax(usub(fa, ge), hg);
@@ -270,7 +280,6 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
                 s2 h = vmax(vmax(add_sat(da, w), E[r]), fa);
                 da = H[r];
                 Ha[r] = h;
-                cma = vmax(cma, h);
                 s2 hg = usub(h, go);
                 E[r] = vmax(usub(E[r], ge), hg);
                 fa = vmax(usub(fa, ge), hg);
@@ -282,7 +291,6 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
                 s2 h = vmax(vmax(add_sat(db, w), E[q]), fb);
                 db = Ha[q];
                 H[q] = h;
-                cmb = vmax(cmb, h);
                 s2 hg = usub(h, go);
                 E[q] = vmax(usub(E[q], ge), hg);
                 fb = vmax(usub(fb, ge), hg);
@@ -294,7 +302,6 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             s2 w = row_w<R>(pb_lo, pb_hi, q);
             s2 h = vmax(vmax(add_sat(db, w), E[q]), fb);
             H[q] = h;
-            cmb = vmax(cmb, h);
             s2 hg = usub(h, go);
             E[q] = vmax(usub(E[q], ge), hg);
             fb = vmax(usub(fb, ge), hg);
@@ -305,6 +312,9 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             bH[(size_t)(i + 1) * 64 + lane] = bits(H[R - 1]);
             bF[(size_t)(i + 1) * 64 + lane] = bits(fb);
         }
+        // column maxima as balanced trees: independent maxes issue back to
+        // back, where a running max would be one dependent chain
+        const s2 cma = tree_max<R>(Ha), cmb = tree_max<R>(H);
         column_event(lds, Ha, cma, bc, i, ci_lo, ci_hi);
         column_event(lds, H, cmb, bc, i + 1, ci_lo, ci_hi);
         ta_h = na_h; ta_f = na_f; tb_h = nb_h; tb_f = nb_f;
@@ -582,6 +592,18 @@ extern "C" hipError_t tpz_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipS
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(tpz::fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
     return hipGetLastError();
+}
+
+extern "C" int tpz_sw_forward_groups_per_cu(int rows) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (rows == 16)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tpz::sw_forward_kernel<16>,
+                                                         tpz::kWaves * 64, 0);
+    else if (rows == 32)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tpz::sw_forward_kernel<32>,
+                                                         tpz::kWaves * 64, 0);
+    return e == hipSuccess ? n : 0;
 }
 
 extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid, int rows,
