@@ -359,27 +359,32 @@ struct TaskView {
     int ncols;               // even, wave-uniform
 };
 
-// Substitution scores of rows j0 .. j0+31 of every query of the task for all
-// 26 residues; rows past a query's end score kPadScore.  Written and read by
-// one wavefront only.
+// Substitution scores of rows j0 .. j0+R-1 of every query slot of the task
+// for all 26 residues; rows past a query's end (and empty slots) score
+// kPadScore.  Each lane fills whole rows: it loads its (slot, row)'s query
+// residue once and copies that residue's column of the LDS matrix.  Written
+// and read by one wavefront only.
 template <int R>
-__device__ __forceinline__ void build_profile(const SwKernelArgs& a, WaveLds<R>& lds,
+__device__ __forceinline__ void build_profile(const int8_t* mat, WaveLds<R>& lds,
                                               const TaskView& t, int j0) {
+    constexpr int kS = WaveLds<R>::kStride;
     const int lane = threadIdx.x % 64;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int k = 0; k < kQ; ++k) {
-        const int8_t* q = t.qs[k];
-        if (!q) break;
-        const int m = t.ms[k];
-        for (int e = lane; e < R * kNRes; e += 64) {
-            const int s = e / R, r = e - s * R;
-            const int j = j0 + r;
-            int w = kPadScore;
-            if (s < kPadResidue && j < m) w = a.matrix[s * 25 + q[j]];
-            lds.prof[(k * kNRes + s) * WaveLds<R>::kStride + r] = (int16_t)w;
-        }
+    for (int base = 0; base < kQ * R; base += 64) {
+        const int idx = base + lane;
+        const int k = idx / R, r = idx % R;
+        const int8_t* q = k == 0 ? t.qs[0] : k == 1 ? t.qs[1] : k == 2 ? t.qs[2] : t.qs[3];
+        const int m = k == 0 ? t.ms[0] : k == 1 ? t.ms[1] : k == 2 ? t.ms[2] : t.ms[3];
+        const int j = j0 + r;
+        const bool real = q != nullptr && j < m;
+        const int qv = real ? q[j] : 0;
+        int16_t* dst = lds.prof + k * kNRes * kS + r;
+#pragma unroll
+        for (int s = 0; s < kPadResidue; ++s)
+            dst[s * kS] = (int16_t)(real ? (int)mat[s * 25 + qv] : kPadScore);
+        dst[kPadResidue * kS] = (int16_t)kPadScore;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -460,6 +465,7 @@ __device__ __forceinline__ void write_best(const SwKernelArgs& a, const TaskView
 template <int R>
 struct GroupLds {
     WaveLds<R> wave[kWaves];
+    int8_t mat[640];                     // the 25 x 25 substitution matrix
     int item;
     int error;
     int progress[kWaves];
@@ -475,7 +481,7 @@ __device__ void run_task_wave(const SwKernelArgs& a, WaveLds<R>& lds, uint32_t* 
     init_best(best, t);
     int chunk = 0;
     for (int j0 = 0; j0 < t.m_max; j0 += R, ++chunk) {
-        build_profile(a, lds, t, j0);
+        build_profile(g.mat, lds, t, j0);
         ChunkIO io{};
         io.top = j0 == 0 ? nullptr : buf;
         io.bottom = j0 + R < t.m_max ? buf : nullptr;
@@ -501,7 +507,7 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds<R>& g, uint32_t* 
     const int nc = t.ncols;
     for (int c = wave; c < n_chunks; c += kWaves) {
         const int j0 = c * R;
-        build_profile(a, g.wave[wave], t, j0);
+        build_profile(g.mat, g.wave[wave], t, j0);
         ChunkIO io{};
         io.top = c == 0 ? nullptr : bufs + (size_t)((c - 1) % kWaves) * buf_words;
         io.bottom = c + 1 < n_chunks ? bufs + (size_t)(c % kWaves) * buf_words : nullptr;
@@ -560,6 +566,7 @@ sw_forward_kernel(SwKernelArgs a) {
     const size_t buf_words = (size_t)a.ws_words_per_slot / kWaves;
     uint32_t* bufs = a.workspace + (size_t)blockIdx.x * a.ws_words_per_slot;
     if (threadIdx.x == 0) g.error = 0;
+    for (int e = threadIdx.x; e < 625; e += blockDim.x) g.mat[e] = a.matrix[e];
     for (;;) {
         if (threadIdx.x == 0) {
             g.item = a.task_counter ? atomicAdd(a.task_counter, 1) : (int)blockIdx.x;
