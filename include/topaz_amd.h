@@ -78,7 +78,8 @@ int tpz_sw_batch_kernel_ms(const tpz_sw_batch* b, double* ms);
 typedef struct {
     int32_t rows_per_chunk;    /* query rows a wavefront holds in registers: 16 or 32  */
     int32_t groups_per_cu;     /* resident 4-wavefront workgroups per CU (persistent):
-                                  1..2 for 32 rows, 1..3 for 16 rows                   */
+                                  1 up to what the kernel's registers and LDS allow
+                                  (2 for both row counts on gfx950)                   */
     int32_t queries_per_task;  /* queries whose pairs may share a wavefront: 1, 2, 4   */
     int32_t use_graph;         /* nonzero: replay the launches from a captured hipGraph */
     double big_fraction;       /* a task costing more than this fraction of one

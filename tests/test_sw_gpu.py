@@ -142,7 +142,7 @@ def test_packed_tasks_mix_queries(schedule):
     compare(w, schedule=schedule)
 
 
-CONFIGS = [dict(rows_per_chunk=16, groups_per_cu=3, queries_per_task=4),
+CONFIGS = [dict(rows_per_chunk=16, groups_per_cu=2, queries_per_task=4),
            dict(rows_per_chunk=16, groups_per_cu=1, queries_per_task=1, big_fraction=1e30),
            dict(rows_per_chunk=32, groups_per_cu=1, queries_per_task=2, big_fraction=0.01),
            dict(rows_per_chunk=16, groups_per_cu=2, queries_per_task=2, big_fraction=0.01)]
