@@ -162,6 +162,31 @@ void align_using_ssw(seq_t* s1, seq_t* s2, hit_t* hit, options_t* opt);
 int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
            int* gapless_score, int* gapless_length);
 
+/* ---- batching extension (no reference counterpart) --------------------
+ * One call per query instead of one call per hit, with the reference's own
+ * types.  Subjects are read from the database (db_seq, db.c:233-247: the
+ * protein's bytes through upper_aa).  A patched query.c calls these in place
+ * of its per-hit loops. */
+
+/* The loop of query_perform_alignments (query.c:199-228): every hit gets
+ * exactly what align_using_ssw_profile writes into it.  Returns how many hits
+ * have evalue < opt->evalue (what perform_alignment_with_profile returns per
+ * hit, query.c:338).  alignment_rank is left to the caller. */
+int align_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, int n,
+                                  options_t* opt);
+
+/* The loop of query_perform_statistics (query.c:276-291): every hit gets
+ * what stats_using_ssw_profile writes (hits with rawscore 32767 are left as
+ * they are, ssw_wrapper.c:136-138). */
+void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, int n,
+                                   options_t* opt);
+
+/* n calls of sa_hsp for one query: seed k is (qoffsets[k], positions[k]);
+ * lengths[k] is sa_hsp's *gapless_length and scores[k] its *gapless_score,
+ * written only where the length is not -1 (search.c:113-116). */
+void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
+                  options_t* opt, int* scores, int* lengths);
+
 #ifdef __cplusplus
 }
 #endif

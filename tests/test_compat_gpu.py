@@ -149,3 +149,178 @@ def test_sa_hsp_compat_matches_oracle():
         es, el = o.sa_hsp(T, pos, Q, off)
         assert ln.value == el
         assert sc.value == (es if el != -1 else -7)
+
+
+def _upper_aa(b):
+    """upper_aa of lexicographical.h: A-Y, '*' and newline kept, a-y raised,
+    anything else 'X'."""
+    if 65 <= b <= 89 or b in (42, 10):
+        return b
+    if 97 <= b <= 121:
+        return b - 32
+    return 88
+
+
+def _db_seq(T, start, n):
+    """db_seq (db.c:233-247): the protein's bytes without newlines, through
+    upper_aa."""
+    return bytes(_upper_aa(c) for c in T[start:start + n] if c != 10)
+
+
+class PositionT(C.Structure):
+    _fields_ = [("header_start", C.c_uint64), ("protein_start", C.c_uint64),
+                ("header_len", C.c_int), ("protein_len", C.c_int)]
+
+
+def _seq(b):
+    sq = SeqT()
+    sq.seq_buf.buf = b
+    sq.seq_buf.len = len(b)
+    sq.seq_len = len(b)
+    return sq
+
+
+HIT_FIELDS = ["db_index", "is_aligned", "alignment_rank", "rawscore", "mismatch", "length",
+              "gapopen", "identity", "bitscore", "evalue", "qstart", "qend", "sstart", "send",
+              "effective_length", "gapless_score"]
+
+
+def _same(a, b):
+    return a == b or (a != a and b != b)
+
+
+def test_batch_entry_points_equal_per_hit_calls():
+    """align_batch_using_ssw_profile / stats_batch_using_ssw_profile /
+    sa_hsp_batch write exactly what the per-hit calls write, on a database
+    with an index (some letters in lower case, one entry spanning a
+    newline), and agree with the reference's compiled per-hit functions."""
+    from tests.refcheck import DbT
+    L = lib()
+    PH = C.POINTER(HitT)
+    L.align_batch_using_ssw_profile.restype = C.c_int
+    L.align_batch_using_ssw_profile.argtypes = [C.c_void_p, C.POINTER(DbT), C.POINTER(PH),
+                                                C.c_int, C.POINTER(OptionsT)]
+    L.stats_batch_using_ssw_profile.argtypes = [C.c_void_p, C.POINTER(DbT), C.POINTER(PH),
+                                                C.c_int, C.POINTER(OptionsT)]
+    L.stats_using_ssw_profile.argtypes = L.align_using_ssw_profile.argtypes
+    PI = C.POINTER(C.c_int)
+    L.sa_hsp.restype = C.c_int
+    L.sa_hsp.argtypes = [C.POINTER(DbT), C.POINTER(SeqT), C.c_int, C.c_int64,
+                         C.POINTER(OptionsT), PI, PI]
+    L.sa_hsp_batch.argtypes = [C.POINTER(DbT), C.POINTER(SeqT), C.c_int, PI,
+                               C.POINTER(C.c_int64), C.POINTER(OptionsT), PI, PI]
+    R = None
+    if REF_SO.exists():
+        R = Reference().L
+        R.get_ssw_profile.restype = C.c_void_p
+        R.get_ssw_profile.argtypes = L.get_ssw_profile.argtypes
+        R.align_using_ssw_profile.argtypes = L.align_using_ssw_profile.argtypes
+        R.stats_using_ssw_profile.argtypes = L.align_using_ssw_profile.argtypes
+
+    rng = np.random.default_rng(6)
+    query_i = synth.random_residues(rng, 260)
+    lens = synth.lognormal_lengths(rng, 120, 250, 150, 5, 900)
+    prots = []
+    for k, n in enumerate(lens):
+        if k % 3 == 0:      # homologs of the query
+            r_, o_, l_ = synth.mutate(rng, query_i, np.array([0]), np.array([len(query_i)]),
+                                      np.array([0.4 + 0.5 * rng.random()]))
+            p = _internal(r_[:l_[0]])
+        else:
+            p = _internal(synth.random_residues(rng, int(n)))
+        if k % 7 == 0:     # a few letters in lower case (the text stays
+            p = p.translate(bytes.maketrans(b"ALK", b"alk"))   # within 31 letters)
+        prots.append(p)
+    T = b"".join(p + b"\n" for p in prots)
+    starts = np.cumsum([0] + [len(p) + 1 for p in prots])[:-1]
+    index = (PositionT * (len(prots) + 1))()
+    for k, p in enumerate(prots):
+        index[k].protein_start = int(starts[k])
+        index[k].protein_len = len(p)
+    # one entry spanning the newline between proteins 4 and 5
+    index[len(prots)].protein_start = int(starts[4])
+    index[len(prots)].protein_len = len(prots[4]) + 1 + len(prots[5])
+    SA = np.arange(len(T), dtype=np.int64)
+    db = DbT()
+    db.T = T
+    db.Tsize = len(T)
+    db.index = C.cast(index, C.c_void_p)
+    db.Psize = len(prots) + 1
+    db.SA = SA.ctypes.data_as(C.POINTER(C.c_int64))
+    db.SAsize = len(T) * 8
+
+    mat = matrix()
+    mat_c = (C.c_int8 * 625)(*mat.tolist())
+    opt = OptionsT()
+    opt.substitution = C.cast(mat_c, C.c_void_p)
+    opt.gap_open, opt.gap_extend = 11, 1
+    opt.lambda_, opt.k, opt.H = 0.267, 0.041, 0.140
+    opt.db_size, opt.db_proteins = 35_000_000, 100_000
+    opt.evalue = 10.0
+    q = _seq(_internal(query_i))
+
+    n = 150
+    db_index = rng.integers(0, len(prots) + 1, n)
+    gscore = rng.choice([10, 39, 40, 100], n)
+
+    def fresh():
+        hs = [HitT() for _ in range(n)]
+        for h, d, g in zip(hs, db_index, gscore):
+            h.db_index, h.gapless_score, h.rawscore = int(d), int(g), 0
+        return hs
+
+    # per hit, as query.c:199-228 and :276-291 drive it
+    def per_hit(impl):
+        prof = impl.get_ssw_profile(C.byref(q), C.byref(opt))
+        hs = fresh()
+        count = 0
+        for h in hs:
+            pos = index[h.db_index]
+            s = _seq(_db_seq(T, pos.protein_start, pos.protein_len))
+            impl.align_using_ssw_profile(prof, C.byref(s), C.byref(h), C.byref(opt))
+            count += h.evalue < opt.evalue
+        for h in hs:
+            if h.evalue <= opt.evalue and h.rawscore != 32767:
+                pos = index[h.db_index]
+                s = _seq(_db_seq(T, pos.protein_start, pos.protein_len))
+                impl.stats_using_ssw_profile(prof, C.byref(s), C.byref(h), C.byref(opt))
+        return hs, count
+
+    mine, count = per_hit(L)
+    assert 10 < count < n
+    prof = L.get_ssw_profile(C.byref(q), C.byref(opt))
+    batch = fresh()
+    got = L.align_batch_using_ssw_profile(prof, C.byref(db), (PH * n)(*[C.pointer(h) for h in batch]),
+                                          n, C.byref(opt))
+    assert got == count
+    todo = [h for h in batch if h.evalue <= opt.evalue]
+    L.stats_batch_using_ssw_profile(prof, C.byref(db),
+                                    (PH * len(todo))(*[C.pointer(h) for h in todo]),
+                                    len(todo), C.byref(opt))
+    for k in range(n):
+        for f in HIT_FIELDS:
+            assert _same(getattr(batch[k], f), getattr(mine[k], f)), (k, f)
+    assert any(h.length > 0 for h in batch)
+    if R is not None:
+        ref, ref_count = per_hit(R)
+        assert ref_count == count
+        for k in range(n):
+            for f in HIT_FIELDS:
+                assert _same(getattr(batch[k], f), getattr(ref[k], f)), (k, f)
+
+    # gapless: n calls of sa_hsp against one call of sa_hsp_batch
+    m = 400
+    offs = rng.integers(0, q.seq_len, m).astype(np.int32)
+    pos = rng.integers(0, len(T), m).astype(np.int64)
+    pos[::2] = (starts[(np.arange(0, m, 2) * 3) % len(prots)] + offs[::2]).clip(0, len(T) - 1)
+    sc = np.full(m, -7, np.int32)
+    ln = np.zeros(m, np.int32)
+    L.sa_hsp_batch(C.byref(db), C.byref(q), m, offs.ctypes.data_as(PI),
+                   pos.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(opt),
+                   sc.ctypes.data_as(PI), ln.ctypes.data_as(PI))
+    assert (ln != -1).sum() > 20
+    for k in range(m):
+        s1, l1 = C.c_int(-7), C.c_int(0)
+        L.sa_hsp(C.byref(db), C.byref(q), int(offs[k]), int(pos[k]), C.byref(opt),
+                 C.byref(s1), C.byref(l1))
+        assert (sc[k], ln[k]) == (s1.value, l1.value), k

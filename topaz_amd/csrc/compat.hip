@@ -474,4 +474,172 @@ int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// batching extension
+// ---------------------------------------------------------------------------
+namespace {
+// Subjects of the hits as matrix indices (db_seq then aa_table2 of upper_aa,
+// db.c:233-247, ssw_wrapper.c:188-201), 4-byte aligned for the batch.
+struct HitSubjects {
+    std::vector<int8_t> res;
+    std::vector<int64_t> off;
+    std::vector<int32_t> len;
+};
+
+static void gather_subjects(const db_t* db, hit_t** hits, const std::vector<int>& which,
+                            HitSubjects& out) {
+    const Alphabet& ab = alphabet();
+    out.res.clear(); out.off.clear(); out.len.clear();
+    for (int k : which) {
+        const hit_t* h = hits[k];
+        if (h->db_index >= db->Psize) {                              // query.c:311-314
+            fprintf(stderr, "Error: protein at index %llu not found!",
+                    (unsigned long long)h->db_index);
+            _exit(EXIT_FAILURE);
+        }
+        const position_t& p = db->index[h->db_index];
+        out.off.push_back((int64_t)out.res.size());
+        int32_t n = 0;
+        for (int i = 0; i < p.protein_len; ++i) {
+            const char c = db->T[p.protein_start + i];
+            if (c == '\n') continue;
+            out.res.push_back(ab.idx(ab.upper[(unsigned char)c]));
+            ++n;
+        }
+        out.len.push_back(n);
+    }
+}
+}  // namespace
+
+int align_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, int n,
+                                  options_t* opt) {
+    check_gaps(opt->gap_open, opt->gap_extend);
+    std::vector<int> todo;
+    for (int k = 0; k < n; ++k) {
+        hit_t* h = hits[k];
+        if (h->gapless_score < 40) {                                 // ssw_wrapper.c:171-176
+            h->bitscore = 0;
+            h->evalue = 1000;
+            h->is_aligned = 1;
+        } else {
+            todo.push_back(k);
+        }
+    }
+    if (!todo.empty()) {
+        HitSubjects sub;
+        gather_subjects(db, hits, todo, sub);
+        const int m = (int)todo.size();
+        std::vector<int32_t> pq(m, 0), ps(m), score(m), ref_end(m), read_end(m);
+        for (int k = 0; k < m; ++k) ps[k] = k;
+        const int64_t qo = 0;
+        const int32_t ql = profile->readLen;
+        tpz_sw_batch* b = nullptr;
+        if (tpz_sw_batch_create(&b, 1, profile->read, &qo, &ql, m, sub.res.data(), sub.off.data(),
+                                sub.len.data(), m, pq.data(), ps.data(), profile->mat,
+                                opt->gap_open, opt->gap_extend) ||
+            tpz_sw_batch_run(b, TPZ_SCHED_PERSISTENT) ||
+            tpz_sw_batch_results(b, score.data(), ref_end.data(), read_end.data()))
+            die("align_batch_using_ssw_profile");
+        tpz_sw_batch_destroy(b);
+        double correction, corrected_q, corrected_db;
+        karlin(nullptr, opt, profile->readLen, &correction, &corrected_q, &corrected_db);
+        for (int k = 0; k < m; ++k) {
+            hit_t* h = hits[todo[k]];
+            if (score[k] >= 32767) {                                 // ssw_wrapper.c:222-226
+                h->rawscore = 32767;
+                continue;
+            }
+            // the forward pass alone leaves the begins at -1 (ssw.c:886-893)
+            h->sstart = -1;
+            h->send = ref_end[k];
+            h->qstart = -1;
+            h->qend = read_end[k];
+            h->rawscore = score[k];
+            h->length = 0;
+            h->gapopen = 0;
+            h->mismatch = 0;
+            h->identity = 0.0;
+            h->bitscore = (opt->lambda * h->rawscore - log(opt->k));  // ssw_wrapper.c:239-245
+            h->evalue = corrected_q * corrected_db * exp(-h->bitscore);
+            h->evalue = h->evalue > 1.0e-180 ? h->evalue : 0.0;
+            h->bitscore /= M_LN2;
+            h->is_aligned = 1;
+            h->effective_length = profile->readLen - correction;
+        }
+    }
+    int count = 0;
+    for (int k = 0; k < n; ++k) count += hits[k]->evalue < opt->evalue;   // query.c:338
+    return count;
+}
+
+void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, int n,
+                                   options_t* opt) {
+    check_gaps(opt->gap_open, opt->gap_extend);
+    std::vector<int> todo;
+    for (int k = 0; k < n; ++k)
+        if (hits[k]->rawscore != 32767) todo.push_back(k);           // ssw_wrapper.c:136-138
+    if (todo.empty()) return;
+    HitSubjects sub;
+    gather_subjects(db, hits, todo, sub);
+    const int m = (int)todo.size();
+    std::vector<int32_t> hq(m, 0), hs(m), re(m), fe(m), sc(m);
+    for (int k = 0; k < m; ++k) {
+        const hit_t* h = hits[todo[k]];
+        hs[k] = k;
+        re[k] = h->qend;
+        fe[k] = h->send;
+        sc[k] = h->rawscore;
+    }
+    const int64_t qo = 0;
+    const int32_t ql = profile->readLen;
+    std::vector<int32_t> rc(m), qb(m), sb(m), len(m), gapo(m), mis(m);
+    std::vector<double> ident(m);
+    tpz_stats_batch* b = nullptr;
+    if (tpz_stats_batch_create(&b, 1, profile->read, &qo, &ql, m, sub.res.data(), sub.off.data(),
+                               sub.len.data(), m, hq.data(), hs.data(), re.data(), fe.data(),
+                               sc.data(), profile->mat, opt->gap_open, opt->gap_extend) ||
+        tpz_stats_batch_run(b) ||
+        tpz_stats_batch_results(b, rc.data(), qb.data(), sb.data(), len.data(), gapo.data(),
+                                mis.data(), ident.data()))
+        die("stats_batch_using_ssw_profile");
+    tpz_stats_batch_destroy(b);
+    for (int k = 0; k < m; ++k) {
+        hit_t* h = hits[todo[k]];
+        h->qstart = qb[k];
+        h->sstart = sb[k];
+        h->length = len[k];
+        h->gapopen = gapo[k];
+        h->mismatch = mis[k];
+        h->identity = ident[k];
+    }
+}
+
+void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
+                  options_t* opt, int* scores, int* lengths) {
+    if (n <= 0) return;
+    const ResidentText& rt = resident_text(db);
+    std::vector<tpz_seed> seeds(n);
+    for (int k = 0; k < n; ++k) {
+        if (db->SA && (positions[k] < 0 || (uint64_t)positions[k] >= db->Tsize)) {
+            fprintf(stderr, "topaz_amd: sa_hsp_batch: suffix-array slot %lld outside the text\n",
+                    (long long)positions[k]);
+            _exit(EXIT_FAILURE);
+        }
+        seeds[k] = tpz_seed{0, qoffsets[k], positions[k]};
+    }
+    const int64_t qo = 0;
+    const int32_t ql = q->seq_len;
+    std::vector<int32_t> sc(n), ln(n);
+    tpz_gapless_batch* b = nullptr;
+    if (tpz_gapless_batch_create(&b, rt.text, 1, q->seq_buf.buf, &qo, &ql, n, seeds.data(),
+                                 (const int8_t*)*opt->substitution) ||
+        tpz_gapless_batch_run(b) || tpz_gapless_batch_results(b, sc.data(), ln.data()))
+        die("sa_hsp_batch");
+    tpz_gapless_batch_destroy(b);
+    for (int k = 0; k < n; ++k) {
+        lengths[k] = ln[k];
+        if (ln[k] != -1) scores[k] = sc[k];                          // search.c:113-116
+    }
+}
+
 }  // extern "C"
