@@ -144,6 +144,33 @@ s_align* ssw_align(const s_profile* prof, const int8_t* ref, int32_t refLen,
                    const uint8_t flag, const uint16_t filters,
                    const int32_t filterd, const int32_t maskLen);
 void align_destroy(s_align* a);
+
+/* ssw.h:23-26, 148-194 -- the BAM encoding of one CIGAR operation (SAM
+ * format specification: length << 4 | op, with the ops "MIDNSHP=X" numbered
+ * 0..8).  An unknown letter encodes as 'M', and op codes above 8 decode as
+ * 'M'.  No function here produces a CIGAR (ssw_align runs with flag 0 on
+ * TOPAZ's path); the helpers serve callers that include this header in place
+ * of ssw.h. */
+#ifndef MAPSTR
+#define MAPSTR "MIDNSHP=X"
+#endif
+#ifndef BAM_CIGAR_SHIFT
+#define BAM_CIGAR_SHIFT 4
+#endif
+static inline uint32_t to_cigar_int(uint32_t length, char op_letter) {
+    uint32_t op = 0;
+    for (uint32_t k = 0; k < sizeof(MAPSTR) - 1; ++k)
+        if (MAPSTR[k] == op_letter) op = k;
+    return (length << BAM_CIGAR_SHIFT) | op;
+}
+static inline char cigar_int_to_op(uint32_t cigar_int) {
+    const uint32_t op = cigar_int & 0xfu;
+    return op < sizeof(MAPSTR) - 1 ? MAPSTR[op] : 'M';
+}
+static inline uint32_t cigar_int_to_len(uint32_t cigar_int) {
+    return cigar_int >> BAM_CIGAR_SHIFT;
+}
+
 int ssw_align_stats(const s_profile* prof, const int8_t* ref, int32_t refLen,
                     const uint8_t weight_gapO, const uint8_t weight_gapE,
                     const int32_t maskLen, int32_t read_end1, int32_t ref_end1,

@@ -2,6 +2,7 @@
 (no compute call: this runs without a GPU)."""
 import ctypes as C
 import re
+import subprocess
 from pathlib import Path
 
 import pytest
@@ -36,3 +37,35 @@ def test_symbol_exported(header, name):
 
 def test_version_string():
     assert b"gfx950" in _lib.lib().tpz_version()
+
+
+def test_cigar_helpers_of_the_compat_header(tmp_path):
+    """The header's inline CIGAR helpers (ssw.h:148-194) follow the BAM
+    encoding of the SAM specification: ops M I D N S H P = X are 0..8,
+    length in the bits above 4; unknown letters encode as M and codes above
+    8 decode as M."""
+    src = tmp_path / "cigar.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "topaz_ssw_compat.h"
+int main(void) {
+    const char* letters = "MIDNSHP=XQm";
+    for (const char* c = letters; *c; ++c) {
+        uint32_t v = to_cigar_int(37u, *c);
+        printf("%c %u %c %u\n", *c, v, cigar_int_to_op(v), cigar_int_to_len(v));
+    }
+    for (uint32_t op = 9; op < 16; ++op) printf("%c", cigar_int_to_op((7u << 4) | op));
+    printf("\n");
+    return 0;
+}
+''')
+    exe = tmp_path / "cigar"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    spec = {"M": 0, "I": 1, "D": 2, "N": 3, "S": 4, "H": 5, "P": 6, "=": 7, "X": 8}
+    for line, letter in zip(out, "MIDNSHP=XQm"):
+        c, v, back, n = line.split(" ")
+        op = spec.get(letter, 0)
+        assert (c, int(v), back, int(n)) == (letter, 37 * 16 + op, "MIDNSHP=X"[op], 37)
+    assert out[11] == "M" * 7
