@@ -196,16 +196,17 @@ def gapless_leg(args):
         b = GaplessBatch(dev_text, queries, seeds, matrix())
         for _ in range(args.warmup):
             b.run()
-        ref = b.results() if args.warmup else None
         times, kms = [], []
         for _ in range(args.steps):
             t1 = time.perf_counter()
             b.run()
             times.append(time.perf_counter() - t1)
             kms.append(b.kernel_ms())
-            if ref is None:
-                ref = b.results()
+        # results are read after the timed steps (a readback between runs
+        # lets the clocks drop), then one more run must reproduce them
         sc, ln = b.results()
+        b.run()
+        ref = b.results()
         assert np.array_equal(sc, ref[0]) and np.array_equal(ln, ref[1])
         b.close()
         right, left = synth.gapless_scan_bytes(seeds, ql, text, nl, rank)
@@ -261,21 +262,23 @@ def sw_leg(args):
     b.set_config(use_graph=1)
     for _ in range(args.warmup):
         b.run(SCHED_PERSISTENT)
-    ref = b.results() if args.warmup else None
     times, kms = [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
         b.run(SCHED_PERSISTENT)
         times.append(time.perf_counter() - t0)
         kms.append(b.kernel_ms())
-        if ref is None:
-            ref = b.results()
-    ref_sc, ref_re, ref_rd = ref
+    # results are read after the timed steps (a readback between runs lets
+    # the clocks drop, and the next launch pays for it), then one more run
+    # must reproduce them bit for bit
     sc, re_, rd = b.results()
+    b.run(SCHED_PERSISTENT)
+    ref_sc, ref_re, ref_rd = b.results()
     assert np.array_equal(sc, ref_sc) and np.array_equal(re_, ref_re) and np.array_equal(rd, ref_rd)
     t_step = float(np.sum(times)) / args.steps
     kernel_s = float(np.mean(kms)) / 1e3
-    print("[bench] persistent: %.2f ms/step (kernel %.2f ms)" % (t_step * 1e3, kernel_s * 1e3),
+    print("[bench] persistent: %.2f ms/step (kernel %.2f ms; per step %s)"
+          % (t_step * 1e3, kernel_s * 1e3, " ".join("%.2f" % k for k in kms)),
           file=sys.stderr, flush=True)
 
     plain = None

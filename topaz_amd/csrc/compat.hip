@@ -85,6 +85,15 @@ int8_t matrix_bias(const int8_t* mat, int32_t n) {
     return (int8_t)(-b);
 }
 
+// Residues are matrix indices 0..24 (aa_table2 never gives anything else);
+// the reference would index its profile out of bounds with another value.
+[[noreturn]] void residue_out_of_range(const char* what, int32_t i, int v) {
+    fprintf(stderr, "topaz_amd: ssw_align: %s residue %d is %d (not a matrix index 0..24)\n",
+            what, i, v);
+    fflush(stderr);
+    _exit(EXIT_FAILURE);
+}
+
 // Forward pass of one pair on the GPU: (score capped at 32767, ref_end1,
 // read_end1) as the byte/word dispatch of ssw.c:861-876 would give.
 void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
@@ -98,6 +107,10 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     char* h = c.host.data();
     memcpy(h + o_q, read, (size_t)m);
     memset(h + o_s, 25, n_pad);
+    for (int32_t i = 0; i < m; ++i)
+        if (read[i] < 0 || read[i] > 24) residue_out_of_range("query", i, read[i]);
+    for (int32_t i = 0; i < n; ++i)
+        if (ref[i] < 0 || ref[i] > 24) residue_out_of_range("subject", i, ref[i]);
     memcpy(h + o_s, ref, (size_t)n);
     memcpy(h + o_mat, mat, 625);
     int64_t* meta64 = reinterpret_cast<int64_t*>(h + o_meta);

@@ -110,22 +110,22 @@ struct LaneBest {
     int chunk[2];
 };
 
-__device__ __forceinline__ uint32_t load4(const int8_t* s, int i, int n) {
-    // subjects start on 4-byte boundaries and are padded with residue 25
-    return i < n ? *reinterpret_cast<const uint32_t*>(s + i) : 0x19191919u;
-}
-
 constexpr int kWaves = 4;          // wavefronts per workgroup
 constexpr int kSyncCols = 8;       // progress is published every 8 columns
 constexpr int kAhead = 16;         // a consumer runs at least 16 columns behind
 constexpr int kSpinLimit = 1 << 24;
+// s_waitcnt vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9
+// encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int kWaitVmem = 0x0f70;
 
 // Where a chunk reads its top boundary and writes its bottom one, and how it
 // synchronises with the wavefront that produces its top (another wavefront
 // of the same workgroup working on the chunk above).
 struct ChunkIO {
-    const uint32_t* top;      // null: first chunk (zero boundary)
-    uint32_t* bottom;         // null: last chunk
+    const uint32_t* top;      // always a valid buffer
+    bool has_top;             // false: first chunk (zero boundary)
+    uint32_t* bottom;         // always a valid buffer
+    bool has_bottom;          // false: last chunk (its bottom row is not needed)
     int* wait_ctr;            // producer progress in LDS, or null
     int wait_base;            // producer progress at column 0 of this chunk's top
     int* pub_ctr;             // own progress in LDS, or null
@@ -192,10 +192,49 @@ __device__ __forceinline__ void column_event(WaveLds<R>& lds, const s2 (&Hc)[R],
     bc = vmax(bc, cm);
 }
 
+// Top boundary (H, F) of columns c and c + 1 of one lane.
+struct TopPair {
+    uint32_t h0, f0, h1, f1;
+};
+
+// The loads are unconditional, so that every iteration issues the same
+// vector memory operations and the compiler's wait counts stay exact: past
+// the last column they read column ncols - 2 again (the values are not
+// used), and the first chunk (keep == 0) reads columns 0 and 1 of its own
+// buffer over and over (cache hits) and the consumer masks the values to
+// zero.
+__device__ __forceinline__ TopPair load_top(const uint32_t* tH, const uint32_t* tF, int c,
+                                            int ncols, uint32_t keep, int lane) {
+    int cc = c < ncols - 2 ? c : (ncols >= 2 ? ncols - 2 : 0);
+    cc = keep ? cc : 0;
+    TopPair p;
+    p.h0 = tH[(size_t)cc * 64 + lane];
+    p.f0 = tF[(size_t)cc * 64 + lane];
+    p.h1 = tH[(size_t)(cc + 1) * 64 + lane];
+    p.f1 = tF[(size_t)(cc + 1) * 64 + lane];
+    return p;
+}
+
+// Residues c .. c + 3 of a subject (c a multiple of 4), residue 25 past its
+// end.  The load itself is unconditional: its address is clamped to the
+// subject's last dword (subjects start on 4-byte boundaries, are padded with
+// residue 25 to the next one, and the residue array has at least 4 bytes
+// after the last subject), so no exec-masked load feeds a phi.
+__device__ __forceinline__ uint32_t load_res4(const int8_t* s, int c, int n) {
+    const int last = n > 0 ? ((n - 1) & ~3) : 0;
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(s + (c < last ? c : last));
+    return c < n ? w : 0x19191919u;
+}
+
 // One chunk of R rows [j0, j0 + R) over all subject columns, two
 // columns at a time: the second column runs one row behind the first, so
 // every step advances two independent dependency chains.  Returns false if
 // a wait on the producer of the top boundary gave up.
+//
+// Each iteration reloads the prefetched top boundary and the residue dword
+// right after their last use, into the same variables: no loaded value is
+// copied at the back edge, so no iteration waits for the loads or stores it
+// has just issued.
 template <int R>
 __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds, const ChunkIO& io,
                                           int ncols, const int8_t* s_lo, int n_lo,
@@ -212,44 +251,30 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
     int ci_lo = -1, ci_hi = -1;
     s2 hdiag = zero;              // H(i-1, j0-1)
 
+    // all ones when the chunk has a top boundary, zero for the first chunk
+    const uint32_t keep = (uint32_t)__builtin_amdgcn_readfirstlane(io.has_top ? -1 : 0);
     const uint32_t* tH = io.top;                         // [col][lane]
-    const uint32_t* tF = io.top ? io.top + (size_t)ncols * 64 : nullptr;
+    const uint32_t* tF = io.top + (size_t)ncols * 64;
     uint32_t* bH = io.bottom;
-    uint32_t* bF = io.bottom ? io.bottom + (size_t)ncols * 64 : nullptr;
+    uint32_t* bF = io.bottom + (size_t)ncols * 64;
+    // the last chunk writes every column pair into columns 0 and 1 (already
+    // read by then, and read by no one after): the stores stay unconditional
+    // but land in one cached line instead of going to memory
+    const int bstep = __builtin_amdgcn_readfirstlane(io.has_bottom ? 1 : 0);
 
     int waited = ncols < kAhead ? ncols : kAhead;       // columns known final
     if (io.wait_ctr && !wait_for(io, io.wait_base + waited)) return false;
-    uint32_t res_lo = load4(s_lo, 0, n_lo), res_hi = load4(s_hi, 0, n_hi);
-    uint32_t ta_h = 0, ta_f = 0, tb_h = 0, tb_f = 0;
-    if (tH) {
-        ta_h = tH[lane]; ta_f = tF[lane];
-        tb_h = tH[64 + lane]; tb_f = tF[64 + lane];
-    }
+    uint32_t res_lo = load_res4(s_lo, 0, n_lo), res_hi = load_res4(s_hi, 0, n_hi);
+    TopPair T = load_top(tH, tF, 0, ncols, keep, lane);
+    // Let these first loads land here: a load still pending at the loop
+    // entry makes the compiler wait for all vector memory at the top of every
+    // iteration.
+    __builtin_amdgcn_s_waitcnt(kWaitVmem);
 
-    // ncols is even (the host pads subjects with residue 25 columns)
-    for (int i = 0; i < ncols; i += 2) {
-        if (io.wait_ctr && (i % kSyncCols) == 0 && waited < ncols) {
-            // columns up to i + kAhead must be final before they are prefetched
-            const int need = i + kAhead < ncols ? i + kAhead : ncols;
-            if (need > waited) {
-                if (!wait_for(io, io.wait_base + need)) return false;
-                waited = need;
-            }
-        }
-        uint32_t na_h = 0, na_f = 0, nb_h = 0, nb_f = 0;
-        if (tH && i + 2 < ncols) {
-            na_h = tH[(size_t)(i + 2) * 64 + lane];
-            na_f = tF[(size_t)(i + 2) * 64 + lane];
-            nb_h = tH[(size_t)(i + 3) * 64 + lane];
-            nb_f = tF[(size_t)(i + 3) * 64 + lane];
-        }
-        const int sh = (i & 3) * 8;
-        const int ra_lo = (res_lo >> sh) & 0xff, ra_hi = (res_hi >> sh) & 0xff;
-        const int rb_lo = (res_lo >> (sh + 8)) & 0xff, rb_hi = (res_hi >> (sh + 8)) & 0xff;
-        if ((i & 3) == 2) {
-            res_lo = load4(s_lo, i + 2, n_lo);
-            res_hi = load4(s_hi, i + 2, n_hi);
-        }
+    // Columns i and i + 1 from the prefetched boundary T, which is then
+    // reloaded with columns i + 2 and i + 3.
+    auto column_pair = [&](int i, int ra_lo, int ra_hi, int rb_lo, int rb_hi,
+                           TopPair& T) __attribute__((always_inline)) {
         constexpr int kS = WaveLds<R>::kStride;
         ProfCol<R> pa_lo, pa_hi, pb_lo, pb_hi;
         load_col<R>(pa_lo, lds.prof + (off_lo + ra_lo) * kS);
@@ -257,10 +282,11 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
         load_col<R>(pb_lo, lds.prof + (off_lo + rb_lo) * kS);
         load_col<R>(pb_hi, lds.prof + (off_hi + rb_hi) * kS);
 
-        s2 fa = from_bits(ta_f), fb = from_bits(tb_f);
+        s2 fa = from_bits(T.f0 & keep), fb = from_bits(T.f1 & keep);
         s2 da = hdiag;                  // H(i-1, j0-1)
-        s2 db = from_bits(ta_h);        // H(i, j0-1)
-        hdiag = from_bits(tb_h);        // H(i+1, j0-1)
+        s2 db = from_bits(T.h0 & keep); // H(i, j0-1)
+        hdiag = from_bits(T.h1 & keep); // H(i+1, j0-1)
+        T = load_top(tH, tF, i + 2, ncols, keep, lane);
 
         // column a, row 0
         {
@@ -306,20 +332,36 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             E[q] = vmax(usub(E[q], ge), hg);
             fb = vmax(usub(fb, ge), hg);
         }
-        if (bH) {
-            bH[(size_t)i * 64 + lane] = bits(Ha[R - 1]);
-            bF[(size_t)i * 64 + lane] = bits(fa);
-            bH[(size_t)(i + 1) * 64 + lane] = bits(H[R - 1]);
-            bF[(size_t)(i + 1) * 64 + lane] = bits(fb);
-        }
+        const size_t bc0 = (size_t)(i * bstep) * 64 + lane;
+        bH[bc0] = bits(Ha[R - 1]);
+        bF[bc0] = bits(fa);
+        bH[bc0 + 64] = bits(H[R - 1]);
+        bF[bc0 + 64] = bits(fb);
         // column maxima as balanced trees: independent maxes issue back to
         // back, where a running max would be one dependent chain
         const s2 cma = tree_max<R>(Ha), cmb = tree_max<R>(H);
         column_event(lds, Ha, cma, bc, i, ci_lo, ci_hi);
         column_event(lds, H, cmb, bc, i + 1, ci_lo, ci_hi);
-        ta_h = na_h; ta_f = na_f; tb_h = nb_h; tb_f = nb_f;
         if (io.pub_ctr && (((i + 2) % kSyncCols) == 0 || i + 2 == ncols))
             publish(io, io.pub_base + i + 2);
+    };
+
+    // ncols is even (the host pads subjects with residue 25 columns)
+    for (int i = 0; i < ncols; i += 2) {
+        if (io.wait_ctr && (i % kSyncCols) == 0 && waited < ncols) {
+            // columns up to i + kAhead must be final before they are prefetched
+            const int need = i + kAhead < ncols ? i + kAhead : ncols;
+            if (need > waited) {
+                if (!wait_for(io, io.wait_base + need)) return false;
+                waited = need;
+            }
+        }
+        const int sh = (i & 2) * 8;
+        const int ra_lo = (res_lo >> sh) & 0xff, ra_hi = (res_hi >> sh) & 0xff;
+        const int rb_lo = (res_lo >> (sh + 8)) & 0xff, rb_hi = (res_hi >> (sh + 8)) & 0xff;
+        res_lo = load_res4(s_lo, (i + 2) & ~3, n_lo);
+        res_hi = load_res4(s_hi, (i + 2) & ~3, n_hi);
+        column_pair(i, ra_lo, ra_hi, rb_lo, rb_hi, T);
     }
 
     // Lowest row of the saved column equal to the chunk best, then merge:
@@ -483,8 +525,10 @@ __device__ void run_task_wave(const SwKernelArgs& a, WaveLds<R>& lds, uint32_t* 
     for (int j0 = 0; j0 < t.m_max; j0 += R, ++chunk) {
         build_profile(g.mat, lds, t, j0);
         ChunkIO io{};
-        io.top = j0 == 0 ? nullptr : buf;
-        io.bottom = j0 + R < t.m_max ? buf : nullptr;
+        io.top = buf;              // in place: column c is read before it is written
+        io.has_top = j0 > 0;
+        io.bottom = buf;
+        io.has_bottom = j0 + R < t.m_max;
         io.error = &g.error;
         io.g_error = a.error;
         run_chunk(a, lds, io, t.ncols, t.s_lo, t.n_lo, t.s_hi, t.n_hi, t.off_lo, t.off_hi, j0,
@@ -509,8 +553,12 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds<R>& g, uint32_t* 
         const int j0 = c * R;
         build_profile(g.mat, g.wave[wave], t, j0);
         ChunkIO io{};
-        io.top = c == 0 ? nullptr : bufs + (size_t)((c - 1) % kWaves) * buf_words;
-        io.bottom = c + 1 < n_chunks ? bufs + (size_t)(c % kWaves) * buf_words : nullptr;
+        // chunk c writes buffer c % kWaves, read by chunk c + 1; the wave of
+        // chunk c - 3 (which read it last) has passed every column first
+        io.bottom = bufs + (size_t)(c % kWaves) * buf_words;
+        io.has_bottom = c + 1 < n_chunks;
+        io.has_top = c > 0;
+        io.top = c > 0 ? bufs + (size_t)((c - 1) % kWaves) * buf_words : io.bottom;
         if (c > 0) {
             io.wait_ctr = &g.progress[(c - 1) % kWaves];
             io.wait_base = ((c - 1) / kWaves) * nc;
