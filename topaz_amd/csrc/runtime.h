@@ -83,7 +83,7 @@ void build_items(const Tasks& t, int n_slots, double big_fraction, std::vector<i
 struct TextCodes {
     uint8_t code[256];        // text byte -> code; 0xff: the byte is not in the text
     uint8_t byte_of[32];      // code -> byte
-    int n_codes = 0;          // codes 0 .. n_codes-1 (kTextTerm is '\n')
+    bool used[32];            // codes of letters of the text (kTextTerm, '\n', excluded)
 };
 
 // Query copies of a gapless batch in the kernel's layout: forward and

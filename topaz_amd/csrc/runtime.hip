@@ -663,17 +663,29 @@ static const AlphabetMaps& maps() {
 static int text_row(uint8_t c) { return maps().tmap[c & 127]; }
 static int query_row(uint8_t c) { return maps().qmap[c & 127]; }
 
+// Codes by letter frequency: the gapless kernel's table lookups hit LDS bank
+// (code >> 1), so the 16 most frequent letters get the even codes 0..30, one
+// bank each, and rarer letters the odd codes 29, 27, ... (sharing banks with
+// the less frequent of the first sixteen).  With codes in byte order the
+// twenty amino acids fell into ten banks.
 static int text_codes(const uint8_t* text, uint64_t n, TextCodes& tc) {
-    bool seen[256] = {false};
-    for (uint64_t i = 0; i < n; ++i) seen[text[i]] = true;
+    uint64_t count[256] = {0};
+    for (uint64_t i = 0; i < n; ++i) ++count[text[i]];
+    int letters[256], n_letters = 0;
+    for (int c = 0; c < 256; ++c)
+        if (count[c] && c != '\n') letters[n_letters++] = c;
+    if (n_letters > kTextTerm)
+        return fail("the text holds more than %d distinct letters", kTextTerm);
+    std::stable_sort(letters, letters + n_letters,
+                     [&](int a, int b) { return count[a] > count[b]; });
     memset(tc.code, 0xff, sizeof tc.code);
-    tc.n_codes = 0;
-    for (int c = 0; c < 256; ++c) {
-        if (!seen[c] || c == '\n') continue;
-        if (tc.n_codes >= kTextTerm)
-            return fail("the text holds more than %d distinct letters", kTextTerm);
-        tc.byte_of[tc.n_codes] = (uint8_t)c;
-        tc.code[c] = (uint8_t)tc.n_codes++;
+    memset(tc.byte_of, 0, sizeof tc.byte_of);
+    memset(tc.used, 0, sizeof tc.used);
+    for (int r = 0; r < n_letters; ++r) {
+        const int code = r < 16 ? 2 * r : 2 * (30 - r) + 1;     // ranks 16..30 -> 29, 27, .., 1
+        tc.byte_of[code] = (uint8_t)letters[r];
+        tc.code[letters[r]] = (uint8_t)code;
+        tc.used[code] = true;
     }
     tc.code['\n'] = kTextTerm;
     tc.byte_of[kTextTerm] = '\n';
@@ -711,10 +723,10 @@ void gapless_table(const TextCodes& tc, const int8_t* matrix, int16_t* table) {
     for (int t = 0; t < kTableRows; ++t) {
         for (int q = 0; q < kTableCols; ++q) {
             int w = kTermScore;
-            if (t < tc.n_codes && q != kQueryTerm) {
+            if (t < kTextTerm && tc.used[t] && q != kQueryTerm) {
                 const int tr = text_row(tc.byte_of[t]);
                 int qr = -1;
-                if (q < tc.n_codes || q == kTextTerm) qr = query_row(tc.byte_of[q]);
+                if ((q < kTextTerm && tc.used[q]) || q == kTextTerm) qr = query_row(tc.byte_of[q]);
                 else if (q >= kForeignCode && q < kForeignCode + 25) qr = q - kForeignCode;
                 if (qr >= 0) w = matrix[tr * 25 + qr];
             }
