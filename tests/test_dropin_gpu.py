@@ -20,13 +20,19 @@ REF = ROOT / "oracle" / "_ref"
 OBJ = REF / "obj"
 
 
-def _link_dropin(tmp):
+def _link_dropin(tmp, batch=False):
+    """TOPAZ's objects minus ssw.o, ssw_wrapper.o and search.o's sa_hsp,
+    linked against the library.  batch: query.o replaced by query_batch.o,
+    whose per-hit loops call the batch entry points."""
+    drop = {"ssw.o", "ssw_wrapper.o", "search.o"} | ({"query.o"} if batch else set())
     objs = sorted(str(p) for p in OBJ.rglob("*.o"))
-    keep = [o for o in objs if Path(o).name not in ("ssw.o", "ssw_wrapper.o", "search.o")]
+    keep = [o for o in objs if Path(o).name not in drop]
+    if batch:
+        keep.append(str(REF / "batch" / "query_batch.o"))
     search_local = str(tmp / "search_local.o")
     # keep the reference's sa_search & co., hide its sa_hsp
     subprocess.run(["objcopy", "-L", "sa_hsp", str(OBJ / "search.o"), search_local], check=True)
-    exe = tmp / "topaz_dropin"
+    exe = tmp / ("topaz_batch" if batch else "topaz_dropin")
     libdir = str(_lib.LIB_PATH.parent)
     subprocess.run(["g++", "-o", str(exe)] + keep + [search_local, "-L" + libdir,
                     "-ltopaz_amd", "-Wl,-rpath," + libdir, "-lm", "-lpthread"], check=True)
@@ -44,7 +50,8 @@ def workspace(tmp_path_factory):
     synth.write_fasta(tmp / "q.fa", queries, "q")
     subprocess.run([str(REF / "topaz"), "index", "-f", str(tmp / "db.fa"), "-p", str(tmp / "DB")],
                    check=True, capture_output=True)
-    return tmp, _link_dropin(tmp)
+    batch = _link_dropin(tmp, batch=True) if (REF / "batch" / "query_batch.o").exists() else None
+    return tmp, _link_dropin(tmp), batch
 
 
 def _search(exe, tmp, *flags):
@@ -73,8 +80,23 @@ def _per_query(text):
 
 @pytest.mark.parametrize("flags", [[], ["-B"], ["-F"]])
 def test_hit_lists_identical(workspace, flags):
-    tmp, dropin = workspace
+    tmp, dropin, _ = workspace
     ref = _search(REF / "topaz", tmp, *flags)
     ours = _search(dropin, tmp, *flags)
+    assert ref.count("\n") > 20
+    assert _per_query(ours) == _per_query(ref)
+
+
+@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"], ["-B", "-T", "4"]])
+def test_batch_loops_hit_lists_identical(workspace, flags):
+    """query.c with its per-hit loops replaced by one call of
+    align_batch_using_ssw_profile / stats_batch_using_ssw_profile per query
+    (oracle/patch_query.py) prints the stock binary's hit lists, also with
+    several worker threads calling at once."""
+    tmp, _, batch = workspace
+    if batch is None:
+        pytest.skip("patched query.o not built (make -C oracle ref)")
+    ref = _search(REF / "topaz", tmp, *flags)
+    ours = _search(batch, tmp, *flags)
     assert ref.count("\n") > 20
     assert _per_query(ours) == _per_query(ref)
