@@ -198,7 +198,9 @@ int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
 /* The loop of query_perform_alignments (query.c:199-228): every hit gets
  * exactly what align_using_ssw_profile writes into it.  Returns how many hits
  * have evalue < opt->evalue (what perform_alignment_with_profile returns per
- * hit, query.c:338).  alignment_rank is left to the caller. */
+ * hit, query.c:338).  alignment_rank is left to the caller.  The first call
+ * for a database (and matrix and gap penalties) puts all its proteins on the
+ * device; calls from several threads at once share GPU launches. */
 int align_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, int n,
                                   options_t* opt);
 

@@ -1,0 +1,104 @@
+"""End-to-end `topaz search` timing on one GPU box: the stock reference
+binary (CPU, the reference's own ssw.c / search.c) against TOPAZ linked with
+libtopaz_amd, both as the link-level drop-in (one GPU call per seed and per
+hit) and with query.c's and suffix.c's per-seed and per-hit loops replaced by
+the batch entry points.  All must print the same hit lists; prints one JSON
+line with the times.
+
+    python tests/search_timing.py [--db 100000] [--queries 1000] [--threads 16]
+
+The database follows the small end-to-end configuration: lognormal lengths
+(mean 350, sd 250, clipped [30, 2000]), 20 % of the queries with 1-5 planted
+homologs at 30-90 % identity; search with 100 hits per query and E < 1e-3.
+The binaries come from the reference sources compiled by oracle/Makefile
+(the checker), so this needs `make -C oracle ref` first.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+from tests.test_dropin_gpu import REF, _link_dropin, _per_query  # noqa: E402
+from topaz_amd import synth  # noqa: E402
+
+
+def run(cmd, cwd, timeout, out_path=None):
+    """Run cmd; its stdout goes to out_path as it is printed (so a long
+    search shows progress there) and is returned with the wall time."""
+    t0 = time.perf_counter()
+    if out_path is None:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=cwd)
+        out, err, rc = r.stdout, r.stderr, r.returncode
+    else:
+        with open(out_path, "w") as f:
+            r = subprocess.run(cmd, stdout=f, stderr=subprocess.PIPE, text=True,
+                               timeout=timeout, cwd=cwd)
+        out, err, rc = open(out_path).read(), r.stderr, r.returncode
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise SystemExit("%s failed: %s" % (cmd[0], err[-2000:]))
+    return out, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--db", type=int, default=100000)
+    ap.add_argument("--queries", type=int, default=1000)
+    ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--timeout", type=int, default=900)
+    ap.add_argument("--skip", nargs="*", default=[], help="binaries not to run (dropin_gpu, ...)")
+    ap.add_argument("--out", default=None, help="directory for each search's output as printed")
+    args = ap.parse_args()
+    batch_objs = [REF / "batch" / f for f in ("query_batch.o", "suffix_batch.o")]
+    if not (REF / "topaz").exists() or not all(p.exists() for p in batch_objs):
+        raise SystemExit("reference checker not built (make -C oracle ref)")
+    if args.out:
+        os.makedirs(args.out, exist_ok=True)
+    with tempfile.TemporaryDirectory() as d:
+        tmp = Path(d)
+        rng = np.random.default_rng(2027)
+        t0 = time.perf_counter()
+        db, queries = synth.search_database(rng, args.db, args.queries, hom_frac=0.2, max_hom=5,
+                                            ident=(0.3, 0.9), hi=2000)
+        synth.write_fasta(tmp / "db.fa", db, "s")
+        synth.write_fasta(tmp / "q.fa", queries, "q")
+        print("[search] %d subjects, %d queries written in %.1f s"
+              % (len(db), len(queries), time.perf_counter() - t0), file=sys.stderr, flush=True)
+        _, t_index = run([str(REF / "topaz"), "index", "-f", str(tmp / "db.fa"),
+                          "-p", str(tmp / "DB")], str(tmp), args.timeout)
+        print("[search] index built in %.1f s" % t_index, file=sys.stderr, flush=True)
+        exes = {"reference_cpu": REF / "topaz", "dropin_gpu": _link_dropin(tmp),
+                "batch_gpu": _link_dropin(tmp, batch=True)}
+        exes = {k: v for k, v in exes.items() if k not in args.skip}
+        flags = ["-T", str(args.threads), "-H", "100", "-E", "0.001"]
+        out, times = {}, {}
+        for name, exe in exes.items():
+            text, dt = run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB")]
+                           + flags, str(tmp), args.timeout,
+                           os.path.join(args.out, name + ".txt") if args.out else None)
+            out[name], times[name] = text, dt
+            print("[search] %s: %.2f s, %d lines" % (name, dt, text.count("\n")),
+                  file=sys.stderr, flush=True)
+        ref = _per_query(out["reference_cpu"])
+        identical = {k: _per_query(v) == ref for k, v in out.items()}
+    res = {"subjects": len(db), "queries": len(queries), "threads": args.threads,
+           "flags": " ".join(flags), "index_s": round(t_index, 2),
+           "search_s": {k: round(v, 2) for k, v in times.items()},
+           "queries_per_s": {k: round(len(queries) / v, 1) for k, v in times.items()},
+           "hit_lines": out["reference_cpu"].count("\n"), "identical": identical,
+           "host_cpus": os.cpu_count()}
+    print(json.dumps(res))
+    return 0 if all(identical.values()) else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

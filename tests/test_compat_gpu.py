@@ -123,7 +123,9 @@ def test_sa_hsp_compat_matches_oracle():
     rng = np.random.default_rng(5)
     T = b"".join(_internal(synth.random_residues(rng, int(L_))) + b"\n"
                  for L_ in synth.lognormal_lengths(rng, 50, 120, 80, 5, 400))
-    SA = np.arange(len(T), dtype=np.int64)
+    # a permuted suffix array: positions are SA slots, the text index SA[p]
+    SA = rng.permutation(len(T)).astype(np.int64)
+    slot_of = np.argsort(SA)
     db = DbT()
     db.T = T
     db.Tsize = len(T)
@@ -141,12 +143,13 @@ def test_sa_hsp_compat_matches_oracle():
     sq.seq_len = len(Q)
     for _ in range(60):
         off = int(rng.integers(0, len(Q)))
-        pos = int(rng.integers(0, len(T)))
+        idx = int(rng.integers(0, len(T)))
         if rng.random() < 0.5:
-            pos = 100 + off
+            idx = 100 + off
         sc, ln = C.c_int(-7), C.c_int(0)
-        L.sa_hsp(C.byref(db), C.byref(sq), off, pos, C.byref(opt), C.byref(sc), C.byref(ln))
-        es, el = o.sa_hsp(T, pos, Q, off)
+        L.sa_hsp(C.byref(db), C.byref(sq), off, int(slot_of[idx]), C.byref(opt), C.byref(sc),
+                 C.byref(ln))
+        es, el = o.sa_hsp(T, idx, Q, off)
         assert ln.value == el
         assert sc.value == (es if el != -1 else -7)
 
@@ -240,7 +243,8 @@ def test_batch_entry_points_equal_per_hit_calls():
     # one entry spanning the newline between proteins 4 and 5
     index[len(prots)].protein_start = int(starts[4])
     index[len(prots)].protein_len = len(prots[4]) + 1 + len(prots[5])
-    SA = np.arange(len(T), dtype=np.int64)
+    SA = rng.permutation(len(T)).astype(np.int64)       # positions are SA slots
+    slot_of = np.argsort(SA)
     db = DbT()
     db.T = T
     db.Tsize = len(T)
@@ -312,15 +316,19 @@ def test_batch_entry_points_equal_per_hit_calls():
     m = 400
     offs = rng.integers(0, q.seq_len, m).astype(np.int32)
     pos = rng.integers(0, len(T), m).astype(np.int64)
-    pos[::2] = (starts[(np.arange(0, m, 2) * 3) % len(prots)] + offs[::2]).clip(0, len(T) - 1)
+    pos[::2] = slot_of[(starts[(np.arange(0, m, 2) * 3) % len(prots)] + offs[::2]).clip(0, len(T) - 1)]
     sc = np.full(m, -7, np.int32)
     ln = np.zeros(m, np.int32)
     L.sa_hsp_batch(C.byref(db), C.byref(q), m, offs.ctypes.data_as(PI),
                    pos.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(opt),
                    sc.ctypes.data_as(PI), ln.ctypes.data_as(PI))
     assert (ln != -1).sum() > 20
+    o = Oracle()
+    Q = _internal(query_i)
     for k in range(m):
         s1, l1 = C.c_int(-7), C.c_int(0)
         L.sa_hsp(C.byref(db), C.byref(q), int(offs[k]), int(pos[k]), C.byref(opt),
                  C.byref(s1), C.byref(l1))
         assert (sc[k], ln[k]) == (s1.value, l1.value), k
+        es, el = o.sa_hsp(T, int(SA[pos[k]]), Q, int(offs[k]))
+        assert (ln[k], sc[k]) == (el, es if el != -1 else -7), k

@@ -22,13 +22,14 @@ OBJ = REF / "obj"
 
 def _link_dropin(tmp, batch=False):
     """TOPAZ's objects minus ssw.o, ssw_wrapper.o and search.o's sa_hsp,
-    linked against the library.  batch: query.o replaced by query_batch.o,
-    whose per-hit loops call the batch entry points."""
-    drop = {"ssw.o", "ssw_wrapper.o", "search.o"} | ({"query.o"} if batch else set())
+    linked against the library.  batch: query.o and suffix.o replaced by
+    query_batch.o and suffix_batch.o, whose per-seed and per-hit loops call
+    the batch entry points."""
+    drop = {"ssw.o", "ssw_wrapper.o", "search.o"} | ({"query.o", "suffix.o"} if batch else set())
     objs = sorted(str(p) for p in OBJ.rglob("*.o"))
     keep = [o for o in objs if Path(o).name not in drop]
     if batch:
-        keep.append(str(REF / "batch" / "query_batch.o"))
+        keep += [str(REF / "batch" / "query_batch.o"), str(REF / "batch" / "suffix_batch.o")]
     search_local = str(tmp / "search_local.o")
     # keep the reference's sa_search & co., hide its sa_hsp
     subprocess.run(["objcopy", "-L", "sa_hsp", str(OBJ / "search.o"), search_local], check=True)
@@ -50,7 +51,8 @@ def workspace(tmp_path_factory):
     synth.write_fasta(tmp / "q.fa", queries, "q")
     subprocess.run([str(REF / "topaz"), "index", "-f", str(tmp / "db.fa"), "-p", str(tmp / "DB")],
                    check=True, capture_output=True)
-    batch = _link_dropin(tmp, batch=True) if (REF / "batch" / "query_batch.o").exists() else None
+    built = all((REF / "batch" / f).exists() for f in ("query_batch.o", "suffix_batch.o"))
+    batch = _link_dropin(tmp, batch=True) if built else None
     return tmp, _link_dropin(tmp), batch
 
 
@@ -89,10 +91,11 @@ def test_hit_lists_identical(workspace, flags):
 
 @pytest.mark.parametrize("flags", [[], ["-B"], ["-F"], ["-B", "-T", "4"]])
 def test_batch_loops_hit_lists_identical(workspace, flags):
-    """query.c with its per-hit loops replaced by one call of
-    align_batch_using_ssw_profile / stats_batch_using_ssw_profile per query
-    (oracle/patch_query.py) prints the stock binary's hit lists, also with
-    several worker threads calling at once."""
+    """query.c and suffix.c with their per-seed and per-hit loops replaced by
+    sa_hsp_batch (seeds of a query, and the seed walk prefetched in batches),
+    align_batch_using_ssw_profile and stats_batch_using_ssw_profile
+    (oracle/patch_query.py, oracle/patch_suffix.py) print the stock binary's
+    hit lists, also with several worker threads calling at once."""
     tmp, _, batch = workspace
     if batch is None:
         pytest.skip("patched query.o not built (make -C oracle ref)")

@@ -48,14 +48,16 @@ void hip_or_die(hipError_t e, const char* what) {
     }
 }
 
-// Grow-only device buffer owned by one host thread.
+// Grow-only device buffer owned by one host thread.  It grows to at least
+// twice its size, so that a run of calls of slowly growing sizes does not
+// reallocate (hipFree waits for the device) on every call.
 struct Scratch {
     void* p = nullptr;
     size_t n = 0;
     void* get(size_t bytes) {
         if (bytes > n) {
             if (p) hip_or_die(hipFree(p), "hipFree");
-            n = std::max(bytes, (size_t)4096);
+            n = std::max(std::max(bytes, 2 * n), (size_t)65536);
             hip_or_die(hipMalloc(&p, n), "hipMalloc");
         }
         return p;
@@ -68,6 +70,15 @@ struct ThreadCtx {
     hipStream_t st = nullptr;
     Scratch in, out, ws, aux, aux2;
     std::vector<char> host;
+    std::vector<int32_t> res;
+    // the gapless query copies and score table of the last call, on the
+    // device (a query's seeds come in many calls in a row): reused while
+    // the text, the query bytes and the matrix are the same
+    Scratch g_dev;
+    const void* g_text = nullptr;
+    std::vector<char> g_query;
+    int8_t g_matrix[625] = {0};
+    size_t g_qr = 0, g_tab = 0, g_meta = 0;
     ThreadCtx() {
         if (ensure_init()) die("device init");
         hip_or_die(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
@@ -428,62 +439,95 @@ static const ResidentText& resident_text(const db_t* db) {
 }
 }  // namespace
 
-int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
-           int* gapless_score, int* gapless_length) {
+namespace {
+// n seeds of one query against the resident text on the calling thread's
+// stream: one upload of the seeds (and of the query copies and score table
+// when the query or matrix changed), one launch, one readback.  lengths[k] is sa_hsp's length; scores[k] is written only where
+// the length is not -1 (search.c:113-116).
+void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
+                   const int64_t* positions, const options_t* opt, int* scores, int* lengths,
+                   const char* who) {
+    if (n <= 0) return;
     const ResidentText& rt = resident_text(db);
-    if (db->SA && (position < 0 || (uint64_t)position >= db->Tsize)) {
-        fprintf(stderr, "topaz_amd: sa_hsp: suffix-array slot %lld outside the text\n",
-                (long long)position);
-        _exit(EXIT_FAILURE);
-    }
+    for (int k = 0; k < n; ++k)
+        if (db->SA && (positions[k] < 0 || (uint64_t)positions[k] >= db->Tsize)) {
+            fprintf(stderr, "topaz_amd: %s: suffix-array slot %lld outside the text\n", who,
+                    (long long)positions[k]);
+            _exit(EXIT_FAILURE);
+        }
     ThreadCtx& c = ctx();
-    // one seed against the resident text: query copies, seed, table, lengths
-    QueryCodes qc;
     const int64_t zero64 = 0;
     const int32_t qlen = q->seq_len;
-    if (gapless_query_codes(rt.text->codes, 1, q->seq_buf.buf, &zero64, &qlen, qc)) {
-        fprintf(stderr, "topaz_amd: sa_hsp: %s\n", tpz_last_error());
-        _exit(EXIT_FAILURE);
+    const int8_t* mat = (const int8_t*)*opt->substitution;
+    const bool same = c.g_text == rt.text && (size_t)qlen == c.g_query.size() &&
+                      memcmp(c.g_query.data(), q->seq_buf.buf, (size_t)qlen) == 0 &&
+                      memcmp(c.g_matrix, mat, 625) == 0;
+    if (!same) {
+        // query copies, score table and their offsets, uploaded once
+        QueryCodes qc;
+        if (gapless_query_codes(rt.text->codes, 1, q->seq_buf.buf, &zero64, &qlen, qc)) {
+            fprintf(stderr, "topaz_amd: %s: %s\n", who, tpz_last_error());
+            _exit(EXIT_FAILURE);
+        }
+        c.g_qr = (qc.fwd.size() + 15) & ~(size_t)15;
+        c.g_tab = (c.g_qr + qc.rev.size() + 15) & ~(size_t)15;
+        c.g_meta = c.g_tab + kTableRows * kTableCols * 2;
+        const size_t gbytes = c.g_meta + 32;
+        c.host.assign(gbytes, 0);
+        char* g = c.host.data();
+        memcpy(g, qc.fwd.data(), qc.fwd.size());
+        memcpy(g + c.g_qr, qc.rev.data(), qc.rev.size());
+        gapless_table(rt.text->codes, mat, (int16_t*)(g + c.g_tab));
+        memcpy(g + c.g_meta, &zero64, 8);          // forward and reversed offsets
+        memcpy(g + c.g_meta + 8, &zero64, 8);
+        memcpy(g + c.g_meta + 16, &qlen, 4);
+        char* gd = (char*)c.g_dev.get(gbytes);
+        hip_or_die(hipMemcpyAsync(gd, g, gbytes, hipMemcpyHostToDevice, c.st), "upload");
+        hip_or_die(hipStreamSynchronize(c.st), "upload");
+        c.g_text = rt.text;
+        c.g_query.assign(q->seq_buf.buf, q->seq_buf.buf + qlen);
+        memcpy(c.g_matrix, mat, 625);
     }
-    const size_t o_qf = 0, o_qr = (qc.fwd.size() + 15) & ~(size_t)15,
-                 o_tab = (o_qr + qc.rev.size() + 15) & ~(size_t)15,
-                 o_seed = o_tab + kTableRows * kTableCols * 2, o_meta = o_seed + 32;
-    const size_t bytes = o_meta + 32;
+    const size_t bytes = (size_t)n * sizeof(SeedRec);
     c.host.assign(bytes, 0);
     char* h = c.host.data();
-    memcpy(h + o_qf, qc.fwd.data(), qc.fwd.size());
-    memcpy(h + o_qr, qc.rev.data(), qc.rev.size());
-    gapless_table(rt.text->codes, (const int8_t*)*opt->substitution, (int16_t*)(h + o_tab));
-    SeedRec seed{0, qoffset, position};
-    memcpy(h + o_seed, &seed, sizeof seed);
-    memcpy(h + o_meta, &zero64, 8);            // forward and reversed offsets
-    memcpy(h + o_meta + 8, &zero64, 8);
-    memcpy(h + o_meta + 16, &qlen, 4);
+    SeedRec* seeds = reinterpret_cast<SeedRec*>(h);
+    for (int k = 0; k < n; ++k) seeds[k] = SeedRec{0, qoffsets[k], positions[k]};
     char* d = (char*)c.in.get(bytes);
-    int32_t* dout = (int32_t*)c.out.get(64);
+    const char* gd = (const char*)c.g_dev.p;
+    int32_t* dout = (int32_t*)c.out.get((size_t)n * 8);
     hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
-    hip_or_die(hipMemsetAsync(dout, 0, 8, c.st), "memset");
     GaplessArgs a{};
     a.t_fwd = (const uint8_t*)rt.text->d_fwd.ptr;
     a.t_head = (const uint8_t*)rt.text->d_head.ptr;
     a.t_size = (int64_t)db->Tsize;
     a.sa = rt.text->has_sa ? (const int64_t*)rt.text->d_sa.ptr : nullptr;
-    a.q_fwd = (const uint8_t*)(d + o_qf);
-    a.q_rev = (const uint8_t*)(d + o_qr);
-    a.qf_off = (const int64_t*)(d + o_meta);
-    a.qr_off = (const int64_t*)(d + o_meta + 8);
-    a.q_len = (const int32_t*)(d + o_meta + 16);
-    a.seeds = (const SeedRec*)(d + o_seed);
-    a.n_seeds = 1;
-    a.table = (const int16_t*)(d + o_tab);
+    a.q_fwd = (const uint8_t*)gd;
+    a.q_rev = (const uint8_t*)(gd + c.g_qr);
+    a.qf_off = (const int64_t*)(gd + c.g_meta);
+    a.qr_off = (const int64_t*)(gd + c.g_meta + 8);
+    a.q_len = (const int32_t*)(gd + c.g_meta + 16);
+    a.seeds = (const SeedRec*)d;
+    a.n_seeds = n;
+    a.table = (const int16_t*)(gd + c.g_tab);
     a.out_score = dout;
-    a.out_length = dout + 1;
-    hip_or_die(tpz_launch_gapless(&a, 1, c.st), "gapless");
-    int32_t r[2];
-    hip_or_die(hipMemcpyAsync(r, dout, 8, hipMemcpyDeviceToHost, c.st), "readback");
+    a.out_length = dout + n;
+    const int grid = std::max(1, std::min((n + 255) / 256, std::max(num_cu(), 1) * 32));
+    hip_or_die(tpz_launch_gapless(&a, grid, c.st), "gapless");
+    c.res.resize((size_t)n * 2);
+    hip_or_die(hipMemcpyAsync(c.res.data(), dout, (size_t)n * 8, hipMemcpyDeviceToHost, c.st),
+               "readback");
     hip_or_die(hipStreamSynchronize(c.st), "gapless");
-    *gapless_length = r[1];
-    if (r[1] != -1) *gapless_score = r[0];                       // search.c:113-116
+    for (int k = 0; k < n; ++k) {
+        lengths[k] = c.res[n + k];
+        if (c.res[n + k] != -1) scores[k] = c.res[k];
+    }
+}
+}  // namespace
+
+int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
+           int* gapless_score, int* gapless_length) {
+    gapless_seeds(db, q, 1, &qoffset, &position, opt, gapless_score, gapless_length, "sa_hsp");
     return 0;
 }
 
@@ -524,6 +568,64 @@ static void gather_subjects(const db_t* db, hit_t** hits, const std::vector<int>
 }
 }  // namespace
 
+namespace {
+// One serving loop (tpz_server_*) per database, matrix and gap penalties,
+// holding every protein of the database on the device.  TOPAZ's worker
+// threads call align_batch_using_ssw_profile at the same time
+// (engine.c:423-457); the loop merges their requests into shared launches,
+// where one query's hits alone would leave most of the GPU idle for the
+// length of its longest alignment.
+struct DbServer {
+    const db_t* db;
+    const char* T;
+    uint64_t psize;
+    int8_t matrix[625];
+    int go, ge;
+    tpz_server* srv;
+};
+std::mutex g_srv_mu;
+std::vector<DbServer*> g_servers;
+
+// Stops the loops' threads before the HIP runtime is torn down at exit.
+void stop_servers() {
+    std::lock_guard<std::mutex> lk(g_srv_mu);
+    for (auto* s : g_servers) tpz_server_destroy(s->srv);
+    g_servers.clear();
+}
+
+tpz_server* db_server(const db_t* db, const int8_t* mat, int go, int ge) {
+    std::lock_guard<std::mutex> lk(g_srv_mu);
+    for (auto* s : g_servers)
+        if (s->db == db && s->T == db->T && s->psize == db->Psize && s->go == go && s->ge == ge &&
+            memcmp(s->matrix, mat, 625) == 0)
+            return s->srv;
+    // every protein as db_seq gives it (db.c:233-247), as matrix indices
+    const Alphabet& ab = alphabet();
+    std::vector<int8_t> res;
+    std::vector<int64_t> off(db->Psize);
+    std::vector<int32_t> len(db->Psize);
+    for (uint64_t p = 0; p < db->Psize; ++p) {
+        const position_t& pos = db->index[p];
+        off[p] = (int64_t)res.size();
+        for (int i = 0; i < pos.protein_len; ++i) {
+            const char ch = db->T[pos.protein_start + i];
+            if (ch != '\n') res.push_back(ab.idx(ab.upper[(unsigned char)ch]));
+        }
+        len[p] = (int32_t)(res.size() - (size_t)off[p]);
+    }
+    tpz_server_config cfg{};
+    cfg.max_inflight = 2;
+    auto* s = new DbServer{db, db->T, db->Psize, {0}, go, ge, nullptr};
+    memcpy(s->matrix, mat, 625);
+    if (tpz_server_create(&s->srv, (int32_t)db->Psize, res.data(), off.data(), len.data(), mat,
+                          go, ge, &cfg))
+        die("align_batch_using_ssw_profile: serving loop");
+    if (g_servers.empty()) atexit(stop_servers);
+    g_servers.push_back(s);
+    return s->srv;
+}
+}  // namespace
+
 int align_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, int n,
                                   options_t* opt) {
     check_gaps(opt->gap_open, opt->gap_extend);
@@ -539,21 +641,25 @@ int align_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, in
         }
     }
     if (!todo.empty()) {
-        HitSubjects sub;
-        gather_subjects(db, hits, todo, sub);
         const int m = (int)todo.size();
         std::vector<int32_t> pq(m, 0), ps(m), score(m), ref_end(m), read_end(m);
-        for (int k = 0; k < m; ++k) ps[k] = k;
+        for (int k = 0; k < m; ++k) {
+            const hit_t* h = hits[todo[k]];
+            if (h->db_index >= db->Psize) {                          // query.c:311-314
+                fprintf(stderr, "Error: protein at index %llu not found!",
+                        (unsigned long long)h->db_index);
+                _exit(EXIT_FAILURE);
+            }
+            ps[k] = (int32_t)h->db_index;
+        }
+        tpz_server* srv = db_server(db, profile->mat, opt->gap_open, opt->gap_extend);
         const int64_t qo = 0;
         const int32_t ql = profile->readLen;
-        tpz_sw_batch* b = nullptr;
-        if (tpz_sw_batch_create(&b, 1, profile->read, &qo, &ql, m, sub.res.data(), sub.off.data(),
-                                sub.len.data(), m, pq.data(), ps.data(), profile->mat,
-                                opt->gap_open, opt->gap_extend) ||
-            tpz_sw_batch_run(b, TPZ_SCHED_PERSISTENT) ||
-            tpz_sw_batch_results(b, score.data(), ref_end.data(), read_end.data()))
+        uint64_t ticket = 0;
+        double latency_ms = 0;
+        if (tpz_server_submit(srv, 1, profile->read, &qo, &ql, m, pq.data(), ps.data(), &ticket) ||
+            tpz_server_wait(srv, ticket, score.data(), ref_end.data(), read_end.data(), &latency_ms))
             die("align_batch_using_ssw_profile");
-        tpz_sw_batch_destroy(b);
         double correction, corrected_q, corrected_db;
         karlin(nullptr, opt, profile->readLen, &correction, &corrected_q, &corrected_db);
         for (int k = 0; k < m; ++k) {
@@ -629,30 +735,7 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
 
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
                   options_t* opt, int* scores, int* lengths) {
-    if (n <= 0) return;
-    const ResidentText& rt = resident_text(db);
-    std::vector<tpz_seed> seeds(n);
-    for (int k = 0; k < n; ++k) {
-        if (db->SA && (positions[k] < 0 || (uint64_t)positions[k] >= db->Tsize)) {
-            fprintf(stderr, "topaz_amd: sa_hsp_batch: suffix-array slot %lld outside the text\n",
-                    (long long)positions[k]);
-            _exit(EXIT_FAILURE);
-        }
-        seeds[k] = tpz_seed{0, qoffsets[k], positions[k]};
-    }
-    const int64_t qo = 0;
-    const int32_t ql = q->seq_len;
-    std::vector<int32_t> sc(n), ln(n);
-    tpz_gapless_batch* b = nullptr;
-    if (tpz_gapless_batch_create(&b, rt.text, 1, q->seq_buf.buf, &qo, &ql, n, seeds.data(),
-                                 (const int8_t*)*opt->substitution) ||
-        tpz_gapless_batch_run(b) || tpz_gapless_batch_results(b, sc.data(), ln.data()))
-        die("sa_hsp_batch");
-    tpz_gapless_batch_destroy(b);
-    for (int k = 0; k < n; ++k) {
-        lengths[k] = ln[k];
-        if (ln[k] != -1) scores[k] = sc[k];                          // search.c:113-116
-    }
+    gapless_seeds(db, q, n, qoffsets, positions, opt, scores, lengths, "sa_hsp_batch");
 }
 
 }  // extern "C"
