@@ -69,3 +69,32 @@ int main(void) {
         op = spec.get(letter, 0)
         assert (c, int(v), back, int(n)) == (letter, 37 * 16 + op, "MIDNSHP=X"[op], 37)
     assert out[11] == "M" * 7
+
+
+BATCH = ROOT / "oracle" / "_ref" / "batch"
+
+
+@pytest.mark.skipif(not (BATCH / "query_batch.o").exists() or not (BATCH / "suffix_batch.o").exists(),
+                    reason="patched TOPAZ objects not built (make -C oracle ref)")
+def test_patched_topaz_objects_call_the_batch_entry_points():
+    """oracle/patch_query.py and patch_suffix.py applied to TOPAZ's query.c
+    and suffix.c: the objects call the batch entry points the compat header
+    declares, and query_batch.o no longer calls the per-hit or per-seed ones
+    from its loops (perform_alignment_with_profile is still defined there,
+    only no longer called by query_perform_alignments)."""
+    def symbols(obj):
+        out = subprocess.run(["nm", str(obj)], check=True, capture_output=True, text=True).stdout
+        undef = {l.split()[-1] for l in out.splitlines() if l.split()[0] == "U"}
+        defined = {l.split()[-1] for l in out.splitlines() if len(l.split()) == 3}
+        return undef, defined
+    qu, qd = symbols(BATCH / "query_batch.o")
+    su, sd = symbols(BATCH / "suffix_batch.o")
+    assert {"align_batch_using_ssw_profile", "stats_batch_using_ssw_profile", "sa_hsp_batch",
+            "suffix_prefetched"} <= qu
+    assert "stats_using_ssw_profile" not in qu and "sa_hsp" not in qu
+    assert {"query_find_suffixes", "query_perform_alignments", "query_perform_statistics"} <= qd
+    assert "sa_hsp_batch" in su and "suffix_prefetched" in sd
+    assert {"suffix_alloc", "suffix_increment", "suffix_duplicate"} <= sd
+    lib = C.CDLL(str(_lib.LIB_PATH))
+    for name in ("align_batch_using_ssw_profile", "stats_batch_using_ssw_profile", "sa_hsp_batch"):
+        assert hasattr(lib, name)
