@@ -365,7 +365,8 @@ def serving_leg(args, rates=(10, 50, 100, 200), seconds=3.0):
     db = (dr.astype(np.int8), do, dn)
     print("[bench] serving workload: %d subjects, %d requests in the pool, generated in %.1f s"
           % (n_db, len(pool), time.perf_counter() - t0), file=sys.stderr, flush=True)
-    srv = Server(db, matrix(), 11, 1, memory_budget=1 << 30, max_inflight=2)
+    srv = Server(db, matrix(), 11, 1, memory_budget=1 << 30, max_inflight=2,
+                 waves_per_group=args.serve_waves)
     offline = []
     for req in pool:
         offline.append(srv.wait(srv.submit(*req))[:3])
@@ -418,8 +419,8 @@ def serving_leg(args, rates=(10, 50, 100, 200), seconds=3.0):
     srv.close()
     return {"workload": "serving: %d resident subjects (lognormal mean 350 sd 250 clipped "
                         "[30,5000]); requests of U{1..500} queries x 200 candidates, Poisson "
-                        "arrivals, %.0f s per rate, memory budget 1 GiB, 2 launches in flight"
-                        % (n_db, seconds),
+                        "arrivals, %.0f s per rate, memory budget 1 GiB, 2 launches in flight, "
+                        "%d-wavefront workgroups" % (n_db, seconds, args.serve_waves),
             "rates": rows, "identical_to_offline": identical,
             "peak_admitted_bytes": st["peak_bytes"], "workspace_bytes": st["workspace_bytes"]}
 
@@ -454,6 +455,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
     ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "copy"], default="all")
+    ap.add_argument("--serve-waves", type=int, default=8, choices=[4, 8],
+                    help="wavefronts per workgroup of the serving loop's launches")
     ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="fraction of the 100,000-subject serving database")
     ap.add_argument("--gapless-scale", type=float, default=1.0,

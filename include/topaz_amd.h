@@ -77,15 +77,17 @@ int tpz_sw_batch_kernel_ms(const tpz_sw_batch* b, double* ms);
  * them (bit-identical for every configuration). */
 typedef struct {
     int32_t rows_per_chunk;    /* query rows a wavefront holds in registers: 16 or 32  */
-    int32_t groups_per_cu;     /* resident 4-wavefront workgroups per CU (persistent):
-                                  1 up to what the kernel's registers and LDS allow
-                                  (2 for both row counts on gfx950)                   */
+    int32_t groups_per_cu;     /* resident workgroups per CU (persistent): 1 up to what
+                                  the kernel's registers and LDS allow (on gfx950 2 with
+                                  4 wavefronts, 1 with 8)                               */
     int32_t queries_per_task;  /* queries whose pairs may share a wavefront: 1, 2, 4   */
     int32_t use_graph;         /* nonzero: replay the launches from a captured hipGraph */
     double big_fraction;       /* a task costing more than this fraction of one
-                                  wavefront's share of the batch is split over the 4
+                                  wavefront's share of the batch is split over the
                                   wavefronts of a workgroup (chunk staircase); 0: all
                                   tasks                                                 */
+    int32_t waves_per_group;   /* wavefronts per workgroup: 4 or 8 (8 halves a
+                                  staircase task's time, one workgroup per CU)         */
 } tpz_sw_config;
 
 /* The configuration the batch runs with (defaults: 32, 2, 4, 0, 0.5). */
@@ -121,6 +123,7 @@ typedef struct {
                                  lists, results) on the device at once; 0: 1 GiB     */
     int32_t max_inflight;     /* launches in flight (one stream each); 0: 2          */
     int32_t max_launch_pairs; /* pairs merged into one launch at most; 0: 4,000,000  */
+    int32_t waves_per_group;  /* wavefronts per workgroup, 4 or 8; 0: 8 (latency)     */
 } tpz_server_config;
 
 typedef struct {

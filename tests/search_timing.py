@@ -54,6 +54,10 @@ def main():
     ap.add_argument("--db", type=int, default=100000)
     ap.add_argument("--queries", type=int, default=1000)
     ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--gpu-threads", type=int, default=None,
+                    help="worker threads of the GPU-backed binaries (default: --threads); their "
+                         "threads mostly wait on the GPU, so more of them than cores keeps more "
+                         "requests in each launch")
     ap.add_argument("--timeout", type=int, default=900)
     ap.add_argument("--skip", nargs="*", default=[], help="binaries not to run (dropin_gpu, ...)")
     ap.add_argument("--out", default=None, help="directory for each search's output as printed")
@@ -79,11 +83,13 @@ def main():
         exes = {"reference_cpu": REF / "topaz", "dropin_gpu": _link_dropin(tmp),
                 "batch_gpu": _link_dropin(tmp, batch=True)}
         exes = {k: v for k, v in exes.items() if k not in args.skip}
-        flags = ["-T", str(args.threads), "-H", "100", "-E", "0.001"]
+        flags = ["-H", "100", "-E", "0.001"]
+        gpu_threads = args.gpu_threads or args.threads
         out, times = {}, {}
         for name, exe in exes.items():
-            text, dt = run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB")]
-                           + flags, str(tmp), args.timeout,
+            t = args.threads if name == "reference_cpu" else gpu_threads
+            text, dt = run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB"),
+                            "-T", str(t)] + flags, str(tmp), args.timeout,
                            os.path.join(args.out, name + ".txt") if args.out else None)
             out[name], times[name] = text, dt
             print("[search] %s: %.2f s, %d lines" % (name, dt, text.count("\n")),
@@ -91,6 +97,7 @@ def main():
         ref = _per_query(out["reference_cpu"])
         identical = {k: _per_query(v) == ref for k, v in out.items()}
     res = {"subjects": len(db), "queries": len(queries), "threads": args.threads,
+           "gpu_threads": gpu_threads,
            "flags": " ".join(flags), "index_s": round(t_index, 2),
            "search_s": {k: round(v, 2) for k, v in times.items()},
            "queries_per_s": {k: round(len(queries) / v, 1) for k, v in times.items()},

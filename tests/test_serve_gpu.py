@@ -49,11 +49,12 @@ def batch_results(db, req):
     return r
 
 
-def test_alone_equals_mix_and_batch():
+@pytest.mark.parametrize("waves", [4, 8])
+def test_alone_equals_mix_and_batch(waves):
     rng = np.random.default_rng(41)
     db = database(rng)
     reqs = requests(rng, db, 14)
-    srv = Server(db, matrix(), 11, 1, max_inflight=2)
+    srv = Server(db, matrix(), 11, 1, max_inflight=2, waves_per_group=waves)
     alone = []
     for q, pq, ps in reqs:
         alone.append(srv.wait(srv.submit(q, pq, ps))[:3])

@@ -145,7 +145,12 @@ def test_packed_tasks_mix_queries(schedule):
 CONFIGS = [dict(rows_per_chunk=16, groups_per_cu=2, queries_per_task=4),
            dict(rows_per_chunk=16, groups_per_cu=1, queries_per_task=1, big_fraction=1e30),
            dict(rows_per_chunk=32, groups_per_cu=1, queries_per_task=2, big_fraction=0.01),
-           dict(rows_per_chunk=16, groups_per_cu=2, queries_per_task=2, big_fraction=0.01)]
+           dict(rows_per_chunk=16, groups_per_cu=2, queries_per_task=2, big_fraction=0.01),
+           dict(rows_per_chunk=32, groups_per_cu=1, queries_per_task=4, waves_per_group=8),
+           dict(rows_per_chunk=32, groups_per_cu=1, queries_per_task=1, waves_per_group=8,
+                big_fraction=0.0),
+           dict(rows_per_chunk=16, groups_per_cu=1, queries_per_task=4, waves_per_group=8,
+                big_fraction=0.01)]
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)

@@ -172,7 +172,7 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.out_score = dout;
     a.out_ref = dout + 1;
     a.out_read = dout + 2;
-    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, c.st), "sw_forward");
+    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, c.st), "sw_forward");
     hip_or_die(hipMemcpyAsync(out, dout, 12, hipMemcpyDeviceToHost, c.st), "readback");
     hip_or_die(hipStreamSynchronize(c.st), "sw_forward");
 }
@@ -615,6 +615,7 @@ tpz_server* db_server(const db_t* db, const int8_t* mat, int go, int ge) {
     }
     tpz_server_config cfg{};
     cfg.max_inflight = 2;
+    cfg.waves_per_group = 8;     // a query's hits are a task or two: 8-wavefront staircases
     auto* s = new DbServer{db, db->T, db->Psize, {0}, go, ge, nullptr};
     memcpy(s->matrix, mat, 625);
     if (tpz_server_create(&s->srv, (int32_t)db->Psize, res.data(), off.data(), len.data(), mat,

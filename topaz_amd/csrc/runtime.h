@@ -32,7 +32,7 @@ struct DeviceBuffer {
 
 // ---- batched Smith-Waterman planning (shared by batches and the server) ----
 constexpr int kGroupsPerCu = 2;     // default resident sw_forward workgroups per CU
-constexpr int kGroupWaves = 4;      // wavefronts per sw_forward workgroup
+constexpr int kGroupWaves = 4;      // default wavefronts per sw_forward workgroup (4 or 8)
 constexpr int kTaskPairs = 128;     // two pairs per lane of a wavefront
 constexpr int kTaskQueries = 4;     // queries whose profiles a wavefront can hold
 
@@ -75,9 +75,10 @@ void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
 void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
                         const std::vector<int32_t>& pair_subject, const std::vector<int32_t>& q_len,
                         const std::vector<int32_t>& s_len, int rows, int nq_max, Tasks& t);
-// Work items of the persistent schedule (4 entries each), most expensive first.
-void build_items(const Tasks& t, int n_slots, double big_fraction, std::vector<int32_t>& items,
-                 int32_t& n_big);
+// Work items of the persistent schedule (`waves` entries each), most
+// expensive first.
+void build_items(const Tasks& t, int n_slots, double big_fraction, int waves,
+                 std::vector<int32_t>& items, int32_t& n_big);
 
 // Letter codes of a text for the gapless kernel (see gapless_kernels.h).
 struct TextCodes {

@@ -231,7 +231,7 @@ namespace tpz {
 // is worked by a whole workgroup (chunks interleaved over 4 wavefronts);
 // the others go four to an item, one per wavefront, neighbours in cost
 // order.  Items run most expensive first.
-void build_items(const Tasks& t, int n_slots, double big_fraction,
+void build_items(const Tasks& t, int n_slots, double big_fraction, int waves,
                         std::vector<int32_t>& items, int32_t& n_big) {
     const size_t n = t.cost.size();
     std::vector<int32_t> order(n);
@@ -240,19 +240,22 @@ void build_items(const Tasks& t, int n_slots, double big_fraction,
                      [&](int32_t x, int32_t y) { return t.cost[x] > t.cost[y]; });
     double total = 0;
     for (int64_t c : t.cost) total += (double)c;
-    const double share = total / std::max(1, n_slots * kGroupWaves);
-    std::vector<std::pair<double, std::array<int32_t, 4>>> list;
+    const double share = total / std::max(1, n_slots * waves);
+    std::vector<std::pair<double, std::vector<int32_t>>> list;
     n_big = 0;
     size_t k = 0;
     while (k < n && t.cost[order[k]] > big_fraction * share && t.cost[order[k]] > 0) {
-        list.push_back({t.cost[order[k]] / (double)kGroupWaves, {order[k], -2, -1, -1}});
+        std::vector<int32_t> it(waves, -1);
+        it[0] = order[k];
+        it[1] = -2;
+        list.push_back({t.cost[order[k]] / (double)waves, it});
         ++n_big;
         ++k;
     }
     while (k < n) {
-        std::array<int32_t, 4> it{-1, -1, -1, -1};
+        std::vector<int32_t> it(waves, -1);
         const double c0 = (double)t.cost[order[k]];
-        for (int w = 0; w < 4 && k < n; ++w) it[w] = order[k++];
+        for (int w = 0; w < waves && k < n; ++w) it[w] = order[k++];
         list.push_back({c0, it});
     }
     std::stable_sort(list.begin(), list.end(),
@@ -268,17 +271,21 @@ static tpz_sw_config default_config() {
     c.queries_per_task = kTaskQueries;
     c.use_graph = 0;
     c.big_fraction = 0.5;
+    c.waves_per_group = kGroupWaves;
     return c;
 }
 
 static int check_config(const tpz_sw_config& c) {
     if (c.rows_per_chunk != 16 && c.rows_per_chunk != 32)
         return fail("rows_per_chunk must be 16 or 32 (got %d)", c.rows_per_chunk);
+    if (c.waves_per_group != 4 && c.waves_per_group != 8)
+        return fail("waves_per_group must be 4 or 8 (got %d)", c.waves_per_group);
     // resident workgroups per CU allowed by the kernel's registers and LDS
-    const int max_groups = std::max(1, tpz_sw_forward_groups_per_cu(c.rows_per_chunk));
+    const int max_groups =
+        std::max(1, tpz_sw_forward_groups_per_cu(c.rows_per_chunk, c.waves_per_group));
     if (c.groups_per_cu < 1 || c.groups_per_cu > max_groups)
-        return fail("groups_per_cu must be 1..%d for %d rows (got %d)", max_groups,
-                    c.rows_per_chunk, c.groups_per_cu);
+        return fail("groups_per_cu must be 1..%d for %d rows and %d waves (got %d)", max_groups,
+                    c.rows_per_chunk, c.waves_per_group, c.groups_per_cu);
     if (c.queries_per_task != 1 && c.queries_per_task != 2 && c.queries_per_task != 4)
         return fail("queries_per_task must be 1, 2 or 4 (got %d)", c.queries_per_task);
     if (!(c.big_fraction >= 0)) return fail("big_fraction must be >= 0");
@@ -295,7 +302,8 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
     build_packed_tasks(b->per_query, b->pair_subject, b->q_len, b->s_len, c.rows_per_chunk,
                        c.queries_per_task, plan->packed_tasks.host);
     int rc = plan->plain_tasks.upload() | plan->packed_tasks.upload();
-    // plain: per query, its tasks four to an item
+    // plain: per query, its tasks one per wavefront of an item
+    const int W = c.waves_per_group;
     size_t max_items_per_query = 0;
     {
         const Tasks& t = plan->plain_tasks.host;
@@ -304,24 +312,24 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
         while (k < n_tasks) {
             int e = k;
             while (e < n_tasks && t.query[kTaskQueries * e] == t.query[kTaskQueries * k]) ++e;
-            plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / 4));
-            for (int j = k; j < e; j += 4)
-                for (int w = 0; w < 4; ++w) plan->items_plain.push_back(j + w < e ? j + w : -1);
-            max_items_per_query = std::max<size_t>(max_items_per_query, (size_t)(e - k + 3) / 4);
+            plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / W));
+            for (int j = k; j < e; j += W)
+                for (int w = 0; w < W; ++w) plan->items_plain.push_back(j + w < e ? j + w : -1);
+            max_items_per_query = std::max<size_t>(max_items_per_query, (size_t)(e - k + W - 1) / W);
             k = e;
         }
-        plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / 4));
+        plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / W));
     }
     const size_t persistent_slots = (size_t)std::max(num_cu(), 1) * c.groups_per_cu;
-    build_items(plan->packed_tasks.host, (int)persistent_slots, c.big_fraction,
+    build_items(plan->packed_tasks.host, (int)persistent_slots, c.big_fraction, W,
                 plan->items_persistent, plan->big_tasks);
     rc |= plan->d_items_plain.upload(plan->items_plain.data(), plan->items_plain.size() * 4);
     rc |= plan->d_items_persistent.upload(plan->items_persistent.data(),
                                           plan->items_persistent.size() * 4);
     // workspace: per workgroup, one boundary buffer (bottom H and F of a
     // chunk: 2 words per column and lane) per wavefront
-    plan->ws_words_per_slot = (int64_t)kGroupWaves * 2 * (b->max_subject + 2) * 64;
-    const size_t n_items_p = plan->items_persistent.size() / 4;
+    plan->ws_words_per_slot = (int64_t)W * 2 * (b->max_subject + 2) * 64;
+    const size_t n_items_p = plan->items_persistent.size() / W;
     plan->ws_slots = std::max<size_t>(std::max(std::min(persistent_slots, n_items_p),
                                                max_items_per_query), 1);
     rc |= plan->d_workspace.alloc(plan->ws_slots * (size_t)plan->ws_words_per_slot * 4);
@@ -472,21 +480,21 @@ static SwKernelArgs make_args(tpz_sw_batch* b) {
 // captures).  The persistent counter is cleared by a kernel, not a memset.
 static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
     const SwPlan& pl = *b->plan;
-    const int rows = pl.cfg.rows_per_chunk;
+    const int rows = pl.cfg.rows_per_chunk, W = pl.cfg.waves_per_group;
     SwKernelArgs a = make_args(b);
     const DeviceTasks& dt = schedule == TPZ_SCHED_PERSISTENT ? pl.packed_tasks : pl.plain_tasks;
     a.pair_index = (const int32_t*)dt.d_index.ptr;
     a.task_begin = (const int32_t*)dt.d_begin.ptr;
     a.task_query = (const int32_t*)dt.d_query.ptr;
     if (schedule == TPZ_SCHED_PERSISTENT) {
-        const int n_items = (int)(pl.items_persistent.size() / 4);
+        const int n_items = (int)(pl.items_persistent.size() / W);
         if (n_items > 0) {
             a.item_task = (const int32_t*)pl.d_items_persistent.ptr;
             a.n_items = n_items;
             a.task_counter = (int32_t*)b->d_counter.ptr;
             if (check(tpz_launch_fill_i32(a.task_counter, 0, 1, stream()), "clear counter")) return -1;
             const int grid = (int)std::min<size_t>((size_t)n_items, pl.ws_slots);
-            if (check(tpz_launch_sw_forward(&a, grid, rows, stream()), "sw_forward (persistent)"))
+            if (check(tpz_launch_sw_forward(&a, grid, rows, W, stream()), "sw_forward (persistent)"))
                 return -1;
         }
     } else {
@@ -494,9 +502,9 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
         a.task_counter = nullptr;
         for (size_t qk = 0; qk + 1 < pl.plain_query_items.size(); ++qk) {
             const int i0 = pl.plain_query_items[qk], i1 = pl.plain_query_items[qk + 1];
-            a.item_task = (const int32_t*)pl.d_items_plain.ptr + 4 * (size_t)i0;
+            a.item_task = (const int32_t*)pl.d_items_plain.ptr + (size_t)W * i0;
             a.n_items = i1 - i0;
-            if (check(tpz_launch_sw_forward(&a, i1 - i0, rows, stream()), "sw_forward (plain)"))
+            if (check(tpz_launch_sw_forward(&a, i1 - i0, rows, W, stream()), "sw_forward (plain)"))
                 return -1;
         }
     }
@@ -562,7 +570,7 @@ int tpz_sw_batch_autotune(tpz_sw_batch* b, tpz_sw_config* best, double* best_ms,
     const tpz_sw_config start = b->plan->cfg;
     std::vector<tpz_sw_config> cands;
     for (int rows : {32, 16}) {
-        const int max_groups = std::max(1, tpz_sw_forward_groups_per_cu(rows));
+        const int max_groups = std::max(1, tpz_sw_forward_groups_per_cu(rows, 4));
         for (int groups = 1; groups <= max_groups; ++groups)
             for (int nq : {4, 2, 1})
                 for (double bf : {0.25, 0.5, 1.0, 1e30}) {
@@ -570,9 +578,18 @@ int tpz_sw_batch_autotune(tpz_sw_batch* b, tpz_sw_config* best, double* best_ms,
                     tpz_sw_config c = start;
                     c.rows_per_chunk = rows; c.groups_per_cu = groups;
                     c.queries_per_task = nq; c.big_fraction = bf; c.use_graph = 0;
+                    c.waves_per_group = 4;
                     cands.push_back(c);
                 }
     }
+    // eight wavefronts per workgroup (one workgroup per CU), 32 rows
+    if (tpz_sw_forward_groups_per_cu(32, 8) >= 1)
+        for (double bf : {0.5, 1.0}) {
+            tpz_sw_config c = start;
+            c.rows_per_chunk = 32; c.groups_per_cu = 1; c.queries_per_task = 4;
+            c.big_fraction = bf; c.use_graph = 0; c.waves_per_group = 8;
+            cands.push_back(c);
+        }
     const size_t n = (size_t)b->n_pairs;
     std::vector<int32_t> ref_sc(n), ref_re(n), ref_rd(n), sc(n), re(n), rd(n);
     double best_t = 1e300;

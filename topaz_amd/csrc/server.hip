@@ -192,8 +192,9 @@ int launch_slot(tpz_server* s, Slot& sl) {
     }
     std::vector<int32_t> items;
     int32_t n_big = 0;
-    build_items(t, s->grid, 1.0, items, n_big);
-    const int n_items = (int)(items.size() / 4);
+    const int W = s->cfg.waves_per_group;
+    build_items(t, s->grid, 1.0, W, items, n_big);
+    const int n_items = (int)(items.size() / W);
 
     // one host blob, one copy: residues | q_off | q_len | pq | ps | index | begin | tquery | items
     const size_t o_res = 0;
@@ -256,7 +257,7 @@ int launch_slot(tpz_server* s, Slot& sl) {
         a.out_ref = dout + np;
         a.out_read = dout + 2 * np;
         const int grid = std::min(n_items, s->grid);
-        if (check(tpz_launch_sw_forward(&a, grid, 32, sl.stream), "sw_forward")) return -1;
+        if (check(tpz_launch_sw_forward(&a, grid, 32, W, sl.stream), "sw_forward")) return -1;
     }
     if (check(hipMemcpyAsync(sl.h_out.ptr, dout, out_words * 4, hipMemcpyDeviceToHost, sl.stream),
               "readback") ||
@@ -385,6 +386,10 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
     if (s->cfg.memory_budget == 0) s->cfg.memory_budget = (uint64_t)1 << 30;
     if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
     if (s->cfg.max_launch_pairs <= 0) s->cfg.max_launch_pairs = 4000000;
+    // 8 wavefronts per staircase: a request's latency is its longest pair's
+    if (s->cfg.waves_per_group == 0) s->cfg.waves_per_group = 8;
+    if (s->cfg.waves_per_group != 4 && s->cfg.waves_per_group != 8)
+        return fail("waves_per_group must be 4 or 8 (got %d)", s->cfg.waves_per_group);
     s->s_len.assign(s_len, s_len + n_subjects);
     // subjects: each starts on a 4-byte boundary, padded with residue 25
     std::vector<int64_t> so(n_subjects);
@@ -404,8 +409,9 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
         }
     int rc = s->d_s_res.upload(sr.data(), sr.size()) | s->d_s_off.upload(so.data(), so.size() * 8) |
              s->d_s_len.upload(s_len, (size_t)n_subjects * 4) | s->d_matrix.upload(matrix, 625);
-    s->grid = std::max(num_cu(), 1) * kGroupsPerCu;
-    s->ws_words_per_slot = (int64_t)kGroupWaves * 2 * (s->max_subject + 2) * 64;
+    s->grid = std::max(num_cu(), 1) *
+              std::max(1, std::min(kGroupsPerCu, tpz_sw_forward_groups_per_cu(32, s->cfg.waves_per_group)));
+    s->ws_words_per_slot = (int64_t)s->cfg.waves_per_group * 2 * (s->max_subject + 2) * 64;
     for (int k = 0; k < s->cfg.max_inflight && !rc; ++k) {
         auto sl = std::make_unique<Slot>();
         rc |= check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");

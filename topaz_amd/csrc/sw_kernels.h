@@ -44,12 +44,13 @@ struct SwKernelArgs {
 
 extern "C" hipError_t tpz_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipStream_t stream);
 
-// Workgroups of sw_forward_kernel<rows> resident per CU (registers and LDS);
-// 0 on error.
-extern "C" int tpz_sw_forward_groups_per_cu(int rows);
+// Workgroups of sw_forward_kernel<rows, waves> resident per CU (registers
+// and LDS); 0 on error.
+extern "C" int tpz_sw_forward_groups_per_cu(int rows, int waves);
 
-// rows: query rows per chunk, 16 or 32.
+// rows: query rows per chunk, 16 or 32; waves: wavefronts per workgroup, 4
+// or 8 (work items then have `waves` entries).
 extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid, int rows,
-                                            hipStream_t stream);
+                                            int waves, hipStream_t stream);
 
 #endif

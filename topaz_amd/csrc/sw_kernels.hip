@@ -110,7 +110,7 @@ struct LaneBest {
     int chunk[2];
 };
 
-constexpr int kWaves = 4;          // wavefronts per workgroup
+// Wavefronts per workgroup (W, 4 or 8) is a template parameter of the kernel.
 constexpr int kSyncCols = 8;       // progress is published every 8 columns
 constexpr int kAhead = 16;         // a consumer runs at least 16 columns behind
 constexpr int kSpinLimit = 1 << 24;
@@ -503,21 +503,21 @@ __device__ __forceinline__ void write_best(const SwKernelArgs& a, const TaskView
     }
 }
 
-// LDS of one workgroup.
-template <int R>
+// LDS of one workgroup of W wavefronts.
+template <int R, int W>
 struct GroupLds {
-    WaveLds<R> wave[kWaves];
+    WaveLds<R> wave[W];
     int8_t mat[640];                     // the 25 x 25 substitution matrix
     int item;
     int error;
-    int progress[kWaves];
-    int bscore[kWaves][2][64], bref[kWaves][2][64], bread[kWaves][2][64], bchunk[kWaves][2][64];
+    int progress[W];
+    int bscore[W][2][64], bref[W][2][64], bread[W][2][64], bchunk[W][2][64];
 };
 
 // A task worked by one wavefront: its chunks in order, boundary in place.
-template <int R>
+template <int R, int W>
 __device__ void run_task_wave(const SwKernelArgs& a, WaveLds<R>& lds, uint32_t* buf, int task,
-                              GroupLds<R>& g) {
+                              GroupLds<R, W>& g) {
     const TaskView t = task_view(a, task);
     LaneBest best;
     init_best(best, t);
@@ -537,11 +537,11 @@ __device__ void run_task_wave(const SwKernelArgs& a, WaveLds<R>& lds, uint32_t* 
     write_best(a, t, best);
 }
 
-// A task worked by all wavefronts of the workgroup: chunk c by wavefront
-// c % kWaves, each one trailing the wavefront of the chunk above by at least
-// kAhead columns; chunk c's bottom boundary lives in buffer c % kWaves.
-template <int R>
-__device__ void run_task_group(const SwKernelArgs& a, GroupLds<R>& g, uint32_t* bufs,
+// A task worked by all W wavefronts of the workgroup: chunk c by wavefront
+// c % W, each one trailing the wavefront of the chunk above by at least
+// kAhead columns; chunk c's bottom boundary lives in buffer c % W.
+template <int R, int W>
+__device__ void run_task_group(const SwKernelArgs& a, GroupLds<R, W>& g, uint32_t* bufs,
                                size_t buf_words, int task) {
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const TaskView t = task_view(a, task);
@@ -549,22 +549,22 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds<R>& g, uint32_t* 
     init_best(best, t);
     const int n_chunks = (t.m_max + R - 1) / R;
     const int nc = t.ncols;
-    for (int c = wave; c < n_chunks; c += kWaves) {
+    for (int c = wave; c < n_chunks; c += W) {
         const int j0 = c * R;
         build_profile(g.mat, g.wave[wave], t, j0);
         ChunkIO io{};
-        // chunk c writes buffer c % kWaves, read by chunk c + 1; the wave of
-        // chunk c - 3 (which read it last) has passed every column first
-        io.bottom = bufs + (size_t)(c % kWaves) * buf_words;
+        // chunk c writes buffer c % W, read by chunk c + 1; the wave of
+        // chunk c - W + 1 (which read it last) has passed every column first
+        io.bottom = bufs + (size_t)(c % W) * buf_words;
         io.has_bottom = c + 1 < n_chunks;
         io.has_top = c > 0;
-        io.top = c > 0 ? bufs + (size_t)((c - 1) % kWaves) * buf_words : io.bottom;
+        io.top = c > 0 ? bufs + (size_t)((c - 1) % W) * buf_words : io.bottom;
         if (c > 0) {
-            io.wait_ctr = &g.progress[(c - 1) % kWaves];
-            io.wait_base = ((c - 1) / kWaves) * nc;
+            io.wait_ctr = &g.progress[(c - 1) % W];
+            io.wait_base = ((c - 1) / W) * nc;
         }
         io.pub_ctr = &g.progress[wave];
-        io.pub_base = (c / kWaves) * nc;
+        io.pub_base = (c / W) * nc;
         io.error = &g.error;
         io.g_error = a.error;
         if (!run_chunk(a, g.wave[wave], io, nc, t.s_lo, t.n_lo, t.s_hi, t.n_hi, t.off_lo,
@@ -583,7 +583,7 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds<R>& g, uint32_t* 
     __syncthreads();
     if (wave == 0) {
         for (int h = 0; h < 2; ++h) {
-            for (int w = 1; w < kWaves; ++w) {
+            for (int w = 1; w < W; ++w) {
                 const int sc = g.bscore[w][h][lane], rf = g.bref[w][h][lane];
                 const int ck = g.bchunk[w][h][lane];
                 if (sc <= 0) continue;
@@ -602,32 +602,33 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds<R>& g, uint32_t* 
     }
 }
 
-// Work items: item k is either one large task worked by the whole workgroup
-// (item_task[4k] >= 0, item_task[4k+1] == -2) or up to four small tasks,
-// one per wavefront (-1 = none).  With a counter the grid is persistent and
-// workgroup b owns workspace slot b; without one, workgroup b runs item b.
-template <int R>
-__global__ void __launch_bounds__(kWaves * 64)
+// Work items (W entries each): item k is either one large task worked by the
+// whole workgroup (item_task[W k] >= 0, item_task[W k + 1] == -2) or up to W
+// small tasks, one per wavefront (-1 = none).  With a counter the grid is
+// persistent and workgroup b owns workspace slot b; without one, workgroup b
+// runs item b.
+template <int R, int W>
+__global__ void __launch_bounds__(W * 64)
 sw_forward_kernel(SwKernelArgs a) {
-    __shared__ GroupLds<R> g;
+    __shared__ GroupLds<R, W> g;
     const int wave = threadIdx.x / 64;
-    const size_t buf_words = (size_t)a.ws_words_per_slot / kWaves;
+    const size_t buf_words = (size_t)a.ws_words_per_slot / W;
     uint32_t* bufs = a.workspace + (size_t)blockIdx.x * a.ws_words_per_slot;
     if (threadIdx.x == 0) g.error = 0;
     for (int e = threadIdx.x; e < 625; e += blockDim.x) g.mat[e] = a.matrix[e];
     for (;;) {
         if (threadIdx.x == 0) {
             g.item = a.task_counter ? atomicAdd(a.task_counter, 1) : (int)blockIdx.x;
-            for (int w = 0; w < kWaves; ++w) g.progress[w] = 0;
+            for (int w = 0; w < W; ++w) g.progress[w] = 0;
         }
         __syncthreads();
         const int item = g.item;
         if (item >= a.n_items || g.error) break;
-        const int t0 = a.item_task[4 * item];
-        if (a.item_task[4 * item + 1] == -2) {
+        const int t0 = a.item_task[W * item];
+        if (a.item_task[W * item + 1] == -2) {
             run_task_group(a, g, bufs, buf_words, t0);
         } else {
-            const int task = a.item_task[4 * item + wave];
+            const int task = a.item_task[W * item + wave];
             if (task >= 0) run_task_wave(a, g.wave[wave], bufs + (size_t)wave * buf_words, task, g);
         }
         __syncthreads();
@@ -649,27 +650,34 @@ extern "C" hipError_t tpz_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipS
     return hipGetLastError();
 }
 
-extern "C" int tpz_sw_forward_groups_per_cu(int rows) {
+namespace {
+template <int R, int W>
+int groups_per_cu() {
     int n = 0;
-    hipError_t e = hipErrorInvalidValue;
-    if (rows == 16)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tpz::sw_forward_kernel<16>,
-                                                         tpz::kWaves * 64, 0);
-    else if (rows == 32)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tpz::sw_forward_kernel<32>,
-                                                         tpz::kWaves * 64, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &n, tpz::sw_forward_kernel<R, W>, W * 64, 0);
     return e == hipSuccess ? n : 0;
+}
+template <int R, int W>
+hipError_t launch(const tpz::SwKernelArgs* args, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL((tpz::sw_forward_kernel<R, W>), dim3(grid), dim3(W * 64), 0, stream, *args);
+    return hipGetLastError();
+}
+}  // namespace
+
+extern "C" int tpz_sw_forward_groups_per_cu(int rows, int waves) {
+    if (rows == 16 && waves == 4) return groups_per_cu<16, 4>();
+    if (rows == 16 && waves == 8) return groups_per_cu<16, 8>();
+    if (rows == 32 && waves == 4) return groups_per_cu<32, 4>();
+    if (rows == 32 && waves == 8) return groups_per_cu<32, 8>();
+    return 0;
 }
 
 extern "C" hipError_t tpz_launch_sw_forward(const tpz::SwKernelArgs* args, int grid, int rows,
-                                            hipStream_t stream) {
-    if (rows == 16)
-        hipLaunchKernelGGL(tpz::sw_forward_kernel<16>, dim3(grid), dim3(tpz::kWaves * 64), 0,
-                           stream, *args);
-    else if (rows == 32)
-        hipLaunchKernelGGL(tpz::sw_forward_kernel<32>, dim3(grid), dim3(tpz::kWaves * 64), 0,
-                           stream, *args);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+                                            int waves, hipStream_t stream) {
+    if (rows == 16 && waves == 4) return launch<16, 4>(args, grid, stream);
+    if (rows == 16 && waves == 8) return launch<16, 8>(args, grid, stream);
+    if (rows == 32 && waves == 4) return launch<32, 4>(args, grid, stream);
+    if (rows == 32 && waves == 8) return launch<32, 8>(args, grid, stream);
+    return hipErrorInvalidValue;
 }

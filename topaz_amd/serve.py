@@ -17,7 +17,7 @@ from .sw import _concat
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("max_inflight", C.c_int32),
-                ("max_launch_pairs", C.c_int32)]
+                ("max_launch_pairs", C.c_int32), ("waves_per_group", C.c_int32)]
 
 
 class ServerStats(C.Structure):
@@ -53,14 +53,14 @@ class Server:
     lengths) triple, kept resident on the device."""
 
     def __init__(self, subjects, matrix, gap_open=11, gap_extend=1, memory_budget=0,
-                 max_inflight=0, max_launch_pairs=0):
+                 max_inflight=0, max_launch_pairs=0, waves_per_group=0):
         L = _lib.lib()
         _declare(L)
         s = subjects if isinstance(subjects, tuple) else _concat(subjects)
         sr, so, sl = (np.ascontiguousarray(s[0], np.int8), np.ascontiguousarray(s[1], np.int64),
                       np.ascontiguousarray(s[2], np.int32))
         mat = np.ascontiguousarray(matrix, np.int8).reshape(-1)
-        cfg = ServerConfig(memory_budget, max_inflight, max_launch_pairs)
+        cfg = ServerConfig(memory_budget, max_inflight, max_launch_pairs, waves_per_group)
         h = C.c_void_p()
         check(L.tpz_server_create(C.byref(h), len(sl), ptr(sr, P8), ptr(so, P64), ptr(sl, P32),
                                   ptr(mat, P8), gap_open, gap_extend, C.byref(cfg)),

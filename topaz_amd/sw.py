@@ -21,7 +21,7 @@ class SwConfig(C.Structure):
     on it)."""
     _fields_ = [("rows_per_chunk", C.c_int32), ("groups_per_cu", C.c_int32),
                 ("queries_per_task", C.c_int32), ("use_graph", C.c_int32),
-                ("big_fraction", C.c_double)]
+                ("big_fraction", C.c_double), ("waves_per_group", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
