@@ -16,6 +16,7 @@ The binaries come from the reference sources compiled by oracle/Makefile
 import argparse
 import json
 import os
+import resource
 import subprocess
 import sys
 import tempfile
@@ -33,7 +34,9 @@ from topaz_amd import synth  # noqa: E402
 
 def run(cmd, cwd, timeout, out_path=None):
     """Run cmd; its stdout goes to out_path as it is printed (so a long
-    search shows progress there) and is returned with the wall time."""
+    search shows progress there) and is returned with the wall time, stderr
+    and the child's CPU seconds (user + system)."""
+    r0 = resource.getrusage(resource.RUSAGE_CHILDREN)
     t0 = time.perf_counter()
     if out_path is None:
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=cwd)
@@ -44,9 +47,11 @@ def run(cmd, cwd, timeout, out_path=None):
                                timeout=timeout, cwd=cwd)
         out, err, rc = open(out_path).read(), r.stderr, r.returncode
     dt = time.perf_counter() - t0
+    r1 = resource.getrusage(resource.RUSAGE_CHILDREN)
+    cpu = (r1.ru_utime - r0.ru_utime) + (r1.ru_stime - r0.ru_stime)
     if rc != 0:
         raise SystemExit("%s failed: %s" % (cmd[0], err[-2000:]))
-    return out, dt
+    return out, dt, err, cpu
 
 
 def main():
@@ -61,6 +66,8 @@ def main():
     ap.add_argument("--timeout", type=int, default=900)
     ap.add_argument("--skip", nargs="*", default=[], help="binaries not to run (dropin_gpu, ...)")
     ap.add_argument("--out", default=None, help="directory for each search's output as printed")
+    ap.add_argument("--workdir", default=None,
+                    help="keep the database, index and binaries here (default: a temporary directory)")
     args = ap.parse_args()
     batch_objs = [REF / "batch" / f for f in ("query_batch.o", "suffix_batch.o")]
     if not (REF / "topaz").exists() or not all(p.exists() for p in batch_objs):
@@ -68,7 +75,8 @@ def main():
     if args.out:
         os.makedirs(args.out, exist_ok=True)
     with tempfile.TemporaryDirectory() as d:
-        tmp = Path(d)
+        tmp = Path(args.workdir or d)
+        tmp.mkdir(parents=True, exist_ok=True)
         rng = np.random.default_rng(2027)
         t0 = time.perf_counter()
         db, queries = synth.search_database(rng, args.db, args.queries, hom_frac=0.2, max_hom=5,
@@ -77,7 +85,7 @@ def main():
         synth.write_fasta(tmp / "q.fa", queries, "q")
         print("[search] %d subjects, %d queries written in %.1f s"
               % (len(db), len(queries), time.perf_counter() - t0), file=sys.stderr, flush=True)
-        _, t_index = run([str(REF / "topaz"), "index", "-f", str(tmp / "db.fa"),
+        _, t_index, _, _ = run([str(REF / "topaz"), "index", "-f", str(tmp / "db.fa"),
                           "-p", str(tmp / "DB")], str(tmp), args.timeout)
         print("[search] index built in %.1f s" % t_index, file=sys.stderr, flush=True)
         exes = {"reference_cpu": REF / "topaz", "dropin_gpu": _link_dropin(tmp),
@@ -85,21 +93,26 @@ def main():
         exes = {k: v for k, v in exes.items() if k not in args.skip}
         flags = ["-H", "100", "-E", "0.001"]
         gpu_threads = args.gpu_threads or args.threads
-        out, times = {}, {}
+        out, times, inner, cpu_s = {}, {}, {}, {}
         for name, exe in exes.items():
             t = args.threads if name == "reference_cpu" else gpu_threads
-            text, dt = run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB"),
+            text, dt, err, cpu_s[name] = run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB"),
                             "-T", str(t)] + flags, str(tmp), args.timeout,
                            os.path.join(args.out, name + ".txt") if args.out else None)
             out[name], times[name] = text, dt
-            print("[search] %s: %.2f s, %d lines" % (name, dt, text.count("\n")),
-                  file=sys.stderr, flush=True)
+            # TOPAZ's own clock: from the search's start, after the index is loaded
+            el = [ln for ln in err.split("\n") if ln.startswith("elapsed time:")]
+            inner[name] = float(el[-1].split()[2]) if el else None
+            print("[search] %s: %.2f s (TOPAZ's elapsed time %s s), %d lines"
+                  % (name, dt, inner[name], text.count("\n")), file=sys.stderr, flush=True)
         ref = _per_query(out["reference_cpu"])
         identical = {k: _per_query(v) == ref for k, v in out.items()}
     res = {"subjects": len(db), "queries": len(queries), "threads": args.threads,
            "gpu_threads": gpu_threads,
            "flags": " ".join(flags), "index_s": round(t_index, 2),
            "search_s": {k: round(v, 2) for k, v in times.items()},
+           "topaz_elapsed_s": inner,
+           "cpu_s": {k: round(v, 2) for k, v in cpu_s.items()},
            "queries_per_s": {k: round(len(queries) / v, 1) for k, v in times.items()},
            "hit_lines": out["reference_cpu"].count("\n"), "identical": identical,
            "host_cpus": os.cpu_count()}

@@ -8,8 +8,10 @@
 //                          stats_using_ssw_profile, align_using_ssw
 //   search.h:14            sa_hsp
 // so that query.c / suffix.c link against this library unchanged.  Each call
-// runs on the GPU (a per-thread HIP stream with reusable device buffers);
-// the batch entry points of topaz_amd.h are the fast path for new callers.
+// runs on the GPU: the calls of all host threads share the library's stream,
+// each thread with its own device and pinned host buffers and its own event
+// to wait on; the batch entry points of topaz_amd.h are the fast path for
+// new callers.
 // Errors are fatal, as in the reference (message + _exit(EXIT_FAILURE)):
 // the calls have no error return and must never return altered scores.
 #include <hip/hip_runtime.h>
@@ -21,6 +23,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -48,29 +51,54 @@ void hip_or_die(hipError_t e, const char* what) {
     }
 }
 
-// Grow-only device buffer owned by one host thread.  It grows to at least
-// twice its size, so that a run of calls of slowly growing sizes does not
-// reallocate (hipFree waits for the device) on every call.
-struct Scratch {
+constexpr size_t kScratchMin = (size_t)1 << 20;
+
+// Grow-only buffers owned by one host thread.  A buffer grows to at least
+// twice its size, so a run of slowly growing calls reallocates rarely, and
+// the buffer it replaces is kept, not freed, until the process ends: hipFree
+// and hipHostFree wait for the whole device, which would hold the thread up
+// behind every other thread's launches.  The kept buffers add up to less
+// than the live one.
+struct Scratch {                      // device memory
     void* p = nullptr;
     size_t n = 0;
+    std::vector<void*> kept;
     void* get(size_t bytes) {
         if (bytes > n) {
-            if (p) hip_or_die(hipFree(p), "hipFree");
-            n = std::max(std::max(bytes, 2 * n), (size_t)65536);
+            if (p) kept.push_back(p);
+            n = std::max(std::max(bytes, 2 * n), kScratchMin);
             hip_or_die(hipMalloc(&p, n), "hipMalloc");
         }
         return p;
     }
 };
+// Page-locked, coherent host memory: copies from it need no staging, and
+// kernels read their inputs from it and write their results into it
+// directly (the event wait after the launch makes the results visible).
+struct Pinned {
+    void* p = nullptr;
+    size_t n = 0;
+    std::vector<void*> kept;
+    char* get(size_t bytes) {
+        if (bytes > n) {
+            if (p) kept.push_back(p);
+            n = std::max(std::max(bytes, 2 * n), 4 * kScratchMin);
+            hip_or_die(hipHostMalloc(&p, n, hipHostMallocCoherent), "hipHostMalloc");
+        }
+        return (char*)p;
+    }
+};
 
-// Per host thread: a stream and the buffers of one call (TOPAZ calls the
-// alignment functions from T worker threads at once, engine.c:423-457).
+// Per host thread: the buffers of one call and the event its wait is on
+// (TOPAZ calls the alignment functions from T worker threads at once,
+// engine.c:423-457).  The work itself goes to the library's one stream: a
+// stream per thread would spread the threads over the few hardware queues
+// HIP multiplexes streams onto, where a thread's short launch would wait
+// behind a serving-loop launch of milliseconds on the same queue.
 struct ThreadCtx {
-    hipStream_t st = nullptr;
-    Scratch in, out, ws, aux, aux2;
-    std::vector<char> host;
-    std::vector<int32_t> res;
+    hipEvent_t done = nullptr;
+    Scratch in, out, ws;
+    Pinned pin;
     // the gapless query copies and score table of the last call, on the
     // device (a query's seeds come in many calls in a row): reused while
     // the text, the query bytes and the matrix are the same
@@ -81,7 +109,15 @@ struct ThreadCtx {
     size_t g_qr = 0, g_tab = 0, g_meta = 0;
     ThreadCtx() {
         if (ensure_init()) die("device init");
-        hip_or_die(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        // blocking: a waiting thread sleeps instead of spinning on a core
+        // that another worker thread's host work could use
+        hip_or_die(hipEventCreateWithFlags(&done, hipEventDisableTiming | hipEventBlockingSync),
+                   "hipEventCreate");
+    }
+    // waits until this thread's work so far on the stream has finished
+    void finish(const char* what) {
+        hip_or_die(hipEventRecord(done, stream()), what);
+        hip_or_die(hipEventSynchronize(done), what);
     }
 };
 
@@ -114,8 +150,9 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     // layout of the input block
     const size_t o_q = 0, o_s = o_q + m_pad, o_mat = o_s + n_pad, o_meta = (o_mat + 625 + 15) & ~(size_t)15;
     const size_t bytes = o_meta + 128;
-    c.host.assign(bytes, 0);
-    char* h = c.host.data();
+    const size_t o_out = (bytes + 15) & ~(size_t)15;
+    char* h = c.pin.get(o_out + 16);
+    memset(h, 0, bytes);
     memcpy(h + o_q, read, (size_t)m);
     memset(h + o_s, 25, n_pad);
     for (int32_t i = 0; i < m; ++i)
@@ -147,7 +184,7 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     int32_t* dout = (int32_t*)c.out.get(64);
     const size_t ws_words = (size_t)4 * 2 * (n + 2) * 64;     // four wavefront buffers
     uint32_t* dws = (uint32_t*)c.ws.get(ws_words * 4);
-    hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
+    hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream()), "upload");
     SwKernelArgs a{};
     a.q_res = (const int8_t*)(d + o_q);
     a.q_off = (const int64_t*)(d + o_meta);
@@ -172,9 +209,11 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.out_score = dout;
     a.out_ref = dout + 1;
     a.out_read = dout + 2;
-    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, c.st), "sw_forward");
-    hip_or_die(hipMemcpyAsync(out, dout, 12, hipMemcpyDeviceToHost, c.st), "readback");
-    hip_or_die(hipStreamSynchronize(c.st), "sw_forward");
+    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, stream()), "sw_forward");
+    int32_t* ho = (int32_t*)(h + o_out);
+    hip_or_die(hipMemcpyAsync(ho, dout, 12, hipMemcpyDeviceToHost, stream()), "readback");
+    c.finish("sw_forward");
+    memcpy(out, ho, 12);
 }
 
 void check_gaps(int go, int ge) {
@@ -440,10 +479,11 @@ static const ResidentText& resident_text(const db_t* db) {
 }  // namespace
 
 namespace {
-// n seeds of one query against the resident text on the calling thread's
-// stream: one upload of the seeds (and of the query copies and score table
-// when the query or matrix changed), one launch, one readback.  lengths[k] is sa_hsp's length; scores[k] is written only where
-// the length is not -1 (search.c:113-116).
+// n seeds of one query against the resident text: one launch that reads the
+// seeds from pinned host memory and writes the results there (and one upload
+// of the query copies and score table when the query or matrix changed).
+// lengths[k] is sa_hsp's length; scores[k] is written only where the length
+// is not -1 (search.c:113-116).
 void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
                    const int64_t* positions, const options_t* opt, int* scores, int* lengths,
                    const char* who) {
@@ -462,9 +502,10 @@ void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
     const bool same = c.g_text == rt.text && (size_t)qlen == c.g_query.size() &&
                       memcmp(c.g_query.data(), q->seq_buf.buf, (size_t)qlen) == 0 &&
                       memcmp(c.g_matrix, mat, 625) == 0;
+    // query copies, score table and their offsets, uploaded once per query
+    QueryCodes qc;
+    size_t q_bytes = 0;
     if (!same) {
-        // query copies, score table and their offsets, uploaded once
-        QueryCodes qc;
         if (gapless_query_codes(rt.text->codes, 1, q->seq_buf.buf, &zero64, &qlen, qc)) {
             fprintf(stderr, "topaz_amd: %s: %s\n", who, tpz_last_error());
             _exit(EXIT_FAILURE);
@@ -472,9 +513,16 @@ void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
         c.g_qr = (qc.fwd.size() + 15) & ~(size_t)15;
         c.g_tab = (c.g_qr + qc.rev.size() + 15) & ~(size_t)15;
         c.g_meta = c.g_tab + kTableRows * kTableCols * 2;
-        const size_t gbytes = c.g_meta + 32;
-        c.host.assign(gbytes, 0);
-        char* g = c.host.data();
+        q_bytes = c.g_meta + 32;
+    }
+    // pinned block: [query block, when it changed | seeds | results]; the
+    // kernel reads the seeds from it and writes the results into it
+    const size_t seed_bytes = (size_t)n * sizeof(SeedRec);
+    char* h = c.pin.get(q_bytes + seed_bytes + (size_t)n * 8);
+    if (!same) {
+        const size_t gbytes = q_bytes;
+        char* g = h;
+        memset(g, 0, gbytes);
         memcpy(g, qc.fwd.data(), qc.fwd.size());
         memcpy(g + c.g_qr, qc.rev.data(), qc.rev.size());
         gapless_table(rt.text->codes, mat, (int16_t*)(g + c.g_tab));
@@ -482,21 +530,15 @@ void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
         memcpy(g + c.g_meta + 8, &zero64, 8);
         memcpy(g + c.g_meta + 16, &qlen, 4);
         char* gd = (char*)c.g_dev.get(gbytes);
-        hip_or_die(hipMemcpyAsync(gd, g, gbytes, hipMemcpyHostToDevice, c.st), "upload");
-        hip_or_die(hipStreamSynchronize(c.st), "upload");
+        hip_or_die(hipMemcpyAsync(gd, g, gbytes, hipMemcpyHostToDevice, stream()), "upload");
         c.g_text = rt.text;
         c.g_query.assign(q->seq_buf.buf, q->seq_buf.buf + qlen);
         memcpy(c.g_matrix, mat, 625);
     }
-    const size_t bytes = (size_t)n * sizeof(SeedRec);
-    c.host.assign(bytes, 0);
-    char* h = c.host.data();
-    SeedRec* seeds = reinterpret_cast<SeedRec*>(h);
+    SeedRec* seeds = reinterpret_cast<SeedRec*>(h + q_bytes);
     for (int k = 0; k < n; ++k) seeds[k] = SeedRec{0, qoffsets[k], positions[k]};
-    char* d = (char*)c.in.get(bytes);
+    int32_t* res = reinterpret_cast<int32_t*>(h + q_bytes + seed_bytes);
     const char* gd = (const char*)c.g_dev.p;
-    int32_t* dout = (int32_t*)c.out.get((size_t)n * 8);
-    hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
     GaplessArgs a{};
     a.t_fwd = (const uint8_t*)rt.text->d_fwd.ptr;
     a.t_head = (const uint8_t*)rt.text->d_head.ptr;
@@ -507,20 +549,17 @@ void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
     a.qf_off = (const int64_t*)(gd + c.g_meta);
     a.qr_off = (const int64_t*)(gd + c.g_meta + 8);
     a.q_len = (const int32_t*)(gd + c.g_meta + 16);
-    a.seeds = (const SeedRec*)d;
+    a.seeds = seeds;
     a.n_seeds = n;
     a.table = (const int16_t*)(gd + c.g_tab);
-    a.out_score = dout;
-    a.out_length = dout + n;
+    a.out_score = res;
+    a.out_length = res + n;
     const int grid = std::max(1, std::min((n + 255) / 256, std::max(num_cu(), 1) * 32));
-    hip_or_die(tpz_launch_gapless(&a, grid, c.st), "gapless");
-    c.res.resize((size_t)n * 2);
-    hip_or_die(hipMemcpyAsync(c.res.data(), dout, (size_t)n * 8, hipMemcpyDeviceToHost, c.st),
-               "readback");
-    hip_or_die(hipStreamSynchronize(c.st), "gapless");
+    hip_or_die(tpz_launch_gapless(&a, grid, stream()), "gapless");
+    c.finish("gapless");
     for (int k = 0; k < n; ++k) {
-        lengths[k] = c.res[n + k];
-        if (c.res[n + k] != -1) scores[k] = c.res[k];
+        lengths[k] = res[n + k];
+        if (res[n + k] != -1) scores[k] = res[k];
     }
 }
 }  // namespace
@@ -624,6 +663,29 @@ tpz_server* db_server(const db_t* db, const int8_t* mat, int go, int ge) {
     if (g_servers.empty()) atexit(stop_servers);
     g_servers.push_back(s);
     return s->srv;
+}
+
+// The first seeds of a search through the batch ABI build the serving loop
+// of its database, so that the proteins are on the device before the first
+// alignments come: one worker thread pays for the build while the others
+// go on with their seeds, instead of all of them waiting at their first
+// alignment call.
+void prefetch_server(const db_t* db, const options_t* opt) {
+    static std::atomic<const db_t*> last{nullptr};
+    static std::mutex mu;
+    static std::vector<const db_t*> claimed;
+    if (last.load(std::memory_order_acquire) == db) return;
+    const int go = opt->gap_open, ge = opt->gap_extend;
+    if (!(ge >= 0 && go > ge && go <= 255)) return;       // the alignment call reports it
+    if (!db->index || db->Psize == 0) return;
+    {
+        // one thread builds; the others do not wait for it here
+        std::lock_guard<std::mutex> lk(mu);
+        if (std::find(claimed.begin(), claimed.end(), db) != claimed.end()) return;
+        claimed.push_back(db);
+    }
+    (void)db_server(db, (const int8_t*)*opt->substitution, go, ge);
+    last.store(db, std::memory_order_release);
 }
 }  // namespace
 
@@ -736,6 +798,7 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
 
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
                   options_t* opt, int* scores, int* lengths) {
+    prefetch_server(db, opt);
     gapless_seeds(db, q, n, qoffsets, positions, opt, scores, lengths, "sa_hsp_batch");
 }
 
