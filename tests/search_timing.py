@@ -5,7 +5,7 @@ hit) and with query.c's and suffix.c's per-seed and per-hit loops replaced by
 the batch entry points.  All must print the same hit lists; prints one JSON
 line with the times.
 
-    python tests/search_timing.py [--db 100000] [--queries 1000] [--threads 16]
+    python tests/search_timing.py [--db 100000] [--queries 1000] [--threads 16] [--blasttab]
 
 The database follows the small end-to-end configuration: lognormal lengths
 (mean 350, sd 250, clipped [30, 2000]), 20 % of the queries with 1-5 planted
@@ -60,9 +60,11 @@ def main():
     ap.add_argument("--queries", type=int, default=1000)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--gpu-threads", type=int, default=None,
-                    help="worker threads of the GPU-backed binaries (default: --threads); their "
-                         "threads mostly wait on the GPU, so more of them than cores keeps more "
-                         "requests in each launch")
+                    help="worker threads of the GPU-backed binaries (default: 4 x --threads); "
+                         "their threads mostly wait on the GPU, so more of them than cores "
+                         "keeps more requests in each launch")
+    ap.add_argument("--blasttab", action="store_true",
+                    help="search with -B (start coordinates and traceback statistics per hit)")
     ap.add_argument("--timeout", type=int, default=900)
     ap.add_argument("--skip", nargs="*", default=[], help="binaries not to run (dropin_gpu, ...)")
     ap.add_argument("--out", default=None, help="directory for each search's output as printed")
@@ -91,8 +93,8 @@ def main():
         exes = {"reference_cpu": REF / "topaz", "dropin_gpu": _link_dropin(tmp),
                 "batch_gpu": _link_dropin(tmp, batch=True)}
         exes = {k: v for k, v in exes.items() if k not in args.skip}
-        flags = ["-H", "100", "-E", "0.001"]
-        gpu_threads = args.gpu_threads or args.threads
+        flags = ["-H", "100", "-E", "0.001"] + (["-B"] if args.blasttab else [])
+        gpu_threads = args.gpu_threads or 4 * args.threads
         out, times, inner, cpu_s = {}, {}, {}, {}
         for name, exe in exes.items():
             t = args.threads if name == "reference_cpu" else gpu_threads

@@ -65,3 +65,34 @@ def test_forward_then_stats_against_oracle():
         exp = o.ssw_align_stats(qq, ss, int(re_[p]), int(fe[p]), int(sc[p]))
         got = tuple(int(x[k]) for x in out[:6]) + (float(out[6][k]),)
         assert got[:6] == tuple(exp[:6]) and _same(got[6], exp[6]), (p, got, exp)
+
+
+def test_long_prefixes_and_other_scores_against_oracle():
+    """Query prefixes longer than one pass of the reverse kernel (2048
+    rows), and score1 one below / one above the forward score (the scan then
+    rises to another value or never stops, as for any caller's input): each
+    against the oracle."""
+    rng = np.random.default_rng(43)
+    w = synth.pair_workload(rng, 3, 4, homolog_frac=0.75, ident_range=(0.5, 0.95),
+                            mean=3000, sd=800, lo=2100, hi=4500)
+    b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], matrix())
+    sc, fe, re_ = b.run().results()
+    b.close()
+    keep = np.nonzero((sc > 1) & (sc < 32766))[0]
+    assert (re_[keep] >= 2048).sum() >= 3, "needs prefixes over 2048 rows"
+    pq, ps, r_, f_, s_ = [], [], [], [], []
+    for p in keep:
+        for d in (0, -1, 1):
+            pq.append(w["pair_query"][p]); ps.append(w["pair_subject"][p])
+            r_.append(re_[p]); f_.append(fe[p]); s_.append(sc[p] + d)
+    out = align_stats(w["q"], w["s"], np.array(pq), np.array(ps), np.array(r_), np.array(f_),
+                      np.array(s_), matrix())
+    o = Oracle()
+    q, s = w["q"], w["s"]
+    for k in range(len(pq)):
+        qi, si = pq[k], ps[k]
+        qq = q[0][q[1][qi]:q[1][qi] + q[2][qi]]
+        ss = s[0][s[1][si]:s[1][si] + s[2][si]]
+        exp = o.ssw_align_stats(qq, ss, int(r_[k]), int(f_[k]), int(s_[k]))
+        got = tuple(int(x[k]) for x in out[:6]) + (float(out[6][k]),)
+        assert got[:6] == tuple(exp[:6]) and _same(got[6], exp[6]), (k, got, exp)

@@ -99,6 +99,7 @@ struct ThreadCtx {
     hipEvent_t done = nullptr;
     Scratch in, out, ws;
     Pinned pin;
+    BlockList run_dev, run_host;      // memory of the traceback-statistics runs
     // the gapless query copies and score table of the last call, on the
     // device (a query's seeds come in many calls in a row): reused while
     // the text, the query bytes and the matrix are the same
@@ -124,6 +125,40 @@ struct ThreadCtx {
 ThreadCtx& ctx() {
     static thread_local ThreadCtx c;
     return c;
+}
+
+// A statistics run on the calling thread's blocks, the shared stream and the
+// thread's event: nothing is allocated once the blocks are large enough.
+struct ThreadMem final : RunMem {
+    ThreadCtx& c;
+    explicit ThreadMem(ThreadCtx& t) : c(t) {}
+    void reset() override { c.run_dev.reset(); c.run_host.reset(); }
+    void* dev(size_t bytes) override {
+        return c.run_dev.take(bytes, [](size_t n, void*) -> void* {
+            void* p = nullptr;
+            hip_or_die(hipMalloc(&p, n), "hipMalloc");
+            return p;
+        }, nullptr);
+    }
+    char* host(size_t bytes) override {
+        return c.run_host.take(bytes, [](size_t n, void*) -> void* {
+            void* p = nullptr;
+            hip_or_die(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc");
+            return p;
+        }, nullptr);
+    }
+    hipStream_t stream() override { return tpz::stream(); }
+    int wait() override {
+        c.finish("statistics");
+        return 0;
+    }
+};
+
+// Traceback statistics of hits on the calling thread (dies on error).
+void stats_run(const StatsInput& in, StatsOutput& out, const char* who) {
+    if (stats_check(in)) die(who);
+    ThreadMem m(ctx());
+    if (stats_compute(in, m, out)) die(who);
 }
 
 int8_t matrix_bias(const int8_t* mat, int32_t n) {
@@ -330,17 +365,16 @@ int ssw_align_stats(const s_profile* prof, const int8_t* ref, int32_t refLen,
     const int64_t qo = 0, so = 0;
     const int32_t ql = prof->readLen, sl = refLen, zero = 0;
     const int32_t re = read_end1, fe = ref_end1, sc = score1;
-    tpz_stats_batch* b = nullptr;
-    if (tpz_stats_batch_create(&b, 1, prof->read, &qo, &ql, 1, ref, &so, &sl, 1, &zero, &zero,
-                               &re, &fe, &sc, prof->mat, weight_gapO, weight_gapE))
-        die("ssw_align_stats");
-    if (tpz_stats_batch_run(b)) die("ssw_align_stats");
-    int32_t rc = 0;
-    if (tpz_stats_batch_results(b, &rc, read_begin1, ref_begin1, length, gapopen, mismatch,
-                                identity))
-        die("ssw_align_stats");
-    tpz_stats_batch_destroy(b);
-    return rc;
+    StatsOutput o;
+    stats_run(StatsInput{1, prof->read, &qo, &ql, 1, ref, &so, &sl, 1, &zero, &zero, &re, &fe, &sc,
+                         prof->mat, weight_gapO, weight_gapE}, o, "ssw_align_stats");
+    if (read_begin1) *read_begin1 = o.read_begin[0];
+    if (ref_begin1) *ref_begin1 = o.ref_begin[0];
+    if (length) *length = o.len[0];
+    if (gapopen) *gapopen = o.gapo[0];
+    if (mismatch) *mismatch = o.mis[0];
+    if (identity) *identity = stats_identity(o.ident[0], o.len[0]);
+    return o.rc[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -653,7 +687,16 @@ tpz_server* db_server(const db_t* db, const int8_t* mat, int go, int ge) {
         len[p] = (int32_t)(res.size() - (size_t)off[p]);
     }
     tpz_server_config cfg{};
+    // launches in flight: TOPAZ_AMD_INFLIGHT (1..8), default 2
     cfg.max_inflight = 2;
+    if (const char* e = getenv("TOPAZ_AMD_INFLIGHT")) {
+        const int v = atoi(e);
+        if (v < 1 || v > 8) {
+            fprintf(stderr, "topaz_amd: TOPAZ_AMD_INFLIGHT=%s is not 1..8\n", e);
+            _exit(EXIT_FAILURE);
+        }
+        cfg.max_inflight = v;
+    }
     cfg.waves_per_group = 8;     // a query's hits are a task or two: 8-wavefront staircases
     auto* s = new DbServer{db, db->T, db->Psize, {0}, go, ge, nullptr};
     memcpy(s->matrix, mat, 625);
@@ -774,25 +817,19 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
     }
     const int64_t qo = 0;
     const int32_t ql = profile->readLen;
-    std::vector<int32_t> rc(m), qb(m), sb(m), len(m), gapo(m), mis(m);
-    std::vector<double> ident(m);
-    tpz_stats_batch* b = nullptr;
-    if (tpz_stats_batch_create(&b, 1, profile->read, &qo, &ql, m, sub.res.data(), sub.off.data(),
-                               sub.len.data(), m, hq.data(), hs.data(), re.data(), fe.data(),
-                               sc.data(), profile->mat, opt->gap_open, opt->gap_extend) ||
-        tpz_stats_batch_run(b) ||
-        tpz_stats_batch_results(b, rc.data(), qb.data(), sb.data(), len.data(), gapo.data(),
-                                mis.data(), ident.data()))
-        die("stats_batch_using_ssw_profile");
-    tpz_stats_batch_destroy(b);
+    StatsOutput o;
+    stats_run(StatsInput{1, profile->read, &qo, &ql, m, sub.res.data(), sub.off.data(),
+                         sub.len.data(), m, hq.data(), hs.data(), re.data(), fe.data(), sc.data(),
+                         profile->mat, opt->gap_open, opt->gap_extend},
+              o, "stats_batch_using_ssw_profile");
     for (int k = 0; k < m; ++k) {
         hit_t* h = hits[todo[k]];
-        h->qstart = qb[k];
-        h->sstart = sb[k];
-        h->length = len[k];
-        h->gapopen = gapo[k];
-        h->mismatch = mis[k];
-        h->identity = ident[k];
+        h->qstart = o.read_begin[k];
+        h->sstart = o.ref_begin[k];
+        h->length = o.len[k];
+        h->gapopen = o.gapo[k];
+        h->mismatch = o.mis[k];
+        h->identity = stats_identity(o.ident[k], o.len[k]);
     }
 }
 

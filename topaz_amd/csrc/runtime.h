@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <utility>
 #include <vector>
 
 namespace tpz {
@@ -97,6 +98,54 @@ int gapless_query_codes(const TextCodes& tc, int32_t n_queries, const char* q_ch
                         const int64_t* q_off, const int32_t* q_len, QueryCodes& out);
 // W(text code, query code) for the kernel, kTableRows x kTableCols int16.
 void gapless_table(const TextCodes& tc, const int8_t* matrix, int16_t* table);
+
+// ---- traceback statistics (shared by tpz_stats_batch and the drop-in ABI) ----
+// Blocks handed out in order and reused after reset(); never freed here.
+struct BlockList {
+    std::vector<std::pair<char*, size_t>> blocks;
+    size_t cur = 0, off = 0;
+    void reset() { cur = 0; off = 0; }
+    // bytes (rounded up to 256) from the current block, or from a new one of
+    // at least twice the last block's size got from grow(size, ctx)
+    char* take(size_t bytes, void* (*grow)(size_t, void*), void* ctx);
+};
+
+// Memory and stream of one statistics run: device and pinned host memory
+// valid until the next reset(), the stream the work goes to, and a wait for
+// the work queued on it so far.  Failures return nullptr / -1 with the
+// error string set.
+struct RunMem {
+    virtual ~RunMem() = default;
+    virtual void reset() = 0;
+    virtual void* dev(size_t bytes) = 0;
+    virtual char* host(size_t bytes) = 0;
+    virtual hipStream_t stream() = 0;
+    virtual int wait() = 0;
+};
+
+struct StatsInput {
+    int32_t n_queries;
+    const int8_t* q_res;
+    const int64_t* q_off;
+    const int32_t* q_len;
+    int32_t n_subjects;
+    const int8_t* s_res;
+    const int64_t* s_off;
+    const int32_t* s_len;
+    int32_t n_hits;
+    const int32_t *hit_query, *hit_subject, *read_end1, *ref_end1, *score1;
+    const int8_t* matrix;
+    int32_t gap_open, gap_extend;
+};
+struct StatsOutput {
+    // per hit: status, begins (0-based), and the counts of the traceback;
+    // ident is the identical count, or -1 / -2 for the size guard / a
+    // failed traceback
+    std::vector<int32_t> rc, read_begin, ref_begin, len, gapo, mis, ident;
+};
+int stats_check(const StatsInput& in);
+int stats_compute(const StatsInput& in, RunMem& m, StatsOutput& out);
+double stats_identity(int32_t ident, int32_t length);
 
 }  // namespace tpz
 

@@ -888,16 +888,285 @@ void tpz_gapless_batch_destroy(tpz_gapless_batch* b) {
 // ---------------------------------------------------------------------------
 // traceback statistics
 // ---------------------------------------------------------------------------
-struct tpz_stats_batch {
-    int32_t n = 0;
-    int32_t gap_open = 11, gap_extend = 1, bias = 0;
-    std::vector<int32_t> read_end, ref_end, score;
-    DeviceBuffer d_q_res, d_q_off, d_s_res, d_s_off, d_hq, d_hs, d_read_end, d_ref_end,
-        d_score, d_matrix;
-    DeviceBuffer d_read_begin, d_ref_begin, d_band, d_rc, d_len, d_gapo, d_mis, d_ident;
-    DeviceBuffer d_ws, d_ws_off, d_dirs, d_dir_off;
-    std::vector<int32_t> h_rc, h_read_begin, h_ref_begin, h_len, h_gapo, h_mis, h_ident;
+}  // extern "C"
+
+namespace tpz {
+
+char* BlockList::take(size_t bytes, void* (*grow)(size_t, void*), void* ctx) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    while (cur < blocks.size() && off + bytes > blocks[cur].second) {
+        ++cur;
+        off = 0;
+    }
+    if (cur == blocks.size()) {
+        const size_t last = blocks.empty() ? 0 : blocks.back().second;
+        const size_t size = std::max(std::max(bytes, 2 * last), (size_t)1 << 20);
+        char* p = (char*)grow(size, ctx);
+        if (!p) return nullptr;
+        blocks.emplace_back(p, size);
+        off = 0;
+    }
+    char* p = blocks[cur].first + off;
+    off += bytes;
+    return p;
+}
+
+int stats_check(const StatsInput& in) {
+    if (in.n_queries < 0 || in.n_subjects < 0 || in.n_hits < 0) return fail("negative counts");
+    if (!(in.gap_extend >= 0 && in.gap_open > in.gap_extend && in.gap_open <= 255))
+        return fail("gap penalties must satisfy 0 <= gap_extend < gap_open <= 255");
+    if (!in.matrix) return fail("null matrix");
+    for (int32_t h = 0; h < in.n_hits; ++h) {
+        const int32_t qi = in.hit_query[h], si = in.hit_subject[h];
+        if (qi < 0 || qi >= in.n_queries || si < 0 || si >= in.n_subjects)
+            return fail("hit %d references query %d / subject %d out of range", h, qi, si);
+        if (in.read_end1[h] < 0 || in.read_end1[h] >= in.q_len[qi] || in.ref_end1[h] < 0 ||
+            in.ref_end1[h] >= in.s_len[si] || in.score1[h] <= 0 || in.score1[h] >= 32767)
+            return fail("hit %d: end positions (%d, %d) / score %d outside the query, "
+                        "subject or (0, 32767)", h, in.read_end1[h], in.ref_end1[h], in.score1[h]);
+    }
+    for (int32_t k = 0; k < in.n_queries; ++k)
+        for (int32_t j = 0; j < in.q_len[k]; ++j)
+            if (in.q_res[in.q_off[k] + j] < 0 || in.q_res[in.q_off[k] + j] > 24)
+                return fail("query %d residue out of range", k);
+    for (int32_t k = 0; k < in.n_subjects; ++k)
+        for (int32_t j = 0; j < in.s_len[k]; ++j)
+            if (in.s_res[in.s_off[k] + j] < 0 || in.s_res[in.s_off[k] + j] > 24)
+                return fail("subject %d residue out of range", k);
+    return 0;
+}
+
+namespace {
+// host block -> device block of the same size, on the run's stream
+template <typename T>
+T* put(RunMem& m, const T* src, size_t count, size_t pad = 0) {
+    const size_t bytes = count * sizeof(T) + pad;
+    char* h = m.host(bytes);
+    char* d = (char*)m.dev(bytes);
+    if (!h || !d) return nullptr;
+    if (count) memcpy(h, src, count * sizeof(T));
+    if (pad) memset(h + count * sizeof(T), 0, pad);
+    if (check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, m.stream()), "upload"))
+        return nullptr;
+    return (T*)d;
+}
+// device -> pinned host block, read after the next wait
+int32_t* get(RunMem& m, const void* d, int32_t n) {
+    int32_t* h = (int32_t*)m.host((size_t)n * 4 + 4);
+    if (!h) return nullptr;
+    if (check(hipMemcpyAsync(h, d, (size_t)n * 4, hipMemcpyDeviceToHost, m.stream()), "readback"))
+        return nullptr;
+    return h;
+}
+}  // namespace
+
+// The reverse pass gives the start coordinates (ssw.c:900-941), the band
+// search the band half-width (ssw.c:570-660), the trace the counts
+// (ssw.c:663-739); each stage's sizes come from the one before, so the
+// host waits three times.  All memory comes from m and is reused after the
+// next reset, so a caller that keeps m allocates nothing per run.
+int stats_compute(const StatsInput& in, RunMem& m, StatsOutput& out) {
+    const int32_t n = in.n_hits;
+    out.rc.assign(n, 0); out.read_begin.assign(n, 0); out.ref_begin.assign(n, 0);
+    out.len.assign(n, 0); out.gapo.assign(n, 0); out.mis.assign(n, 0); out.ident.assign(n, 0);
+    if (n == 0) return 0;
+    m.reset();
+    int32_t minw = 0;
+    for (int i = 0; i < 625; ++i) minw = std::min<int32_t>(minw, in.matrix[i]);
+    int64_t qtot = 0, stot = 0;
+    for (int32_t k = 0; k < in.n_queries; ++k) qtot = std::max<int64_t>(qtot, in.q_off[k] + in.q_len[k]);
+    for (int32_t k = 0; k < in.n_subjects; ++k) stot = std::max<int64_t>(stot, in.s_off[k] + in.s_len[k]);
+    // reverse-pass workspace: bottom-row H and F between passes and the
+    // column maxima, one word per subject column each
+    std::vector<int64_t> ws_off(n);
+    int64_t ws = 0;
+    for (int32_t h = 0; h < n; ++h) {
+        ws_off[h] = ws;
+        ws += 3 * ((int64_t)in.ref_end1[h] + 1);
+    }
+    ReverseArgs r{};
+    r.q_res = put(m, in.q_res, (size_t)qtot, 16);
+    r.q_off = put(m, in.q_off, (size_t)in.n_queries);
+    r.s_res = put(m, in.s_res, (size_t)stot, 16);
+    r.s_off = put(m, in.s_off, (size_t)in.n_subjects);
+    r.pair_query = put(m, in.hit_query, (size_t)n);
+    r.pair_subject = put(m, in.hit_subject, (size_t)n);
+    r.read_end = put(m, in.read_end1, (size_t)n);
+    r.ref_end = put(m, in.ref_end1, (size_t)n);
+    r.score = put(m, in.score1, (size_t)n);
+    r.matrix = put(m, in.matrix, 625);
+    r.ws_off = put(m, ws_off.data(), (size_t)n);
+    r.workspace = (int32_t*)m.dev((size_t)ws * 4 + 16);
+    r.out_read_begin = (int32_t*)m.dev((size_t)n * 4 + 4);
+    r.out_ref_begin = (int32_t*)m.dev((size_t)n * 4 + 4);
+    if (!r.q_res || !r.q_off || !r.s_res || !r.s_off || !r.pair_query || !r.pair_subject ||
+        !r.read_end || !r.ref_end || !r.score || !r.matrix || !r.ws_off || !r.workspace ||
+        !r.out_read_begin || !r.out_ref_begin)
+        return -1;
+    r.n_pairs = n;
+    r.gap_open = in.gap_open; r.gap_extend = in.gap_extend; r.bias = -minw;
+    if (check(tpz_launch_sw_reverse(&r, m.stream()), "sw_reverse")) return -1;
+    const int32_t* hrb = get(m, r.out_ref_begin, n);
+    const int32_t* hqb = get(m, r.out_read_begin, n);
+    if (!hrb || !hqb || m.wait()) return -1;
+    out.ref_begin.assign(hrb, hrb + n);
+    out.read_begin.assign(hqb, hqb + n);
+    const std::vector<int32_t>& rb = out.ref_begin;
+    const std::vector<int32_t>& qb = out.read_begin;
+    for (int32_t h = 0; h < n; ++h)
+        if (rb[h] < 0 || qb[h] < 0 || qb[h] > in.read_end1[h])
+            return fail("hit %d: reverse pass found no start (score1 %d not reached)", h, in.score1[h]);
+    // band rows: the band half-width stays below refLen + readLen before the
+    // search stops, so 4 (refLen + readLen) + 8 words per row suffice
+    std::vector<int64_t> bo(n);
+    int64_t bw = 0;
+    int32_t band_words = 8;
+    for (int32_t h = 0; h < n; ++h) {
+        const int32_t sum = (in.ref_end1[h] - rb[h] + 1) + (in.read_end1[h] - qb[h] + 1);
+        band_words = std::max(band_words, 4 * sum + 8);
+    }
+    for (int32_t h = 0; h < n; ++h) { bo[h] = bw; bw += 3 * (int64_t)band_words; }
+    BandArgs a{};
+    a.q_res = r.q_res; a.q_off = r.q_off; a.s_res = r.s_res; a.s_off = r.s_off;
+    a.pair_query = r.pair_query; a.pair_subject = r.pair_subject;
+    a.read_begin = r.out_read_begin;
+    a.read_end = r.read_end;
+    a.ref_begin = r.out_ref_begin;
+    a.ref_end = r.ref_end;
+    a.score = r.score;
+    a.n_pairs = n;
+    a.matrix = r.matrix;
+    a.gap_open = in.gap_open; a.gap_extend = in.gap_extend;
+    a.ws_band_words = band_words;
+    a.out_band = (int32_t*)m.dev((size_t)n * 4 + 4);
+    if (!a.out_band) return -1;
+    // band search with a wavefront per hit; a band wider than the LDS rows
+    // (status -2) goes again through the one-thread-per-hit kernel, whose
+    // rows live in the workspace
+    a.hits = nullptr;
+    a.n_list = n;
+    if (check(tpz_launch_band_search(&a, 1, m.stream()), "band_search")) return -1;
+    const int32_t* band = get(m, a.out_band, n);
+    if (!band || m.wait()) return -1;
+    std::vector<int32_t> wide;
+    for (int32_t h = 0; h < n; ++h)
+        if (band[h] == -2) wide.push_back(h);
+    if (!wide.empty()) {
+        a.ws_off = put(m, bo.data(), (size_t)n);
+        a.workspace = (int32_t*)m.dev((size_t)bw * 4 + 16);
+        a.hits = put(m, wide.data(), wide.size());
+        a.n_list = (int32_t)wide.size();
+        if (!a.ws_off || !a.workspace || !a.hits ||
+            check(tpz_launch_band_search(&a, 0, m.stream()), "band_search"))
+            return -1;
+        band = get(m, a.out_band, n);
+        if (!band || m.wait()) return -1;
+    }
+    // direction buffers: the reference's size, 1024 bytes grown to the next
+    // power of two above each pass's (2w+1) * readLen * 3 (ssw.c:555, 591-597)
+    std::vector<int64_t> dofs(n), dsize(n);
+    int64_t dtot = 0;
+    for (int32_t h = 0; h < n; ++h) {
+        dofs[h] = dtot;
+        int64_t s2 = 1024;
+        if (band[h] >= 0) {
+            const int64_t ncell = (int64_t)(2 * band[h] + 1) * (in.read_end1[h] - qb[h] + 1) * 3;
+            while (ncell >= s2) {
+                int64_t v = s2;           // ++s2 then round up to a power of two
+                v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+                s2 = v + 1;
+            }
+        }
+        dsize[h] = band[h] >= 0 ? s2 : 0;
+        dtot += dsize[h];
+    }
+    a.dirs = (int8_t*)m.dev((size_t)dtot + 16);
+    a.dir_off = put(m, dofs.data(), (size_t)n);
+    a.dir_size = put(m, dsize.data(), (size_t)n);
+    int32_t* d_res = (int32_t*)m.dev(5 * ((size_t)n * 4 + 4));
+    if (!a.dirs || !a.dir_off || !a.dir_size || !d_res ||
+        check(hipMemsetAsync(a.dirs, 0, (size_t)dtot + 16, m.stream()), "memset"))
+        return -1;
+    a.out_rc = d_res;
+    a.out_length = d_res + (n + 1);
+    a.out_gapopen = d_res + 2 * (n + 1);
+    a.out_mismatch = d_res + 3 * (n + 1);
+    a.out_identical = d_res + 4 * (n + 1);
+    // trace: a wavefront per hit whose final band fits the LDS rows
+    std::vector<int32_t> narrow;
+    wide.clear();
+    for (int32_t h = 0; h < n; ++h)
+        (band[h] >= 0 && 2 * band[h] + 3 > kBandSlots ? wide : narrow).push_back(h);
+    for (int pass = 0; pass < 2; ++pass) {
+        const std::vector<int32_t>& list = pass == 0 ? narrow : wide;
+        if (list.empty()) continue;
+        if (pass == 1 && !a.workspace) {
+            a.ws_off = put(m, bo.data(), (size_t)n);
+            a.workspace = (int32_t*)m.dev((size_t)bw * 4 + 16);
+            if (!a.ws_off || !a.workspace) return -1;
+        }
+        a.hits = put(m, list.data(), list.size());
+        a.n_list = (int32_t)list.size();
+        if (!a.hits || check(tpz_launch_band_trace(&a, pass == 0, m.stream()), "band_trace"))
+            return -1;
+    }
+    const int32_t* res = get(m, d_res, 5 * (n + 1));
+    if (!res || m.wait()) return -1;
+    out.rc.assign(res, res + n);
+    out.len.assign(res + (n + 1), res + (n + 1) + n);
+    out.gapo.assign(res + 2 * (n + 1), res + 2 * (n + 1) + n);
+    out.mis.assign(res + 3 * (n + 1), res + 3 * (n + 1) + n);
+    out.ident.assign(res + 4 * (n + 1), res + 4 * (n + 1) + n);
+    return 0;
+}
+
+double stats_identity(int32_t ident, int32_t length) {
+    // ssw.c:741: identical / (double) length; -1.0 / 0.0 for the size guard /
+    // trace failure (identity is never assigned there)
+    return ident == -1 ? -1.0 : ident == -2 ? 0.0 : ident / (double)length;
+}
+
+}  // namespace tpz
+
+namespace {
+// Memory of a tpz_stats_batch: its own device and pinned blocks, freed
+// with the batch; waits synchronise the library stream.
+struct BatchMem final : tpz::RunMem {
+    tpz::BlockList d, h;
+    ~BatchMem() override {
+        for (auto& b : d.blocks) (void)hipFree(b.first);
+        for (auto& b : h.blocks) (void)hipHostFree(b.first);
+    }
+    void reset() override { d.reset(); h.reset(); }
+    void* dev(size_t bytes) override {
+        return d.take(bytes, [](size_t n, void*) -> void* {
+            void* p = nullptr;
+            return tpz::check(hipMalloc(&p, n), "hipMalloc") ? nullptr : p;
+        }, nullptr);
+    }
+    char* host(size_t bytes) override {
+        return h.take(bytes, [](size_t n, void*) -> void* {
+            void* p = nullptr;
+            return tpz::check(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc") ? nullptr : p;
+        }, nullptr);
+    }
+    hipStream_t stream() override { return tpz::stream(); }
+    int wait() override { return tpz::check(hipStreamSynchronize(tpz::stream()), "stats"); }
 };
+}  // namespace
+
+struct tpz_stats_batch {
+    // inputs, copied at create
+    std::vector<int8_t> q_res, s_res, matrix;
+    std::vector<int64_t> q_off, s_off;
+    std::vector<int32_t> q_len, s_len, hq, hs, read_end, ref_end, score;
+    tpz::StatsInput in{};
+    BatchMem mem;
+    tpz::StatsOutput out;
+    bool ran = false;
+};
+
+extern "C" {
 
 int tpz_stats_batch_create(tpz_stats_batch** out, int32_t n_queries, const int8_t* q_res,
                            const int64_t* q_off, const int32_t* q_len, int32_t n_subjects,
@@ -909,170 +1178,39 @@ int tpz_stats_batch_create(tpz_stats_batch** out, int32_t n_queries, const int8_
     if (!out) return fail("tpz_stats_batch_create: null output pointer");
     *out = nullptr;
     if (ensure_init()) return -1;
-    if (n_queries < 0 || n_subjects < 0 || n_hits < 0) return fail("negative counts");
-    if (!(gap_extend >= 0 && gap_open > gap_extend && gap_open <= 255))
-        return fail("gap penalties must satisfy 0 <= gap_extend < gap_open <= 255");
-    if (!matrix) return fail("null matrix");
-    int32_t minw = 0;
-    for (int i = 0; i < 625; ++i) minw = std::min<int32_t>(minw, matrix[i]);
-    for (int32_t h = 0; h < n_hits; ++h) {
-        const int32_t qi = hit_query[h], si = hit_subject[h];
-        if (qi < 0 || qi >= n_queries || si < 0 || si >= n_subjects)
-            return fail("hit %d references query %d / subject %d out of range", h, qi, si);
-        if (read_end1[h] < 0 || read_end1[h] >= q_len[qi] || ref_end1[h] < 0 ||
-            ref_end1[h] >= s_len[si] || score1[h] <= 0 || score1[h] >= 32767)
-            return fail("hit %d: end positions (%d, %d) / score %d outside the query, "
-                        "subject or (0, 32767)", h, read_end1[h], ref_end1[h], score1[h]);
-    }
-    for (int32_t k = 0; k < n_queries; ++k)
-        for (int32_t j = 0; j < q_len[k]; ++j)
-            if (q_res[q_off[k] + j] < 0 || q_res[q_off[k] + j] > 24)
-                return fail("query %d residue out of range", k);
-    for (int32_t k = 0; k < n_subjects; ++k)
-        for (int32_t j = 0; j < s_len[k]; ++j)
-            if (s_res[s_off[k] + j] < 0 || s_res[s_off[k] + j] > 24)
-                return fail("subject %d residue out of range", k);
+    StatsInput in{n_queries, q_res, q_off, q_len, n_subjects, s_res, s_off, s_len, n_hits,
+                  hit_query, hit_subject, read_end1, ref_end1, score1, matrix, gap_open,
+                  gap_extend};
+    if (stats_check(in)) return -1;
     auto b = new tpz_stats_batch();
-    b->n = n_hits;
-    b->gap_open = gap_open; b->gap_extend = gap_extend; b->bias = -minw;
-    b->read_end.assign(read_end1, read_end1 + n_hits);
-    b->ref_end.assign(ref_end1, ref_end1 + n_hits);
-    b->score.assign(score1, score1 + n_hits);
     int64_t qtot = 0, stot = 0;
     for (int32_t k = 0; k < n_queries; ++k) qtot = std::max<int64_t>(qtot, q_off[k] + q_len[k]);
     for (int32_t k = 0; k < n_subjects; ++k) stot = std::max<int64_t>(stot, s_off[k] + s_len[k]);
-    // reverse-pass workspace: H and E over the lane-padded reversed prefix
-    std::vector<int64_t> ws_off(n_hits);
-    int64_t ws = 0;
-    for (int32_t h = 0; h < n_hits; ++h) {
-        ws_off[h] = ws;
-        ws += 2 * 16 * ((int64_t)(read_end1[h] + 1 + 7) / 8);
-    }
-    int rc = 0;
-    rc |= b->d_q_res.upload(q_res, (size_t)qtot + 16);
-    rc |= b->d_q_off.upload(q_off, (size_t)n_queries * 8);
-    rc |= b->d_s_res.upload(s_res, (size_t)stot + 16);
-    rc |= b->d_s_off.upload(s_off, (size_t)n_subjects * 8);
-    rc |= b->d_hq.upload(hit_query, (size_t)n_hits * 4);
-    rc |= b->d_hs.upload(hit_subject, (size_t)n_hits * 4);
-    rc |= b->d_read_end.upload(read_end1, (size_t)n_hits * 4);
-    rc |= b->d_ref_end.upload(ref_end1, (size_t)n_hits * 4);
-    rc |= b->d_score.upload(score1, (size_t)n_hits * 4);
-    rc |= b->d_matrix.upload(matrix, 625);
-    rc |= b->d_ws.alloc((size_t)ws * 4 + 16);
-    rc |= b->d_ws_off.upload(ws_off.data(), ws_off.size() * 8);
-    for (DeviceBuffer* d : {&b->d_read_begin, &b->d_ref_begin, &b->d_band, &b->d_rc, &b->d_len,
-                            &b->d_gapo, &b->d_mis, &b->d_ident})
-        rc |= d->alloc((size_t)n_hits * 4 + 4);
-    rc |= check(hipStreamSynchronize(stream()), "stats upload");
-    if (rc) { delete b; return -1; }
+    b->q_res.assign(q_res, q_res + qtot);
+    b->s_res.assign(s_res, s_res + stot);
+    b->matrix.assign(matrix, matrix + 625);
+    b->q_off.assign(q_off, q_off + n_queries);
+    b->s_off.assign(s_off, s_off + n_subjects);
+    b->q_len.assign(q_len, q_len + n_queries);
+    b->s_len.assign(s_len, s_len + n_subjects);
+    b->hq.assign(hit_query, hit_query + n_hits);
+    b->hs.assign(hit_subject, hit_subject + n_hits);
+    b->read_end.assign(read_end1, read_end1 + n_hits);
+    b->ref_end.assign(ref_end1, ref_end1 + n_hits);
+    b->score.assign(score1, score1 + n_hits);
+    b->in = StatsInput{n_queries, b->q_res.data(), b->q_off.data(), b->q_len.data(), n_subjects,
+                       b->s_res.data(), b->s_off.data(), b->s_len.data(), n_hits, b->hq.data(),
+                       b->hs.data(), b->read_end.data(), b->ref_end.data(), b->score.data(),
+                       b->matrix.data(), gap_open, gap_extend};
     *out = b;
     return 0;
 }
 
 int tpz_stats_batch_run(tpz_stats_batch* b) {
     if (!b) return fail("null batch");
-    const int32_t n = b->n;
-    if (n == 0) return 0;
-    ReverseArgs r{};
-    r.q_res = (const int8_t*)b->d_q_res.ptr;
-    r.q_off = (const int64_t*)b->d_q_off.ptr;
-    r.s_res = (const int8_t*)b->d_s_res.ptr;
-    r.s_off = (const int64_t*)b->d_s_off.ptr;
-    r.pair_query = (const int32_t*)b->d_hq.ptr;
-    r.pair_subject = (const int32_t*)b->d_hs.ptr;
-    r.read_end = (const int32_t*)b->d_read_end.ptr;
-    r.ref_end = (const int32_t*)b->d_ref_end.ptr;
-    r.score = (const int32_t*)b->d_score.ptr;
-    r.n_pairs = n;
-    r.matrix = (const int8_t*)b->d_matrix.ptr;
-    r.gap_open = b->gap_open; r.gap_extend = b->gap_extend; r.bias = b->bias;
-    r.workspace = (int32_t*)b->d_ws.ptr;
-    r.ws_off = (const int64_t*)b->d_ws_off.ptr;
-    r.out_read_begin = (int32_t*)b->d_read_begin.ptr;
-    r.out_ref_begin = (int32_t*)b->d_ref_begin.ptr;
-    if (check(tpz_launch_sw_reverse(&r, stream()), "sw_reverse")) return -1;
-    std::vector<int32_t> rb(n), qb(n);
-    if (check(hipMemcpyAsync(rb.data(), b->d_ref_begin.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipMemcpyAsync(qb.data(), b->d_read_begin.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipStreamSynchronize(stream()), "sw_reverse"))
-        return -1;
-    for (int32_t h = 0; h < n; ++h)
-        if (rb[h] < 0 || qb[h] < 0 || qb[h] > b->read_end[h])
-            return fail("hit %d: reverse pass found no start (score1 %d not reached)", h, b->score[h]);
-    // band rows: the band half-width stays below refLen + readLen before the
-    // search stops, so 4 (refLen + readLen) + 8 words per row suffice
-    std::vector<int64_t> bo(n);
-    int64_t bw = 0;
-    int32_t band_words = 8;
-    for (int32_t h = 0; h < n; ++h) {
-        const int32_t sum = (b->ref_end[h] - rb[h] + 1) + (b->read_end[h] - qb[h] + 1);
-        band_words = std::max(band_words, 4 * sum + 8);
-    }
-    for (int32_t h = 0; h < n; ++h) { bo[h] = bw; bw += 3 * (int64_t)band_words; }
-    DeviceBuffer d_bo, d_bws;
-    if (d_bo.upload(bo.data(), bo.size() * 8) || d_bws.alloc((size_t)bw * 4 + 16)) return -1;
-    BandArgs a{};
-    a.q_res = r.q_res; a.q_off = r.q_off; a.s_res = r.s_res; a.s_off = r.s_off;
-    a.pair_query = r.pair_query; a.pair_subject = r.pair_subject;
-    a.read_begin = (const int32_t*)b->d_read_begin.ptr;
-    a.read_end = r.read_end;
-    a.ref_begin = (const int32_t*)b->d_ref_begin.ptr;
-    a.ref_end = r.ref_end;
-    a.score = r.score;
-    a.n_pairs = n;
-    a.matrix = r.matrix;
-    a.gap_open = b->gap_open; a.gap_extend = b->gap_extend;
-    a.workspace = (int32_t*)d_bws.ptr;
-    a.ws_off = (const int64_t*)d_bo.ptr;
-    a.ws_band_words = band_words;
-    a.out_band = (int32_t*)b->d_band.ptr;
-    if (check(tpz_launch_band_search(&a, stream()), "band_search")) return -1;
-    std::vector<int32_t> band(n);
-    if (check(hipMemcpyAsync(band.data(), b->d_band.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipStreamSynchronize(stream()), "band_search"))
-        return -1;
-    // direction buffers: the reference's size, 1024 bytes grown to the next
-    // power of two above each pass's (2w+1) * readLen * 3 (ssw.c:555, 591-597)
-    std::vector<int64_t> dofs(n), dsize(n);
-    int64_t dtot = 0;
-    for (int32_t h = 0; h < n; ++h) {
-        dofs[h] = dtot;
-        int64_t s2 = 1024;
-        if (band[h] >= 0) {
-            const int64_t ncell = (int64_t)(2 * band[h] + 1) * (b->read_end[h] - qb[h] + 1) * 3;
-            while (ncell >= s2) {
-                int64_t v = s2;           // ++s2 then round up to a power of two
-                v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
-                s2 = v + 1;
-            }
-        }
-        dsize[h] = band[h] >= 0 ? s2 : 0;
-        dtot += dsize[h];
-    }
-    DeviceBuffer d_dirs, d_doff, d_dsize;
-    if (d_dirs.alloc((size_t)dtot + 16) || d_doff.upload(dofs.data(), dofs.size() * 8) ||
-        d_dsize.upload(dsize.data(), dsize.size() * 8) ||
-        check(hipMemsetAsync(d_dirs.ptr, 0, (size_t)dtot + 16, stream()), "memset"))
-        return -1;
-    a.dirs = (int8_t*)d_dirs.ptr;
-    a.dir_off = (const int64_t*)d_doff.ptr;
-    a.dir_size = (const int64_t*)d_dsize.ptr;
-    a.out_rc = (int32_t*)b->d_rc.ptr;
-    a.out_length = (int32_t*)b->d_len.ptr;
-    a.out_gapopen = (int32_t*)b->d_gapo.ptr;
-    a.out_mismatch = (int32_t*)b->d_mis.ptr;
-    a.out_identical = (int32_t*)b->d_ident.ptr;
-    if (check(tpz_launch_band_trace(&a, stream()), "band_trace")) return -1;
-    b->h_rc.resize(n); b->h_read_begin = qb; b->h_ref_begin = rb;
-    b->h_len.resize(n); b->h_gapo.resize(n); b->h_mis.resize(n); b->h_ident.resize(n);
-    if (check(hipMemcpyAsync(b->h_rc.data(), b->d_rc.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipMemcpyAsync(b->h_len.data(), b->d_len.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipMemcpyAsync(b->h_gapo.data(), b->d_gapo.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipMemcpyAsync(b->h_mis.data(), b->d_mis.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipMemcpyAsync(b->h_ident.data(), b->d_ident.ptr, n * 4, hipMemcpyDeviceToHost, stream()), "readback") ||
-        check(hipStreamSynchronize(stream()), "band_trace"))
-        return -1;
+    b->ran = false;
+    if (stats_compute(b->in, b->mem, b->out)) return -1;
+    b->ran = true;
     return 0;
 }
 
@@ -1080,21 +1218,16 @@ int tpz_stats_batch_results(const tpz_stats_batch* b, int32_t* rc, int32_t* read
                             int32_t* ref_begin1, int32_t* length, int32_t* gapopen,
                             int32_t* mismatch, double* identity) {
     if (!b) return fail("null batch");
-    if ((int32_t)b->h_rc.size() != b->n) return fail("tpz_stats_batch_results before run");
-    for (int32_t h = 0; h < b->n; ++h) {
-        if (rc) rc[h] = b->h_rc[h];
-        if (read_begin1) read_begin1[h] = b->h_read_begin[h];
-        if (ref_begin1) ref_begin1[h] = b->h_ref_begin[h];
-        if (length) length[h] = b->h_len[h];
-        if (gapopen) gapopen[h] = b->h_gapo[h];
-        if (mismatch) mismatch[h] = b->h_mis[h];
-        if (identity) {
-            // ssw.c:741: identical / (double) length; -1.0 / 0.0 for the
-            // size guard / trace failure (identity is never assigned there)
-            const int32_t ident = b->h_ident[h];
-            identity[h] = ident == -1 ? -1.0 : ident == -2 ? 0.0
-                                                           : ident / (double)b->h_len[h];
-        }
+    if (!b->ran && b->in.n_hits > 0) return fail("tpz_stats_batch_results before run");
+    const StatsOutput& o = b->out;
+    for (int32_t h = 0; h < b->in.n_hits; ++h) {
+        if (rc) rc[h] = o.rc[h];
+        if (read_begin1) read_begin1[h] = o.read_begin[h];
+        if (ref_begin1) ref_begin1[h] = o.ref_begin[h];
+        if (length) length[h] = o.len[h];
+        if (gapopen) gapopen[h] = o.gapo[h];
+        if (mismatch) mismatch[h] = o.mis[h];
+        if (identity) identity[h] = stats_identity(o.ident[h], o.len[h]);
     }
     return 0;
 }

@@ -20,7 +20,7 @@ struct ReverseArgs {
     int32_t n_pairs;
     const int8_t* matrix;
     int32_t gap_open, gap_extend, bias;
-    int32_t* workspace;            // 2 * 16 * ceil((read_end1 + 1) / 8) words per hit
+    int32_t* workspace;            // 3 * (ref_end1 + 1) words per hit
     const int64_t* ws_off;
     int32_t* out_read_begin;
     int32_t* out_ref_begin;
@@ -49,12 +49,20 @@ struct BandArgs {
     const int64_t* dir_off;
     const int64_t* dir_size;       // bytes of each hit's buffer
     int32_t* out_rc, *out_length, *out_gapopen, *out_mismatch, *out_identical;
+    const int32_t* hits;           // hit numbers the launch covers (n_list of them); null: all
+    int32_t n_list;
 };
+
+// Band rows the wavefront kernels hold in LDS: a band wider than this goes
+// to the one-thread-per-hit kernels (band status -2 from the search).
+constexpr int kBandSlots = 5120;
 
 }  // namespace tpz
 
 extern "C" hipError_t tpz_launch_sw_reverse(const tpz::ReverseArgs* a, hipStream_t stream);
-extern "C" hipError_t tpz_launch_band_search(const tpz::BandArgs* a, hipStream_t stream);
-extern "C" hipError_t tpz_launch_band_trace(const tpz::BandArgs* a, hipStream_t stream);
+// wave: one wavefront per hit with the band rows in LDS; otherwise one
+// thread per hit with the rows in the workspace
+extern "C" hipError_t tpz_launch_band_search(const tpz::BandArgs* a, int wave, hipStream_t stream);
+extern "C" hipError_t tpz_launch_band_trace(const tpz::BandArgs* a, int wave, hipStream_t stream);
 
 #endif
