@@ -24,7 +24,11 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/topaz_amd.h"
@@ -99,7 +103,6 @@ struct ThreadCtx {
     hipEvent_t done = nullptr;
     Scratch in, out, ws;
     Pinned pin;
-    BlockList run_dev, run_host;      // memory of the traceback-statistics runs
     // the gapless query copies and score table of the last call, on the
     // device (a query's seeds come in many calls in a row): reused while
     // the text, the query bytes and the matrix are the same
@@ -127,38 +130,195 @@ ThreadCtx& ctx() {
     return c;
 }
 
-// A statistics run on the calling thread's blocks, the shared stream and the
-// thread's event: nothing is allocated once the blocks are large enough.
-struct ThreadMem final : RunMem {
-    ThreadCtx& c;
-    explicit ThreadMem(ThreadCtx& t) : c(t) {}
-    void reset() override { c.run_dev.reset(); c.run_host.reset(); }
+// Traceback statistics of all host threads go to one worker thread with a
+// stream of its own: each turn it merges the requests waiting into one
+// stats_compute run, whose launches hold every waiting hit (a wavefront or
+// thread per hit, so a hit takes as long among others as alone).  The
+// worker's device and pinned blocks are reused from run to run.
+struct StatsReq {
+    const StatsInput* in;
+    StatsOutput* out;
+    bool done = false;
+    int rc = 0;
+    std::string err;
+};
+
+struct StatsWorker final : RunMem {
+    std::mutex mu;
+    std::condition_variable cv_req, cv_done;
+    std::deque<StatsReq*> pending;
+    bool stop = false;
+    std::thread th;
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    BlockList d, h;
+    // RunMem on the worker's blocks and stream
+    void reset() override { d.reset(); h.reset(); }
     void* dev(size_t bytes) override {
-        return c.run_dev.take(bytes, [](size_t n, void*) -> void* {
+        return d.take(bytes, [](size_t n, void*) -> void* {
             void* p = nullptr;
             hip_or_die(hipMalloc(&p, n), "hipMalloc");
             return p;
         }, nullptr);
     }
     char* host(size_t bytes) override {
-        return c.run_host.take(bytes, [](size_t n, void*) -> void* {
+        return h.take(bytes, [](size_t n, void*) -> void* {
             void* p = nullptr;
             hip_or_die(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc");
             return p;
         }, nullptr);
     }
-    hipStream_t stream() override { return tpz::stream(); }
+    hipStream_t stream() override { return st; }
     int wait() override {
-        c.finish("statistics");
-        return 0;
+        return check(hipEventRecord(ev, st), "statistics") ||
+               check(hipEventSynchronize(ev), "statistics");
     }
 };
 
-// Traceback statistics of hits on the calling thread (dies on error).
+bool same_settings(const StatsInput& a, const StatsInput& b) {
+    return a.gap_open == b.gap_open && a.gap_extend == b.gap_extend &&
+           memcmp(a.matrix, b.matrix, 625) == 0;
+}
+
+// the requests' queries, subjects and hits back to back, numbers shifted
+struct Merged {
+    std::vector<int8_t> q_res, s_res;
+    std::vector<int64_t> q_off, s_off;
+    std::vector<int32_t> q_len, s_len, hq, hs, re, fe, sc;
+    std::vector<int32_t> first_hit;            // per request, then the total
+    StatsInput in{};
+};
+
+void merge(const std::vector<StatsReq*>& batch, Merged& m) {
+    for (StatsReq* r : batch) {
+        const StatsInput& x = *r->in;
+        const int32_t q0 = (int32_t)m.q_len.size(), s0 = (int32_t)m.s_len.size();
+        int64_t qtot = 0, stot = 0;
+        for (int32_t k = 0; k < x.n_queries; ++k) qtot = std::max<int64_t>(qtot, x.q_off[k] + x.q_len[k]);
+        for (int32_t k = 0; k < x.n_subjects; ++k) stot = std::max<int64_t>(stot, x.s_off[k] + x.s_len[k]);
+        const int64_t qb = (int64_t)m.q_res.size(), sb = (int64_t)m.s_res.size();
+        m.q_res.insert(m.q_res.end(), x.q_res, x.q_res + qtot);
+        m.s_res.insert(m.s_res.end(), x.s_res, x.s_res + stot);
+        for (int32_t k = 0; k < x.n_queries; ++k) {
+            m.q_off.push_back(qb + x.q_off[k]);
+            m.q_len.push_back(x.q_len[k]);
+        }
+        for (int32_t k = 0; k < x.n_subjects; ++k) {
+            m.s_off.push_back(sb + x.s_off[k]);
+            m.s_len.push_back(x.s_len[k]);
+        }
+        m.first_hit.push_back((int32_t)m.hq.size());
+        for (int32_t h = 0; h < x.n_hits; ++h) {
+            m.hq.push_back(q0 + x.hit_query[h]);
+            m.hs.push_back(s0 + x.hit_subject[h]);
+            m.re.push_back(x.read_end1[h]);
+            m.fe.push_back(x.ref_end1[h]);
+            m.sc.push_back(x.score1[h]);
+        }
+    }
+    m.first_hit.push_back((int32_t)m.hq.size());
+    const StatsInput& x = *batch[0]->in;
+    m.in = StatsInput{(int32_t)m.q_len.size(), m.q_res.data(), m.q_off.data(), m.q_len.data(),
+                      (int32_t)m.s_len.size(), m.s_res.data(), m.s_off.data(), m.s_len.data(),
+                      (int32_t)m.hq.size(), m.hq.data(), m.hs.data(), m.re.data(), m.fe.data(),
+                      m.sc.data(), x.matrix, x.gap_open, x.gap_extend};
+}
+
+void stats_loop(StatsWorker* w) {
+    (void)hipSetDevice(device_id());
+    std::unique_lock<std::mutex> lk(w->mu);
+    for (;;) {
+        w->cv_req.wait(lk, [&] { return w->stop || !w->pending.empty(); });
+        if (w->pending.empty()) break;                 // stopping
+        // the waiting requests with the first one's matrix and gaps
+        std::vector<StatsReq*> batch;
+        for (auto it = w->pending.begin(); it != w->pending.end();) {
+            if (batch.empty() || same_settings(*(*it)->in, *batch[0]->in)) {
+                batch.push_back(*it);
+                it = w->pending.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        lk.unlock();
+        Merged m;
+        merge(batch, m);
+        StatsOutput o;
+        const int rc = stats_compute(m.in, *w, o);
+        const std::string err = rc ? std::string(tpz_last_error()) : std::string();
+        for (size_t b = 0; b < batch.size() && !rc; ++b) {
+            const int32_t h0 = m.first_hit[b], h1 = m.first_hit[b + 1];
+            StatsOutput& out = *batch[b]->out;
+            out.rc.assign(o.rc.begin() + h0, o.rc.begin() + h1);
+            out.read_begin.assign(o.read_begin.begin() + h0, o.read_begin.begin() + h1);
+            out.ref_begin.assign(o.ref_begin.begin() + h0, o.ref_begin.begin() + h1);
+            out.len.assign(o.len.begin() + h0, o.len.begin() + h1);
+            out.gapo.assign(o.gapo.begin() + h0, o.gapo.begin() + h1);
+            out.mis.assign(o.mis.begin() + h0, o.mis.begin() + h1);
+            out.ident.assign(o.ident.begin() + h0, o.ident.begin() + h1);
+        }
+        lk.lock();
+        for (StatsReq* r : batch) {
+            r->rc = rc;
+            r->err = err;
+            r->done = true;
+        }
+        w->cv_done.notify_all();
+    }
+}
+
+StatsWorker* g_stats = nullptr;
+std::mutex g_stats_mu;
+
+// Stops the worker before the HIP runtime is torn down at exit.
+void stop_stats_worker() {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    if (!g_stats) return;
+    {
+        std::lock_guard<std::mutex> wl(g_stats->mu);
+        g_stats->stop = true;
+    }
+    g_stats->cv_req.notify_all();
+    if (g_stats->th.joinable()) g_stats->th.join();
+}
+
+StatsWorker& stats_worker() {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    if (!g_stats) {
+        if (ensure_init()) die("device init");
+        auto* w = new StatsWorker();
+        hip_or_die(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking), "hipStreamCreate");
+        hip_or_die(hipEventCreateWithFlags(&w->ev, hipEventDisableTiming | hipEventBlockingSync),
+                   "hipEventCreate");
+        w->th = std::thread(stats_loop, w);
+        g_stats = w;
+        atexit(stop_stats_worker);
+    }
+    return *g_stats;
+}
+
+// Traceback statistics of hits through the worker (dies on error).
 void stats_run(const StatsInput& in, StatsOutput& out, const char* who) {
     if (stats_check(in)) die(who);
-    ThreadMem m(ctx());
-    if (stats_compute(in, m, out)) die(who);
+    if (in.n_hits == 0) {
+        out = StatsOutput{};
+        return;
+    }
+    StatsWorker& w = stats_worker();
+    StatsReq r;
+    r.in = &in;
+    r.out = &out;
+    {
+        std::unique_lock<std::mutex> lk(w.mu);
+        w.pending.push_back(&r);
+        w.cv_req.notify_one();
+        w.cv_done.wait(lk, [&] { return r.done; });
+    }
+    if (r.rc) {
+        fprintf(stderr, "topaz_amd: %s: %s\n", who, r.err.c_str());
+        fflush(stderr);
+        _exit(EXIT_FAILURE);
+    }
 }
 
 int8_t matrix_bias(const int8_t* mat, int32_t n) {
