@@ -425,6 +425,49 @@ def serving_leg(args, rates=(10, 50, 100, 200), seconds=3.0):
             "peak_admitted_bytes": st["peak_bytes"], "workspace_bytes": st["workspace_bytes"]}
 
 
+def stats_leg(args, n_queries=200, top=100):
+    """Traceback statistics of reported hits (reverse pass, banded_sw): for
+    each of the first n_queries queries of the 2M-pair workload, its `top`
+    best-scoring pairs, as TOPAZ reports -H 100 hits per query, in one batch.
+    A step is one tpz_stats_batch_run: upload, reverse pass, band search,
+    trace, three host waits.  The reverse pass covers each hit's prefix
+    rectangle, (read_end1 + 1) x (ref_end1 + 1) cells."""
+    from topaz_amd import matrix
+    from topaz_amd.stats import StatsBatch
+    from topaz_amd.sw import SwBatch
+    w = make_workload(n_queries / 2000.0)
+    b = SwBatch(w["q"], w["s"], w["pair_query"], w["pair_subject"], matrix())
+    sc, fe, re_ = b.run().results()
+    b.close()
+    pick = []
+    for qi in range(len(w["q"][2])):
+        idx = np.nonzero((w["pair_query"] == qi) & (sc > 0) & (sc < 32767))[0]
+        pick.extend(idx[np.argsort(-sc[idx], kind="stable")][:top].tolist())
+    pick = np.array(pick, np.int64)
+    cells = int(np.sum((re_[pick].astype(np.int64) + 1) * (fe[pick].astype(np.int64) + 1)))
+    st = StatsBatch(w["q"], w["s"], w["pair_query"][pick], w["pair_subject"][pick], re_[pick],
+                    fe[pick], sc[pick], matrix())
+    for _ in range(max(1, args.warmup)):
+        st.run()
+    ref = st.results()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        st.run()
+        times.append(time.perf_counter() - t0)
+    got = st.results()
+    st.close()
+    assert all(np.array_equal(x, y) for x, y in zip(got, ref))
+    t = float(np.mean(times))
+    print("[bench] stats: %d hits, %.3g reverse-pass cells, %.2f ms per run"
+          % (len(pick), cells, t * 1e3), file=sys.stderr, flush=True)
+    return {"workload": "traceback statistics: the %d best-scoring pairs of each of %d queries "
+                        "of the 2M-pair workload, one batch" % (top, n_queries),
+            "hits": int(len(pick)), "reverse_cells": cells, "ms_per_run": round(t * 1e3, 3),
+            "hits_per_s": round(len(pick) / t, 1),
+            "reverse_gcups": round(cells / t / 1e9, 2)}
+
+
 def copy_leg(args):
     """A plain device-to-device copy of 1 GiB, three times: the calibration
     run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch of
@@ -454,7 +497,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
-    ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "copy"], default="all")
+    ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "stats", "copy"],
+                    default="all")
     ap.add_argument("--serve-waves", type=int, default=8, choices=[4, 8],
                     help="wavefronts per workgroup of the serving loop's launches")
     ap.add_argument("--serve-scale", type=float, default=1.0,
@@ -471,6 +515,10 @@ def main():
     if args.leg == "copy":
         print(json.dumps(copy_leg(args)), flush=True)
         return
+    if args.leg == "stats":
+        print(json.dumps({"metric": "traceback statistics per batch of reported hits",
+                          "stats": stats_leg(args)}), flush=True)
+        return
     if args.leg == "serve":
         sv = serving_leg(args)
         print(json.dumps({"metric": "serving p50/p99 latency vs sustained queries/s",
@@ -481,6 +529,7 @@ def main():
         if args.leg == "all":
             out["gapless"] = gapless_leg(args)
             out["serving"] = serving_leg(args)
+            out["stats"] = stats_leg(args)
             top = [r for r in out["serving"]["rates"] if r["offered_requests_per_s"] == 100]
             if top:
                 out["latency_p50_ms"] = top[0]["p50_ms"]

@@ -1005,7 +1005,23 @@ int stats_compute(const StatsInput& in, RunMem& m, StatsOutput& out) {
         return -1;
     r.n_pairs = n;
     r.gap_open = in.gap_open; r.gap_extend = in.gap_extend; r.bias = -minw;
-    if (check(tpz_launch_sw_reverse(&r, m.stream()), "sw_reverse")) return -1;
+    // one launch per rows-per-lane class of the hits' lane-padded prefixes
+    {
+        std::vector<int32_t> cls[3];
+        for (int32_t h = 0; h < n; ++h) {
+            const int32_t L = in.read_end1[h] + 1;
+            const int32_t P = -minw + in.score1[h] < 255 ? 16 : 8;
+            const int rpl = reverse_rows_per_lane((L + P - 1) / P * P);
+            cls[rpl == 8 ? 0 : rpl == 16 ? 1 : 2].push_back(h);
+        }
+        for (int c = 0; c < 3; ++c) {
+            if (cls[c].empty()) continue;
+            r.hits = put(m, cls[c].data(), cls[c].size());
+            r.n_list = (int32_t)cls[c].size();
+            if (!r.hits || check(tpz_launch_sw_reverse(&r, 8 << c, m.stream()), "sw_reverse"))
+                return -1;
+        }
+    }
     const int32_t* hrb = get(m, r.out_ref_begin, n);
     const int32_t* hqb = get(m, r.out_read_begin, n);
     if (!hrb || !hqb || m.wait()) return -1;

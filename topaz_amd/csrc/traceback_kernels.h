@@ -24,7 +24,14 @@ struct ReverseArgs {
     const int64_t* ws_off;
     int32_t* out_read_begin;
     int32_t* out_ref_begin;
+    const int32_t* hits;           // hit numbers the launch covers (n_list of them); null: all
+    int32_t n_list;
 };
+
+// Query rows per lane of the reverse-pass kernel for a hit whose (lane-
+// padded) prefix has `rows` rows: the fewest of 8, 16, 32 holding it in one
+// pass of 64 lanes; longer prefixes take passes of 32 rows per lane.
+inline int reverse_rows_per_lane(int rows) { return rows <= 512 ? 8 : rows <= 1024 ? 16 : 32; }
 
 struct BandArgs {
     const int8_t* q_res;
@@ -55,11 +62,12 @@ struct BandArgs {
 
 // Band rows the wavefront kernels hold in LDS: a band wider than this goes
 // to the one-thread-per-hit kernels (band status -2 from the search).
-constexpr int kBandSlots = 5120;
+constexpr int kBandSlots = 2048;
 
 }  // namespace tpz
 
-extern "C" hipError_t tpz_launch_sw_reverse(const tpz::ReverseArgs* a, hipStream_t stream);
+// one wavefront per listed hit, all of them of rows-per-lane class rpl
+extern "C" hipError_t tpz_launch_sw_reverse(const tpz::ReverseArgs* a, int rpl, hipStream_t stream);
 // wave: one wavefront per hit with the band rows in LDS; otherwise one
 // thread per hit with the rows in the workspace
 extern "C" hipError_t tpz_launch_band_search(const tpz::BandArgs* a, int wave, hipStream_t stream);

@@ -50,8 +50,9 @@ __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 // over the recorded maxima; (c) only when the scan's last rise is not to
 // score1 (a score1 that is not the pair's forward score), sweeps again up
 // to that column to find the row.
-constexpr int kRevRows = 32;                    // rows per lane
-constexpr int kRevPassRows = 64 * kRevRows;     // rows per pass
+// Rows per lane (reverse_rows_per_lane): a kernel instance per class, so a
+// short prefix's launch holds a small LDS profile (25 x 64 x rows bytes)
+// and more hits fit a CU.
 
 struct RevTrack {
     int k, j;                                   // first (column, row) with H == target
@@ -65,10 +66,13 @@ __device__ __forceinline__ int extract_s8(const uint32_t* w, int r) {
 // se - k, row j query residue qe - j, rows >= L score 0, rows >= rows do
 // not exist).  Writes each column's max to cmax; returns this lane's
 // first (column, row) with H == target among real rows.
-__device__ __forceinline__ RevTrack rev_sweep(const int8_t* q, const int8_t* s, int qe, int se, int L, int rows,
-                              const int8_t* mat_lds, int go, int ge, int8_t* prof,
-                              int32_t* bot_h, int32_t* bot_f, int32_t* cmax, int k_last,
-                              int target) {
+template <int kRevRows>
+__device__ __forceinline__ RevTrack rev_sweep(const int8_t* q, const int8_t* s, int qe, int se,
+                                              int L, int rows, const int8_t* mat_lds, int go,
+                                              int ge, int8_t* prof, int32_t* bot_h,
+                                              int32_t* bot_f, int32_t* cmax, int k_last,
+                                              int target) {
+    constexpr int kRevPassRows = 64 * kRevRows;
     const int lane = threadIdx.x;
     const int n_pass = (rows + kRevPassRows - 1) / kRevPassRows;
     RevTrack tr{INT_MAX, INT_MAX};
@@ -132,9 +136,18 @@ __device__ __forceinline__ RevTrack rev_sweep(const int8_t* q, const int8_t* s, 
                 rs = s0;
             }
             if (active) {
-                const uint4* pr = reinterpret_cast<const uint4*>(prof + rs * kRevPassRows + lane * kRevRows);
-                const uint4 p0 = pr[0], p1 = pr[1];
-                const uint32_t pw[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+                uint32_t pw[kRevRows / 4];
+                const int8_t* src = prof + rs * kRevPassRows + lane * kRevRows;
+                if constexpr (kRevRows == 8) {
+                    const uint2 p0 = *reinterpret_cast<const uint2*>(src);
+                    pw[0] = p0.x; pw[1] = p0.y;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < kRevRows / 16; ++c) {
+                        const uint4 pc = reinterpret_cast<const uint4*>(src)[c];
+                        pw[4 * c] = pc.x; pw[4 * c + 1] = pc.y; pw[4 * c + 2] = pc.z; pw[4 * c + 3] = pc.w;
+                    }
+                }
                 int diag = prev_top, f = rf, mymax = 0;
 #pragma unroll
                 for (int r = 0; r < kRevRows; ++r) {
@@ -179,11 +192,12 @@ __device__ __forceinline__ RevTrack wave_min(RevTrack t) {
     return t;
 }
 
+template <int kRows>
 __global__ void __launch_bounds__(64)
 sw_reverse_kernel(ReverseArgs a) {
-    __shared__ __attribute__((aligned(16))) int8_t prof[25 * kRevPassRows];
+    __shared__ __attribute__((aligned(16))) int8_t prof[25 * 64 * kRows];
     __shared__ int8_t mat[625];
-    const int p = blockIdx.x;
+    const int p = a.hits ? a.hits[blockIdx.x] : blockIdx.x;
     const int lane = threadIdx.x;
     for (int i = lane; i < 625; i += 64) mat[i] = a.matrix[i];
     __syncthreads();
@@ -205,9 +219,9 @@ sw_reverse_kernel(ReverseArgs a) {
     int end_read = L - 1;
     int k_last = se, target = score1;
     for (int phase = 0; phase < 2; ++phase) {
-        const RevTrack t = wave_min(rev_sweep(q, s, qe, se, L, rows, mat, a.gap_open,
-                                              a.gap_extend, prof, bot_h, bot_f, cmax, k_last,
-                                              target));
+        const RevTrack t = wave_min(rev_sweep<kRows>(q, s, qe, se, L, rows, mat, a.gap_open,
+                                                     a.gap_extend, prof, bot_h, bot_f, cmax,
+                                                     k_last, target));
         if (phase == 1) {
             if (t.k == rise_k) end_read = t.j;
             break;
@@ -576,9 +590,17 @@ banded_trace_wave_kernel(BandArgs a) {
 
 }  // namespace tpz
 
-extern "C" hipError_t tpz_launch_sw_reverse(const tpz::ReverseArgs* a, hipStream_t stream) {
-    if (a->n_pairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tpz::sw_reverse_kernel, dim3(a->n_pairs), dim3(64), 0, stream, *a);
+extern "C" hipError_t tpz_launch_sw_reverse(const tpz::ReverseArgs* a, int rpl, hipStream_t stream) {
+    const int n = a->hits ? a->n_list : a->n_pairs;
+    if (n <= 0) return hipSuccess;
+    if (rpl == 8)
+        hipLaunchKernelGGL(tpz::sw_reverse_kernel<8>, dim3(n), dim3(64), 0, stream, *a);
+    else if (rpl == 16)
+        hipLaunchKernelGGL(tpz::sw_reverse_kernel<16>, dim3(n), dim3(64), 0, stream, *a);
+    else if (rpl == 32)
+        hipLaunchKernelGGL(tpz::sw_reverse_kernel<32>, dim3(n), dim3(64), 0, stream, *a);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
