@@ -224,6 +224,19 @@ int tpz_gapless_batch_results(const tpz_gapless_batch* b, int32_t* score,
 int tpz_gapless_batch_kernel_ms(const tpz_gapless_batch* b, double* ms);
 void tpz_gapless_batch_destroy(tpz_gapless_batch* b);
 
+/* ---- suffix array of the database text (topaz index) -------------------- */
+
+/* sa[0..n-1] = the suffix array of text[0..n-1]: the order divsufsort64
+ * gives the .SA file (index.c:66, divsufsort64.h:79), suffixes compared as
+ * unsigned bytes, a suffix before every longer suffix it is a prefix of.
+ * Built on the device by prefix doubling; n may be up to 4,294,967,294.
+ * Device memory: about 33 n bytes.  stats may be null. */
+typedef struct {
+    int32_t rounds;          /* doubling rounds: sorts by 2, 4, 8, ... bytes */
+    int64_t device_bytes;    /* device memory used */
+} tpz_sa_stats;
+int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,3 +103,46 @@ def test_batch_loops_hit_lists_identical(workspace, flags):
     ours = _search(batch, tmp, *flags)
     assert ref.count("\n") > 20
     assert _per_query(ours) == _per_query(ref)
+
+
+def _link_index(tmp):
+    """TOPAZ's objects minus divsufsort.o, sssort.o and trsort.o (the
+    vendored libdivsufsort), linked against the library's divsufsort64."""
+    drop = {"divsufsort.o", "sssort.o", "trsort.o"}
+    objs = [o for o in sorted(str(p) for p in OBJ.rglob("*.o")) if Path(o).name not in drop]
+    exe = tmp / "topaz_index_gpu"
+    libdir = str(_lib.LIB_PATH.parent)
+    subprocess.run(["g++", "-o", str(exe)] + objs + ["-L" + libdir, "-ltopaz_amd",
+                    "-Wl,-rpath," + libdir, "-lm", "-lpthread"], check=True)
+    return exe
+
+
+def test_index_files_identical(tmp_path):
+    """`topaz index` with the suffix array built on the GPU writes the stock
+    binary's index files byte for byte (.SA included), and a search over
+    that index prints the same hit lists."""
+    if not (REF / "topaz").exists():
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    rng = np.random.default_rng(2031)
+    db, queries = synth.search_database(rng, 3000, 12, hom_frac=0.5, max_hom=4)
+    db += [db[k] for k in rng.integers(0, len(db), 50)]     # identical sequences
+    synth.write_fasta(tmp_path / "db.fa", db, "s")
+    synth.write_fasta(tmp_path / "q.fa", queries, "q")
+    exe = _link_index(tmp_path)
+    for name, binary in (("ref", REF / "topaz"), ("gpu", exe)):
+        (tmp_path / name).mkdir()
+        subprocess.run([str(binary), "index", "-f", str(tmp_path / "db.fa"), "-p",
+                        str(tmp_path / name / "DB")], check=True, capture_output=True,
+                       cwd=str(tmp_path), timeout=300)
+    files = sorted(p.name for p in (tmp_path / "ref").iterdir())
+    assert "DB.SA" in files and files == sorted(p.name for p in (tmp_path / "gpu").iterdir())
+    for f in files:
+        assert (tmp_path / "ref" / f).read_bytes() == (tmp_path / "gpu" / f).read_bytes(), f
+    out = {}
+    for name in ("ref", "gpu"):
+        r = subprocess.run([str(REF / "topaz"), "search", "-f", str(tmp_path / "q.fa"), "-p",
+                            str(tmp_path / name / "DB"), "-T", "1"], capture_output=True,
+                           text=True, timeout=300, cwd=str(tmp_path))
+        assert r.returncode == 0
+        out[name] = r.stdout
+    assert out["gpu"].count("\n") >= 3 and out["gpu"] == out["ref"]
