@@ -216,6 +216,14 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
                   options_t* opt, int* scores, int* lengths);
 
+/* n calls of sa_search(db, q, offsets[k], 0, ...) (search.c:237-361) for one
+ * query, run on the GPU: idx[k], num_matches[k] and last_position[k] as that
+ * call writes them (idx[k] = -1 and the other two untouched where it
+ * returns -1).  Returns 0, or -1 when any call would.  num_matches and
+ * last_position may be null. */
+int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
+                    int64_t* num_matches, int64_t* last_position);
+
 #ifdef __cplusplus
 }
 #endif

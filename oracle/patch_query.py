@@ -9,8 +9,9 @@ compiles it straight into oracle/_ref/batch/query_batch.o, which the drop-in
 test links in place of query.o.
 
 The code is found by definitions and calls, not by line numbers:
-  * query_find_suffixes (query.c:94-131) is rewritten: the suffix-array
-    search of every offset as before, then one sa_hsp_batch call for both
+  * query_find_suffixes (query.c:94-131) is rewritten: one sa_search_batch
+    call for the suffix-array search of every offset, then one sa_hsp_batch
+    call for both
     seeds of every offset and the first FIND_K slots of each new suffix's
     walk (handed to the suffix with suffix_prefetched, patch_suffix.py),
     then the suffixes in the original order;
@@ -32,6 +33,8 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
                                    options_t* opt);
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
                   options_t* opt, int* scores, int* lengths);
+int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
+                    int64_t* num_matches, int64_t* last_position);
 /* suffix_batch.o (patch_suffix.py): the next extensions of a new suffix's walk */
 void suffix_prefetched(suffix_t* suf, int n, const int* scores, const int* lengths);
 """
@@ -40,12 +43,11 @@ void suffix_prefetched(suffix_t* suf, int n, const int* scores, const int* lengt
 FIND_K = 64
 
 FIND = """int query_find_suffixes(query_t* q) {
-    /* 1. the suffix-array interval of every query offset, as before */
-    int i, h, j, n_hit = 0, n_seed = 0;
+    /* 1. the suffix-array interval of every query offset, in one batch */
+    int i, h, j, n_cand = 0, n_hit = 0, n_seed = 0;
     int num_suffixes = seq_len(q->query) - q->opt->min_hit_length;
     const int K = %(K)d;
     const saidx64_t tsize = (saidx64_t) q->db->Tsize;
-    saidx64_t pos, num_mat, last_pos;
     size_t cap = num_suffixes > 0 ? (size_t) num_suffixes : 1;
     int* hit_off = malloc(sizeof(int) * cap);
     saidx64_t* hit_pos = malloc(sizeof(saidx64_t) * cap);
@@ -53,16 +55,18 @@ FIND = """int query_find_suffixes(query_t* q) {
     int64_t* slots;
 
     for(i = 0; i < num_suffixes; ++i) {
-        pos = -1;
         if(_bad_prefix(seq_ptr(q->query) + i))
             continue;
-        if(sa_search(q->db, q->query, i, 0, &pos, &num_mat, &last_pos) != 0) {
-            fprintf(stderr, "Error: internal error in suffix array search\\n");
-            _exit(EXIT_FAILURE);
-        }
-        if(pos != -1) {
-            hit_off[n_hit] = i;
-            hit_pos[n_hit] = pos;
+        hit_off[n_cand++] = i;
+    }
+    if(n_cand > 0 && sa_search_batch(q->db, q->query, n_cand, hit_off, hit_pos, NULL, NULL) != 0) {
+        fprintf(stderr, "Error: internal error in suffix array search\\n");
+        _exit(EXIT_FAILURE);
+    }
+    for(i = 0; i < n_cand; ++i) {
+        if(hit_pos[i] != -1) {
+            hit_off[n_hit] = hit_off[i];
+            hit_pos[n_hit] = hit_pos[i];
             ++n_hit;
         }
     }

@@ -332,3 +332,62 @@ def test_batch_entry_points_equal_per_hit_calls():
         assert (sc[k], ln[k]) == (s1.value, l1.value), k
         es, el = o.sa_hsp(T, int(SA[pos[k]]), Q, int(offs[k]))
         assert (ln[k], sc[k]) == (el, es if el != -1 else -7), k
+
+
+def test_sa_search_batch_equals_reference():
+    """sa_search_batch gives, offset by offset, what the reference's own
+    sa_search (search.c:237-361, compiled from its sources) writes: the
+    interval start, the number of matches and the last match length, for
+    suffixes that occur (query parts copied from the text), suffixes that do
+    not, repeats, and an offset past the query's end (return -1)."""
+    from tests.refcheck import DbT
+    if not REF_SO.exists():
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    L, R = lib(), Reference().L
+    rng = np.random.default_rng(61)
+    prots = [_internal(synth.random_residues(rng, int(n)))
+             for n in synth.lognormal_lengths(rng, 300, 200, 120, 5, 900)]
+    prots[7] = b"A" * 40 + prots[7]                   # a run the queries repeat
+    T = b"".join(p + b"\n" for p in prots)
+    SA = np.zeros(len(T), np.int64)
+    R.divsufsort64.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.c_int64]
+    R.divsufsort64.restype = C.c_int32
+    assert R.divsufsort64(T, SA.ctypes.data_as(C.POINTER(C.c_int64)), len(T)) == 0
+    db = DbT()
+    db.T = T
+    db.Tsize = len(T)
+    db.SA = SA.ctypes.data_as(C.POINTER(C.c_int64))
+    db.SAsize = len(T) * 8
+    PI64 = C.POINTER(C.c_int64)
+    R.sa_search.argtypes = [C.POINTER(DbT), C.POINTER(SeqT), C.c_int64, C.c_int64, PI64, PI64, PI64]
+    R.sa_search.restype = C.c_int
+    L.sa_search_batch.argtypes = [C.POINTER(DbT), C.POINTER(SeqT), C.c_int, C.POINTER(C.c_int),
+                                  PI64, PI64, PI64]
+    L.sa_search_batch.restype = C.c_int
+    queries = [prots[3][10:200] + _internal(synth.random_residues(rng, 60)) + prots[11][:50],
+               _internal(synth.random_residues(rng, 300)),
+               b"A" * 45 + prots[7][40:90],
+               prots[20]]
+    checked = 0
+    for qb in queries:
+        q = _seq(qb)
+        offs = np.arange(len(qb), dtype=np.int32)
+        n = len(offs)
+        idx = np.full(n, -9, np.int64)
+        nm = np.full(n, -9, np.int64)
+        lp = np.full(n, -9, np.int64)
+        assert L.sa_search_batch(C.byref(db), C.byref(q), n, offs.ctypes.data_as(C.POINTER(C.c_int)),
+                                 idx.ctypes.data_as(PI64), nm.ctypes.data_as(PI64),
+                                 lp.ctypes.data_as(PI64)) == 0
+        for k in range(n):
+            ri, rn, rl = C.c_int64(-9), C.c_int64(-9), C.c_int64(-9)
+            assert R.sa_search(C.byref(db), C.byref(q), int(offs[k]), 0, C.byref(ri), C.byref(rn),
+                               C.byref(rl)) == 0
+            assert (idx[k], nm[k], lp[k]) == (ri.value, rn.value, rl.value), (k, qb[k:k + 20])
+            checked += 1
+        # past the end: both return -1 with idx -1
+        end = np.array([len(qb)], np.int32)
+        one = np.full(1, -9, np.int64)
+        assert L.sa_search_batch(C.byref(db), C.byref(q), 1, end.ctypes.data_as(C.POINTER(C.c_int)),
+                                 one.ctypes.data_as(PI64), None, None) == -1 and one[0] == -1
+    assert checked > 500

@@ -145,4 +145,5 @@ def test_index_files_identical(tmp_path):
                            text=True, timeout=300, cwd=str(tmp_path))
         assert r.returncode == 0
         out[name] = r.stdout
-    assert out["gpu"].count("\n") >= 3 and out["gpu"] == out["ref"]
+    # per query: the reference prints the queries' blocks in varying order
+    assert out["gpu"].count("\n") >= 3 and _per_query(out["gpu"]) == _per_query(out["ref"])

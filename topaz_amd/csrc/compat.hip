@@ -651,6 +651,7 @@ struct ResidentText {
     uint64_t t_size = 0;
     const int64_t* host_SA = nullptr;
     tpz_text* text = nullptr;
+    char* raw = nullptr;            // db->T's bytes, for the suffix-array search
 };
 std::mutex g_text_mu;
 std::vector<ResidentText*> g_texts;
@@ -669,6 +670,20 @@ static const ResidentText& resident_text(const db_t* db) {
     }
     g_texts.push_back(r);
     return *r;
+}
+
+// db->T's bytes on the device, uploaded at the first suffix-array search
+const char* resident_raw(const db_t* db) {
+    const ResidentText& rt = resident_text(db);
+    std::lock_guard<std::mutex> lk(g_text_mu);
+    ResidentText& w = const_cast<ResidentText&>(rt);
+    if (!w.raw) {
+        char* d = nullptr;
+        hip_or_die(hipMalloc(&d, (size_t)db->Tsize + 1), "hipMalloc");
+        hip_or_die(hipMemcpy(d, db->T, (size_t)db->Tsize, hipMemcpyHostToDevice), "text upload");
+        w.raw = d;
+    }
+    return w.raw;
 }
 }  // namespace
 
@@ -991,6 +1006,50 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
         h->mismatch = o.mis[k];
         h->identity = stats_identity(o.ident[k], o.len[k]);
     }
+}
+
+int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
+                    int64_t* num_matches, int64_t* last_position) {
+    if (n <= 0) return 0;
+    if (!db->T || !db->SA || db->Tsize <= 0) {                     // search.c:274-276
+        for (int k = 0; k < n; ++k) idx[k] = -1;
+        return -1;
+    }
+    const ResidentText& rt = resident_text(db);
+    const char* raw = resident_raw(db);
+    ThreadCtx& c = ctx();
+    // pinned block: [query bytes | offsets | idx, num_matches, last_position | status]
+    const size_t qb = ((size_t)q->seq_len + 16) & ~(size_t)15;
+    const size_t ob = ((size_t)n * 4 + 15) & ~(size_t)15;
+    char* h = c.pin.get(qb + ob + (size_t)n * 24 + (size_t)n * 4);
+    memcpy(h, q->seq_buf.buf, (size_t)q->seq_len);
+    int32_t* ho = reinterpret_cast<int32_t*>(h + qb);
+    for (int k = 0; k < n; ++k) ho[k] = offsets[k];
+    int64_t* hr = reinterpret_cast<int64_t*>(h + qb + ob);
+    int32_t* hs = reinterpret_cast<int32_t*>(h + qb + ob + (size_t)n * 24);
+    SaSearchArgs a{};
+    a.text = raw;
+    a.t_size = (int64_t)db->Tsize;
+    a.sa = (const int64_t*)rt.text->d_sa.ptr;
+    a.query = h;
+    a.q_len = q->seq_len;
+    a.offsets = ho;
+    a.n = n;
+    a.out = hr;
+    a.status = hs;
+    hip_or_die(tpz_launch_sa_search(&a, stream()), "sa_search");
+    c.finish("sa_search");
+    int rc = 0;
+    for (int k = 0; k < n; ++k) {
+        idx[k] = hr[3 * k];
+        if (hs[k] != 0) {
+            rc = -1;
+            continue;
+        }
+        if (num_matches) num_matches[k] = hr[3 * k + 1];
+        if (last_position) last_position[k] = hr[3 * k + 2];
+    }
+    return rc;
 }
 
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,

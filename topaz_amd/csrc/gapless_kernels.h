@@ -51,9 +51,25 @@ struct GaplessArgs {
     int32_t* out_length;
 };
 
+// sa_search (search.c:237-361) for offsets of one query: the suffix-array
+// interval of the query suffix at each offset.
+struct SaSearchArgs {
+    const char* text;           // db->T as bytes
+    int64_t t_size;
+    const int64_t* sa;          // db->SA
+    const char* query;          // the query's bytes
+    int64_t q_len;
+    const int32_t* offsets;
+    int32_t n;
+    int64_t* out;               // per offset: idx, num_matches, last_position
+    int32_t* status;            // per offset: sa_search's return value
+};
+constexpr int kSaSearchQueryLds = 32768;   // a longer query is read from memory
+
 }  // namespace tpz
 
 extern "C" hipError_t tpz_launch_gapless(const tpz::GaplessArgs* args, int grid,
                                          hipStream_t stream);
+extern "C" hipError_t tpz_launch_sa_search(const tpz::SaSearchArgs* args, hipStream_t stream);
 
 #endif

@@ -193,6 +193,104 @@ gapless_kernel(GaplessArgs a) {
 
 }  // namespace tpz
 
+namespace tpz {
+
+// search.c:17-30: compare the suffix at suf with P from *match on; bytes as
+// the reference's (signed) char differences.
+__device__ __forceinline__ int sa_compare(const char* T, int64_t Tsize, const char* P,
+                                          int64_t Psize, int64_t suf, int64_t* match) {
+    int64_t i = suf + *match, j = *match;
+    int r = 0;
+    for (; i < Tsize && j < Psize && (r = (int)(signed char)T[i] - (int)(signed char)P[j]) == 0;
+         ++i, ++j) {}
+    *match = j;
+    return r == 0 ? -(j != Psize) : r;
+}
+
+// One thread per query offset, the reference's binary search step for step
+// (search.c:266-344 without USE_LCP): which SA entries it compares, and the
+// match lengths it carries, decide the reported index and last_position
+// when the suffix does not occur.
+__global__ void __launch_bounds__(64) sa_search_kernel(SaSearchArgs a) {
+    extern __shared__ char qs[];
+    const bool in_lds = a.q_len <= kSaSearchQueryLds;
+    if (in_lds) {
+        for (int64_t x = threadIdx.x; x < a.q_len; x += blockDim.x) qs[x] = a.query[x];
+        __syncthreads();
+    }
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const char* T = a.text;
+    const int64_t* SA = a.sa;
+    const int64_t Tsize = a.t_size;
+    const int64_t offset = a.offsets[k];
+    const char* P = (in_lds ? qs : a.query) + offset;
+    const int64_t Psize = a.q_len - offset;
+    int64_t* out = a.out + 3 * (int64_t)k;
+    out[0] = -1;
+    if (T == nullptr || SA == nullptr || Tsize <= 0 || Psize <= 0) {
+        a.status[k] = -1;
+        return;
+    }
+    int64_t size, lsize, rsize, half;
+    int64_t match, lmatch, rmatch, llmatch, lrmatch, rlmatch, rrmatch;
+    int64_t i, j, kk;
+    int r;
+    for (i = j = kk = 0, match = lmatch = rmatch = 0, size = Tsize, half = size >> 1; 0 < size;
+         size = half, half >>= 1) {
+        match = lmatch < rmatch ? lmatch : rmatch;
+        r = sa_compare(T, Tsize, P, Psize, SA[i + half], &match);
+        if (r < 0) {
+            i += half + 1;
+            half -= (size & 1) ^ 1;
+            lmatch = match;
+        } else if (r > 0) {
+            rmatch = match;
+        } else {
+            lsize = half, j = i, rsize = size - half - 1, kk = i + half + 1;
+            for (llmatch = lmatch, lrmatch = match, half = lsize >> 1; 0 < lsize;
+                 lsize = half, half >>= 1) {
+                lmatch = llmatch < lrmatch ? llmatch : lrmatch;
+                r = sa_compare(T, Tsize, P, match, SA[j + half], &lmatch);
+                if (r < 0) {
+                    j += half + 1;
+                    half -= (lsize & 1) ^ 1;
+                    llmatch = lmatch;
+                } else {
+                    lrmatch = lmatch;
+                }
+            }
+            for (rlmatch = match, rrmatch = rmatch, half = rsize >> 1; 0 < rsize;
+                 rsize = half, half >>= 1) {
+                rmatch = rlmatch < rrmatch ? rlmatch : rrmatch;
+                r = sa_compare(T, Tsize, P, match, SA[kk + half], &rmatch);
+                if (r <= 0) {
+                    kk += half + 1;
+                    half -= (rsize & 1) ^ 1;
+                    rlmatch = rmatch;
+                } else {
+                    rrmatch = rmatch;
+                }
+            }
+            break;
+        }
+    }
+    out[0] = (0 < (kk - j)) ? j : i;
+    out[1] = kk - j;
+    out[2] = match;
+    a.status[k] = 0;
+}
+
+}  // namespace tpz
+
+extern "C" hipError_t tpz_launch_sa_search(const tpz::SaSearchArgs* args, hipStream_t stream) {
+    if (args->n <= 0) return hipSuccess;
+    const size_t lds = args->q_len <= tpz::kSaSearchQueryLds ? (size_t)args->q_len + 1 : 0;
+    hipLaunchKernelGGL(tpz::sa_search_kernel, dim3((args->n + 63) / 64), dim3(64), lds, stream,
+                       *args);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t tpz_launch_gapless(const tpz::GaplessArgs* args, int grid,
                                          hipStream_t stream) {
     hipLaunchKernelGGL(tpz::gapless_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream, *args);
