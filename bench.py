@@ -468,6 +468,50 @@ def stats_leg(args, n_queries=200, top=100):
             "reverse_gcups": round(cells / t / 1e9, 2)}
 
 
+def sa_leg(args, t_residues=100_000_000):
+    """Suffix array of a database text (the .SA of topaz index, index.c:66):
+    a 100 M-byte newline-separated protein text with 1 % of its proteins
+    repeated exactly.  A step is one tpz_suffix_array call, text upload and
+    suffix-array download (8 bytes per position) included."""
+    import ctypes as C
+    from topaz_amd import _lib, synth
+
+    class SaStats(C.Structure):
+        _fields_ = [("rounds", C.c_int32), ("device_bytes", C.c_int64)]
+
+    rng = np.random.default_rng(2030)
+    text, _, _, _ = synth.internal_text_and_seeds(rng, t_residues, 10, 10)
+    t = text.tobytes()
+    prots = t.split(b"\n")[:-1]
+    t += b"".join(prots[k] + b"\n" for k in rng.integers(0, len(prots), len(prots) // 100))
+    L = _lib.lib()
+    L.tpz_suffix_array.argtypes = [C.c_char_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(SaStats)]
+    L.tpz_suffix_array.restype = C.c_int
+    sa = np.empty(len(t), np.int64)
+    st = SaStats()
+    times = []
+    for k in range(max(1, args.warmup) + args.steps):
+        t0 = time.perf_counter()
+        _lib.check(L.tpz_suffix_array(t, len(t), sa.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      C.byref(st)), "tpz_suffix_array")
+        if k >= max(1, args.warmup):
+            times.append(time.perf_counter() - t0)
+    # a permutation whose neighbouring suffixes are in order on their first 8 bytes
+    assert np.array_equal(np.bincount(sa, minlength=len(t)), np.ones(len(t), np.int64))
+    pad = np.frombuffer(t + b"\0" * 8, np.uint8)
+    key = np.zeros(len(t), np.uint64)
+    for b in range(8):
+        key = (key << np.uint64(8)) | pad[np.minimum(sa + b, len(t))].astype(np.uint64)
+    assert np.all(key[1:] >= key[:-1])
+    dt = float(np.mean(times))
+    print("[bench] suffix array: %d bytes, %d rounds, %.3f s" % (len(t), st.rounds, dt),
+          file=sys.stderr, flush=True)
+    return {"workload": "suffix array of a %d-byte protein text (lognormal lengths mean 350, "
+                        "1%% of proteins repeated exactly), upload and download included" % len(t),
+            "text_bytes": len(t), "rounds": st.rounds, "s_per_build": round(dt, 3),
+            "mbytes_per_s": round(len(t) / dt / 1e6, 1), "device_bytes": st.device_bytes}
+
+
 def copy_leg(args):
     """A plain device-to-device copy of 1 GiB, three times: the calibration
     run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch of
@@ -497,7 +541,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
-    ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "stats", "copy"],
+    ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "stats", "sa", "copy"],
                     default="all")
     ap.add_argument("--serve-waves", type=int, default=8, choices=[4, 8],
                     help="wavefronts per workgroup of the serving loop's launches")
@@ -515,6 +559,10 @@ def main():
     if args.leg == "copy":
         print(json.dumps(copy_leg(args)), flush=True)
         return
+    if args.leg == "sa":
+        print(json.dumps({"metric": "suffix array of the database text", "suffix_array": sa_leg(args)}),
+              flush=True)
+        return
     if args.leg == "stats":
         print(json.dumps({"metric": "traceback statistics per batch of reported hits",
                           "stats": stats_leg(args)}), flush=True)
@@ -530,6 +578,7 @@ def main():
             out["gapless"] = gapless_leg(args)
             out["serving"] = serving_leg(args)
             out["stats"] = stats_leg(args)
+            out["suffix_array"] = sa_leg(args)
             top = [r for r in out["serving"]["rates"] if r["offered_requests_per_s"] == 100]
             if top:
                 out["latency_p50_ms"] = top[0]["p50_ms"]
