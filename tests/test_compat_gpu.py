@@ -391,3 +391,45 @@ def test_sa_search_batch_equals_reference():
         assert L.sa_search_batch(C.byref(db), C.byref(q), 1, end.ctypes.data_as(C.POINTER(C.c_int)),
                                  one.ctypes.data_as(PI64), None, None) == -1 and one[0] == -1
     assert checked > 500
+
+
+def test_statistics_from_threads_equal_one_at_a_time():
+    """ssw_align_stats called from 8 threads at once, with two gap settings
+    mixed, gives what the same calls give one at a time: the library's
+    statistics worker merges concurrent calls into shared runs, and keeps
+    calls of different settings apart."""
+    from concurrent.futures import ThreadPoolExecutor
+    L = lib()
+    mat = matrix()
+    rng = np.random.default_rng(71)
+    jobs = []
+    keep = []
+    for k in range(48):
+        q = synth.random_residues(rng, int(rng.integers(80, 700)))
+        r_, o_, l_ = synth.mutate(rng, q, np.array([0]), np.array([len(q)]),
+                                  np.array([rng.uniform(0.4, 0.9)]))
+        s = r_[:l_[0]].copy()
+        go, ge = (11, 1) if k % 2 else (10, 2)
+        prof = L.ssw_init(_p8(q), len(q), _p8(mat), 25, 2)
+        a = L.ssw_align(prof, _p8(s), len(s), go, ge, 0, 0, 0, 15)
+        sc, fe, re_ = a.contents.score1, a.contents.ref_end1, a.contents.read_end1
+        L.align_destroy(a)
+        keep.append((q, s, prof))
+        if sc > 0:
+            jobs.append((prof, s, go, ge, re_, fe, sc))
+
+    def stats(job):
+        prof, s, go, ge, re_, fe, sc = job
+        v = [C.c_int32() for _ in range(5)]
+        ident = C.c_double()
+        rc = L.ssw_align_stats(prof, _p8(s), len(s), go, ge, 15, re_, fe, sc,
+                               *[C.byref(x) for x in v], C.byref(ident))
+        return (rc,) + tuple(x.value for x in v) + (repr(ident.value),)
+
+    alone = [stats(j) for j in jobs]
+    with ThreadPoolExecutor(8) as ex:
+        together = list(ex.map(stats, jobs * 3))
+    assert len(jobs) > 40
+    assert together == alone * 3
+    for _, _, prof in keep:
+        L.init_destroy(prof)
