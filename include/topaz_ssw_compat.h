@@ -216,6 +216,14 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
                   options_t* opt, int* scores, int* lengths);
 
+/* db.c:176-205: the protein whose letters (or its '\n') hold text position
+ * pos: 0 and *index set, or -1 for pos >= db->Tsize.  For an index that
+ * tiles the text as `topaz index` writes it, a lookup in a table of every
+ * 64th position's protein (built once per database) in place of the binary
+ * search, which every hit of the seed walk pays (hit.c:138); the same
+ * protein, as it is unique.  Link with db.o's own db_index hidden. */
+int db_index(db_t* db, uint64_t pos, int64_t* index);
+
 /* n calls of sa_search(db, q, offsets[k], 0, ...) (search.c:237-361) for one
  * query, run on the GPU: idx[k], num_matches[k] and last_position[k] as that
  * call writes them (idx[k] = -1 and the other two untouched where it

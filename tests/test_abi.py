@@ -98,3 +98,38 @@ def test_patched_topaz_objects_call_the_batch_entry_points():
     lib = C.CDLL(str(_lib.LIB_PATH))
     for name in ("align_batch_using_ssw_profile", "stats_batch_using_ssw_profile", "sa_hsp_batch"):
         assert hasattr(lib, name)
+
+
+def test_db_index_equals_reference():
+    """db_index (host code, no GPU): the protein holding each text position,
+    from the table of every 64th position, equals the reference's own binary
+    search (db.c:176-205, compiled from its sources) at every position of a
+    tiled index (short, long and empty proteins); an index that does not
+    tile the text takes the binary search step for step."""
+    from tests.refcheck import REF_SO, DbT
+    from tests.test_compat_gpu import PositionT
+    if not REF_SO.exists():
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    import numpy as np
+    L, R = C.CDLL(str(_lib.LIB_PATH)), C.CDLL(str(REF_SO))
+    for f in (L.db_index, R.db_index):
+        f.argtypes = [C.POINTER(DbT), C.c_uint64, C.POINTER(C.c_int64)]
+        f.restype = C.c_int
+    rng = np.random.default_rng(81)
+    lens = np.concatenate([rng.integers(0, 5, 40), rng.integers(30, 700, 300),
+                           rng.integers(0, 3, 20)]).astype(np.int64)
+    rng.shuffle(lens)
+    starts = np.concatenate([[0], np.cumsum(lens + 1)[:-1]])
+    for tiled in (True, False):
+        index = (PositionT * len(lens))()
+        for k, (s0, n) in enumerate(zip(starts, lens)):
+            index[k].protein_start = int(s0) + (0 if tiled or k % 7 else 3)
+            index[k].protein_len = int(n)
+        db = DbT()
+        db.index = C.cast(index, C.c_void_p)
+        db.Psize = len(lens)
+        db.Tsize = int(starts[-1] + lens[-1] + 1)
+        for pos in list(range(db.Tsize + 2)):
+            a, b = C.c_int64(-5), C.c_int64(-5)
+            ra, rb = L.db_index(C.byref(db), pos, C.byref(a)), R.db_index(C.byref(db), pos, C.byref(b))
+            assert (ra, a.value) == (rb, b.value), (tiled, pos)

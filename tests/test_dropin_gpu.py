@@ -33,9 +33,16 @@ def _link_dropin(tmp, batch=False):
     search_local = str(tmp / "search_local.o")
     # keep the reference's sa_search & co., hide its sa_hsp
     subprocess.run(["objcopy", "-L", "sa_hsp", str(OBJ / "search.o"), search_local], check=True)
+    extra = [search_local]
+    if batch:
+        # the batch build also takes db_index from the library (db.o keeps the rest)
+        db_local = str(tmp / "db_local.o")
+        subprocess.run(["objcopy", "-L", "db_index", str(OBJ / "db.o"), db_local], check=True)
+        keep = [o for o in keep if Path(o).name != "db.o"]
+        extra.append(db_local)
     exe = tmp / ("topaz_batch" if batch else "topaz_dropin")
     libdir = str(_lib.LIB_PATH.parent)
-    subprocess.run(["g++", "-o", str(exe)] + keep + [search_local, "-L" + libdir,
+    subprocess.run(["g++", "-o", str(exe)] + keep + extra + ["-L" + libdir,
                     "-ltopaz_amd", "-Wl,-rpath," + libdir, "-lm", "-lpthread"], check=True)
     return exe
 

@@ -1008,6 +1008,105 @@ void stats_batch_using_ssw_profile(s_profile* profile, db_t* db, hit_t** hits, i
     }
 }
 
+}  // extern "C"
+
+namespace {
+// Protein of every 64th text position, for db_index: the protein holding
+// position p is the one holding p & ~63 or one of the next few.  Built once
+// per database; used only when the index tiles the text the way TOPAZ
+// writes it (each protein's letters and its '\n', back to back from 0).
+struct ProteinMap {
+    const db_t* db = nullptr;
+    const void* index = nullptr;
+    uint64_t t_size = 0, p_size = 0;
+    uint64_t print = 0;             // a few entries, against a reused address
+    bool tiled = false;
+    std::vector<uint32_t> block;
+};
+
+uint64_t index_print(const db_t* db) {
+    uint64_t h = 0;
+    if (db->Psize == 0 || !db->index) return h;
+    for (uint64_t k : {(uint64_t)0, db->Psize / 2, db->Psize - 1})
+        h = h * 1000003 + db->index[k].protein_start * 31 + (uint64_t)db->index[k].protein_len;
+    return h;
+}
+std::mutex g_pmap_mu;
+std::vector<ProteinMap*> g_pmaps;
+std::atomic<const ProteinMap*> g_pmap_last{nullptr};
+
+bool same_db(const ProteinMap* m, const db_t* db) {
+    return m->db == db && m->index == db->index && m->t_size == db->Tsize &&
+           m->p_size == db->Psize && m->print == index_print(db);
+}
+
+const ProteinMap& protein_map(const db_t* db) {
+    // every hit of a search asks: the last map answers without the lock
+    const ProteinMap* last = g_pmap_last.load(std::memory_order_acquire);
+    if (last && same_db(last, db)) return *last;
+    std::lock_guard<std::mutex> lk(g_pmap_mu);
+    for (auto* m : g_pmaps)
+        if (same_db(m, db)) {
+            g_pmap_last.store(m, std::memory_order_release);
+            return *m;
+        }
+    auto* m = new ProteinMap();
+    m->db = db;
+    m->index = db->index;
+    m->t_size = db->Tsize;
+    m->p_size = db->Psize;
+    m->print = index_print(db);
+    uint64_t next = 0;
+    bool tiled = db->Psize > 0 && db->Psize < UINT32_MAX;
+    for (uint64_t k = 0; tiled && k < db->Psize; ++k) {
+        tiled = db->index[k].protein_start == next && db->index[k].protein_len >= 0;
+        next = db->index[k].protein_start + (uint64_t)db->index[k].protein_len + 1;
+    }
+    m->tiled = tiled && next == db->Tsize;
+    if (m->tiled) {
+        m->block.resize((db->Tsize + 63) / 64);
+        uint64_t k = 0;
+        for (uint64_t b = 0; b < m->block.size(); ++b) {
+            const uint64_t pos = b * 64;
+            while (pos > db->index[k].protein_start + (uint64_t)db->index[k].protein_len) ++k;
+            m->block[b] = (uint32_t)k;
+        }
+    }
+    g_pmaps.push_back(m);
+    g_pmap_last.store(m, std::memory_order_release);
+    return *m;
+}
+}  // namespace
+
+extern "C" {
+
+int db_index(db_t* db, uint64_t pos, int64_t* index) {
+    if (pos >= db->Tsize) return -1;                                 // db.c:180-182
+    const ProteinMap& m = protein_map(db);
+    if (m.tiled) {
+        // the protein whose letters or '\n' hold pos (db.c:184-203 finds the
+        // same one: in a tiled index it is unique)
+        uint64_t k = m.block[pos >> 6];
+        while (pos > db->index[k].protein_start + (uint64_t)db->index[k].protein_len) ++k;
+        *index = (int64_t)k;
+        return 0;
+    }
+    // any other index: the reference's binary search, step for step
+    uint64_t i, half, size;
+    for (i = 0, size = db->Psize, half = size >> 1; 0 < size; size = half, half >>= 1) {
+        const position_t* t = db->index + i + half;
+        if (pos < t->protein_start) continue;
+        if (pos > t->protein_start + (uint64_t)t->protein_len) {
+            i += half + 1;
+            half -= (size & 1) ^ 1;
+            continue;
+        }
+        break;
+    }
+    *index = (int64_t)(i + half);
+    return 0;
+}
+
 int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
                     int64_t* num_matches, int64_t* last_position) {
     if (n <= 0) return 0;
