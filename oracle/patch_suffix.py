@@ -13,9 +13,10 @@ filled by one sa_hsp_batch call when it runs out (or up front by the patched
 query_find_suffixes, patch_query.py).  The cache sits behind the suffix_t in
 the same allocation, so suffix.h and the other objects are unchanged.
 
-suffix_alloc, suffix_duplicate and suffix_increment (found by their
-definitions, not by line numbers) are replaced by the code below, and a
-helper, suffix_prefetched, is added for query_find_suffixes.  The patched
+suffix_alloc, suffix_duplicate, suffix_free and suffix_increment (found by
+their definitions, not by line numbers) are replaced by the code below, and
+a helper, suffix_prefetched, is added for query_find_suffixes.  A suffix's
+refills grow from TPZ_WALK_K slots, doubling each time.  The patched
 text goes to stdout only: oracle/Makefile compiles it straight into
 oracle/_ref/batch/suffix_batch.o.
 """
@@ -26,13 +27,17 @@ HEADER = """
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
                   options_t* opt, int* scores, int* lengths);
 
+/* a refill extends TPZ_WALK_K slots, twice as many as the suffix's last
+   refill after that, up to TPZ_WALK_MAX: a walk that has gone far tends to
+   go on, and each refill is a GPU round trip */
 #define TPZ_WALK_K 128
+#define TPZ_WALK_MAX 16384
 
 /* extensions of the slots first, first + direction, ..., n of them */
 typedef struct {
     saidx64_t first;
-    int n, next;
-    int score[TPZ_WALK_K], length[TPZ_WALK_K];
+    int n, next, cap, want;
+    int *score, *length;
 } tpz_walk_cache;
 
 static tpz_walk_cache* tpz_cache(suffix_t* suf) {
@@ -41,9 +46,22 @@ static tpz_walk_cache* tpz_cache(suffix_t* suf) {
 
 static suffix_t* tpz_suffix_new(void) {
     suffix_t* suf = (suffix_t*) malloc(sizeof(suffix_t) + sizeof(tpz_walk_cache));
-    tpz_cache(suf)->n = 0;
-    tpz_cache(suf)->next = 0;
+    tpz_walk_cache* c = tpz_cache(suf);
+    c->n = 0;
+    c->next = 0;
+    c->cap = 0;
+    c->want = TPZ_WALK_K;
+    c->score = NULL;
+    c->length = NULL;
     return suf;
+}
+
+static void tpz_reserve(tpz_walk_cache* c, int n) {
+    if(n > c->cap) {
+        c->score = (int*) realloc(c->score, sizeof(int) * n);
+        c->length = (int*) realloc(c->length, sizeof(int) * n);
+        c->cap = n;
+    }
 }
 
 /* the extensions of the n slots after suf->position in its direction,
@@ -51,10 +69,11 @@ static suffix_t* tpz_suffix_new(void) {
 void suffix_prefetched(suffix_t* suf, int n, const int* scores, const int* lengths) {
     tpz_walk_cache* c = tpz_cache(suf);
     int k;
+    tpz_reserve(c, n);
     c->first = suf->position + suf->direction;
-    c->n = n < TPZ_WALK_K ? n : TPZ_WALK_K;
+    c->n = n;
     c->next = 0;
-    for(k = 0; k < c->n; ++k) {
+    for(k = 0; k < n; ++k) {
         c->score[k] = scores[k];
         c->length[k] = lengths[k];
     }
@@ -80,6 +99,14 @@ DUPLICATE = """suffix_t* suffix_duplicate(suffix_t* suf) {
     return tmp;
 }"""
 
+FREE = """void suffix_free(suffix_t *suf) {
+    if(suf) {
+        free(tpz_cache(suf)->score);
+        free(tpz_cache(suf)->length);
+        free(suf);
+    }
+}"""
+
 INCREMENT = """int suffix_increment(suffix_t *suf, db_t* db, options_t* opt) {
     tpz_walk_cache* c = tpz_cache(suf);
     int j;
@@ -98,20 +125,25 @@ INCREMENT = """int suffix_increment(suffix_t *suf, db_t* db, options_t* opt) {
 
     if(c->next >= c->n || c->first + (saidx64_t) c->next * suf->direction != suf->position) {
         /* refill: this slot and the next ones in the walk's direction */
-        int offs[TPZ_WALK_K];
-        int64_t pos[TPZ_WALK_K];
+        const int want = c->want;
+        int* offs = (int*) malloc(sizeof(int) * want);
+        int64_t* pos = (int64_t*) malloc(sizeof(int64_t) * want);
         saidx64_t p = suf->position;
         int k = 0;
-        while(k < TPZ_WALK_K && p >= 0 && p < (saidx64_t) db->Tsize) {
+        while(k < want && p >= 0 && p < (saidx64_t) db->Tsize) {
             offs[k] = suf->offset;
             pos[k] = p;
             ++k;
             p += suf->direction;
         }
+        tpz_reserve(c, k);
         sa_hsp_batch(db, suf->seq, k, offs, pos, opt, c->score, c->length);
+        free(offs);
+        free(pos);
         c->first = suf->position;
         c->n = k;
         c->next = 0;
+        c->want = want < TPZ_WALK_MAX ? 2 * want : TPZ_WALK_MAX;
     }
 
     j = c->next++;
@@ -145,6 +177,7 @@ def main(path):
     spans = [(definition(lines, net, head), body) for head, body in (
         ("suffix_t* suffix_alloc(", ALLOC),
         ("suffix_t* suffix_duplicate(", DUPLICATE),
+        ("void suffix_free(", FREE),
         ("int suffix_increment(", INCREMENT))]
     spans.sort()
     out, at = [], 0
