@@ -1,4 +1,4 @@
-"""`topaz index` timing on one GPU box: the stock reference binary (its own
+"""`topaz index` timing on a machine with one MI355X: the stock reference binary (its own
 libdivsufsort, one core) against TOPAZ linked with libtopaz_amd's
 divsufsort64 (suffix array built on the GPU).  Both must write identical
 index files; prints one JSON line.

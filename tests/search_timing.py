@@ -1,4 +1,4 @@
-"""End-to-end `topaz search` timing on one GPU box: the stock reference
+"""End-to-end `topaz search` timing on a machine with one MI355X: the stock reference
 binary (CPU, the reference's own ssw.c / search.c) against TOPAZ linked with
 libtopaz_amd, both as the link-level drop-in (one GPU call per seed and per
 hit) and with query.c's and suffix.c's per-seed and per-hit loops replaced by
