@@ -32,19 +32,19 @@ from tests.test_dropin_gpu import REF, _link_dropin, _per_query  # noqa: E402
 from topaz_amd import synth  # noqa: E402
 
 
-def run(cmd, cwd, timeout, out_path=None):
+def run(cmd, cwd, timeout, out_path=None, env=None):
     """Run cmd; its stdout goes to out_path as it is printed (so a long
     search shows progress there) and is returned with the wall time, stderr
     and the child's CPU seconds (user + system)."""
     r0 = resource.getrusage(resource.RUSAGE_CHILDREN)
     t0 = time.perf_counter()
     if out_path is None:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=cwd)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=cwd, env=env)
         out, err, rc = r.stdout, r.stderr, r.returncode
     else:
         with open(out_path, "w") as f:
             r = subprocess.run(cmd, stdout=f, stderr=subprocess.PIPE, text=True,
-                               timeout=timeout, cwd=cwd)
+                               timeout=timeout, cwd=cwd, env=env)
         out, err, rc = open(out_path).read(), r.stderr, r.returncode
     dt = time.perf_counter() - t0
     r1 = resource.getrusage(resource.RUSAGE_CHILDREN)
@@ -63,6 +63,9 @@ def main():
                     help="worker threads of the GPU-backed binaries (default: 4 x --threads); "
                          "their threads mostly wait on the GPU, so more of them than cores "
                          "keeps more requests in each launch")
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="GPU_MAX_HW_QUEUES for the GPU-backed binaries (HIP's default is 4; "
+                         "with 8 the library spreads its calls over two streams)")
     ap.add_argument("--blasttab", action="store_true",
                     help="search with -B (start coordinates and traceback statistics per hit)")
     ap.add_argument("--timeout", type=int, default=900)
@@ -96,11 +99,13 @@ def main():
         flags = ["-H", "100", "-E", "0.001"] + (["-B"] if args.blasttab else [])
         gpu_threads = args.gpu_threads or 4 * args.threads
         out, times, inner, cpu_s = {}, {}, {}, {}
+        gpu_env = dict(os.environ, GPU_MAX_HW_QUEUES=str(args.hw_queues))
         for name, exe in exes.items():
             t = args.threads if name == "reference_cpu" else gpu_threads
             text, dt, err, cpu_s[name] = run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB"),
                             "-T", str(t)] + flags, str(tmp), args.timeout,
-                           os.path.join(args.out, name + ".txt") if args.out else None)
+                           os.path.join(args.out, name + ".txt") if args.out else None,
+                           None if name == "reference_cpu" else gpu_env)
             out[name], times[name] = text, dt
             # TOPAZ's own clock: from the search's start, after the index is loaded
             el = [ln for ln in err.split("\n") if ln.startswith("elapsed time:")]
@@ -110,7 +115,7 @@ def main():
         ref = _per_query(out["reference_cpu"])
         identical = {k: _per_query(v) == ref for k, v in out.items()}
     res = {"subjects": len(db), "queries": len(queries), "threads": args.threads,
-           "gpu_threads": gpu_threads,
+           "gpu_threads": gpu_threads, "gpu_hw_queues": args.hw_queues,
            "flags": " ".join(flags), "index_s": round(t_index, 2),
            "search_s": {k: round(v, 2) for k, v in times.items()},
            "topaz_elapsed_s": inner,

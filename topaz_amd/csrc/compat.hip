@@ -56,6 +56,28 @@ void hip_or_die(hipError_t e, const char* what) {
 }
 
 constexpr size_t kScratchMin = (size_t)1 << 20;
+// Pinned blocks: each hipHostMalloc call costs milliseconds whatever its
+// size, and TOPAZ's worker threads all make their first call at once, so
+// the first block of a thread comes from a pool pinned 16 blocks at a time.
+constexpr size_t kPinnedMin = (size_t)4 << 20;
+constexpr int kPinnedPoolBlocks = 16;
+
+std::mutex g_pin_mu;
+std::vector<char*> g_pin_free;
+
+char* pooled_pinned_block() {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    if (g_pin_free.empty()) {
+        void* p = nullptr;
+        hip_or_die(hipHostMalloc(&p, kPinnedMin * kPinnedPoolBlocks, hipHostMallocCoherent),
+                   "hipHostMalloc");
+        for (int k = kPinnedPoolBlocks - 1; k >= 0; --k)
+            g_pin_free.push_back((char*)p + (size_t)k * kPinnedMin);
+    }
+    char* b = g_pin_free.back();
+    g_pin_free.pop_back();
+    return b;
+}
 
 // Grow-only buffers owned by one host thread.  A buffer grows to at least
 // twice its size, so a run of slowly growing calls reallocates rarely, and
@@ -86,21 +108,56 @@ struct Pinned {
     char* get(size_t bytes) {
         if (bytes > n) {
             if (p) kept.push_back(p);
-            n = std::max(std::max(bytes, 2 * n), 4 * kScratchMin);
-            hip_or_die(hipHostMalloc(&p, n, hipHostMallocCoherent), "hipHostMalloc");
+            if (!p && bytes <= kPinnedMin) {
+                p = pooled_pinned_block();
+                n = kPinnedMin;
+            } else {
+                n = std::max(std::max(bytes, 2 * n), kPinnedMin);
+                hip_or_die(hipHostMalloc(&p, n, hipHostMallocCoherent), "hipHostMalloc");
+            }
         }
         return (char*)p;
     }
 };
 
-// Per host thread: the buffers of one call and the event its wait is on
-// (TOPAZ calls the alignment functions from T worker threads at once,
-// engine.c:423-457).  The work itself goes to the library's one stream: a
-// stream per thread would spread the threads over the few hardware queues
-// HIP multiplexes streams onto, where a thread's short launch would wait
-// behind a serving-loop launch of milliseconds on the same queue.
+// The streams the calls go to: the library's stream, plus
+// TOPAZ_AMD_CALL_STREAMS - 1 more, thread k using stream k mod the count.
+// Few streams: a stream per thread would spread the threads over the few
+// hardware queues HIP multiplexes streams onto (GPU_MAX_HW_QUEUES, 4 by
+// default), where a thread's short launch would wait behind a serving-loop
+// launch of milliseconds on the same queue.  With 8 or more hardware
+// queues, 2 streams (the default then) spread the threads' enqueues over
+// two stream locks without sharing a queue with the serving loop.
+hipStream_t call_stream(int ordinal) {
+    static std::vector<hipStream_t> streams = [] {
+        if (ensure_init()) die("device init");
+        std::vector<hipStream_t> v{stream()};
+        int n = 1;
+        if (const char* q = getenv("GPU_MAX_HW_QUEUES"))
+            if (atoi(q) >= 8) n = 2;
+        if (const char* e = getenv("TOPAZ_AMD_CALL_STREAMS")) {
+            n = atoi(e);
+            if (n < 1 || n > 16) {
+                fprintf(stderr, "topaz_amd: TOPAZ_AMD_CALL_STREAMS=%s is not 1..16\n", e);
+                _exit(EXIT_FAILURE);
+            }
+        }
+        for (int k = 1; k < n; ++k) {
+            hipStream_t st;
+            hip_or_die(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+            v.push_back(st);
+        }
+        return v;
+    }();
+    return streams[(size_t)ordinal % streams.size()];
+}
+
+// Per host thread: the buffers of one call, the stream its work goes to and
+// the event its wait is on (TOPAZ calls the alignment functions from T
+// worker threads at once, engine.c:423-457).
 struct ThreadCtx {
     hipEvent_t done = nullptr;
+    hipStream_t st = nullptr;
     Scratch in, out, ws;
     Pinned pin;
     // the gapless query copies and score table of the last call, on the
@@ -113,14 +170,16 @@ struct ThreadCtx {
     size_t g_qr = 0, g_tab = 0, g_meta = 0;
     ThreadCtx() {
         if (ensure_init()) die("device init");
+        static std::atomic<int> threads{0};
+        st = call_stream(threads.fetch_add(1));
         // blocking: a waiting thread sleeps instead of spinning on a core
         // that another worker thread's host work could use
         hip_or_die(hipEventCreateWithFlags(&done, hipEventDisableTiming | hipEventBlockingSync),
                    "hipEventCreate");
     }
-    // waits until this thread's work so far on the stream has finished
+    // waits until this thread's work so far on its stream has finished
     void finish(const char* what) {
-        hip_or_die(hipEventRecord(done, stream()), what);
+        hip_or_die(hipEventRecord(done, st), what);
         hip_or_die(hipEventSynchronize(done), what);
     }
 };
@@ -379,7 +438,7 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     int32_t* dout = (int32_t*)c.out.get(64);
     const size_t ws_words = (size_t)4 * 2 * (n + 2) * 64;     // four wavefront buffers
     uint32_t* dws = (uint32_t*)c.ws.get(ws_words * 4);
-    hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream()), "upload");
+    hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
     SwKernelArgs a{};
     a.q_res = (const int8_t*)(d + o_q);
     a.q_off = (const int64_t*)(d + o_meta);
@@ -404,9 +463,9 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.out_score = dout;
     a.out_ref = dout + 1;
     a.out_read = dout + 2;
-    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, stream()), "sw_forward");
+    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, c.st), "sw_forward");
     int32_t* ho = (int32_t*)(h + o_out);
-    hip_or_die(hipMemcpyAsync(ho, dout, 12, hipMemcpyDeviceToHost, stream()), "readback");
+    hip_or_die(hipMemcpyAsync(ho, dout, 12, hipMemcpyDeviceToHost, c.st), "readback");
     c.finish("sw_forward");
     memcpy(out, ho, 12);
 }
@@ -739,7 +798,7 @@ void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
         memcpy(g + c.g_meta + 8, &zero64, 8);
         memcpy(g + c.g_meta + 16, &qlen, 4);
         char* gd = (char*)c.g_dev.get(gbytes);
-        hip_or_die(hipMemcpyAsync(gd, g, gbytes, hipMemcpyHostToDevice, stream()), "upload");
+        hip_or_die(hipMemcpyAsync(gd, g, gbytes, hipMemcpyHostToDevice, c.st), "upload");
         c.g_text = rt.text;
         c.g_query.assign(q->seq_buf.buf, q->seq_buf.buf + qlen);
         memcpy(c.g_matrix, mat, 625);
@@ -764,7 +823,7 @@ void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
     a.out_score = res;
     a.out_length = res + n;
     const int grid = std::max(1, std::min((n + 255) / 256, std::max(num_cu(), 1) * 32));
-    hip_or_die(tpz_launch_gapless(&a, grid, stream()), "gapless");
+    hip_or_die(tpz_launch_gapless(&a, grid, c.st), "gapless");
     c.finish("gapless");
     for (int k = 0; k < n; ++k) {
         lengths[k] = res[n + k];
@@ -1136,7 +1195,7 @@ int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
     a.n = n;
     a.out = hr;
     a.status = hs;
-    hip_or_die(tpz_launch_sa_search(&a, stream()), "sa_search");
+    hip_or_die(tpz_launch_sa_search(&a, c.st), "sa_search");
     c.finish("sa_search");
     int rc = 0;
     for (int k = 0; k < n; ++k) {
