@@ -12,8 +12,9 @@ The code is found by definitions and calls, not by line numbers:
   * query_find_suffixes (query.c:94-131) is rewritten: one sa_search_batch
     call for the suffix-array search of every offset, then one sa_hsp_batch
     call for both
-    seeds of every offset and the first FIND_K slots of each new suffix's
-    walk (handed to the suffix with suffix_prefetched, patch_suffix.py),
+    seeds of every offset and the first slots of each new suffix's walk
+    (FIND_K, more in a larger text; handed to the suffix with
+    suffix_prefetched, patch_suffix.py),
     then the suffixes in the original order;
   * in query_perform_alignments (query.c:178-248), the `while` loop over the
     hit-list traversal that calls perform_alignment_with_profile per hit
@@ -39,21 +40,26 @@ int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
 void suffix_prefetched(suffix_t* suf, int n, const int* scores, const int* lengths);
 """
 
-# slots of each new suffix's walk prefetched with the first extensions
+# slots of each new suffix's walk prefetched with the first extensions: at
+# least FIND_K, doubled while FIND_K slots per 1.4 M text bytes is more
+# (walks run longer in a larger database), at most FIND_K_MAX
 FIND_K = 64
+FIND_K_MAX = 1024
 
 FIND = """int query_find_suffixes(query_t* q) {
     /* 1. the suffix-array interval of every query offset, in one batch */
     int i, h, j, n_cand = 0, n_hit = 0, n_seed = 0;
     int num_suffixes = seq_len(q->query) - q->opt->min_hit_length;
-    const int K = %(K)d;
     const saidx64_t tsize = (saidx64_t) q->db->Tsize;
+    int K = %(K)d;
     size_t cap = num_suffixes > 0 ? (size_t) num_suffixes : 1;
     int* hit_off = malloc(sizeof(int) * cap);
     saidx64_t* hit_pos = malloc(sizeof(saidx64_t) * cap);
     int *offs, *scores, *lengths, *first;
     int64_t* slots;
 
+    while(K < %(KMAX)d && (saidx64_t) K * 1400000 < tsize)
+        K *= 2;
     for(i = 0; i < num_suffixes; ++i) {
         if(_bad_prefix(seq_ptr(q->query) + i))
             continue;
@@ -114,7 +120,7 @@ FIND = """int query_find_suffixes(query_t* q) {
     free(first);
     q->state = QUERY_GET_ALIGNMENTS;
     return 0;
-}""" % {"K": FIND_K}
+}""" % {"K": FIND_K, "KMAX": FIND_K_MAX}
 
 ALIGN = """    {
         /* every hit not yet aligned, in traversal order, ranked as before */
