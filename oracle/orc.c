@@ -603,3 +603,117 @@ void orc_sa_hsp_batch(const char* T, int64_t Tsize, const char* Qcat,
                             q_len[s->query], s->offset, mat, &score[k]);
     }
 }
+
+
+/* ---- suffix array (index.c:66) ------------------------------------------ */
+
+typedef struct {
+    int64_t pos;
+    int64_t r1, r2;        /* rank of the first h bytes, of the next h */
+} orc_sa_rec;
+
+static int orc_sa_cmp(const void* a, const void* b) {
+    const orc_sa_rec* x = (const orc_sa_rec*) a;
+    const orc_sa_rec* y = (const orc_sa_rec*) b;
+    if (x->r1 != y->r1) return x->r1 < y->r1 ? -1 : 1;
+    if (x->r2 != y->r2) return x->r2 < y->r2 ? -1 : 1;
+    return 0;
+}
+
+int orc_suffix_array(const unsigned char* T, int64_t n, int64_t* SA) {
+    orc_sa_rec* rec;
+    int64_t* rank;
+    int64_t h, k;
+    if (n <= 0) return 0;
+    rec = (orc_sa_rec*) malloc(sizeof(orc_sa_rec) * (size_t) n);
+    rank = (int64_t*) malloc(sizeof(int64_t) * (size_t) n);
+    if (!rec || !rank) { free(rec); free(rank); return -1; }
+    /* rank 0 past the end: a prefix sorts before the longer suffix */
+    for (k = 0; k < n; ++k) rank[k] = (int64_t) T[k] + 1;
+    for (h = 1;; h *= 2) {
+        int64_t groups = 1;
+        for (k = 0; k < n; ++k) {
+            rec[k].pos = k;
+            rec[k].r1 = rank[k];
+            rec[k].r2 = k + h < n ? rank[k + h] : 0;
+        }
+        qsort(rec, (size_t) n, sizeof(orc_sa_rec), orc_sa_cmp);
+        rank[rec[0].pos] = 1;
+        for (k = 1; k < n; ++k) {
+            if (orc_sa_cmp(&rec[k], &rec[k - 1]) != 0) ++groups;
+            rank[rec[k].pos] = groups;
+        }
+        if (groups == n || h >= n) break;
+    }
+    for (k = 0; k < n; ++k) SA[k] = rec[k].pos;
+    free(rec);
+    free(rank);
+    return 0;
+}
+
+/* ---- sa_search (search.c:17-30, 237-361) ---------------------------------- */
+
+static int orc_sa_compare(const char* T, int64_t Tsize, const char* P, int64_t Psize,
+                          int64_t suf, int64_t* match) {
+    int64_t i = suf + *match, j = *match;
+    int r = 0;
+    /* the reference subtracts (signed) chars */
+    for (; i < Tsize && j < Psize && (r = (int) (signed char) T[i] - (int) (signed char) P[j]) == 0;
+         ++i, ++j) {}
+    *match = j;
+    return r == 0 ? -(j != Psize) : r;
+}
+
+int orc_sa_search(const char* T, int64_t Tsize, const int64_t* SA,
+                  const char* P, int64_t Psize, int64_t* idx,
+                  int64_t* num_matches, int64_t* last_position) {
+    int64_t size, lsize, rsize, half;
+    int64_t match, lmatch, rmatch, llmatch, lrmatch, rlmatch, rrmatch;
+    int64_t i, j, k;
+    int r;
+    if (idx) *idx = -1;
+    if (!T || !P || !SA || Tsize <= 0 || Psize <= 0) return -1;
+    for (i = j = k = 0, match = lmatch = rmatch = 0, size = Tsize, half = size >> 1; 0 < size;
+         size = half, half >>= 1) {
+        match = lmatch < rmatch ? lmatch : rmatch;
+        r = orc_sa_compare(T, Tsize, P, Psize, SA[i + half], &match);
+        if (r < 0) {                      /* the upper half */
+            i += half + 1;
+            half -= (size & 1) ^ 1;
+            lmatch = match;
+        } else if (r > 0) {               /* the lower half */
+            rmatch = match;
+        } else {                          /* a match: widen to the interval */
+            lsize = half, j = i, rsize = size - half - 1, k = i + half + 1;
+            for (llmatch = lmatch, lrmatch = match, half = lsize >> 1; 0 < lsize;
+                 lsize = half, half >>= 1) {
+                lmatch = llmatch < lrmatch ? llmatch : lrmatch;
+                r = orc_sa_compare(T, Tsize, P, match, SA[j + half], &lmatch);
+                if (r < 0) {
+                    j += half + 1;
+                    half -= (lsize & 1) ^ 1;
+                    llmatch = lmatch;
+                } else {
+                    lrmatch = lmatch;
+                }
+            }
+            for (rlmatch = match, rrmatch = rmatch, half = rsize >> 1; 0 < rsize;
+                 rsize = half, half >>= 1) {
+                rmatch = rlmatch < rrmatch ? rlmatch : rrmatch;
+                r = orc_sa_compare(T, Tsize, P, match, SA[k + half], &rmatch);
+                if (r <= 0) {
+                    k += half + 1;
+                    half -= (rsize & 1) ^ 1;
+                    rlmatch = rmatch;
+                } else {
+                    rrmatch = rmatch;
+                }
+            }
+            break;
+        }
+    }
+    if (idx) *idx = (0 < (k - j)) ? j : i;
+    if (num_matches) *num_matches = k - j;
+    if (last_position) *last_position = match;
+    return 0;
+}

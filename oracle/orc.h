@@ -25,6 +25,9 @@
  *   orc_sa_gapless -- brute force, and the compiled reference's sa_gapless.
  *   orc_evalue -- closed form, against compiled align_using_ssw_profile.
  *   orc_aa_table2 / orc_upper_aa_only -- against the committed tables.
+ *   orc_suffix_array -- brute-force sorting of the suffixes (small texts)
+ *        and the reference's compiled divsufsort64.
+ *   orc_sa_search -- the reference's compiled sa_search, offset by offset.
  */
 #ifndef TOPAZ_ORACLE_ORC_H
 #define TOPAZ_ORACLE_ORC_H
@@ -131,6 +134,19 @@ int orc_sa_gapless(const char* T, int64_t Tsize, int64_t idx,
 void orc_evalue(int32_t rawscore, int32_t qlen, uint64_t db_size,
                 uint64_t db_proteins, double lambda, double k, double H,
                 double* bitscore, double* evalue, int32_t* effective_length);
+
+/* Suffix array of T[0..n-1] in divsufsort64's order (index.c:66): suffixes
+ * as unsigned bytes, a suffix before every longer suffix it is a prefix of.
+ * Prefix doubling with qsort (the suffix array is unique, so the method is
+ * free).  Returns 0, or -1 when memory runs out. */
+int orc_suffix_array(const unsigned char* T, int64_t n, int64_t* SA);
+
+/* sa_search (search.c:237-361, without USE_LCP) for the suffix P[0..Psize-1]
+ * of a query: the same binary search, step for step.  Returns 0 with *idx,
+ * *num_matches, *last_position, or -1 (and *idx = -1) for empty inputs. */
+int orc_sa_search(const char* T, int64_t Tsize, const int64_t* SA,
+                  const char* P, int64_t Psize, int64_t* idx,
+                  int64_t* num_matches, int64_t* last_position);
 
 /* Alphabet maps (lexicographical.h:19-74): internal char -> matrix index. */
 int8_t orc_aa_table2(unsigned char c);

@@ -58,6 +58,12 @@ class Oracle:
         L.orc_sa_hsp.restype = C.c_int
         L.orc_sa_hsp.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_char_p,
                                  I32, I32, P8, PI]
+        P64 = C.POINTER(C.c_int64)
+        L.orc_suffix_array.restype = C.c_int
+        L.orc_suffix_array.argtypes = [C.c_char_p, C.c_int64, P64]
+        L.orc_sa_search.restype = C.c_int
+        L.orc_sa_search.argtypes = [C.c_char_p, C.c_int64, P64, C.c_char_p, C.c_int64,
+                                    P64, P64, P64]
         L.orc_sa_gapless.restype = C.c_int
         L.orc_sa_gapless.argtypes = L.orc_sa_hsp.argtypes
         L.orc_sa_hsp_batch.restype = None
@@ -106,6 +112,18 @@ class Oracle:
             _ptr8(q), len(q), _ptr8(s), len(s), _ptr8(mat), 25, go, ge,
             read_end, ref_end, score, *[C.byref(x) for x in v], C.byref(ident))
         return (rc,) + tuple(x.value for x in v) + (ident.value,)
+
+    def suffix_array(self, T):
+        sa = np.zeros(max(len(T), 1), np.int64)
+        assert self.L.orc_suffix_array(T, len(T), sa.ctypes.data_as(C.POINTER(C.c_int64))) == 0
+        return sa[:len(T)]
+
+    def sa_search(self, T, SA, P):
+        """(return, idx, num_matches, last_position) for the suffix P."""
+        v = [C.c_int64(-9) for _ in range(3)]
+        rc = self.L.orc_sa_search(T, len(T), SA.ctypes.data_as(C.POINTER(C.c_int64)), P, len(P),
+                                  *[C.byref(x) for x in v])
+        return (rc,) + tuple(x.value for x in v)
 
     def sa_hsp(self, T, idx, Q, o, mat=None):
         mat = matrix() if mat is None else mat

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from tests.refcheck import REF_SO, Oracle, Reference, matrix
+from topaz_amd import synth
 from topaz_amd.matrices import NAMES, aa_table2
 
 GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
@@ -403,3 +404,63 @@ def test_stats_vs_compiled_reference_homologs(orc):
         assert x[:6] == y[:6] and (x[6] == y[6] or (math.isnan(x[6]) and math.isnan(y[6]))), it
         n += 1
     assert n > 2500
+
+
+def _ref_lib():
+    if not REF_SO.exists():
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    return Reference().L
+
+
+def test_suffix_array_against_reference_and_brute_force():
+    """orc_suffix_array equals the reference's compiled divsufsort64 on
+    random bytes, a two-letter text, one letter x 3,000, a repeated block and
+    a protein text with '\\n' separators, and brute-force sorting of the
+    suffixes where that is cheap."""
+    R = _ref_lib()
+    R.divsufsort64.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.c_int64]
+    R.divsufsort64.restype = C.c_int32
+    o = Oracle()
+    rng = np.random.default_rng(91)
+    block = rng.integers(0, 256, 700, dtype=np.uint8).tobytes()
+    text, _, _, _ = synth.internal_text_and_seeds(rng, 200_000, 4, 4)
+    cases = [b"", b"a", b"ba", b"banana", rng.integers(0, 256, 50_000, dtype=np.uint8).tobytes(),
+             rng.integers(65, 67, 20_000, dtype=np.uint8).tobytes(), b"Q" * 3000,
+             block * 12, text.tobytes()]
+    for t in cases:
+        got = o.suffix_array(t)
+        ref = np.zeros(max(len(t), 1), np.int64)
+        assert R.divsufsort64(t, ref.ctypes.data_as(C.POINTER(C.c_int64)), len(t)) == 0
+        assert np.array_equal(got, ref[:len(t)]), len(t)
+        if len(t) <= 3000:
+            assert got.tolist() == sorted(range(len(t)), key=lambda i: t[i:])
+
+
+def test_sa_search_against_reference():
+    """orc_sa_search equals the reference's compiled sa_search (search.c:
+    237-361) at every offset of queries that occur in the text, that do not,
+    and that run along a repeat, with the index and the last match length
+    of absent suffixes."""
+    from tests.refcheck import DbT, SeqT
+    R = _ref_lib()
+    o = Oracle()
+    rng = np.random.default_rng(92)
+    text, queries, _, _ = synth.internal_text_and_seeds(rng, 100_000, 3, 3)
+    t = text.tobytes() + b"A" * 30 + b"\n"
+    sa = o.suffix_array(t)
+    db = DbT()
+    db.T = t
+    db.Tsize = len(t)
+    db.SA = sa.ctypes.data_as(C.POINTER(C.c_int64))
+    P64 = C.POINTER(C.c_int64)
+    R.sa_search.argtypes = [C.POINTER(DbT), C.POINTER(SeqT), C.c_int64, C.c_int64, P64, P64, P64]
+    R.sa_search.restype = C.c_int
+    for qb in [t[5000:5300], queries[0], b"A" * 40 + t[7000:7100], queries[1][:50] + t[100:200]]:
+        q = SeqT()
+        q.seq_buf.buf = qb
+        q.seq_buf.len = len(qb)
+        q.seq_len = len(qb)
+        for off in range(len(qb) + 1):
+            v = [C.c_int64(-9) for _ in range(3)]
+            rc = R.sa_search(C.byref(db), C.byref(q), off, 0, *[C.byref(x) for x in v])
+            assert o.sa_search(t, sa, qb[off:]) == (rc,) + tuple(x.value for x in v), off
