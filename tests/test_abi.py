@@ -90,13 +90,14 @@ def test_patched_topaz_objects_call_the_batch_entry_points():
     qu, qd = symbols(BATCH / "query_batch.o")
     su, sd = symbols(BATCH / "suffix_batch.o")
     assert {"align_batch_using_ssw_profile", "stats_batch_using_ssw_profile", "sa_hsp_batch",
-            "suffix_prefetched"} <= qu
-    assert "stats_using_ssw_profile" not in qu and "sa_hsp" not in qu
+            "sa_search_batch", "suffix_prefetched"} <= qu
+    assert not {"stats_using_ssw_profile", "sa_hsp", "sa_search"} & qu
     assert {"query_find_suffixes", "query_perform_alignments", "query_perform_statistics"} <= qd
     assert "sa_hsp_batch" in su and "suffix_prefetched" in sd
-    assert {"suffix_alloc", "suffix_increment", "suffix_duplicate"} <= sd
+    assert {"suffix_alloc", "suffix_increment", "suffix_duplicate", "suffix_free"} <= sd
     lib = C.CDLL(str(_lib.LIB_PATH))
-    for name in ("align_batch_using_ssw_profile", "stats_batch_using_ssw_profile", "sa_hsp_batch"):
+    for name in ("align_batch_using_ssw_profile", "stats_batch_using_ssw_profile", "sa_hsp_batch",
+                 "sa_search_batch", "db_index", "divsufsort64"):
         assert hasattr(lib, name)
 
 
