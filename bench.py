@@ -477,7 +477,8 @@ def sa_leg(args, t_residues=100_000_000):
     from topaz_amd import _lib, synth
 
     class SaStats(C.Structure):
-        _fields_ = [("rounds", C.c_int32), ("device_bytes", C.c_int64)]
+        _fields_ = [("rounds", C.c_int32), ("device_bytes", C.c_int64), ("alloc_s", C.c_double),
+                    ("sort_s", C.c_double), ("readback_s", C.c_double)]
 
     rng = np.random.default_rng(2030)
     text, _, _, _ = synth.internal_text_and_seeds(rng, t_residues, 10, 10)
@@ -489,13 +490,14 @@ def sa_leg(args, t_residues=100_000_000):
     L.tpz_suffix_array.restype = C.c_int
     sa = np.empty(len(t), np.int64)
     st = SaStats()
-    times = []
+    times, phases = [], []
     for k in range(max(1, args.warmup) + args.steps):
         t0 = time.perf_counter()
         _lib.check(L.tpz_suffix_array(t, len(t), sa.ctypes.data_as(C.POINTER(C.c_int64)),
                                       C.byref(st)), "tpz_suffix_array")
         if k >= max(1, args.warmup):
             times.append(time.perf_counter() - t0)
+            phases.append((st.alloc_s, st.sort_s, st.readback_s))
     # a permutation whose neighbouring suffixes are in order on their first 8 bytes
     assert np.array_equal(np.bincount(sa, minlength=len(t)), np.ones(len(t), np.int64))
     pad = np.frombuffer(t + b"\0" * 8, np.uint8)
@@ -509,7 +511,9 @@ def sa_leg(args, t_residues=100_000_000):
     return {"workload": "suffix array of a %d-byte protein text (lognormal lengths mean 350, "
                         "1%% of proteins repeated exactly), upload and download included" % len(t),
             "text_bytes": len(t), "rounds": st.rounds, "s_per_build": round(dt, 3),
-            "mbytes_per_s": round(len(t) / dt / 1e6, 1), "device_bytes": st.device_bytes}
+            "mbytes_per_s": round(len(t) / dt / 1e6, 1), "device_bytes": st.device_bytes,
+            "phase_s": dict(zip(("alloc_upload", "rounds", "readback"),
+                                (round(float(x), 4) for x in np.mean(phases, axis=0))))}
 
 
 def copy_leg(args):

@@ -234,6 +234,9 @@ void tpz_gapless_batch_destroy(tpz_gapless_batch* b);
 typedef struct {
     int32_t rounds;          /* doubling rounds: sorts by 2, 4, 8, ... bytes */
     int64_t device_bytes;    /* device memory used */
+    double alloc_s;          /* wall seconds: device allocation and text upload, */
+    double sort_s;           /* the doubling rounds, */
+    double readback_s;       /* and the suffix array's copy into sa */
 } tpz_sa_stats;
 int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* stats);
 

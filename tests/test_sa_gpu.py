@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 
 class SaStats(C.Structure):
-    _fields_ = [("rounds", C.c_int32), ("device_bytes", C.c_int64)]
+    _fields_ = [("rounds", C.c_int32), ("device_bytes", C.c_int64), ("alloc_s", C.c_double),
+                ("sort_s", C.c_double), ("readback_s", C.c_double)]
 
 
 def _gpu_sa(text, via_dropin=False):
@@ -52,7 +53,10 @@ def _brute_sa(text):
 
 
 @pytest.mark.parametrize("text", [b"", b"a", b"ab", b"ba", b"aa", b"aaa", b"abab", b"banana",
-                                  b"mississippi", bytes([255, 0, 255, 0, 1]), b"\n\n\n"])
+                                  b"mississippi", bytes([255, 0, 255, 0, 1]), b"\n\n\n",
+                                  # one repeated pair of ten: the later rounds sort the
+                                  # repeated suffixes only
+                                  b"abcdefghab", b"xyzxyzxyzxyzxyzxyzq", b"abracadabra" * 3])
 def test_tiny_texts_brute_force(text):
     sa, _ = _gpu_sa(text)
     assert sa.tolist() == _brute_sa(text).tolist()
@@ -76,7 +80,9 @@ def test_random_and_repetitive_texts_against_reference():
         if len(text) <= 5000:
             assert np.array_equal(sa, _brute_sa(text)), name
     # 'A' * 5000: ranks all distinct only once the prefix length reaches 5000
-    assert _gpu_sa(b"A" * 5000)[1].rounds == 13
+    st = _gpu_sa(b"A" * 5000)[1]
+    assert st.rounds == 13
+    assert min(st.alloc_s, st.sort_s, st.readback_s) >= 0
 
 
 def test_protein_text_against_reference():

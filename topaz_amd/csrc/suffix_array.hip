@@ -12,6 +12,14 @@
 // records in HBM; a round's sort covers only the bits the current ranks
 // need (2 x bit width of the largest rank), 18 bits in the first.  Rounds:
 // about log2 of the longest repeated substring, plus one.
+//
+// Once at most a quarter of the suffixes share their group with another
+// (n - groups <= n / 8), the ranks become slot numbers: a group's rank is
+// 1 + the index of its first suffix in the sorted order, which is where
+// the group's suffixes sit in the final suffix array.  The later rounds
+// then sort only the suffixes of groups of two or more, by (rank[i],
+// rank[i + h]), and split each group inside its own slots.  In a protein
+// database that is the repeated sequences, a few percent of the text.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -19,8 +27,10 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
+#include <chrono>
 
 #include "../../include/topaz_amd.h"
 #include "../../include/topaz_index_compat.h"
@@ -60,6 +70,86 @@ __global__ void sa_rerank(const uint32_t* __restrict__ pos, const uint32_t* __re
         rank[pos[k]] = group[k];
 }
 
+// Slot ranks over the whole sorted array: mark[k] = k where the sorted
+// pair differs from the one before, else 0; a max-scan then gives the
+// first index of each group.
+__global__ void sa_marks_whole(const uint64_t* __restrict__ key, int64_t n,
+                               uint32_t* __restrict__ mark) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        mark[k] = (k == 0 || key[k] != key[k - 1]) ? (uint32_t)k : 0u;
+}
+
+// 1 for a suffix whose sorted pair equals a neighbour's (its group has two
+// or more suffixes)
+__device__ inline uint8_t sa_shared(const uint64_t* key, int64_t k, int64_t m) {
+    const bool first = k == 0 || key[k] != key[k - 1];
+    const bool last = k + 1 == m || key[k + 1] != key[k];
+    return (first && last) ? 0 : 1;
+}
+
+__global__ void sa_place_whole(const uint64_t* __restrict__ key, const uint32_t* __restrict__ pos,
+                               const uint32_t* __restrict__ start, int64_t n,
+                               uint32_t* __restrict__ rank, uint8_t* __restrict__ shared) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        rank[pos[k]] = start[k] + 1;
+        shared[k] = sa_shared(key, k, n);
+    }
+}
+
+// pairs of the unresolved suffixes list[0..m): ranks are slot numbers of
+// at most `bits` bits
+__global__ void sa_list_pairs(const uint32_t* __restrict__ list, int64_t m,
+                              const uint32_t* __restrict__ rank, int64_t n, int64_t h, int bits,
+                              uint64_t* __restrict__ key, uint32_t* __restrict__ pos) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = list[j];
+        const uint64_t second = i + h < n ? rank[i + h] : 0;
+        key[j] = ((uint64_t)rank[i] << bits) | second;
+        pos[j] = (uint32_t)i;
+    }
+}
+
+// Two marks per sorted entry j, packed high | low: j where the group rank
+// (key >> bits) changes, j where the whole pair changes; 0 elsewhere.
+__global__ void sa_marks(const uint64_t* __restrict__ key, int64_t m, int bits,
+                         uint64_t* __restrict__ mark) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const bool group = j == 0 || (key[j] >> bits) != (key[j - 1] >> bits);
+        const bool pair = j == 0 || key[j] != key[j - 1];
+        mark[j] = ((group ? (uint64_t)j : 0) << 32) | (pair ? (uint64_t)j : 0);
+    }
+}
+
+// componentwise max of the two 32-bit halves
+struct HalvesMax {
+    __host__ __device__ uint64_t operator()(uint64_t a, uint64_t b) const {
+        const uint64_t hi = (a >> 32) > (b >> 32) ? (a >> 32) : (b >> 32);
+        const uint64_t lo = (uint32_t)a > (uint32_t)b ? (uint32_t)a : (uint32_t)b;
+        return (hi << 32) | lo;
+    }
+};
+
+// Entry j of a group whose slots begin at g = rank - 1 and whose entries
+// begin at r in the sorted list goes to slot g + j - r; its new rank is
+// 1 + the slot of the first entry with the same pair.
+__global__ void sa_place(const uint64_t* __restrict__ key, const uint32_t* __restrict__ pos,
+                         const uint64_t* __restrict__ start, int64_t m, int bits,
+                         uint32_t* __restrict__ rank, uint32_t* __restrict__ sa,
+                         uint8_t* __restrict__ shared) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t g = (key[j] >> bits) - 1;
+        const uint64_t r = start[j] >> 32, first = (uint32_t)start[j];
+        rank[pos[j]] = (uint32_t)(g + (first - r) + 1);
+        sa[g + (j - r)] = pos[j];
+        shared[j] = sa_shared(key, j, m);
+    }
+}
+
 __global__ void sa_widen(const uint32_t* __restrict__ pos, int64_t n, int64_t* __restrict__ sa) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x)
@@ -73,8 +163,13 @@ int bit_width(uint64_t v) {
 }
 
 struct SaBuffers {
-    DeviceBuffer text, rank, key0, key1, pos0, pos1, head, tmp;
+    DeviceBuffer text, rank, key0, key1, pos0, pos1, head, tmp, count;
 };
+
+double seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
 
 }  // namespace
 }  // namespace tpz
@@ -91,40 +186,60 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
         return fail("tpz_suffix_array: %lld bytes; positions and ranks are 32-bit (at most "
                     "4,294,967,294 bytes)", (long long)n);
     if (ensure_init()) return -1;
+    const double t0 = seconds();
     hipStream_t st = stream();
     SaBuffers b;
     const size_t N = (size_t)n;
     if (b.text.upload(text, N) || b.rank.alloc(N * 4) || b.key0.alloc(N * 8) ||
-        b.key1.alloc(N * 8) || b.pos0.alloc(N * 4) || b.pos1.alloc(N * 4) || b.head.alloc(N * 4))
+        b.key1.alloc(N * 8) || b.pos0.alloc(N * 4) || b.pos1.alloc(N * 4) ||
+        b.head.alloc(N * 4) || b.count.alloc(8))
         return -1;
-    // scratch of the sort and the scan, the larger of the two
-    size_t sort_bytes = 0, scan_bytes = 0;
+    // scratch of the sorts, scans and selection, the largest of them
+    size_t sort_bytes = 0, scan_bytes = 0, max_bytes = 0, pair_bytes = 0, select_bytes = 0;
     {
         rocprim::double_buffer<uint64_t> kb((uint64_t*)b.key0.ptr, (uint64_t*)b.key1.ptr);
         rocprim::double_buffer<uint32_t> vb((uint32_t*)b.pos0.ptr, (uint32_t*)b.pos1.ptr);
+        uint32_t* u = (uint32_t*)b.head.ptr;
+        uint64_t* w = (uint64_t*)b.key1.ptr;
         if (check(rocprim::radix_sort_pairs(nullptr, sort_bytes, kb, vb, N, 0, 64, st),
                   "radix sort size") ||
-            check(rocprim::inclusive_scan(nullptr, scan_bytes, (uint32_t*)b.head.ptr,
-                                          (uint32_t*)b.key1.ptr, N, rocprim::plus<uint32_t>(), st),
-                  "scan size"))
+            check(rocprim::inclusive_scan(nullptr, scan_bytes, u, u, N, rocprim::plus<uint32_t>(),
+                                          st), "scan size") ||
+            check(rocprim::inclusive_scan(nullptr, max_bytes, u, u, N,
+                                          rocprim::maximum<uint32_t>(), st), "scan size") ||
+            check(rocprim::inclusive_scan(nullptr, pair_bytes, w, w, N / 4 + 1, HalvesMax(), st),
+                  "scan size") ||
+            check(rocprim::select(nullptr, select_bytes, u, (const uint8_t*)w, u,
+                                  (uint32_t*)b.count.ptr, N, st), "select size"))
             return -1;
     }
-    if (b.tmp.alloc(std::max(sort_bytes, scan_bytes) + 256)) return -1;
+    const size_t tmp_bytes = std::max({sort_bytes, scan_bytes, max_bytes, pair_bytes, select_bytes});
+    if (b.tmp.alloc(tmp_bytes + 256)) return -1;
     const int block = 256;
-    const int grid = (int)std::min<int64_t>((n + block - 1) / block, (int64_t)num_cu() * 64);
+    auto grid_for = [&](int64_t m) {
+        return (int)std::max<int64_t>(1, std::min<int64_t>((m + block - 1) / block,
+                                                           (int64_t)num_cu() * 64));
+    };
+    const int grid = grid_for(n);
     hipLaunchKernelGGL(sa_first_rank, dim3(grid), dim3(block), 0, st, (const uint8_t*)b.text.ptr,
                        n, (uint32_t*)b.rank.ptr);
-    if (check(hipGetLastError(), "sa_first_rank")) return -1;
+    if (check(hipGetLastError(), "sa_first_rank") || check(hipStreamSynchronize(st), "upload"))
+        return -1;
+    const double t1 = seconds();
     uint64_t max_rank = 256;
     int rounds = 0;
     uint32_t* pos_sorted = nullptr;
-    for (int64_t h = 1;; h *= 2) {
+    bool compact = false;
+    int64_t h = 1;
+    rocprim::double_buffer<uint64_t> kb((uint64_t*)b.key0.ptr, (uint64_t*)b.key1.ptr);
+    rocprim::double_buffer<uint32_t> vb((uint32_t*)b.pos0.ptr, (uint32_t*)b.pos1.ptr);
+    for (;; h *= 2) {
         const int bits = bit_width(max_rank);
-        rocprim::double_buffer<uint64_t> kb((uint64_t*)b.key0.ptr, (uint64_t*)b.key1.ptr);
-        rocprim::double_buffer<uint32_t> vb((uint32_t*)b.pos0.ptr, (uint32_t*)b.pos1.ptr);
+        kb = rocprim::double_buffer<uint64_t>((uint64_t*)b.key0.ptr, (uint64_t*)b.key1.ptr);
+        vb = rocprim::double_buffer<uint32_t>((uint32_t*)b.pos0.ptr, (uint32_t*)b.pos1.ptr);
         hipLaunchKernelGGL(sa_pairs, dim3(grid), dim3(block), 0, st, (const uint32_t*)b.rank.ptr,
                            n, h, bits, kb.current(), vb.current());
-        size_t tb = sort_bytes;
+        size_t tb = tmp_bytes;
         if (check(hipGetLastError(), "sa_pairs") ||
             check(rocprim::radix_sort_pairs(b.tmp.ptr, tb, kb, vb, N, 0, 2 * bits, st),
                   "radix sort"))
@@ -133,17 +248,12 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
                            (const uint64_t*)kb.current(), n, (uint32_t*)b.head.ptr);
         // group numbers (1-based, in sorted order) into the free key buffer
         uint32_t* group = reinterpret_cast<uint32_t*>(kb.alternate());
-        size_t sb = scan_bytes;
+        size_t sb = tmp_bytes;
+        uint32_t groups = 0;
         if (check(hipGetLastError(), "sa_heads") ||
             check(rocprim::inclusive_scan(b.tmp.ptr, sb, (uint32_t*)b.head.ptr, group, N,
                                           rocprim::plus<uint32_t>(), st),
-                  "scan"))
-            return -1;
-        hipLaunchKernelGGL(sa_rerank, dim3(grid), dim3(block), 0, st,
-                           (const uint32_t*)vb.current(), (const uint32_t*)group, n,
-                           (uint32_t*)b.rank.ptr);
-        uint32_t groups = 0;
-        if (check(hipGetLastError(), "sa_rerank") ||
+                  "scan") ||
             check(hipMemcpyAsync(&groups, group + (n - 1), 4, hipMemcpyDeviceToHost, st),
                   "readback") ||
             check(hipStreamSynchronize(st), "suffix array round"))
@@ -151,9 +261,96 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
         ++rounds;
         pos_sorted = vb.current();
         if ((int64_t)groups == n || h >= n) break;
+        if ((n - (int64_t)groups) * 8 <= n) {
+            compact = true;
+            break;
+        }
+        hipLaunchKernelGGL(sa_rerank, dim3(grid), dim3(block), 0, st,
+                           (const uint32_t*)vb.current(), (const uint32_t*)group, n,
+                           (uint32_t*)b.rank.ptr);
+        if (check(hipGetLastError(), "sa_rerank")) return -1;
         max_rank = groups;
     }
-    // positions as the reference's 64-bit entries, into a key buffer
+    if (compact) {
+        // Slot ranks for all suffixes, and the list of those still sharing a
+        // group (at most 2 (n - groups) <= n / 4 of them).  Buffers: the
+        // suffix array in slot order stays in pos_sorted; the sorted keys'
+        // buffer holds the list rounds' keys (16 m bytes), marks and starts
+        // (8 m each); the other key buffer holds the flags (n bytes) and the
+        // list (4 m bytes from byte n rounded up to 16); the other position
+        // buffer and the head buffer hold the list rounds' positions.
+        uint64_t* K = kb.current();
+        uint8_t* shared = reinterpret_cast<uint8_t*>(kb.alternate());
+        uint32_t* list = reinterpret_cast<uint32_t*>(shared + ((N + 15) & ~(size_t)15));
+        uint32_t* mark32 = (uint32_t*)b.head.ptr;
+        uint32_t* start32 = vb.alternate();
+        uint32_t* sa32 = pos_sorted;
+        uint32_t* posA = vb.alternate();
+        uint32_t* posB = (uint32_t*)b.head.ptr;
+        hipLaunchKernelGGL(sa_marks_whole, dim3(grid), dim3(block), 0, st, (const uint64_t*)K, n,
+                           mark32);
+        size_t sb = tmp_bytes;
+        if (check(hipGetLastError(), "sa_marks_whole") ||
+            check(rocprim::inclusive_scan(b.tmp.ptr, sb, mark32, start32, N,
+                                          rocprim::maximum<uint32_t>(), st), "max scan"))
+            return -1;
+        hipLaunchKernelGGL(sa_place_whole, dim3(grid), dim3(block), 0, st, (const uint64_t*)K,
+                           (const uint32_t*)sa32, (const uint32_t*)start32, n,
+                           (uint32_t*)b.rank.ptr, shared);
+        uint32_t m = 0;
+        sb = tmp_bytes;
+        if (check(hipGetLastError(), "sa_place_whole") ||
+            check(rocprim::select(b.tmp.ptr, sb, sa32, (const uint8_t*)shared, list,
+                                  (uint32_t*)b.count.ptr, N, st), "select") ||
+            check(hipMemcpyAsync(&m, b.count.ptr, 4, hipMemcpyDeviceToHost, st), "readback") ||
+            check(hipStreamSynchronize(st), "suffix array round"))
+            return -1;
+        if ((int64_t)m * 4 > n)
+            return fail("tpz_suffix_array: %u unresolved suffixes of %lld", m, (long long)n);
+        const int bits = bit_width((uint64_t)n);
+        while (m > 0) {
+            h *= 2;
+            if (h >= 2 * n) return fail("tpz_suffix_array: rounds did not converge");
+            const int64_t M = m;
+            const int g = grid_for(M);
+            rocprim::double_buffer<uint64_t> lk(K, K + M);
+            rocprim::double_buffer<uint32_t> lv(posA, posB);
+            uint64_t* marks = K + 2 * M;
+            uint64_t* starts = K + 3 * M;
+            hipLaunchKernelGGL(sa_list_pairs, dim3(g), dim3(block), 0, st, (const uint32_t*)list,
+                               M, (const uint32_t*)b.rank.ptr, n, h, bits, lk.current(),
+                               lv.current());
+            size_t tb = tmp_bytes;
+            if (check(hipGetLastError(), "sa_list_pairs") ||
+                check(rocprim::radix_sort_pairs(b.tmp.ptr, tb, lk, lv, (size_t)M, 0, 2 * bits,
+                                                st), "radix sort"))
+                return -1;
+            hipLaunchKernelGGL(sa_marks, dim3(g), dim3(block), 0, st,
+                               (const uint64_t*)lk.current(), M, bits, marks);
+            sb = tmp_bytes;
+            if (check(hipGetLastError(), "sa_marks") ||
+                check(rocprim::inclusive_scan(b.tmp.ptr, sb, marks, starts, (size_t)M,
+                                              HalvesMax(), st), "pair scan"))
+                return -1;
+            hipLaunchKernelGGL(sa_place, dim3(g), dim3(block), 0, st,
+                               (const uint64_t*)lk.current(), (const uint32_t*)lv.current(),
+                               (const uint64_t*)starts, M, bits, (uint32_t*)b.rank.ptr, sa32,
+                               shared);
+            sb = tmp_bytes;
+            if (check(hipGetLastError(), "sa_place") ||
+                check(rocprim::select(b.tmp.ptr, sb, lv.current(), (const uint8_t*)shared, list,
+                                      (uint32_t*)b.count.ptr, (size_t)M, st), "select") ||
+                check(hipMemcpyAsync(&m, b.count.ptr, 4, hipMemcpyDeviceToHost, st),
+                      "readback") ||
+                check(hipStreamSynchronize(st), "suffix array round"))
+                return -1;
+            ++rounds;
+        }
+        pos_sorted = sa32;
+    }
+    const double t2 = seconds();
+    // positions as the reference's 64-bit entries, into a key buffer that
+    // no longer holds anything needed
     int64_t* out = (int64_t*)b.key0.ptr;
     hipLaunchKernelGGL(sa_widen, dim3(grid), dim3(block), 0, st, (const uint32_t*)pos_sorted, n, out);
     if (check(hipGetLastError(), "sa_widen") ||
@@ -162,7 +359,10 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
         return -1;
     if (stats) {
         stats->rounds = rounds;
-        stats->device_bytes = (int64_t)(N * 33 + std::max(sort_bytes, scan_bytes));
+        stats->device_bytes = (int64_t)(N * 33 + 8 + tmp_bytes);
+        stats->alloc_s = t1 - t0;
+        stats->sort_s = t2 - t1;
+        stats->readback_s = seconds() - t2;
     }
     return 0;
 }
