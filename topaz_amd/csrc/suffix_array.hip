@@ -31,6 +31,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
+#include <thread>
 
 #include "../../include/topaz_amd.h"
 #include "../../include/topaz_index_compat.h"
@@ -169,6 +171,84 @@ struct SaBuffers {
 double seconds() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
+}
+
+// Download of a large suffix array: the 32-bit positions cross PCIe into
+// pinned staging buffers (half the bytes of the 64-bit entries), and host
+// threads widen them into the caller's array.  Each thread owns a stream,
+// two 4 MB halves of the staging memory and a contiguous range of the
+// array, so the copy of one half overlaps the widening of the other.  The
+// staging memory, streams and events are made on the first large call and
+// kept until the process ends; calls from several threads take turns.
+constexpr int kReadThreads = 8;
+constexpr int64_t kReadChunk = int64_t(1) << 20;          // entries per half
+constexpr int64_t kReadStaged = int64_t(4) << 20;         // from this many entries on
+
+struct Staging {
+    std::mutex mu;
+    uint32_t* host = nullptr;
+    hipStream_t st[kReadThreads] = {};
+    hipEvent_t ev[kReadThreads][2] = {};
+};
+
+Staging& staging() {
+    static Staging* s = new Staging;   // never destroyed: its memory lives with the process
+    return *s;
+}
+
+int staging_init(Staging& s) {
+    if (s.host) return 0;
+    void* p = nullptr;
+    if (check(hipHostMalloc(&p, kReadThreads * 2 * kReadChunk * 4, hipHostMallocDefault),
+              "hipHostMalloc"))
+        return -1;
+    for (int t = 0; t < kReadThreads; ++t)
+        if (check(hipStreamCreateWithFlags(&s.st[t], hipStreamNonBlocking), "hipStreamCreate") ||
+            check(hipEventCreateWithFlags(&s.ev[t][0], hipEventDisableTiming), "hipEventCreate") ||
+            check(hipEventCreateWithFlags(&s.ev[t][1], hipEventDisableTiming), "hipEventCreate"))
+            return -1;
+    s.host = (uint32_t*)p;
+    return 0;
+}
+
+hipError_t read_range(Staging& s, int t, const uint32_t* dev, int64_t lo, int64_t hi,
+                      int64_t* sa) {
+    uint32_t* half[2] = {s.host + (2 * t) * kReadChunk, s.host + (2 * t + 1) * kReadChunk};
+    const int64_t chunks = (hi - lo + kReadChunk - 1) / kReadChunk;
+    auto issue = [&](int64_t c) {
+        const int64_t a = lo + c * kReadChunk, m = std::min(kReadChunk, hi - a);
+        hipError_t e = hipMemcpyAsync(half[c & 1], dev + a, m * 4, hipMemcpyDeviceToHost, s.st[t]);
+        return e == hipSuccess ? hipEventRecord(s.ev[t][c & 1], s.st[t]) : e;
+    };
+    for (int64_t c = 0; c < std::min<int64_t>(2, chunks); ++c)
+        if (hipError_t e = issue(c)) return e;
+    for (int64_t c = 0; c < chunks; ++c) {
+        if (hipError_t e = hipEventSynchronize(s.ev[t][c & 1])) return e;
+        const int64_t a = lo + c * kReadChunk, m = std::min(kReadChunk, hi - a);
+        const uint32_t* src = half[c & 1];
+        int64_t* dst = sa + a;
+        for (int64_t i = 0; i < m; ++i) dst[i] = src[i];
+        if (c + 2 < chunks)
+            if (hipError_t e = issue(c + 2)) return e;
+    }
+    return hipSuccess;
+}
+
+// sa[0..n) = pos[0..n) widened; pos is complete on the device
+int sa_download(const uint32_t* pos, int64_t n, int64_t* sa) {
+    Staging& s = staging();
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (staging_init(s)) return -1;
+    hipError_t err[kReadThreads] = {};
+    std::thread th[kReadThreads];
+    for (int t = 0; t < kReadThreads; ++t)
+        th[t] = std::thread([&, t] {
+            err[t] = read_range(s, t, pos, n * t / kReadThreads, n * (t + 1) / kReadThreads, sa);
+        });
+    for (auto& x : th) x.join();
+    for (int t = 0; t < kReadThreads; ++t)
+        if (check(err[t], "suffix array download")) return -1;
+    return 0;
 }
 
 }  // namespace
@@ -349,14 +429,19 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
         pos_sorted = sa32;
     }
     const double t2 = seconds();
-    // positions as the reference's 64-bit entries, into a key buffer that
-    // no longer holds anything needed
-    int64_t* out = (int64_t*)b.key0.ptr;
-    hipLaunchKernelGGL(sa_widen, dim3(grid), dim3(block), 0, st, (const uint32_t*)pos_sorted, n, out);
-    if (check(hipGetLastError(), "sa_widen") ||
-        check(hipMemcpyAsync(sa, out, N * 8, hipMemcpyDeviceToHost, st), "readback") ||
-        check(hipStreamSynchronize(st), "suffix array"))
-        return -1;
+    if (n >= kReadStaged) {
+        if (sa_download(pos_sorted, n, sa)) return -1;
+    } else {
+        // positions as the reference's 64-bit entries, into a key buffer
+        // that no longer holds anything needed
+        int64_t* out = (int64_t*)b.key0.ptr;
+        hipLaunchKernelGGL(sa_widen, dim3(grid), dim3(block), 0, st, (const uint32_t*)pos_sorted,
+                           n, out);
+        if (check(hipGetLastError(), "sa_widen") ||
+            check(hipMemcpyAsync(sa, out, N * 8, hipMemcpyDeviceToHost, st), "readback") ||
+            check(hipStreamSynchronize(st), "suffix array"))
+            return -1;
+    }
     if (stats) {
         stats->rounds = rounds;
         stats->device_bytes = (int64_t)(N * 33 + 8 + tmp_bytes);
