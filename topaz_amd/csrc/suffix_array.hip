@@ -220,16 +220,22 @@ hipError_t read_range(Staging& s, int t, const uint32_t* dev, int64_t lo, int64_
         hipError_t e = hipMemcpyAsync(half[c & 1], dev + a, m * 4, hipMemcpyDeviceToHost, s.st[t]);
         return e == hipSuccess ? hipEventRecord(s.ev[t][c & 1], s.st[t]) : e;
     };
+    // on an error, let the copies already issued land before the staging
+    // memory is used again
+    auto drain = [&](hipError_t e) {
+        (void)hipStreamSynchronize(s.st[t]);
+        return e;
+    };
     for (int64_t c = 0; c < std::min<int64_t>(2, chunks); ++c)
-        if (hipError_t e = issue(c)) return e;
+        if (hipError_t e = issue(c)) return drain(e);
     for (int64_t c = 0; c < chunks; ++c) {
-        if (hipError_t e = hipEventSynchronize(s.ev[t][c & 1])) return e;
+        if (hipError_t e = hipEventSynchronize(s.ev[t][c & 1])) return drain(e);
         const int64_t a = lo + c * kReadChunk, m = std::min(kReadChunk, hi - a);
         const uint32_t* src = half[c & 1];
         int64_t* dst = sa + a;
         for (int64_t i = 0; i < m; ++i) dst[i] = src[i];
         if (c + 2 < chunks)
-            if (hipError_t e = issue(c + 2)) return e;
+            if (hipError_t e = issue(c + 2)) return drain(e);
     }
     return hipSuccess;
 }
