@@ -5,12 +5,13 @@
 // gives the reference's .SA byte for byte.
 //
 // Prefix doubling: rank[i] orders the suffixes by their first h bytes
-// (the byte + 1 to begin with, 0 past the end); each round sorts the
-// positions by the pair (rank[i], rank[i + h]) with a device radix sort,
-// numbers the distinct pairs in sorted order with a scan, and doubles h,
-// until every rank is distinct.  Each round is a few passes over 16-byte
-// records in HBM; a round's sort covers only the bits the current ranks
-// need (2 x bit width of the largest rank), 18 bits in the first.  Rounds:
+// (bytes count as byte + 1, 0 past the end).  The first round ranks the
+// two-byte prefixes from a bitmap of the byte pairs present; each later
+// round sorts the positions by the pair (rank[i], rank[i + h]) with a
+// device radix sort, numbers the distinct pairs in sorted order with a
+// scan, and doubles h, until every rank is distinct.  Each round is a few
+// passes over 16-byte records in HBM; a round's sort covers only the bits
+// the current ranks need (2 x bit width of the largest rank).  Rounds:
 // about log2 of the longest repeated substring, plus one.
 //
 // Once at most a quarter of the suffixes share their group with another
@@ -41,10 +42,51 @@
 namespace tpz {
 namespace {
 
-__global__ void sa_first_rank(const uint8_t* __restrict__ T, int64_t n, uint32_t* __restrict__ rank) {
+// The first round needs no sort: the rank of a suffix by its first two
+// bytes is the number of distinct byte pairs (T[i] + 1, T[i + 1] + 1, 0
+// past the end) below its own, plus one.  A bitmap of the 257 x 257 pair
+// codes marks the pairs present; word prefix counts of the bitmap then
+// number them.
+constexpr int kPairCodes = 257 * 257;
+constexpr int kPairWords = (kPairCodes + 31) / 32;
+
+__device__ inline uint32_t sa_pair_code(const uint8_t* T, int64_t n, int64_t i) {
+    return ((uint32_t)T[i] + 1) * 257 + (i + 1 < n ? (uint32_t)T[i + 1] + 1 : 0);
+}
+
+__global__ void sa_pair_bits(const uint8_t* __restrict__ T, int64_t n, uint32_t* __restrict__ bits) {
+    __shared__ uint32_t b[kPairWords];
+    for (int w = threadIdx.x; w < kPairWords; w += blockDim.x) b[w] = 0;
+    __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        rank[i] = (uint32_t)T[i] + 1;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = sa_pair_code(T, n, i);
+        atomicOr(&b[c >> 5], 1u << (c & 31));
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < kPairWords; w += blockDim.x)
+        if (b[w]) atomicOr(&bits[w], b[w]);
+}
+
+// prefix[w] = pairs present in words 0..w-1; prefix[kPairWords] = all
+__global__ void sa_pair_count(const uint32_t* __restrict__ bits, uint32_t* __restrict__ prefix) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t sum = 0;
+    for (int w = 0; w < kPairWords; ++w) {
+        prefix[w] = sum;
+        sum += __popc(bits[w]);
+    }
+    prefix[kPairWords] = sum;
+}
+
+__global__ void sa_pair_rank(const uint8_t* __restrict__ T, int64_t n,
+                             const uint32_t* __restrict__ bits, const uint32_t* __restrict__ prefix,
+                             uint32_t* __restrict__ rank) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = sa_pair_code(T, n, i);
+        rank[i] = prefix[c >> 5] + __popc(bits[c >> 5] & ((1u << (c & 31)) - 1)) + 1;
+    }
 }
 
 // the pair (rank[i], rank[i + h]) packed in 2 * bits bits, and position i
@@ -165,7 +207,7 @@ int bit_width(uint64_t v) {
 }
 
 struct SaBuffers {
-    DeviceBuffer text, rank, key0, key1, pos0, pos1, head, tmp, count;
+    DeviceBuffer text, rank, key0, key1, pos0, pos1, head, tmp, count, pairs;
 };
 
 double seconds() {
@@ -278,7 +320,7 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
     const size_t N = (size_t)n;
     if (b.text.upload(text, N) || b.rank.alloc(N * 4) || b.key0.alloc(N * 8) ||
         b.key1.alloc(N * 8) || b.pos0.alloc(N * 4) || b.pos1.alloc(N * 4) ||
-        b.head.alloc(N * 4) || b.count.alloc(8))
+        b.head.alloc(N * 4) || b.count.alloc(8) || b.pairs.alloc((2 * kPairWords + 1) * 4))
         return -1;
     // scratch of the sorts, scans and selection, the largest of them
     size_t sort_bytes = 0, scan_bytes = 0, max_bytes = 0, pair_bytes = 0, select_bytes = 0;
@@ -307,16 +349,31 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
                                                            (int64_t)num_cu() * 64));
     };
     const int grid = grid_for(n);
-    hipLaunchKernelGGL(sa_first_rank, dim3(grid), dim3(block), 0, st, (const uint8_t*)b.text.ptr,
-                       n, (uint32_t*)b.rank.ptr);
-    if (check(hipGetLastError(), "sa_first_rank") || check(hipStreamSynchronize(st), "upload"))
-        return -1;
+    if (check(hipStreamSynchronize(st), "upload")) return -1;
     const double t1 = seconds();
-    uint64_t max_rank = 256;
-    int rounds = 0;
+    // round 1: ranks by the first two bytes
+    uint32_t* pair_bits = (uint32_t*)b.pairs.ptr;
+    uint32_t* pair_prefix = pair_bits + kPairWords;
+    uint32_t pairs = 0;
+    if (check(hipMemsetAsync(pair_bits, 0, kPairWords * 4, st), "clear"))
+        return -1;
+    hipLaunchKernelGGL(sa_pair_bits, dim3(std::min(grid, num_cu() * 4)), dim3(block), 0, st,
+                       (const uint8_t*)b.text.ptr, n, pair_bits);
+    hipLaunchKernelGGL(sa_pair_count, dim3(1), dim3(64), 0, st, (const uint32_t*)pair_bits,
+                       pair_prefix);
+    hipLaunchKernelGGL(sa_pair_rank, dim3(grid), dim3(block), 0, st, (const uint8_t*)b.text.ptr,
+                       n, (const uint32_t*)pair_bits, (const uint32_t*)pair_prefix,
+                       (uint32_t*)b.rank.ptr);
+    if (check(hipGetLastError(), "pair ranks") ||
+        check(hipMemcpyAsync(&pairs, pair_prefix + kPairWords, 4, hipMemcpyDeviceToHost, st),
+              "readback") ||
+        check(hipStreamSynchronize(st), "pair ranks"))
+        return -1;
+    uint64_t max_rank = pairs;
+    int rounds = 1;
     uint32_t* pos_sorted = nullptr;
     bool compact = false;
-    int64_t h = 1;
+    int64_t h = 2;
     rocprim::double_buffer<uint64_t> kb((uint64_t*)b.key0.ptr, (uint64_t*)b.key1.ptr);
     rocprim::double_buffer<uint32_t> vb((uint32_t*)b.pos0.ptr, (uint32_t*)b.pos1.ptr);
     for (;; h *= 2) {
