@@ -232,7 +232,7 @@ void tpz_gapless_batch_destroy(tpz_gapless_batch* b);
  * Built on the device by prefix doubling; n may be up to 4,294,967,294.
  * Device memory: about 33 n bytes.  stats may be null. */
 typedef struct {
-    int32_t rounds;          /* doubling rounds: sorts by 2, 4, 8, ... bytes */
+    int32_t rounds;          /* doubling rounds: ranks by 2, 4, 8, ... bytes */
     int64_t device_bytes;    /* device memory used */
     double alloc_s;          /* wall seconds: device allocation and text upload, */
     double sort_s;           /* the doubling rounds, */
