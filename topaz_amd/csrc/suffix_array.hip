@@ -68,15 +68,53 @@ __global__ void sa_pair_bits(const uint8_t* __restrict__ T, int64_t n, uint32_t*
         if (b[w]) atomicOr(&bits[w], b[w]);
 }
 
-// prefix[w] = pairs present in words 0..w-1; prefix[kPairWords] = all
-__global__ void sa_pair_count(const uint32_t* __restrict__ bits, uint32_t* __restrict__ prefix) {
+// prefix[w] = codes present in words 0..w-1; prefix[words] = all
+__global__ void sa_bit_count(const uint32_t* __restrict__ bits, int words,
+                             uint32_t* __restrict__ prefix) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     uint32_t sum = 0;
-    for (int w = 0; w < kPairWords; ++w) {
+    for (int w = 0; w < words; ++w) {
         prefix[w] = sum;
         sum += __popc(bits[w]);
     }
-    prefix[kPairWords] = sum;
+    prefix[words] = sum;
+}
+
+// Round 2 the same way when the text has few distinct byte pairs (p <=
+// 511: protein and DNA text): the rank by four bytes is 1 + the number of
+// distinct codes (rank[i] - 1) x (p + 1) + rank[i + 2] (0 past the end)
+// below the suffix's own.  The bitmap of the p (p + 1) codes fits in
+// 32 KB of LDS.
+constexpr int kQuadWords = 8192;
+constexpr uint32_t kQuadMaxPairs = 511;   // 511 x 512 codes <= 32 x kQuadWords
+
+__device__ inline uint32_t sa_quad_code(const uint32_t* rank, int64_t n, int64_t i, uint32_t p) {
+    return (rank[i] - 1) * (p + 1) + (i + 2 < n ? rank[i + 2] : 0);
+}
+
+__global__ void sa_quad_bits(const uint32_t* __restrict__ rank, int64_t n, uint32_t p,
+                             uint32_t* __restrict__ bits) {
+    __shared__ uint32_t b[kQuadWords];
+    for (int w = threadIdx.x; w < kQuadWords; w += blockDim.x) b[w] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = sa_quad_code(rank, n, i, p);
+        atomicOr(&b[c >> 5], 1u << (c & 31));
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < kQuadWords; w += blockDim.x)
+        if (b[w]) atomicOr(&bits[w], b[w]);
+}
+
+__global__ void sa_quad_rank(const uint32_t* __restrict__ rank, int64_t n, uint32_t p,
+                             const uint32_t* __restrict__ bits, const uint32_t* __restrict__ prefix,
+                             uint32_t* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = sa_quad_code(rank, n, i, p);
+        out[i] = prefix[c >> 5] + __popc(bits[c >> 5] & ((1u << (c & 31)) - 1)) + 1;
+    }
 }
 
 __global__ void sa_pair_rank(const uint8_t* __restrict__ T, int64_t n,
@@ -320,7 +358,7 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
     const size_t N = (size_t)n;
     if (b.text.upload(text, N) || b.rank.alloc(N * 4) || b.key0.alloc(N * 8) ||
         b.key1.alloc(N * 8) || b.pos0.alloc(N * 4) || b.pos1.alloc(N * 4) ||
-        b.head.alloc(N * 4) || b.count.alloc(8) || b.pairs.alloc((2 * kPairWords + 1) * 4))
+        b.head.alloc(N * 4) || b.count.alloc(8) || b.pairs.alloc((2 * kQuadWords + 1) * 4))
         return -1;
     // scratch of the sorts, scans and selection, the largest of them
     size_t sort_bytes = 0, scan_bytes = 0, max_bytes = 0, pair_bytes = 0, select_bytes = 0;
@@ -359,8 +397,8 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
         return -1;
     hipLaunchKernelGGL(sa_pair_bits, dim3(std::min(grid, num_cu() * 4)), dim3(block), 0, st,
                        (const uint8_t*)b.text.ptr, n, pair_bits);
-    hipLaunchKernelGGL(sa_pair_count, dim3(1), dim3(64), 0, st, (const uint32_t*)pair_bits,
-                       pair_prefix);
+    hipLaunchKernelGGL(sa_bit_count, dim3(1), dim3(64), 0, st, (const uint32_t*)pair_bits,
+                       kPairWords, pair_prefix);
     hipLaunchKernelGGL(sa_pair_rank, dim3(grid), dim3(block), 0, st, (const uint8_t*)b.text.ptr,
                        n, (const uint32_t*)pair_bits, (const uint32_t*)pair_prefix,
                        (uint32_t*)b.rank.ptr);
@@ -371,9 +409,34 @@ int tpz_suffix_array(const uint8_t* text, int64_t n, int64_t* sa, tpz_sa_stats* 
         return -1;
     uint64_t max_rank = pairs;
     int rounds = 1;
+    int64_t h = 2;
+    if (pairs <= kQuadMaxPairs && (int64_t)pairs < n) {
+        // round 2: ranks by four bytes, into the head buffer, which then
+        // becomes the rank buffer
+        uint32_t* quad_bits = (uint32_t*)b.pairs.ptr;
+        uint32_t* quad_prefix = quad_bits + kQuadWords;
+        uint32_t* out = (uint32_t*)b.head.ptr;
+        uint32_t quads = 0;
+        if (check(hipMemsetAsync(quad_bits, 0, kQuadWords * 4, st), "clear")) return -1;
+        hipLaunchKernelGGL(sa_quad_bits, dim3(std::min(grid, num_cu() * 2)), dim3(block), 0, st,
+                           (const uint32_t*)b.rank.ptr, n, pairs, quad_bits);
+        hipLaunchKernelGGL(sa_bit_count, dim3(1), dim3(64), 0, st, (const uint32_t*)quad_bits,
+                           kQuadWords, quad_prefix);
+        hipLaunchKernelGGL(sa_quad_rank, dim3(grid), dim3(block), 0, st,
+                           (const uint32_t*)b.rank.ptr, n, pairs, (const uint32_t*)quad_bits,
+                           (const uint32_t*)quad_prefix, out);
+        if (check(hipGetLastError(), "four-byte ranks") ||
+            check(hipMemcpyAsync(&quads, quad_prefix + kQuadWords, 4, hipMemcpyDeviceToHost,
+                                 st), "readback") ||
+            check(hipStreamSynchronize(st), "four-byte ranks"))
+            return -1;
+        std::swap(b.rank.ptr, b.head.ptr);
+        max_rank = quads;
+        rounds = 2;
+        h = 4;
+    }
     uint32_t* pos_sorted = nullptr;
     bool compact = false;
-    int64_t h = 2;
     rocprim::double_buffer<uint64_t> kb((uint64_t*)b.key0.ptr, (uint64_t*)b.key1.ptr);
     rocprim::double_buffer<uint32_t> vb((uint32_t*)b.pos0.ptr, (uint32_t*)b.pos1.ptr);
     for (;; h *= 2) {
