@@ -6,10 +6,12 @@
 //
 // Prefix doubling: rank[i] orders the suffixes by their first h bytes
 // (bytes count as byte + 1, 0 past the end).  The first round ranks the
-// two-byte prefixes from a bitmap of the byte pairs present; each later
-// round sorts the positions by the pair (rank[i], rank[i + h]) with a
-// device radix sort, numbers the distinct pairs in sorted order with a
-// scan, and doubles h, until every rank is distinct.  Each round is a few
+// two-byte prefixes from a bitmap of the byte pairs present, and in a text
+// of few distinct byte pairs (protein, DNA) the second round ranks the
+// four-byte prefixes the same way.  Each later round sorts the positions
+// by the pair (rank[i], rank[i + h]) with a device radix sort, numbers the
+// distinct pairs in sorted order with a scan, and doubles h, until every
+// rank is distinct.  Each round is a few
 // passes over 16-byte records in HBM; a round's sort covers only the bits
 // the current ranks need (2 x bit width of the largest rank).  Rounds:
 // about log2 of the longest repeated substring, plus one.
