@@ -98,3 +98,30 @@ def test_protein_text_against_reference():
     sa, st = _gpu_sa(t)
     assert st.rounds >= 10
     assert np.array_equal(sa, _ref_sa(t))
+
+
+def _distinct_pairs(t):
+    b = np.frombuffer(t, np.uint8).astype(np.int64)
+    nxt = np.r_[b[1:] + 1, 0]
+    return len(np.unique((b + 1) * 257 + nxt))
+
+
+@pytest.mark.parametrize("target", [511, 512])
+def test_byte_pair_count_at_the_bitmap_limit(target):
+    """Round 2 reads four-byte ranks off a bitmap when the text has at most
+    511 distinct byte pairs (the last byte with the end counted), and sorts
+    otherwise.  Texts with exactly 511 and 512 pairs: 22 letters in random
+    order (485 pairs), then one fresh byte value per missing pair (each
+    adds the pairs (last, fresh) and (fresh, end) and drops (last, end))."""
+    rng = np.random.default_rng(53 + target)
+    letters = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWYBZ", np.uint8)
+    t = bytes(letters[rng.integers(0, len(letters), 200_000)])
+    assert _distinct_pairs(t) == len(letters) ** 2 + 1
+    fresh = iter(range(128, 256))
+    while _distinct_pairs(t) < target:
+        t += bytes([next(fresh)])
+    assert _distinct_pairs(t) == target
+    t = t[:100_000] + t[:5_000] + t[100_000:]       # a 5,000-byte repeat
+    assert _distinct_pairs(t) == target
+    sa, st = _gpu_sa(t)
+    assert np.array_equal(sa, _ref_sa(t))
