@@ -134,3 +134,31 @@ def test_db_index_equals_reference():
             a, b = C.c_int64(-5), C.c_int64(-5)
             ra, rb = L.db_index(C.byref(db), pos, C.byref(a)), R.db_index(C.byref(db), pos, C.byref(b))
             assert (ra, a.value) == (rb, b.value), (tiled, pos)
+
+
+def test_suffix_array_argument_checks_match_the_reference():
+    """divsufsort64's argument checks (divsufsort.c:338-339) come before any
+    device work: NULL text or array or n < 0 give -1, n = 0 gives 0.  The
+    library returns the same codes as the reference's own compiled
+    divsufsort64, and tpz_suffix_array rejects texts too long for 32-bit
+    positions with a message, without a GPU."""
+    from tests.refcheck import REF_SO
+    L = _lib.lib()
+    L.divsufsort64.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.c_int64]
+    L.divsufsort64.restype = C.c_int32
+    L.tpz_suffix_array.argtypes = [C.c_char_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]
+    L.tpz_suffix_array.restype = C.c_int
+    sa = (C.c_int64 * 4)()
+    cases = [(None, sa, 3), (b"abc", None, 3), (b"abc", sa, -1), (b"abc", sa, 0), (None, None, 0)]
+    ours = [L.divsufsort64(t, a, n) for t, a, n in cases]
+    assert ours == [-1, -1, -1, 0, -1]
+    if REF_SO.exists():
+        R = C.CDLL(str(REF_SO))
+        R.divsufsort64.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.c_int64]
+        R.divsufsort64.restype = C.c_int32
+        assert [R.divsufsort64(t, a, n) for t, a, n in cases] == ours
+    assert L.tpz_suffix_array(None, 3, sa, None) == -1
+    assert b"bad arguments" in L.tpz_last_error()
+    assert L.tpz_suffix_array(b"abc", 1 << 32, sa, None) == -1
+    assert b"32-bit" in L.tpz_last_error()
+    assert L.tpz_suffix_array(b"", 0, sa, None) == 0
