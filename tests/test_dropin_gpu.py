@@ -112,6 +112,43 @@ def test_batch_loops_hit_lists_identical(workspace, flags):
     assert _per_query(ours) == _per_query(ref)
 
 
+@pytest.fixture(scope="module")
+def full_config(tmp_path_factory):
+    """The small end-to-end configuration at its full size: 100,000 database
+    proteins (lognormal mean 350, sd 250, clipped [30, 2000]) and 1,000
+    queries, 20 % of them with 1-5 planted homologs at 30-90 % identity."""
+    if not (REF / "topaz").exists():
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    if not all((REF / "batch" / f).exists() for f in ("query_batch.o", "suffix_batch.o")):
+        pytest.skip("patched query.o not built (make -C oracle ref)")
+    tmp = tmp_path_factory.mktemp("config1")
+    rng = np.random.default_rng(2027)
+    db, queries = synth.search_database(rng, 100000, 1000, hom_frac=0.2, max_hom=5,
+                                        ident=(0.3, 0.9), hi=2000)
+    synth.write_fasta(tmp / "db.fa", db, "s")
+    synth.write_fasta(tmp / "q.fa", queries, "q")
+    subprocess.run([str(REF / "topaz"), "index", "-f", str(tmp / "db.fa"), "-p", str(tmp / "DB")],
+                   check=True, capture_output=True, timeout=600)
+    return tmp, _link_dropin(tmp, batch=True)
+
+
+@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"]])
+def test_batch_loops_full_size_config(full_config, flags):
+    """The batch build against the stock binary on the whole 100,000 x 1,000
+    configuration (-H 100 -E 1e-3), default, -B and -F; both with 16 worker
+    threads, the GPU build's calls spread over 8 hardware queues."""
+    tmp, batch = full_config
+    common = ["-H", "100", "-E", "0.001", "-T", "16"] + list(flags)
+    ref = _search(REF / "topaz", tmp, *common)
+    env = dict(os.environ, GPU_MAX_HW_QUEUES="8")
+    r = subprocess.run([str(batch), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB")] +
+                       common, capture_output=True, text=True, timeout=600, cwd=str(tmp), env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    head, blocks = _per_query(ref)
+    assert len(blocks) > 150 and sum(len(v) for v in blocks.values()) > 400
+    assert _per_query(r.stdout) == (head, blocks)
+
+
 def _link_index(tmp):
     """TOPAZ's objects minus divsufsort.o, sssort.o and trsort.o (the
     vendored libdivsufsort), linked against the library's divsufsort64."""

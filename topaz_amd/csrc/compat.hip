@@ -817,6 +817,7 @@ void gapless_seeds(const db_t* db, const seq_t* q, int n, const int* qoffsets,
     a.qf_off = (const int64_t*)(gd + c.g_meta);
     a.qr_off = (const int64_t*)(gd + c.g_meta + 8);
     a.q_len = (const int32_t*)(gd + c.g_meta + 16);
+    a.n_queries = 1;
     a.seeds = seeds;
     a.n_seeds = n;
     a.table = (const int16_t*)(gd + c.g_tab);

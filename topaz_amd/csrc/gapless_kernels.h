@@ -44,10 +44,11 @@ struct GaplessArgs {
     const uint8_t* q_rev;       // per query: doubled codes of q[m-1..1], then two terminators
     const int64_t* qr_off;
     const int32_t* q_len;
+    int32_t n_queries;          // seeds naming a query outside [0, n_queries) get length -1
     const SeedRec* seeds;
     int64_t n_seeds;
     const int16_t* table;       // kTableRows x kTableCols, W(text code, query code)
-    int32_t* out_score;
+    int32_t* out_score;         // 0 where the length is -1
     int32_t* out_length;
 };
 

@@ -140,12 +140,18 @@ gapless_kernel(GaplessArgs a) {
             const SeedRec s = a.seeds[sid];
             qid = s.query;
             o = s.offset;
-            idx = a.sa ? a.sa[s.position] : s.position;
-            // positions outside the text, or query offsets outside the
-            // query, are not seeds (the reference reads out of bounds there)
-            if (idx >= 0 && idx < a.t_size && o >= 0 && o < a.q_len[qid])
+            // slots or positions outside the text, and queries or query
+            // offsets that do not exist, are not seeds (the reference reads
+            // out of bounds there)
+            const bool in_sa = s.position >= 0 && s.position < a.t_size;
+            idx = a.sa ? (in_sa ? a.sa[s.position] : -1) : s.position;
+            if (idx >= 0 && idx < a.t_size && qid >= 0 && qid < a.n_queries && o >= 0 &&
+                o < a.q_len[qid])
                 pass = 2 * (int)a.t_fwd[idx] == (int)a.q_fwd[a.qf_off[qid] + o];
-            if (!pass) a.out_length[sid] = -1;
+            if (!pass) {
+                a.out_length[sid] = -1;
+                a.out_score[sid] = 0;
+            }
         }
         const uint64_t mask = __ballot(pass);
         const int n_pass = __popcll(mask);

@@ -98,6 +98,9 @@ int gapless_query_codes(const TextCodes& tc, int32_t n_queries, const char* q_ch
                         const int64_t* q_off, const int32_t* q_len, QueryCodes& out);
 // W(text code, query code) for the kernel, kTableRows x kTableCols int16.
 void gapless_table(const TextCodes& tc, const int8_t* matrix, int16_t* table);
+// Matrix rows of text bytes and of query bytes (lexicographical.h:19-74),
+// 128 entries each; bytes >= 128 use their low seven bits.
+void alphabet_rows(int8_t* text_rows, int8_t* query_rows);
 
 // ---- traceback statistics (shared by tpz_stats_batch and the drop-in ABI) ----
 // Blocks handed out in order and reused after reset(); never freed here.

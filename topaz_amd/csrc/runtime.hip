@@ -486,6 +486,9 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
     a.pair_index = (const int32_t*)dt.d_index.ptr;
     a.task_begin = (const int32_t*)dt.d_begin.ptr;
     a.task_query = (const int32_t*)dt.d_query.ptr;
+    // the timeout flag of the staircase waits is read after every run: clear
+    // it with a kernel too (a graph replays it)
+    if (check(tpz_launch_fill_i32(a.error, 0, 1, stream()), "clear error flag")) return -1;
     if (schedule == TPZ_SCHED_PERSISTENT) {
         const int n_items = (int)(pl.items_persistent.size() / W);
         if (n_items > 0) {
@@ -678,6 +681,10 @@ static const AlphabetMaps& maps() {
     return m;
 }
 static int text_row(uint8_t c) { return maps().tmap[c & 127]; }
+void alphabet_rows(int8_t* text_rows, int8_t* query_rows) {
+    memcpy(text_rows, maps().tmap, 128);
+    memcpy(query_rows, maps().qmap, 128);
+}
 static int query_row(uint8_t c) { return maps().qmap[c & 127]; }
 
 // Codes by letter frequency: the gapless kernel's table lookups hit LDS bank
@@ -788,6 +795,7 @@ void tpz_text_destroy(tpz_text* t) { delete t; }
 
 struct tpz_gapless_batch {
     const tpz_text* text = nullptr;
+    int32_t n_queries = 0;
     int64_t n_seeds = 0;
     DeviceBuffer d_qf, d_qf_off, d_qr, d_qr_off, d_q_len, d_seeds, d_table, d_score, d_len;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -817,6 +825,7 @@ int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
     gapless_table(text->codes, matrix, table.data());
     auto b = new tpz_gapless_batch();
     b->text = text;
+    b->n_queries = n_queries;
     b->n_seeds = n_seeds;
     int rc = 0;
     rc |= b->d_qf.upload(qc.fwd.data(), qc.fwd.size());
@@ -828,7 +837,6 @@ int tpz_gapless_batch_create(tpz_gapless_batch** out, const tpz_text* text,
     rc |= b->d_table.upload(table.data(), table.size() * 2);
     rc |= b->d_score.alloc((size_t)n_seeds * 4 + 4);
     rc |= b->d_len.alloc((size_t)n_seeds * 4 + 4);
-    if (!rc) rc |= check(hipMemsetAsync(b->d_score.ptr, 0, b->d_score.bytes, stream()), "memset");
     for (auto& e : b->ev) rc |= check(hipEventCreate(&e), "hipEventCreate");
     rc |= check(hipStreamSynchronize(stream()), "gapless upload");
     if (rc) { tpz_gapless_batch_destroy(b); return -1; }
@@ -848,6 +856,7 @@ int tpz_gapless_batch_run(tpz_gapless_batch* b) {
     a.q_rev = (const uint8_t*)b->d_qr.ptr;
     a.qr_off = (const int64_t*)b->d_qr_off.ptr;
     a.q_len = (const int32_t*)b->d_q_len.ptr;
+    a.n_queries = b->n_queries;
     a.seeds = (const SeedRec*)b->d_seeds.ptr;
     a.n_seeds = b->n_seeds;
     a.table = (const int16_t*)b->d_table.ptr;
@@ -1032,16 +1041,6 @@ int stats_compute(const StatsInput& in, RunMem& m, StatsOutput& out) {
     for (int32_t h = 0; h < n; ++h)
         if (rb[h] < 0 || qb[h] < 0 || qb[h] > in.read_end1[h])
             return fail("hit %d: reverse pass found no start (score1 %d not reached)", h, in.score1[h]);
-    // band rows: the band half-width stays below refLen + readLen before the
-    // search stops, so 4 (refLen + readLen) + 8 words per row suffice
-    std::vector<int64_t> bo(n);
-    int64_t bw = 0;
-    int32_t band_words = 8;
-    for (int32_t h = 0; h < n; ++h) {
-        const int32_t sum = (in.ref_end1[h] - rb[h] + 1) + (in.read_end1[h] - qb[h] + 1);
-        band_words = std::max(band_words, 4 * sum + 8);
-    }
-    for (int32_t h = 0; h < n; ++h) { bo[h] = bw; bw += 3 * (int64_t)band_words; }
     BandArgs a{};
     a.q_res = r.q_res; a.q_off = r.q_off; a.s_res = r.s_res; a.s_off = r.s_off;
     a.pair_query = r.pair_query; a.pair_subject = r.pair_subject;
@@ -1053,7 +1052,6 @@ int stats_compute(const StatsInput& in, RunMem& m, StatsOutput& out) {
     a.n_pairs = n;
     a.matrix = r.matrix;
     a.gap_open = in.gap_open; a.gap_extend = in.gap_extend;
-    a.ws_band_words = band_words;
     a.out_band = (int32_t*)m.dev((size_t)n * 4 + 4);
     if (!a.out_band) return -1;
     // band search with a wavefront per hit; a band wider than the LDS rows
@@ -1067,6 +1065,15 @@ int stats_compute(const StatsInput& in, RunMem& m, StatsOutput& out) {
     std::vector<int32_t> wide;
     for (int32_t h = 0; h < n; ++h)
         if (band[h] == -2) wide.push_back(h);
+    // workspace band rows for those hits only, each sized by its own
+    // rectangle (the trace's wide hits are among them: their final band is
+    // at least the one that did not fit the LDS rows)
+    std::vector<int64_t> bo(n, 0);
+    int64_t bw = 0;
+    for (int32_t h : wide) {
+        bo[h] = bw;
+        bw += 3 * (int64_t)band_row_words(in.ref_end1[h] - rb[h] + 1, in.read_end1[h] - qb[h] + 1);
+    }
     if (!wide.empty()) {
         a.ws_off = put(m, bo.data(), (size_t)n);
         a.workspace = (int32_t*)m.dev((size_t)bw * 4 + 16);

@@ -48,9 +48,9 @@ struct BandArgs {
     int32_t n_pairs;
     const int8_t* matrix;
     int32_t gap_open, gap_extend;
-    int32_t* workspace;            // three band rows (H, E, H of the row) per hit
+    int32_t* workspace;            // three band rows (H, E, H of the row) of
+                                   // band_row_words() words each, per listed hit
     const int64_t* ws_off;
-    int32_t ws_band_words;         // words per band row available per hit
     int32_t* out_band;             // final band half-width, -1: size guard
     int8_t* dirs;                  // direction buffers, zeroed
     const int64_t* dir_off;
@@ -59,6 +59,13 @@ struct BandArgs {
     const int32_t* hits;           // hit numbers the launch covers (n_list of them); null: all
     int32_t n_list;
 };
+
+// Words of one band row in the workspace of a hit whose rectangle is
+// ref_len x read_len.  The band half-width stays below ref_len + read_len
+// until the search stops, and a row needs 2w + 3 slots.
+__host__ __device__ inline int32_t band_row_words(int32_t ref_len, int32_t read_len) {
+    return 4 * (ref_len + read_len) + 8;
+}
 
 // Band rows the wavefront kernels hold in LDS: a band wider than this goes
 // to the one-thread-per-hit kernels (band status -2 from the search).

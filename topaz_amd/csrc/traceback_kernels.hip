@@ -346,11 +346,12 @@ __device__ __forceinline__ BandView band_view(const BandArgs& a, int p) {
     v.ref = a.s_res + a.s_off[a.pair_subject[p]] + rb;
     v.read = a.q_res + a.q_off[a.pair_query[p]] + qb;
     v.w0 = v.refLen > v.readLen ? v.refLen - v.readLen + 1 : v.readLen - v.refLen + 1;
+    const int32_t words = band_row_words(v.refLen, v.readLen);
     v.hb = a.workspace + (size_t)a.ws_off[p];
-    v.eb = v.hb + a.ws_band_words;
-    v.hc = v.eb + a.ws_band_words;
+    v.eb = v.hb + words;
+    v.hc = v.eb + words;
     // the reference's band rows start as fresh allocations: zero here
-    for (int j = 0; j < a.ws_band_words; ++j) { v.hb[j] = 0; v.eb[j] = 0; v.hc[j] = 0; }
+    for (int j = 0; j < words; ++j) { v.hb[j] = 0; v.eb[j] = 0; v.hc[j] = 0; }
     return v;
 }
 
@@ -364,7 +365,7 @@ banded_search_kernel(BandArgs a) {
     for (;;) {
         const int64_t cells = (int64_t)(2 * w + 1) * v.readLen * 3;
         // the reference's direction buffer stops growing at 2^31 bytes
-        if (cells >= ((int64_t)1 << 30) || 2 * w + 4 > a.ws_band_words) { status = -1; break; }
+        if (cells >= ((int64_t)1 << 30) || 2 * w + 4 > band_row_words(v.refLen, v.readLen)) { status = -1; break; }
         best = imax(best, band_pass(v.ref, v.read, v.refLen, v.readLen, w, a.matrix, a.gap_open,
                                     a.gap_extend, v.hb, v.eb, v.hc, nullptr));
         if (best >= a.score[p]) break;
@@ -552,7 +553,7 @@ banded_search_wave_kernel(BandArgs a) {
     int w = v.w0, best = 0, status = 0;
     for (;;) {
         const int64_t cells = (int64_t)(2 * w + 1) * v.readLen * 3;
-        if (cells >= ((int64_t)1 << 30) || 2 * w + 4 > a.ws_band_words) { status = -1; break; }
+        if (cells >= ((int64_t)1 << 30) || 2 * w + 4 > band_row_words(v.refLen, v.readLen)) { status = -1; break; }
         if (2 * w + 3 > kBandSlots) { status = -2; break; }
         best = imax(best, band_pass_wave(v.ref, v.read, v.refLen, v.readLen, w, a.matrix,
                                          a.gap_open, a.gap_extend, L, nullptr));
