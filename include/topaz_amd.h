@@ -224,6 +224,49 @@ int tpz_gapless_batch_results(const tpz_gapless_batch* b, int32_t* score,
 int tpz_gapless_batch_kernel_ms(const tpz_gapless_batch* b, double* ms);
 void tpz_gapless_batch_destroy(tpz_gapless_batch* b);
 
+/* ---- the seed walk of one query (host logic) ---------------------------- */
+
+/* The walk of query_fetch_alignment_batch (query.c:134-174): pqueue_next
+ * over the query's suffix-array streams (priority_queue.c:36-56,
+ * suffix.c:40-65) into hitlist_add (hit.c:126-193) until number_of_seeds
+ * hits have counted or the streams end -- the same pops, ties and
+ * evictions, popped a run at a time.  This entry point drives it for one
+ * query over extensions the caller computes (tpz_search computes them on the
+ * GPU); it is the host half of the search, exposed so that it can be checked
+ * on its own.
+ *
+ * extend(ctx, offset, first_slot, dir, count, out) writes, for the slots
+ * first_slot + k * dir (k < count), the gapless score of sa_hsp at query
+ * offset `offset` when its length is >= min_hit_length, else -1 (also for a
+ * slot outside [0, t_size)); it returns 0, or nonzero to abort. */
+typedef int32_t (*tpz_extend_fn)(void* ctx, int32_t offset, int64_t first_slot, int32_t dir,
+                                 int32_t count, int16_t* out);
+typedef struct {
+    int32_t n_offsets;              /* the offsets query_find_suffixes searched   */
+    const int32_t* offsets;         /* (bad prefixes skipped, query.c:86-91, 104) */
+    const int64_t* positions;       /* sa_search's index per offset, -1: none      */
+    uint64_t t_size;                /* db->Tsize                                   */
+    const int64_t* sa;              /* db->SA (host memory)                        */
+    uint64_t n_proteins;            /* db->Psize, and the index (db.h:10-17)       */
+    const uint64_t* protein_start;
+    const int32_t* protein_len;
+    int32_t number_of_seeds;        /* opt->number_of_seeds                        */
+    int32_t capacity;               /* the hit list's max_length (hit.c:109-113)   */
+    int32_t max_score;              /* bound on gapless scores (255 x max matrix)  */
+    int32_t first_chunk;            /* slots extended per new stream, >= 1         */
+    int32_t refill;                 /* slots of a stream's first refill (doubling) */
+} tpz_walk_input;
+typedef struct {
+    int64_t pops, runs, refills, elements;
+    int32_t counted;
+} tpz_walk_stats;
+/* On entry *n_hits is the capacity of db_index / score; on return the
+ * number of hits, listed in the hit list's traversal order (rbtree.c:646-669:
+ * gapless score descending, the latest insertion first among equals). */
+int tpz_walk_query(const tpz_walk_input* in, tpz_extend_fn extend, void* ctx,
+                   uint64_t* db_index, int32_t* score, int32_t* n_hits,
+                   tpz_walk_stats* stats);
+
 /* ---- suffix array of the database text (topaz index) -------------------- */
 
 /* sa[0..n-1] = the suffix array of text[0..n-1]: the order divsufsort64
