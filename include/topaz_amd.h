@@ -224,6 +224,30 @@ int tpz_gapless_batch_results(const tpz_gapless_batch* b, int32_t* score,
 int tpz_gapless_batch_kernel_ms(const tpz_gapless_batch* b, double* ms);
 void tpz_gapless_batch_destroy(tpz_gapless_batch* b);
 
+/* The gapless batch above without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Inputs and outputs are the caller's DEVICE arrays, laid out
+ * as for tpz_gapless_batch_create: q_chars holds q_bytes bytes, query k is
+ * q_chars[q_off[k] .. q_off[k] + q_len[k]), queries in increasing offset
+ * order without overlap; seeds, score and length hold n_seeds entries.  The
+ * matrix (625 bytes) is read on the host.  Results equal
+ * tpz_gapless_batch_results' for the same inputs (score 0 where length is
+ * -1).  Work is queued on `stream` (a hipStream_t; null: the library's own)
+ * and the call returns without waiting.  status (a device int32, may be
+ * null) becomes 0, or 1 when a query lies outside q_bytes or overlaps the
+ * next (its seeds then get length -1).
+ * The library's staging for this call -- score table, query code copies,
+ * their offsets -- sits in a fixed 32 MiB device array per device, so
+ * 2 * (q_bytes + n_queries) + 12 * n_queries must stay below about 32 MiB;
+ * larger calls fail with an error naming the limit.  Calls on one device
+ * share that array: calls that overlap on one device race (queue them on
+ * one stream). */
+int tpz_gapless_run_noalloc(const tpz_text* text, int32_t n_queries, const char* q_chars,
+                            int64_t q_bytes, const int64_t* q_off, const int32_t* q_len,
+                            int64_t n_seeds, const tpz_seed* seeds, const int8_t* matrix,
+                            int32_t* score, int32_t* length, int32_t* status, void* stream);
+
 /* ---- the seed walk of one query (host logic) ---------------------------- */
 
 /* The walk of query_fetch_alignment_batch (query.c:134-174): pqueue_next
