@@ -65,110 +65,88 @@ def cells_of(w):
                       s[2][w["pair_subject"]].astype(np.int64)))
 
 
-def cpu_baseline(w, seconds=12.0):
-    """The reference's own compiled ssw_align (oracle/_ref) on a bounded sample
-    of the same pairs, one profile per query as query.c:194-197 does, on all
-    host cores; falls back to the oracle restatement when the reference
-    checker was not built."""
-    from tests.refcheck import REF_SO, Oracle, Reference, matrix
-    kind = "reference" if REF_SO.exists() else "port"
-    eng = Reference() if kind == "reference" else Oracle()
-    mat = matrix()
+CPU_BENCH = os.path.join(ROOT, "oracle", "_ref", "cpu_bench")
+
+
+def _cores():
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(w, seconds=12.0, n_queries=1000):
+    """The reference's own compiled ssw_align (oracle/_ref/cpu_bench, a C
+    harness over the reference's objects) on all host cores, one profile per
+    query as query.c:194-197 builds it, over the pairs of a random sample of
+    the workload's queries, for about `seconds`."""
+    import subprocess
+    import tempfile
     q, s = w["q"], w["s"]
     pq, ps = w["pair_query"], w["pair_subject"]
-    order = np.argsort(pq, kind="stable")
-    cores = len(os.sched_getaffinity(0))
-    lock = threading.Lock()
-    state = {"next": 0, "cells": 0, "pairs": 0}
-    deadline = [0.0]
-    by_query = np.split(order, np.nonzero(np.diff(pq[order]))[0] + 1)
     rng = np.random.default_rng(7)
-    rng.shuffle(by_query)
-
-    def worker():
-        while True:
-            with lock:
-                k = state["next"]
-                state["next"] += 1
-            if k >= len(by_query) or time.perf_counter() > deadline[0]:
-                return
-            grp = by_query[k]
-            qi = pq[grp[0]]
-            qq = q[0][q[1][qi]:q[1][qi] + q[2][qi]]
-            cells = pairs = 0
-            if kind == "reference":
-                prof = eng.profile(qq, mat)
-                for p in grp:
-                    ss = s[0][s[1][ps[p]]:s[1][ps[p]] + s[2][ps[p]]]
-                    eng.ssw_align(qq, ss, 11, 1, mat, prof=prof)
-                    cells += len(qq) * len(ss)
-                    pairs += 1
-                eng.L.init_destroy(prof[0])
-            else:
-                for p in grp[:50]:
-                    ss = s[0][s[1][ps[p]]:s[1][ps[p]] + s[2][ps[p]]]
-                    eng.ssw_align(qq, ss, 11, 1, mat)
-                    cells += len(qq) * len(ss)
-                    pairs += 1
-            with lock:
-                state["cells"] += cells
-                state["pairs"] += pairs
-
-    t0 = time.perf_counter()
-    deadline[0] = t0 + seconds
-    th = [threading.Thread(target=worker) for _ in range(cores)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    return {"value": round(state["cells"] / dt / 1e9, 3), "unit": "GCUPS", "cores": cores,
-            "kind": kind,
-            "sample": "%d pairs of %d queries (%.3g cells) of the same workload, %.1f s, "
-                      "ssw_align with one profile per query, %d threads"
-                      % (state["pairs"], min(state["next"], len(by_query)), state["cells"],
-                         dt, cores)}
+    pick_q = rng.permutation(len(q[2]))[:n_queries]
+    order = np.argsort(pq, kind="stable")
+    starts = np.searchsorted(pq[order], np.arange(len(q[2]) + 1))
+    pairs = np.concatenate([order[starts[k]:starts[k + 1]] for k in pick_q])
+    uq = pick_q
+    us = np.unique(ps[pairs])
+    q_map = {int(k): i for i, k in enumerate(uq)}
+    s_map = np.full(len(s[2]), -1, np.int64)
+    s_map[us] = np.arange(len(us))
+    ql = q[2][uq].astype(np.int32)
+    sl = s[2][us].astype(np.int32)
+    qo = np.concatenate([[0], np.cumsum(ql[:-1], dtype=np.int64)])
+    so = np.concatenate([[0], np.cumsum(sl[:-1], dtype=np.int64)])
+    qr = np.concatenate([q[0][q[1][k]:q[1][k] + q[2][k]] for k in uq]).astype(np.int8)
+    sr = np.concatenate([s[0][s[1][k]:s[1][k] + s[2][k]] for k in us]).astype(np.int8)
+    npq = np.array([q_map[int(k)] for k in pq[pairs]], np.int32)
+    nps = s_map[ps[pairs]].astype(np.int32)
+    cores = _cores()
+    with tempfile.NamedTemporaryFile(suffix=".pairs") as f:
+        f.write(np.array([len(ql), len(sl), len(npq), len(qr), len(sr)], np.int64).tobytes())
+        for a in (qo, so, ql, sl, npq, nps, qr, sr):
+            f.write(np.ascontiguousarray(a).tobytes())
+        f.flush()
+        out = subprocess.run([CPU_BENCH, "sw", f.name, str(cores), str(seconds)],
+                             capture_output=True, text=True, check=True).stdout.split()
+    cells, n_pairs, n_done, dt = int(out[0]), int(out[1]), int(out[2]), float(out[3])
+    return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": cores,
+            "kind": "reference",
+            "sample": "%d pairs of %d queries (%.3g cells) of the same workload, %.1f s: the "
+                      "reference's ssw.c (ssw_init once per query, ssw_align per pair) in a C "
+                      "pthread harness, %d threads" % (n_pairs, n_done, cells, dt, cores)}
 
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 SEED_BYTES, RESULT_BYTES = 16, 8
 
 
-def gapless_cpu_baseline(text_bytes, queries, seeds, per_seed_bytes, seconds=8.0):
-    """The oracle's sa_hsp loop (a scalar port of search.c:99-190) on all host
-    cores over a bounded prefix of the seeds."""
-    from tests.refcheck import Oracle
-    orc = Oracle()
-    cores = len(os.sched_getaffinity(0))
-    chunk = 200_000
-    lock = threading.Lock()
-    state = {"next": 0, "seeds": 0, "bytes": 0}
-    deadline = time.perf_counter() + seconds
-
-    def worker():
-        while time.perf_counter() < deadline:
-            with lock:
-                k = state["next"]
-                state["next"] += chunk
-            if k >= len(seeds):
-                return
-            part = seeds[k:k + chunk]
-            orc.sa_hsp_batch(text_bytes, queries, part)
-            with lock:
-                state["seeds"] += len(part)
-                state["bytes"] += int(per_seed_bytes[k:k + chunk].sum())
-
-    t0 = time.perf_counter()
-    th = [threading.Thread(target=worker) for _ in range(cores)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    return {"value": round(state["bytes"] / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
-            "kind": "port",
-            "sample": "first %d seeds of the first-residue-matched set, %.1f s, oracle "
-                      "sa_hsp loop, %d threads" % (state["seeds"], dt, cores)}
+def gapless_cpu_baseline(text, queries, seeds, per_seed_bytes, seconds=8.0, max_seeds=40_000_000):
+    """The reference's own compiled sa_hsp (search.c:99-190, in the C harness
+    oracle/_ref/cpu_bench) on all host cores over a prefix of the seeds, for
+    about `seconds`; the rate counts the same algorithmic bytes as the GPU's."""
+    import subprocess
+    import tempfile
+    n = min(len(seeds), max_seeds)
+    ql = np.array([len(x) for x in queries], np.int32)
+    qo = np.concatenate([[0], np.cumsum(ql[:-1], dtype=np.int64)])
+    cores = _cores()
+    with tempfile.NamedTemporaryFile(suffix=".seeds") as f:
+        f.write(np.array([len(text), len(queries), int(ql.sum()), n], np.int64).tobytes())
+        f.write(np.ascontiguousarray(text).tobytes())
+        f.write(qo.tobytes())
+        f.write(ql.tobytes())
+        f.write(b"".join(queries))
+        f.write(np.ascontiguousarray(seeds["query"][:n], np.int32).tobytes())
+        f.write(np.ascontiguousarray(seeds["offset"][:n], np.int32).tobytes())
+        f.write(np.ascontiguousarray(seeds["position"][:n], np.int64).tobytes())
+        f.flush()
+        out = subprocess.run([CPU_BENCH, "gapless", f.name, str(cores), str(seconds)],
+                             capture_output=True, text=True, check=True).stdout.split()
+    done, dt = int(out[0]), float(out[1])
+    total = int(per_seed_bytes[:done].sum())
+    return {"value": round(total / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
+            "kind": "reference",
+            "sample": "first %d seeds of the first-residue-matched set, %.1f s: the "
+                      "reference's sa_hsp in a C pthread harness, %d threads" % (done, dt, cores)}
 
 
 def gapless_leg(args):
@@ -227,7 +205,7 @@ def gapless_leg(args):
         print("[bench] gapless %s: kernel %.3f ms, %.1f GB/s"
               % (name, kernel_s * 1e3, gbs), file=sys.stderr, flush=True)
         if name == "matched" and not args.no_cpu:
-            out["cpu_baseline"] = gapless_cpu_baseline(text_bytes, queries, seeds, per_seed)
+            out["cpu_baseline"] = gapless_cpu_baseline(text, queries, seeds, per_seed)
     dev_text.close()
     out["workload"] = ("gapless HSP extension: %d-byte text (lognormal proteins, mean 350, "
                        "clipped [30,5000]), 10000 queries, %d seeds per set grouped by query; "

@@ -248,6 +248,82 @@ int tpz_gapless_run_noalloc(const tpz_text* text, int32_t n_queries, const char*
                             int64_t n_seeds, const tpz_seed* seeds, const int8_t* matrix,
                             int32_t* score, int32_t* length, int32_t* status, void* stream);
 
+/* ---- search: the whole path for a batch of queries ---------------------- */
+
+/* A database resident on the device for searching: TOPAZ's db_t (db.h:19-39)
+ * -- the '\n'-separated internal-alphabet text (db->T, Tsize bytes), its
+ * suffix array (db->SA, Tsize entries) and the protein index
+ * (db->index[k].protein_start / protein_len) -- plus the scoring the
+ * searches use (the substitution matrix, 25 x 25, and gap penalties with
+ * gap_open > gap_extend).  The text and suffix array are read on the host as
+ * well (the hit list maps suffix-array slots to proteins), so the caller
+ * keeps them alive until tpz_db_destroy.  The proteins must tile the text as
+ * TOPAZ's index writes it: each protein's letters and its newline, back to
+ * back from position 0. */
+typedef struct tpz_db tpz_db;
+int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t* sa,
+                  uint64_t n_proteins, const uint64_t* protein_start,
+                  const int32_t* protein_len, const int8_t* matrix, int32_t gap_open,
+                  int32_t gap_extend);
+void tpz_db_destroy(tpz_db* db);
+
+/* options_t (options.h:26-53) as far as the search reads it; db_size and
+ * db_proteins as engine.c:52-55 sets them (Tsize - Psize, Psize). */
+typedef struct {
+    int32_t num_hits;              /* -H */
+    double evalue;                 /* -E */
+    double lambda, k, H;           /* Karlin-Altschul parameters */
+    uint64_t db_size, db_proteins;
+    int32_t min_hit_length;        /* -l */
+    int32_t number_of_alignments;  /* -A (hit-list capacity) */
+    int32_t number_of_seeds;       /* -S */
+    int32_t superfast;             /* 1: no traceback statistics (no -B) */
+    /* GPU stream expansion (no reference counterpart; results do not depend
+     * on them): slots of every stream in the first round (doubling in the
+     * next), and the elements a stream is expanded to before the walk asks
+     * for more on demand */
+    int32_t first_chunk;
+    int64_t max_stream_elements;
+} tpz_search_options;
+/* TOPAZ's defaults (options.h:6-23), superfast output. */
+void tpz_search_defaults(tpz_search_options* opt);
+
+/* One reported hit: hit_t (hit.h:27-60) as engine.c:139-167 prints it.
+ * Coordinates are 0-based (printed + 1); with superfast set only db_index,
+ * rawscore, qend, send, evalue and bitscore are computed. */
+typedef struct {
+    uint64_t db_index;
+    int32_t gapless_score;
+    int32_t rawscore;
+    int32_t qstart, qend, sstart, send;
+    int32_t length, mismatch, gapopen;
+    double identity, evalue, bitscore;
+} tpz_hit;
+
+typedef struct {
+    int64_t offsets, streams, elements, rounds;   /* seeding */
+    int64_t pops, runs, refills, hits;            /* walks */
+    int64_t pairs, cells;                         /* forward Smith-Waterman */
+    double gpu_s;                                 /* suffix-array search + stream kernels */
+    double seed_s, walk_s, align_s, stats_s, total_s;
+} tpz_search_stats;
+
+typedef struct tpz_search_result tpz_search_result;
+
+/* Search queries (internal alphabet as seq_2internal leaves them, after SEG;
+ * query k = q_chars[q_off[k] .. q_off[k] + q_len[k])) and return, per query,
+ * the hits output_top_hits prints (engine.c:193-234), in printed order.
+ * schedule TPZ_SCHED_PERSISTENT: all queries through each stage together;
+ * TPZ_SCHED_PLAIN: one query at a time.  Results do not depend on it.
+ * Calls from several threads may run at once. */
+int tpz_search(tpz_db* db, const tpz_search_options* opt, int32_t n_queries,
+               const char* q_chars, const int64_t* q_off, const int32_t* q_len,
+               int32_t schedule, tpz_search_result** out);
+int32_t tpz_search_result_count(const tpz_search_result* r, int32_t query);
+const tpz_hit* tpz_search_result_hits(const tpz_search_result* r, int32_t query);
+int tpz_search_result_stats(const tpz_search_result* r, tpz_search_stats* st);
+void tpz_search_result_destroy(tpz_search_result* r);
+
 /* ---- the seed walk of one query (host logic) ---------------------------- */
 
 /* The walk of query_fetch_alignment_batch (query.c:134-174): pqueue_next

@@ -149,6 +149,61 @@ def test_batch_loops_full_size_config(full_config, flags):
     assert _per_query(r.stdout) == (head, blocks)
 
 
+def _link_engine(tmp):
+    """TOPAZ's objects with engine.o replaced by engine_batch.o, whose
+    workers search batches of queries through tpz_search
+    (oracle/patch_engine.py), linked against the library."""
+    if not (REF / "batch" / "engine_batch.o").exists():
+        pytest.skip("patched engine.o not built (make -C oracle ref)")
+    objs = [o for o in sorted(str(p) for p in OBJ.rglob("*.o")) if Path(o).name != "engine.o"]
+    exe = tmp / "topaz_search_gpu"
+    libdir = str(_lib.LIB_PATH.parent)
+    subprocess.run(["g++", "-o", str(exe)] + objs + [str(REF / "batch" / "engine_batch.o"),
+                    "-L" + libdir, "-ltopaz_amd", "-Wl,-rpath," + libdir, "-lm", "-lpthread"],
+                   check=True)
+    return exe
+
+
+def _run(exe, tmp, args, env=None):
+    r = subprocess.run([str(exe), "search", "-f", str(tmp / "q.fa"), "-p", str(tmp / "DB")] +
+                       list(args), capture_output=True, text=True, timeout=900, cwd=str(tmp),
+                       env=dict(os.environ, **(env or {})))
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.mark.parametrize("flags,env", [
+    ([], {}), (["-B"], {}), (["-F"], {}),
+    (["-T", "3"], {"TOPAZ_AMD_SEARCH_BATCH": "5"}),      # several batches at once
+    (["-B"], {"TOPAZ_AMD_SEARCH_PLAIN": "1"}),           # one query at a time
+])
+def test_search_engine_hit_lists_identical(workspace, flags, env):
+    """TOPAZ with its workers searching whole batches through tpz_search
+    (suffix-array search, stream expansion, walk, SW, statistics) prints the
+    stock binary's hit lists."""
+    tmp = workspace[0]
+    exe = _link_engine(tmp)
+    ref = _search(REF / "topaz", tmp, *[f for f in flags if f.startswith("-B") or f == "-F"])
+    ours = _run(exe, tmp, flags, env)
+    assert ref.count("\n") > 20
+    assert _per_query(ours) == _per_query(ref)
+
+
+@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"]])
+def test_search_engine_full_size_config(full_config, flags):
+    """tpz_search through TOPAZ's own driver on the whole 100,000 x 1,000
+    configuration (-H 100 -E 1e-3), default, -B and -F, against the stock
+    binary; batches of 256 queries from 2 worker threads."""
+    tmp, _ = full_config
+    exe = _link_engine(tmp)
+    common = ["-H", "100", "-E", "0.001"] + list(flags)
+    ref = _search(REF / "topaz", tmp, *(common + ["-T", "16"]))
+    ours = _run(exe, tmp, common + ["-T", "2"], {"TOPAZ_AMD_SEARCH_BATCH": "256"})
+    head, blocks = _per_query(ref)
+    assert len(blocks) > 150
+    assert _per_query(ours) == (head, blocks)
+
+
 def _link_index(tmp):
     """TOPAZ's objects minus divsufsort.o, sssort.o and trsort.o (the
     vendored libdivsufsort), linked against the library's divsufsort64."""

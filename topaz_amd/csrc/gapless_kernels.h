@@ -67,10 +67,49 @@ struct SaSearchArgs {
 };
 constexpr int kSaSearchQueryLds = 32768;   // a longer query is read from memory
 
+// Where query_find_suffixes starts its suffixes (query.c:104-131): for each
+// (query, offset) record, the number of suffixes of the text that sort
+// below the query's suffix at that offset -- the index sa_search reports,
+// whether or not the suffix occurs.
+struct LowerBoundArgs {
+    const char* text;           // db->T
+    int64_t t_size;
+    const int64_t* sa;          // db->SA
+    const char* q_chars;        // the queries' bytes, back to back
+    const int64_t* q_off;
+    const int32_t* q_len;
+    const int32_t* rec_query;   // per record: query number and offset
+    const int32_t* rec_offset;
+    int64_t n;
+    int64_t* out;               // per record: the index
+};
+
+// A segment of a walk stream: up to 64 consecutive suffix-array slots
+// first, first + dir, ... of one (query, offset), extended with sa_hsp.
+struct StreamSeg {
+    int32_t query;
+    int32_t offset;
+    int64_t first;
+    int64_t out;                // element index of the first slot in StreamArgs::out
+    int32_t count;              // 1..64
+    int32_t dir;                // +1 or -1
+};
+
+struct StreamArgs {
+    GaplessArgs g;              // text, suffix array (required), queries, table
+    const StreamSeg* segs;
+    int64_t n_segs;
+    int16_t* out;               // per slot: the score when length >= min_len, else -1
+                                // (also for a slot outside the suffix array)
+    int32_t min_len;
+};
+
 }  // namespace tpz
 
 extern "C" hipError_t tpz_launch_gapless(const tpz::GaplessArgs* args, int grid,
                                          hipStream_t stream);
 extern "C" hipError_t tpz_launch_sa_search(const tpz::SaSearchArgs* args, hipStream_t stream);
+extern "C" hipError_t tpz_launch_lower_bound(const tpz::LowerBoundArgs* args, hipStream_t stream);
+extern "C" hipError_t tpz_launch_streams(const tpz::StreamArgs* args, int grid, hipStream_t stream);
 
 #endif

@@ -121,6 +121,42 @@ __device__ __forceinline__ int scan_side(const uint8_t* t, const uint8_t* q, boo
     return key;
 }
 
+// Phase 2 of both kernels: the wavefront's n_pass compacted seeds (first
+// residues equal), 32 at a time, two lanes per seed: the even lane scans the
+// right side, the odd lane the left; write(sid, score, length) stores a
+// seed's result.
+template <typename Write>
+__device__ __forceinline__ void extend_passing(const GaplessArgs& a, const GaplessLds& lds,
+                                               const Slot* slots, int n_pass, int lane,
+                                               Write&& write) {
+    for (int r = 0; r < n_pass; r += 32) {
+        const int j = r + lane / 2;
+        const bool active = j < n_pass;
+        const bool left = lane & 1;
+        int key = -1;
+        Slot s{};
+        if (active) {
+            s = slots[j];
+            const int m = a.q_len[s.query];
+            // left: the 128 text bytes before idx (the head window when
+            // idx < 128), scanned from the end
+            const uint8_t* t = !left ? a.t_fwd + s.idx
+                              : s.idx >= kWindow ? a.t_fwd + (s.idx - kWindow)
+                                                 : a.t_head + s.idx;
+            const uint8_t* q = left ? a.q_rev + a.qr_off[s.query] + (m - s.offset)
+                                    : a.q_fwd + a.qf_off[s.query] + s.offset;
+            key = scan_side(t, q, left, lds);
+        }
+        const int other = __shfl_xor(key, 1);
+        if (active && !left) {
+            // right: i = position; left: i = position + 1; -1: (0, 0)
+            const int sr = key < 0 ? 0 : key >> 8, lr = key < 0 ? 0 : key & 255;
+            const int sl = other < 0 ? 0 : other >> 8, ll = other < 0 ? 0 : (other & 255) + 1;
+            write(s.sid, sr + sl, lr + ll);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kWaves * 64)
 gapless_kernel(GaplessArgs a) {
     __shared__ GaplessLds lds;
@@ -164,37 +200,107 @@ gapless_kernel(GaplessArgs a) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // phase 2: two lanes per passing seed, even = right side, odd = left
-        for (int r = 0; r < n_pass; r += 32) {
-            const int j = r + lane / 2;
-            const bool active = j < n_pass;
-            const bool left = lane & 1;
-            int key = -1;
-            Slot s{};
-            if (active) {
-                s = slots[j];
-                const int m = a.q_len[s.query];
-                // left: the 128 text bytes before idx (the head window when
-                // idx < 128), scanned from the end
-                const uint8_t* t = !left ? a.t_fwd + s.idx
-                                  : s.idx >= kWindow ? a.t_fwd + (s.idx - kWindow)
-                                                     : a.t_head + s.idx;
-                const uint8_t* q = left ? a.q_rev + a.qr_off[s.query] + (m - s.offset)
-                                        : a.q_fwd + a.qf_off[s.query] + s.offset;
-                key = scan_side(t, q, left, lds);
-            }
-            const int other = __shfl_xor(key, 1);
-            if (active && !left) {
-                // right: i = position; left: i = position + 1; -1: (0, 0)
-                const int sr = key < 0 ? 0 : key >> 8, lr = key < 0 ? 0 : key & 255;
-                const int sl = other < 0 ? 0 : other >> 8, ll = other < 0 ? 0 : (other & 255) + 1;
-                a.out_score[s.sid] = sr + sl;
-                a.out_length[s.sid] = lr + ll;
-            }
-        }
+        extend_passing(a, lds, slots, n_pass, lane, [&](int64_t sid, int score, int length) {
+            a.out_score[sid] = score;
+            a.out_length[sid] = length;
+        });
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+}
+
+// Walk streams: one segment (up to 64 consecutive suffix-array slots of one
+// query offset) per wavefront iteration, lane l on slot first + l * dir;
+// then the same compaction and two-lane scans as gapless_kernel.  The
+// output is one int16 per slot: the score where the extension is a seed
+// (length >= min_len), -1 otherwise.
+__global__ void __launch_bounds__(kWaves * 64)
+stream_kernel(StreamArgs sa) {
+    __shared__ GaplessLds lds;
+    const GaplessArgs& a = sa.g;
+    for (int e = threadIdx.x; e < kTableRows * kTableCols; e += blockDim.x) lds.table[e] = a.table[e];
+    __syncthreads();
+
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    Slot* slots = lds.slot[wave];
+    const int64_t waves_total = (int64_t)gridDim.x * kWaves;
+    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < sa.n_segs; g += waves_total) {
+        const StreamSeg seg = sa.segs[g];
+        bool pass = false;
+        int64_t idx = 0;
+        const int64_t sid = seg.out + lane;
+        if (lane < seg.count) {
+            const int64_t slot = seg.first + (int64_t)lane * seg.dir;
+            if (slot >= 0 && slot < a.t_size) {
+                idx = a.sa[slot];
+                if (idx >= 0 && idx < a.t_size && seg.query >= 0 && seg.query < a.n_queries &&
+                    seg.offset >= 0 && seg.offset < a.q_len[seg.query])
+                    pass = 2 * (int)a.t_fwd[idx] ==
+                           (int)a.q_fwd[a.qf_off[seg.query] + seg.offset];
+            }
+            if (!pass) sa.out[sid] = -1;
+        }
+        const uint64_t mask = __ballot(pass);
+        const int n_pass = __popcll(mask);
+        if (pass) {
+            const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+            slots[rank] = Slot{idx, sid, seg.offset, seg.query};
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int min_len = sa.min_len;
+        extend_passing(a, lds, slots, n_pass, lane, [&](int64_t s_id, int score, int length) {
+            sa.out[s_id] = (int16_t)(length >= min_len ? score : -1);
+        });
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// One thread per (query, offset) record: binary search over the suffix
+// array for the first suffix not below the query's suffix P (bytes compared
+// as the reference's signed chars; a suffix that ends inside P sorts below
+// it, one that P is a prefix of does not).  The bytes P shares with the
+// suffixes on both sides of the interval are skipped (the smaller of the two
+// common prefixes holds for every suffix between them).
+__global__ void __launch_bounds__(256) lower_bound_kernel(LowerBoundArgs a) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const int32_t q = a.rec_query[k], o = a.rec_offset[k];
+    const int64_t m = (int64_t)a.q_len[q] - o;
+    if (m <= 0 || o < 0 || a.t_size <= 0) {
+        a.out[k] = -1;
+        return;
+    }
+    const signed char* T = reinterpret_cast<const signed char*>(a.text);
+    const signed char* P = reinterpret_cast<const signed char*>(a.q_chars + a.q_off[q] + o);
+    const int64_t n = a.t_size;
+    int64_t lo = 0, hi = n, ml = 0, mr = 0;
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        int64_t j = ml < mr ? ml : mr;
+        int64_t i = a.sa[mid] + j;
+        int r = 0;
+        while (i < n && j < m) {
+            r = (int)T[i] - (int)P[j];
+            if (r != 0) break;
+            ++i;
+            ++j;
+        }
+        const bool below = r != 0 ? r < 0 : j < m;
+        if (below) {
+            lo = mid + 1;
+            ml = j;
+        } else {
+            hi = mid;
+            mr = j;
+        }
+    }
+    a.out[k] = lo;
 }
 
 }  // namespace tpz
@@ -300,5 +406,19 @@ extern "C" hipError_t tpz_launch_sa_search(const tpz::SaSearchArgs* args, hipStr
 extern "C" hipError_t tpz_launch_gapless(const tpz::GaplessArgs* args, int grid,
                                          hipStream_t stream) {
     hipLaunchKernelGGL(tpz::gapless_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream, *args);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t tpz_launch_lower_bound(const tpz::LowerBoundArgs* args, hipStream_t stream) {
+    if (args->n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tpz::lower_bound_kernel, dim3((unsigned)((args->n + 255) / 256)), dim3(256),
+                       0, stream, *args);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t tpz_launch_streams(const tpz::StreamArgs* args, int grid,
+                                         hipStream_t stream) {
+    if (args->n_segs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tpz::stream_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream, *args);
     return hipGetLastError();
 }

@@ -1,0 +1,894 @@
+// TOPAZ's search of a batch of queries against a resident database, on one
+// MI355X: the path of engine.c:193-248 (_full_search) per query --
+// query_find_suffixes, query_fetch_alignment_batch, query_perform_alignments,
+// query_perform_statistics, output_top_hits -- with every query of the batch
+// going through each stage together.
+//
+//   1. Suffix-array search: for every (query, offset) the index sa_search
+//      reports (lower_bound_kernel), query.c:104-110.
+//   2. Streams: per offset the forward suffix at that index and the backward
+//      one before it (query.c:112-128).  Every stream is extended on the GPU
+//      slot by slot until its first non-seed (stream_kernel), in rounds of
+//      doubling length shared by all streams still open: the walk pops
+//      nearly every element of every stream, so they are computed up front.
+//   3. The walk and hit list per query on the host (walk.h), in parallel
+//      over queries; a stream that reaches the round cap is refilled on
+//      demand.
+//   4. Forward Smith-Waterman of every hit whose gapless score is >= 40
+//      (ssw_wrapper.c:171-176) through the library's serving loop over the
+//      resident proteins; E-values and bitscores on the host with the
+//      reference's expressions (ssw_wrapper.c:179-183, 239-245).
+//   5. Output order: the hit list's traversal order, stable sort by bitscore,
+//      the first -H, then the E-value filter (engine.c:193-234); with -B the
+//      traceback statistics of those hits on the GPU (stats_compute).
+//
+// Schedules: TPZ_SCHED_PERSISTENT puts the whole batch through each stage
+// in one set of launches; TPZ_SCHED_PLAIN runs the queries one at a time
+// (one set of launches per query), the comparison form.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/topaz_amd.h"
+#include "gapless_kernels.h"
+#include "runtime.h"
+#include "walk.h"
+
+namespace tpz {
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// ---------------------------------------------------------------------------
+// host thread pool: parallel loops over queries
+// ---------------------------------------------------------------------------
+class Pool {
+public:
+    explicit Pool(int n) {
+        for (int k = 0; k < n; ++k) th_.emplace_back([this] { worker(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    // fn(i) for i in [0, n); the caller works too.  Returns when all are done.
+    void run(int64_t n, const std::function<void(int64_t)>& fn) {
+        if (n <= 0) return;
+        auto job = std::make_shared<Job>();
+        job->n = n;
+        job->fn = &fn;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_.push_back(job);
+        }
+        cv_.notify_all();
+        work(*job);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return job->done.load() == job->n; });
+        jobs_.erase(std::remove(jobs_.begin(), jobs_.end(), job), jobs_.end());
+    }
+
+private:
+    struct Job {
+        int64_t n = 0;
+        const std::function<void(int64_t)>* fn = nullptr;
+        std::atomic<int64_t> next{0}, done{0};
+    };
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::vector<std::shared_ptr<Job>> jobs_;
+    bool stop_ = false;
+
+    void work(Job& j) {
+        int64_t i;
+        while ((i = j.next.fetch_add(1)) < j.n) {
+            (*j.fn)(i);
+            if (j.done.fetch_add(1) + 1 == j.n) {
+                std::lock_guard<std::mutex> lk(mu_);
+                done_cv_.notify_all();
+            }
+        }
+    }
+    void worker() {
+        for (;;) {
+            std::shared_ptr<Job> j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] {
+                    if (stop_) return true;
+                    for (auto& x : jobs_)
+                        if (x->next.load() < x->n) return true;
+                    return false;
+                });
+                if (stop_) return;
+                for (auto& x : jobs_)
+                    if (x->next.load() < x->n) { j = x; break; }
+            }
+            if (j) work(*j);
+        }
+    }
+};
+
+Pool& pool() {
+    static Pool* p = [] {
+        int n = (int)std::thread::hardware_concurrency();
+        if (const char* e = getenv("TOPAZ_AMD_HOST_THREADS")) n = atoi(e);
+        return new Pool(std::max(1, std::min(n, 256) - 1));
+    }();
+    return *p;
+}
+
+// ---------------------------------------------------------------------------
+// grow-only buffers of one search call
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    void* get(size_t n) {
+        if (n > bytes) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            const size_t want = std::max(n + n / 4, (size_t)1 << 20);
+            if (check(hipMalloc(&p, want), "hipMalloc")) return nullptr;
+            bytes = want;
+        }
+        return p;
+    }
+};
+struct HostBuf {
+    char* p = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { if (p) (void)hipHostFree(p); }
+    char* get(size_t n) {
+        if (n > bytes) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            bytes = 0;
+            const size_t want = std::max(n + n / 4, (size_t)1 << 20);
+            if (check(hipHostMalloc((void**)&p, want, hipHostMallocDefault), "hipHostMalloc"))
+                return nullptr;
+            bytes = want;
+        }
+        return p;
+    }
+};
+
+// Device and pinned memory of a statistics run (RunMem over grow-only blocks).
+struct StatsMem final : RunMem {
+    BlockList d, h;
+    hipStream_t st = nullptr;
+    ~StatsMem() override {
+        for (auto& b : d.blocks) (void)hipFree(b.first);
+        for (auto& b : h.blocks) (void)hipHostFree(b.first);
+    }
+    void reset() override { d.reset(); h.reset(); }
+    void* dev(size_t bytes) override {
+        return d.take(bytes, [](size_t n, void*) -> void* {
+            void* p = nullptr;
+            return check(hipMalloc(&p, n), "hipMalloc") ? nullptr : p;
+        }, nullptr);
+    }
+    char* host(size_t bytes) override {
+        return h.take(bytes, [](size_t n, void*) -> void* {
+            void* p = nullptr;
+            return check(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc") ? nullptr : p;
+        }, nullptr);
+    }
+    hipStream_t stream() override { return st; }
+    int wait() override { return check(hipStreamSynchronize(st), "stats"); }
+};
+
+// Everything one search call uses; calls take one from the database's pool.
+struct Slot {
+    hipStream_t stream = nullptr;
+    DevBuf d_q, d_qoff, d_qlen, d_rq, d_ro, d_idx;
+    DevBuf d_qf, d_qr, d_qf_off, d_qr_off, d_segs, d_out;
+    HostBuf h_stage, h_out, h_idx;
+    StatsMem stats;
+    ~Slot() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace
+}  // namespace tpz
+
+using namespace tpz;
+
+struct tpz_db {
+    uint64_t t_size = 0, n_proteins = 0;
+    const char* text = nullptr;          // the caller's (kept alive by the caller)
+    const int64_t* sa = nullptr;         // the caller's host suffix array
+    std::vector<uint64_t> start;
+    std::vector<int32_t> len;
+    ProteinIndex proteins;
+    tpz_text* dtext = nullptr;           // letter codes, head window, suffix array
+    DeviceBuffer d_raw;                  // the text's bytes, for the suffix-array search
+    std::vector<int16_t> table;          // gapless scores per (text code, query code)
+    DeviceBuffer d_table;
+    tpz_server* sw = nullptr;            // stage S over the proteins
+    int8_t matrix[625];
+    int32_t gap_open = 11, gap_extend = 1, max_w = 1;
+    std::mutex mu;
+    std::vector<std::unique_ptr<Slot>> free_slots;
+    ~tpz_db() {
+        if (sw) tpz_server_destroy(sw);
+        if (dtext) tpz_text_destroy(dtext);
+    }
+};
+
+struct tpz_search_result {
+    std::vector<std::vector<tpz_hit>> hits;
+    tpz_search_stats stats{};
+};
+
+namespace tpz {
+namespace {
+
+// lexicographical.h:19-33, 35-74 (aa_table2, upper_aa, upper_aa_only)
+struct Maps {
+    int8_t aa2[128];
+    char upper[256], upper_only[256];
+    Maps() {
+        static const char internal_order[] = "ACMLJIVTSKRQZEDBNHYFWXP*G";
+        static const char matrix_order[] = "ARNDCQEGHILKMFPSTWYVBJZX*";
+        for (int c = 0; c < 128; ++c) aa2[c] = 23;
+        for (int k = 0; k < 25; ++k) {
+            const int8_t idx = (int8_t)(strchr(matrix_order, internal_order[k]) - matrix_order);
+            aa2['A' + k] = idx;
+            aa2['a' + k] = idx;
+        }
+        for (int c = 0; c < 256; ++c) {
+            const bool keep = (c >= 'A' && c <= 'Y') || c == '*' || c == '\n';
+            const bool lower = c >= 'a' && c <= 'y';
+            upper[c] = keep ? (char)c : lower ? (char)(c - 32) : 'X';
+            upper_only[c] = keep ? (char)c : 'V';
+        }
+    }
+    int8_t idx(unsigned char c) const { return aa2[c & 127]; }
+};
+const Maps& maps() {
+    static const Maps m;
+    return m;
+}
+
+Slot* take_slot(tpz_db* db) {
+    {
+        std::lock_guard<std::mutex> lk(db->mu);
+        if (!db->free_slots.empty()) {
+            Slot* s = db->free_slots.back().release();
+            db->free_slots.pop_back();
+            return s;
+        }
+    }
+    auto s = std::make_unique<Slot>();
+    if (check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate"))
+        return nullptr;
+    s->stats.st = s->stream;
+    return s.release();
+}
+
+void give_slot(tpz_db* db, Slot* s) {
+    std::lock_guard<std::mutex> lk(db->mu);
+    db->free_slots.emplace_back(s);
+}
+
+// One stream of the walk (query.c:112-128): offset, first slot, direction.
+struct StreamDef {
+    int32_t query, offset;
+    int64_t first;
+    int32_t dir;
+};
+
+struct Batch {
+    int32_t n = 0;
+    const char* q_chars = nullptr;
+    const int64_t* q_off = nullptr;
+    const int32_t* q_len = nullptr;
+};
+
+// Upload the batch's queries in both layouts the kernels read: raw bytes
+// (suffix-array search) and doubled letter codes (stream_kernel).
+struct QueryData {
+    std::vector<int64_t> off;      // offsets into the raw copy
+    QueryCodes codes;
+};
+
+int upload_queries(tpz_db* db, Slot& sl, const Batch& b, QueryData& qd) {
+    int64_t tot = 0;
+    qd.off.assign(b.n, 0);
+    for (int32_t k = 0; k < b.n; ++k) {
+        qd.off[k] = tot;
+        tot += b.q_len[k];
+    }
+    char* h = sl.h_stage.get((size_t)tot + 64);
+    if (!h) return -1;
+    for (int32_t k = 0; k < b.n; ++k) memcpy(h + qd.off[k], b.q_chars + b.q_off[k], b.q_len[k]);
+    void* dq = sl.d_q.get((size_t)tot + 64);
+    void* doff = sl.d_qoff.get((size_t)b.n * 8 + 8);
+    void* dlen = sl.d_qlen.get((size_t)b.n * 4 + 4);
+    if (!dq || !doff || !dlen) return -1;
+    if (check(hipMemcpyAsync(dq, h, (size_t)tot, hipMemcpyHostToDevice, sl.stream), "upload") ||
+        check(hipMemcpyAsync(doff, qd.off.data(), (size_t)b.n * 8, hipMemcpyHostToDevice,
+                             sl.stream), "upload") ||
+        check(hipMemcpyAsync(dlen, b.q_len, (size_t)b.n * 4, hipMemcpyHostToDevice, sl.stream),
+              "upload"))
+        return -1;
+    // pageable copies above are staged before the call returns; the codes
+    // go the same way
+    if (gapless_query_codes(db->dtext->codes, b.n, b.q_chars, b.q_off, b.q_len, qd.codes))
+        return -1;
+    void* qf = sl.d_qf.get(qd.codes.fwd.size());
+    void* qr = sl.d_qr.get(qd.codes.rev.size());
+    void* qfo = sl.d_qf_off.get(qd.codes.fwd_off.size() * 8 + 8);
+    void* qro = sl.d_qr_off.get(qd.codes.rev_off.size() * 8 + 8);
+    if (!qf || !qr || !qfo || !qro) return -1;
+    if (check(hipMemcpyAsync(qf, qd.codes.fwd.data(), qd.codes.fwd.size(), hipMemcpyHostToDevice,
+                             sl.stream), "upload") ||
+        check(hipMemcpyAsync(qr, qd.codes.rev.data(), qd.codes.rev.size(), hipMemcpyHostToDevice,
+                             sl.stream), "upload") ||
+        check(hipMemcpyAsync(qfo, qd.codes.fwd_off.data(), qd.codes.fwd_off.size() * 8,
+                             hipMemcpyHostToDevice, sl.stream), "upload") ||
+        check(hipMemcpyAsync(qro, qd.codes.rev_off.data(), qd.codes.rev_off.size() * 8,
+                             hipMemcpyHostToDevice, sl.stream), "upload"))
+        return -1;
+    return check(hipStreamSynchronize(sl.stream), "query upload");
+}
+
+GaplessArgs stream_gapless_args(tpz_db* db, Slot& sl, int32_t n_queries, const int32_t* q_len_dev) {
+    GaplessArgs a{};
+    a.t_fwd = (const uint8_t*)db->dtext->d_fwd.ptr;
+    a.t_head = (const uint8_t*)db->dtext->d_head.ptr;
+    a.t_size = (int64_t)db->t_size;
+    a.sa = (const int64_t*)db->dtext->d_sa.ptr;
+    a.q_fwd = (const uint8_t*)sl.d_qf.p;
+    a.qf_off = (const int64_t*)sl.d_qf_off.p;
+    a.q_rev = (const uint8_t*)sl.d_qr.p;
+    a.qr_off = (const int64_t*)sl.d_qr_off.p;
+    a.q_len = q_len_dev;
+    a.n_queries = n_queries;
+    a.table = (const int16_t*)db->d_table.ptr;
+    return a;
+}
+
+// Extend `count` slots of each listed stream from slot `from[i]` on; the
+// elements land in out[i * ... ] per the offsets given (host pinned copy).
+struct ChunkReq {
+    int32_t stream;
+    int64_t from;
+    int32_t count;
+    int64_t out;        // element index in the round's output
+};
+
+int run_chunks(tpz_db* db, Slot& sl, const std::vector<StreamDef>& defs,
+               const std::vector<ChunkReq>& reqs, int64_t n_out, int32_t n_queries,
+               int32_t min_len, const int16_t** host_out, double* gpu_s) {
+    std::vector<StreamSeg> segs;
+    segs.reserve((size_t)n_out / 32 + reqs.size());
+    for (const ChunkReq& r : reqs) {
+        const StreamDef& d = defs[r.stream];
+        for (int32_t c = 0; c < r.count; c += 64) {
+            StreamSeg g;
+            g.query = d.query;
+            g.offset = d.offset;
+            g.first = r.from + (int64_t)c * d.dir;
+            g.out = r.out + c;
+            g.count = std::min(64, r.count - c);
+            g.dir = d.dir;
+            segs.push_back(g);
+        }
+    }
+    const size_t seg_bytes = segs.size() * sizeof(StreamSeg);
+    char* hs = sl.h_stage.get(seg_bytes + 64);
+    void* ds = sl.d_segs.get(seg_bytes + 64);
+    void* dout = sl.d_out.get((size_t)n_out * 2 + 64);
+    int16_t* hout = (int16_t*)sl.h_out.get((size_t)n_out * 2 + 64);
+    if (!hs || !ds || !dout || !hout) return -1;
+    memcpy(hs, segs.data(), seg_bytes);
+    const double t0 = now_s();
+    if (check(hipMemcpyAsync(ds, hs, seg_bytes, hipMemcpyHostToDevice, sl.stream), "upload"))
+        return -1;
+    StreamArgs a{};
+    a.g = stream_gapless_args(db, sl, n_queries, (const int32_t*)sl.d_qlen.p);
+    a.segs = (const StreamSeg*)ds;
+    a.n_segs = (int64_t)segs.size();
+    a.out = (int16_t*)dout;
+    a.min_len = min_len;
+    const int grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>(((int64_t)segs.size() + 3) / 4, (int64_t)std::max(num_cu(), 1) * 32));
+    if (check(tpz_launch_streams(&a, grid, sl.stream), "stream_kernel") ||
+        check(hipMemcpyAsync(hout, dout, (size_t)n_out * 2, hipMemcpyDeviceToHost, sl.stream),
+              "download") ||
+        check(hipStreamSynchronize(sl.stream), "streams"))
+        return -1;
+    *gpu_s += now_s() - t0;
+    *host_out = hout;
+    return 0;
+}
+
+struct Options {
+    tpz_search_options o;
+    int32_t capacity(int32_t qlen) const {
+        // hit.c:109-113
+        return qlen > 3000 ? (int32_t)(1.5 * o.num_hits) : o.number_of_alignments;
+    }
+};
+
+// Per-query state through the stages.
+struct QueryWork {
+    int32_t first_stream = 0, n_streams = 0;   // range in the batch's stream list
+    std::vector<std::vector<int16_t>> vals;     // per stream: elements computed
+    std::vector<uint8_t> open;                  // per stream: more elements may exist
+    std::vector<uint64_t> hit_index;            // hit list, traversal order
+    std::vector<int32_t> hit_gapless;
+    WalkStats walk;
+    int32_t counted = 0;
+};
+
+// The whole search of one batch of queries (all stages).
+int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
+                 std::vector<std::vector<tpz_hit>>& out, tpz_search_stats& st) {
+    const Maps& M = maps();
+    const int32_t min_len = opt.o.min_hit_length;
+    out.assign(b.n, {});
+    double t0 = now_s();
+    QueryData qd;
+    if (upload_queries(db, sl, b, qd)) return -1;
+
+    // 1. the suffix-array index of every offset (query.c:94-110)
+    std::vector<int32_t> rq, ro;
+    std::vector<int32_t> rec_begin(b.n + 1, 0);
+    for (int32_t k = 0; k < b.n; ++k) {
+        rec_begin[k] = (int32_t)rq.size();
+        const char* q = b.q_chars + b.q_off[k];
+        const int32_t num_suffixes = b.q_len[k] - min_len;
+        for (int32_t i = 0; i < num_suffixes; ++i) {
+            if (M.upper_only[(unsigned char)q[i]] == 'V') continue;     // _bad_prefix, query.c:86-91
+            rq.push_back(k);
+            ro.push_back(i);
+        }
+    }
+    rec_begin[b.n] = (int32_t)rq.size();
+    const int64_t n_rec = (int64_t)rq.size();
+    std::vector<int64_t> idx((size_t)n_rec);
+    if (n_rec > 0) {
+        void* drq = sl.d_rq.get((size_t)n_rec * 4);
+        void* dro = sl.d_ro.get((size_t)n_rec * 4);
+        void* didx = sl.d_idx.get((size_t)n_rec * 8);
+        if (!drq || !dro || !didx) return -1;
+        LowerBoundArgs a{};
+        a.text = (const char*)db->d_raw.ptr;
+        a.t_size = (int64_t)db->t_size;
+        a.sa = (const int64_t*)db->dtext->d_sa.ptr;
+        a.q_chars = (const char*)sl.d_q.p;
+        a.q_off = (const int64_t*)sl.d_qoff.p;
+        a.q_len = (const int32_t*)sl.d_qlen.p;
+        a.rec_query = (const int32_t*)drq;
+        a.rec_offset = (const int32_t*)dro;
+        a.n = n_rec;
+        a.out = (int64_t*)didx;
+        const double g0 = now_s();
+        if (check(hipMemcpyAsync(drq, rq.data(), (size_t)n_rec * 4, hipMemcpyHostToDevice,
+                                 sl.stream), "upload") ||
+            check(hipMemcpyAsync(dro, ro.data(), (size_t)n_rec * 4, hipMemcpyHostToDevice,
+                                 sl.stream), "upload") ||
+            check(tpz_launch_lower_bound(&a, sl.stream), "lower_bound") ||
+            check(hipMemcpyAsync(idx.data(), didx, (size_t)n_rec * 8, hipMemcpyDeviceToHost,
+                                 sl.stream), "download") ||
+            check(hipStreamSynchronize(sl.stream), "suffix-array search"))
+            return -1;
+        st.gpu_s += now_s() - g0;
+    }
+    st.offsets += n_rec;
+
+    // 2. streams in creation order: per offset forward at idx, backward at
+    // idx - 1 (a slot outside [0, Tsize) has no suffix)
+    std::vector<StreamDef> defs;
+    std::vector<QueryWork> qw(b.n);
+    for (int32_t k = 0; k < b.n; ++k) {
+        qw[k].first_stream = (int32_t)defs.size();
+        for (int32_t r = rec_begin[k]; r < rec_begin[k + 1]; ++r) {
+            const int64_t pos = idx[r];
+            if (pos == -1) continue;
+            if (pos < (int64_t)db->t_size) defs.push_back(StreamDef{k, ro[r], pos, 1});
+            if (pos != 0 && pos - 1 < (int64_t)db->t_size)
+                defs.push_back(StreamDef{k, ro[r], pos - 1, -1});
+        }
+        qw[k].n_streams = (int32_t)defs.size() - qw[k].first_stream;
+        qw[k].vals.resize(qw[k].n_streams);
+        qw[k].open.assign(qw[k].n_streams, 1);
+    }
+    st.streams += (int64_t)defs.size();
+    auto query_of = [&](int32_t s) { return defs[s].query; };
+
+    // rounds: every open stream gets its next `chunk` slots
+    int32_t chunk = std::max(1, opt.o.first_chunk);
+    const int64_t cap_elems = opt.o.max_stream_elements;
+    for (int round = 0;; ++round) {
+        std::vector<ChunkReq> reqs;
+        int64_t n_out = 0;
+        for (int32_t s = 0; s < (int32_t)defs.size(); ++s) {
+            QueryWork& w = qw[query_of(s)];
+            const int32_t ls = s - w.first_stream;
+            if (!w.open[ls]) continue;
+            const int64_t have = (int64_t)w.vals[ls].size();
+            if (have >= cap_elems) continue;       // left to the walk's refills
+            const int32_t c = (int32_t)std::min<int64_t>(chunk, cap_elems - have);
+            reqs.push_back(ChunkReq{s, defs[s].first + have * defs[s].dir, c, n_out});
+            n_out += c;
+        }
+        if (reqs.empty()) break;
+        const int16_t* hv = nullptr;
+        if (run_chunks(db, sl, defs, reqs, n_out, b.n, min_len, &hv, &st.gpu_s)) return -1;
+        st.elements += n_out;
+        ++st.rounds;
+        for (const ChunkReq& r : reqs) {
+            QueryWork& w = qw[query_of(r.stream)];
+            const int32_t ls = r.stream - w.first_stream;
+            std::vector<int16_t>& v = w.vals[ls];
+            // a stream that is dead at its first slot is never created
+            for (int32_t i = 0; i < r.count; ++i) {
+                v.push_back(hv[r.out + i]);
+                if (hv[r.out + i] == kStreamEnd) {
+                    w.open[ls] = 0;
+                    break;
+                }
+            }
+        }
+        chunk = (int32_t)std::min<int64_t>((int64_t)chunk * 2, 1 << 20);
+    }
+    const double t_streams = now_s();
+    st.seed_s += t_streams - t0;
+
+    // 3. the walk of every query (host, parallel); a stream cut at the cap
+    // is refilled on demand, one GPU call per refill
+    std::mutex gpu_mu;
+    int walk_rc = 0;
+    const int32_t max_score = 255 * std::max(1, db->max_w);
+    pool().run(b.n, [&](int64_t kk) {
+        const int32_t k = (int32_t)kk;
+        QueryWork& w = qw[k];
+        Walk walk;
+        walk.reset(opt.o.number_of_seeds, opt.capacity(b.q_len[k]), max_score);
+        std::vector<int32_t> local;      // walk stream -> batch stream
+        for (int32_t ls = 0; ls < w.n_streams; ++ls) {
+            std::vector<int16_t>& v = w.vals[ls];
+            if (v.empty() || v[0] == kStreamEnd) continue;
+            const StreamDef& d = defs[w.first_stream + ls];
+            walk.add_stream(d.offset, d.first, d.dir, v.data(), (int32_t)v.size(), !w.open[ls]);
+            local.push_back(w.first_stream + ls);
+            std::vector<int16_t>().swap(v);
+        }
+        const int64_t* sa = db->sa;
+        const ProteinIndex& P = db->proteins;
+        auto protein_of = [&](int64_t slot) { return P.of((uint64_t)sa[slot]); };
+        std::vector<int32_t> want(local.size(), 0);
+        while (walk.run(protein_of) == Walk::kNeed) {
+            const int32_t ws = walk.waiting();
+            const int32_t n = want[ws] = std::max(2 * want[ws], 4096);
+            const StreamDef& d = defs[local[ws]];
+            std::vector<ChunkReq> one{ChunkReq{local[ws], walk.next_slot(ws), n, 0}};
+            const int16_t* hv = nullptr;
+            std::lock_guard<std::mutex> lk(gpu_mu);
+            double dummy = 0;
+            if (run_chunks(db, sl, defs, one, n, b.n, min_len, &hv, &dummy)) {
+                walk_rc = -1;
+                return;
+            }
+            (void)d;
+            walk.supply(ws, hv, n, false);
+        }
+        walk.hits().traversal(w.hit_index, w.hit_gapless);
+        w.walk = walk.stats();
+        w.counted = walk.counted();
+    });
+    if (walk_rc) return -1;
+    for (const QueryWork& w : qw) {
+        st.pops += w.walk.pops;
+        st.runs += w.walk.runs;
+        st.refills += w.walk.refills;
+        st.hits += (int64_t)w.hit_index.size();
+    }
+    const double t_walk = now_s();
+    st.walk_s += t_walk - t_streams;
+
+    // 4. forward Smith-Waterman of the hits with gapless score >= 40
+    std::vector<int8_t> q_res;
+    std::vector<int64_t> q_off(b.n);
+    std::vector<int32_t> pq, ps;
+    std::vector<int64_t> pair_of;          // per (query, hit) -> pair number or -1
+    std::vector<int64_t> hit_base(b.n + 1, 0);
+    for (int32_t k = 0; k < b.n; ++k) {
+        q_off[k] = (int64_t)q_res.size();
+        const char* q = b.q_chars + b.q_off[k];
+        for (int32_t j = 0; j < b.q_len[k]; ++j)                     // ssw_wrapper.c:115-121
+            q_res.push_back(M.idx((unsigned char)M.upper[(unsigned char)q[j]]));
+        hit_base[k + 1] = hit_base[k] + (int64_t)qw[k].hit_index.size();
+    }
+    pair_of.assign((size_t)hit_base[b.n], -1);
+    for (int32_t k = 0; k < b.n; ++k) {
+        const QueryWork& w = qw[k];
+        for (size_t h = 0; h < w.hit_index.size(); ++h) {
+            if (w.hit_gapless[h] < 40) continue;
+            pair_of[hit_base[k] + h] = (int64_t)pq.size();
+            pq.push_back(k);
+            ps.push_back((int32_t)w.hit_index[h]);
+        }
+    }
+    const int64_t n_pairs = (int64_t)pq.size();
+    std::vector<int32_t> sc((size_t)n_pairs), re((size_t)n_pairs), rd((size_t)n_pairs);
+    if (n_pairs > 0) {
+        uint64_t ticket = 0;
+        double lat = 0;
+        if (tpz_server_submit(db->sw, b.n, q_res.data(), q_off.data(), b.q_len, (int32_t)n_pairs,
+                              pq.data(), ps.data(), &ticket) ||
+            tpz_server_wait(db->sw, ticket, sc.data(), re.data(), rd.data(), &lat))
+            return -1;
+    }
+    st.pairs += n_pairs;
+    for (int64_t p = 0; p < n_pairs; ++p)
+        st.cells += (int64_t)b.q_len[pq[p]] * db->len[ps[p]];
+    const double t_sw = now_s();
+    st.align_s += t_sw - t_walk;
+
+    // 5. hits in output order (engine.c:193-234), E-values (ssw_wrapper.c:171-245)
+    const tpz_search_options& o = opt.o;
+    std::vector<std::vector<int32_t>> stat_hits(b.n);   // reported hits needing statistics
+    for (int32_t k = 0; k < b.n; ++k) {
+        const QueryWork& w = qw[k];
+        const int32_t m = b.q_len[k];
+        const double correction = log(o.k * m * (double)o.db_size) / o.H;
+        const double corrected_q = fmax(m - correction, 1.0);
+        const double corrected_db = fmax((double)o.db_size - (correction * (double)o.db_proteins), 1.0);
+        std::vector<tpz_hit> all;
+        for (size_t h = 0; h < w.hit_index.size(); ++h) {
+            tpz_hit x{};
+            x.db_index = w.hit_index[h];
+            x.gapless_score = w.hit_gapless[h];
+            x.qstart = x.qend = x.sstart = x.send = -1;
+            x.length = x.mismatch = x.gapopen = -1;
+            const int64_t p = pair_of[hit_base[k] + h];
+            if (p < 0) {
+                // not aligned (gapless score < 40): hit_alloc's fields
+                // (hit.c:15-38) but for bitscore and E-value
+                x.rawscore = -1;
+                x.identity = INFINITY;
+                x.bitscore = 0;
+                x.evalue = 1000;
+                all.push_back(x);
+                continue;
+            }
+            if (sc[p] >= 32767) continue;             // NULL alignment: not aligned, not printed
+            x.rawscore = sc[p];
+            x.send = re[p];
+            x.qend = rd[p];
+            x.length = x.gapopen = x.mismatch = 0;
+            x.identity = 0.0;
+            x.bitscore = (o.lambda * x.rawscore - log(o.k));
+            x.evalue = corrected_q * corrected_db * exp(-x.bitscore);
+            x.evalue = x.evalue > 1.0e-180 ? x.evalue : 0.0;
+            x.bitscore /= M_LN2;
+            all.push_back(x);
+        }
+        // qsort with compare_bitscore (utils.c:140-152): glibc sorts these
+        // pointer arrays by merging, which keeps equal bitscores in order
+        std::stable_sort(all.begin(), all.end(),
+                         [](const tpz_hit& a, const tpz_hit& c) { return a.bitscore > c.bitscore; });
+        if ((int64_t)all.size() > o.num_hits) all.resize((size_t)std::max(o.num_hits, 0));
+        std::vector<tpz_hit>& res = out[k];
+        for (const tpz_hit& x : all)
+            if (x.evalue <= o.evalue) res.push_back(x);
+        if (!o.superfast)
+            for (size_t i = 0; i < res.size(); ++i) stat_hits[k].push_back((int32_t)i);
+    }
+
+    // 6. traceback statistics of the reported hits (query.c:250-299)
+    if (!o.superfast) {
+        std::vector<int32_t> hq, hs, r1, f1, s1;
+        std::vector<std::pair<int32_t, int32_t>> who;
+        std::vector<int8_t> s_res;
+        std::vector<int64_t> s_off;
+        std::vector<int32_t> s_len;
+        for (int32_t k = 0; k < b.n; ++k)
+            for (int32_t i : stat_hits[k]) {
+                const tpz_hit& x = out[k][i];
+                if (x.rawscore <= 0 || x.rawscore >= 32767) continue;
+                const uint64_t pidx = x.db_index;
+                s_off.push_back((int64_t)s_res.size());
+                s_len.push_back(db->len[pidx]);
+                const char* t = db->text + db->start[pidx];
+                for (int32_t j = 0; j < db->len[pidx]; ++j) s_res.push_back(M.idx((unsigned char)t[j]));
+                hq.push_back(k);
+                hs.push_back((int32_t)s_len.size() - 1);
+                r1.push_back(x.qend);
+                f1.push_back(x.send);
+                s1.push_back(x.rawscore);
+                who.emplace_back(k, i);
+            }
+        if (!hq.empty()) {
+            StatsInput in{b.n, q_res.data(), q_off.data(), b.q_len, (int32_t)s_len.size(),
+                          s_res.data(), s_off.data(), s_len.data(), (int32_t)hq.size(),
+                          hq.data(), hs.data(), r1.data(), f1.data(), s1.data(), db->matrix,
+                          db->gap_open, db->gap_extend};
+            StatsOutput so;
+            if (stats_check(in) || stats_compute(in, sl.stats, so)) return -1;
+            for (size_t h = 0; h < who.size(); ++h) {
+                tpz_hit& x = out[who[h].first][who[h].second];
+                x.qstart = so.read_begin[h];
+                x.sstart = so.ref_begin[h];
+                x.length = so.len[h];
+                x.gapopen = so.gapo[h];
+                x.mismatch = so.mis[h];
+                x.identity = stats_identity(so.ident[h], so.len[h]);
+            }
+        }
+        st.stats_s += now_s() - t_sw;
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace tpz
+
+extern "C" {
+
+int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t* sa,
+                  uint64_t n_proteins, const uint64_t* protein_start, const int32_t* protein_len,
+                  const int8_t* matrix, int32_t gap_open, int32_t gap_extend) {
+    if (!out) return fail("tpz_db_create: null output pointer");
+    *out = nullptr;
+    if (ensure_init()) return -1;
+    if (!text || !sa || !protein_start || !protein_len || !matrix || t_size == 0)
+        return fail("tpz_db_create: null or empty input");
+    if (!(gap_extend >= 0 && gap_open > gap_extend && gap_open <= 255))
+        return fail("tpz_db_create: gap penalties must satisfy 0 <= gap_extend < gap_open <= 255 "
+                    "(got %d/%d)", gap_open, gap_extend);
+    if (n_proteins > (uint64_t)INT32_MAX) return fail("tpz_db_create: too many proteins");
+    auto db = std::make_unique<tpz_db>();
+    db->t_size = t_size;
+    db->n_proteins = n_proteins;
+    db->text = text;
+    db->sa = sa;
+    db->start.assign(protein_start, protein_start + n_proteins);
+    db->len.assign(protein_len, protein_len + n_proteins);
+    if (!db->proteins.build(t_size, n_proteins, protein_start, protein_len))
+        return fail("tpz_db_create: the proteins do not tile the text (each protein's letters "
+                    "and its newline, back to back from position 0)");
+    memcpy(db->matrix, matrix, 625);
+    db->gap_open = gap_open;
+    db->gap_extend = gap_extend;
+    for (int i = 0; i < 625; ++i) db->max_w = std::max<int32_t>(db->max_w, matrix[i]);
+    if (tpz_text_create(&db->dtext, text, t_size, sa)) return -1;
+    if (db->d_raw.upload(text, t_size)) return -1;
+    db->table.assign(kTableRows * kTableCols, 0);
+    gapless_table(db->dtext->codes, matrix, db->table.data());
+    if (db->d_table.upload(db->table.data(), db->table.size() * 2)) return -1;
+    // the proteins as matrix rows (aa_table2 of the text, ssw_wrapper.c:193-201)
+    const Maps& M = maps();
+    std::vector<int8_t> res(t_size);
+    std::vector<int64_t> off(n_proteins);
+    for (uint64_t k = 0; k < n_proteins; ++k) off[k] = (int64_t)protein_start[k];
+    pool().run((int64_t)((t_size + (1 << 20) - 1) >> 20), [&](int64_t c) {
+        const uint64_t a = (uint64_t)c << 20, e = std::min<uint64_t>(t_size, a + (1 << 20));
+        for (uint64_t i = a; i < e; ++i) res[i] = M.idx((unsigned char)text[i]);
+    });
+    tpz_server_config cfg{};
+    cfg.memory_budget = (uint64_t)16 << 30;
+    cfg.max_inflight = 2;
+    cfg.max_launch_pairs = 64 << 20;
+    cfg.waves_per_group = 4;
+    if (const char* e = getenv("TOPAZ_AMD_SEARCH_WAVES")) cfg.waves_per_group = atoi(e);
+    if (tpz_server_create(&db->sw, (int32_t)n_proteins, res.data(), off.data(), protein_len,
+                          matrix, gap_open, gap_extend, &cfg))
+        return -1;
+    if (check(hipStreamSynchronize(stream()), "database upload")) return -1;
+    *out = db.release();
+    return 0;
+}
+
+void tpz_db_destroy(tpz_db* db) {
+    delete db;
+}
+
+void tpz_search_defaults(tpz_search_options* o) {
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->num_hits = 100;                  // options.h:6-23
+    o->evalue = 1e-3;
+    o->lambda = 0.267;
+    o->k = 0.041;
+    o->H = 0.140;
+    o->min_hit_length = 5;
+    o->number_of_alignments = 5000;
+    o->number_of_seeds = 300000;
+    o->superfast = 1;
+    o->first_chunk = 32;
+    o->max_stream_elements = 1 << 20;
+}
+
+int tpz_search(tpz_db* db, const tpz_search_options* opt, int32_t n_queries, const char* q_chars,
+               const int64_t* q_off, const int32_t* q_len, int32_t schedule,
+               tpz_search_result** out) {
+    if (!db || !opt || !out) return fail("tpz_search: null argument");
+    *out = nullptr;
+    if (n_queries < 0) return fail("tpz_search: negative query count");
+    if (n_queries > 0 && (!q_chars || !q_off || !q_len)) return fail("tpz_search: null queries");
+    if (schedule != TPZ_SCHED_PLAIN && schedule != TPZ_SCHED_PERSISTENT)
+        return fail("tpz_search: unknown schedule %d", schedule);
+    if (opt->min_hit_length < 1 || opt->number_of_alignments < 0 || opt->number_of_seeds < 0 ||
+        opt->first_chunk < 1 || opt->max_stream_elements < 1)
+        return fail("tpz_search: invalid options");
+    for (int32_t k = 0; k < n_queries; ++k)
+        if (q_len[k] < 0) return fail("tpz_search: query %d has negative length", k);
+    Options o{*opt};
+    auto res = std::make_unique<tpz_search_result>();
+    res->hits.resize(n_queries);
+    Slot* sl = take_slot(db);
+    if (!sl) return -1;
+    const double t0 = now_s();
+    int rc = 0;
+    if (schedule == TPZ_SCHED_PERSISTENT) {
+        rc = search_batch(db, o, *sl, Batch{n_queries, q_chars, q_off, q_len}, res->hits,
+                          res->stats);
+    } else {
+        for (int32_t k = 0; k < n_queries && !rc; ++k) {
+            std::vector<std::vector<tpz_hit>> one;
+            rc = search_batch(db, o, *sl, Batch{1, q_chars, q_off + k, q_len + k}, one,
+                              res->stats);
+            if (!rc) res->hits[k] = std::move(one[0]);
+        }
+    }
+    res->stats.total_s = now_s() - t0;
+    give_slot(db, sl);
+    if (rc) return -1;
+    *out = res.release();
+    return 0;
+}
+
+int32_t tpz_search_result_count(const tpz_search_result* r, int32_t query) {
+    if (!r || query < 0 || query >= (int32_t)r->hits.size()) return -1;
+    return (int32_t)r->hits[query].size();
+}
+
+const tpz_hit* tpz_search_result_hits(const tpz_search_result* r, int32_t query) {
+    if (!r || query < 0 || query >= (int32_t)r->hits.size()) return nullptr;
+    return r->hits[query].data();
+}
+
+int tpz_search_result_stats(const tpz_search_result* r, tpz_search_stats* st) {
+    if (!r || !st) return fail("null argument");
+    *st = r->stats;
+    return 0;
+}
+
+void tpz_search_result_destroy(tpz_search_result* r) { delete r; }
+
+}  // extern "C"
