@@ -17,7 +17,8 @@ def declared_functions():
     for h in sorted((ROOT / "include").glob("*.h")):
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
-        for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text):
+        # (a function-pointer typedef, "type (*name)(...)", declares no function)
+        for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\((?!\s*\*)[^;{]*\)\s*;", text):
             name = m.group(1)
             if name in ("if", "for", "while", "return", "sizeof"):
                 continue
