@@ -494,6 +494,118 @@ def sa_leg(args, t_residues=100_000_000):
                                 (round(float(x), 4) for x in np.mean(phases, axis=0))))}
 
 
+def _per_query(text):
+    """A search's output as {query id: its lines in printed order}: the
+    reference prints finished queries' blocks in varying order."""
+    blocks = {}
+    for line in text.splitlines():
+        f = line.split("\t")
+        if len(f) >= 2:
+            blocks.setdefault(f[0], []).append(line)
+    return blocks
+
+
+def _run_search(exe, work, prefix, qfa, flags, threads, env, label):
+    """One `topaz search`: wall seconds, TOPAZ's own elapsed time, stdout, and
+    the library's totals when the binary prints them."""
+    import subprocess
+    t0 = time.perf_counter()
+    r = subprocess.run([str(exe), "search", "-f", str(qfa), "-p", str(prefix), "-T", str(threads)]
+                       + flags, capture_output=True, text=True, cwd=str(work),
+                       env=dict(os.environ, **env), timeout=1500)
+    wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise SystemExit("%s failed: %s" % (label, r.stderr[-3000:]))
+    el = [ln for ln in r.stderr.split("\n") if ln.startswith("elapsed time:")]
+    st = [ln for ln in r.stderr.split("\n") if ln.startswith("tpz-stats ")]
+    up = [ln for ln in r.stderr.split("\n") if ln.startswith("tpz: database resident")]
+    info = {"wall_s": round(wall, 3),
+            "topaz_elapsed_s": float(el[-1].split()[2]) if el else None,
+            "db_upload_s": float(up[-1].split()[-2]) if up else None}
+    if st:
+        info["library"] = json.loads(st[-1][len("tpz-stats "):])
+    print("[bench] search %s: %.2f s wall (%s)" % (label, wall, info["topaz_elapsed_s"]),
+          file=sys.stderr, flush=True)
+    return r.stdout, info
+
+
+def search_leg(args):
+    """End-to-end `topaz search` on a synthetic database: the reference
+    binary on the CPU (all host cores) against TOPAZ with its workers
+    searching batches of queries through tpz_search (the GPU), persistent
+    and plain schedules; hit lists compared per query on the sample the CPU
+    searches.  The index is written by topaz_amd.index.write_index (the
+    files `topaz index` writes, suffix array built on the GPU)."""
+    import shutil
+    import tempfile
+    from pathlib import Path
+    from tests.test_dropin_gpu import REF, _link_engine
+    from topaz_amd import index, synth
+    n_db, n_q = args.search_db, args.search_queries
+    sample = min(args.search_sample, n_q)
+    hi = 2000 if n_db <= 100000 else 5000
+    hom = 0.2 if n_db <= 100000 else 0.1
+    work = Path(tempfile.mkdtemp(prefix="tpz_search_", dir=args.workdir))
+    out = {"workload": "search: %d database proteins (lognormal mean 350 sd 250 clipped [30,%d]), "
+                       "%d queries, %d%% with 1-5 planted homologs at 30-90%% identity; -H 100 "
+                       "-E 1e-3, default and -F" % (n_db, hi, n_q, int(hom * 100)),
+           "proteins": n_db, "queries": n_q, "sample": sample}
+    try:
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(2040)
+        text, lens, q = synth.search_text(rng, n_db, n_q, hom_frac=hom, hi=hi)
+        out["text_bytes"] = int(len(text))
+        t1 = time.perf_counter()
+        print("[bench] search: %d-byte text generated in %.1f s" % (len(text), t1 - t0),
+              file=sys.stderr, flush=True)
+        st = index.write_index(work / "DB", text, lens, ["s%d" % k for k in range(len(lens))])
+        out["index"] = {"write_s": round(time.perf_counter() - t1, 2), "sa_rounds": st.rounds,
+                        "sa_s": round(st.alloc_s + st.sort_s + st.readback_s, 2)}
+        del text
+        print("[bench] search: index written in %.1f s" % out["index"]["write_s"],
+              file=sys.stderr, flush=True)
+        synth.write_queries_fasta(work / "q.fa", q)
+        qs = (q[0], q[1][:sample], q[2][:sample])
+        synth.write_queries_fasta(work / "qs.fa", qs)
+        exe = _link_engine(work)
+        cores = _cores()
+        batch = str(max(1, min(2048, (n_q + 3) // 4)))
+        gpu_env = {"TOPAZ_AMD_SEARCH_STATS": "1", "TOPAZ_AMD_SEARCH_BATCH": batch}
+        for mode, flags in (("default", []), ("fast", ["-F"])):
+            f = ["-H", "100", "-E", "0.001"] + flags
+            ref_txt, ref_i = _run_search(REF / "topaz", work, work / "DB", work / "qs.fa", f, cores,
+                                         {}, "%s reference_cpu x%d" % (mode, sample))
+            gpu_txt, gpu_i = _run_search(exe, work, work / "DB", work / "qs.fa", f, 4, gpu_env,
+                                         "%s gpu x%d" % (mode, sample))
+            pl_txt, pl_i = _run_search(exe, work, work / "DB", work / "qs.fa", f, 1,
+                                       dict(gpu_env, TOPAZ_AMD_SEARCH_PLAIN="1"),
+                                       "%s gpu plain x%d" % (mode, sample))
+            ref_b = _per_query(ref_txt)
+            row = {"reference_cpu": dict(ref_i, cores=cores,
+                                         queries_per_s=round(sample / ref_i["wall_s"], 2)),
+                   "gpu_sample": dict(gpu_i, queries_per_s=round(sample / gpu_i["wall_s"], 2)),
+                   "gpu_plain_sample": dict(pl_i, queries_per_s=round(sample / pl_i["wall_s"], 2)),
+                   "identical": _per_query(gpu_txt) == ref_b and _per_query(pl_txt) == ref_b,
+                   "hit_lines": sum(len(v) for v in ref_b.values())}
+            if n_q > sample:
+                _, all_i = _run_search(exe, work, work / "DB", work / "q.fa", f, 4, gpu_env,
+                                       "%s gpu x%d" % (mode, n_q))
+                row["gpu_all"] = dict(all_i, queries_per_s=round(n_q / all_i["wall_s"], 2))
+            big = row.get("gpu_all", row["gpu_sample"])
+            lib = big.get("library") or {}
+            if lib.get("align_s"):
+                row["sw_gcups_stage_wall"] = round(lib["cells"] / lib["align_s"] / 1e9, 2)
+            row["speedup_vs_reference"] = round(big["queries_per_s"] /
+                                                row["reference_cpu"]["queries_per_s"], 2)
+            row["batched_vs_plain"] = round(row["gpu_sample"]["queries_per_s"] /
+                                            row["gpu_plain_sample"]["queries_per_s"], 2)
+            out[mode] = row
+    finally:
+        if not args.keep:
+            shutil.rmtree(work, ignore_errors=True)
+    return out
+
+
 def copy_leg(args):
     """A plain device-to-device copy of 1 GiB, three times: the calibration
     run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch of
@@ -523,8 +635,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
-    ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "stats", "sa", "copy"],
-                    default="all")
+    ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "stats", "sa", "copy",
+                                      "search"], default="all")
+    ap.add_argument("--search-db", type=int, default=100000,
+                    help="database proteins of the search leg")
+    ap.add_argument("--search-queries", type=int, default=1000)
+    ap.add_argument("--search-sample", type=int, default=1000,
+                    help="queries the reference CPU binary (and the plain schedule) search")
+    ap.add_argument("--workdir", default=None, help="where the search leg writes its index")
+    ap.add_argument("--keep", action="store_true", help="keep the search leg's files")
     ap.add_argument("--serve-waves", type=int, default=8, choices=[4, 8],
                     help="wavefronts per workgroup of the serving loop's launches")
     ap.add_argument("--serve-scale", type=float, default=1.0,
@@ -538,6 +657,10 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.leg == "search":
+        print(json.dumps({"metric": "topaz search queries/s, GPU vs reference CPU",
+                          "search": search_leg(args)}), flush=True)
+        return
     if args.leg == "copy":
         print(json.dumps(copy_leg(args)), flush=True)
         return

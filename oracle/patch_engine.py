@@ -14,6 +14,8 @@ time -- waiting for that many unless the input is all read -- and hands them
 to one tpz_search call, which returns each query's hits in printed order.
 With TOPAZ_AMD_SEARCH_PLAIN set the batch runs one query at a time
 (TPZ_SCHED_PLAIN).  The database goes to the GPU once, on the first batch.
+With TOPAZ_AMD_SEARCH_STATS set, each batch prints its time and the
+library's running totals (tpz_search_stats) to stderr.
 
 The function is found by its definition, not by line number, and replaced
 up to its matching closing brace.  The patched text goes to stdout only:
@@ -27,6 +29,14 @@ HEADER = """
 
 static pthread_mutex_t tpz_db_mutex = PTHREAD_MUTEX_INITIALIZER;
 static tpz_db* tpz_resident_db = NULL;
+static tpz_search_stats tpz_total;      /* summed over batches, printed with TOPAZ_AMD_SEARCH_STATS */
+static int tpz_batches = 0;
+
+static double tpz_now(void) {
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec + tv.tv_usec * 1e-6;
+}
 
 /* the database and its scoring, on the device once for the whole search */
 static tpz_db* tpz_database(search_t* s) {
@@ -40,11 +50,13 @@ static tpz_db* tpz_database(search_t* s) {
             start[k] = db->index[k].protein_start;
             len[k] = db->index[k].protein_len;
         }
+        double t0 = tpz_now();
         if(tpz_db_create(&tpz_resident_db, db->T, db->Tsize, db->SA, db->Psize, start, len,
                          *s->opt->substitution, s->opt->gap_open, s->opt->gap_extend) != 0) {
             fprintf(stderr, "Error: %s\\n", tpz_last_error());
             _exit(EXIT_FAILURE);
         }
+        fprintf(stderr, "tpz: database resident on the device in %.2f s\\n", tpz_now() - t0);
         free(start);
         free(len);
     }
@@ -92,6 +104,43 @@ static void tpz_search_queries(search_t* s, query_t** qs, int n) {
                   &r) != 0) {
         fprintf(stderr, "Error: %s\\n", tpz_last_error());
         _exit(EXIT_FAILURE);
+    }
+
+    if(getenv("TOPAZ_AMD_SEARCH_STATS")) {
+        tpz_search_stats st;
+        tpz_search_result_stats(r, &st);
+        pthread_mutex_lock(&tpz_db_mutex);
+        tpz_total.offsets += st.offsets;
+        tpz_total.streams += st.streams;
+        tpz_total.elements += st.elements;
+        tpz_total.rounds += st.rounds;
+        tpz_total.pops += st.pops;
+        tpz_total.runs += st.runs;
+        tpz_total.refills += st.refills;
+        tpz_total.hits += st.hits;
+        tpz_total.pairs += st.pairs;
+        tpz_total.cells += st.cells;
+        tpz_total.gpu_s += st.gpu_s;
+        tpz_total.seed_s += st.seed_s;
+        tpz_total.walk_s += st.walk_s;
+        tpz_total.align_s += st.align_s;
+        tpz_total.stats_s += st.stats_s;
+        tpz_total.total_s += st.total_s;
+        ++tpz_batches;
+        fprintf(stderr, "tpz: batch %d of %d queries in %.2f s\\n", tpz_batches, n, st.total_s);
+        fprintf(stderr, "tpz-stats {\\"batches\\": %d, \\"offsets\\": %lld, \\"streams\\": %lld, "
+                "\\"elements\\": %lld, \\"rounds\\": %lld, \\"pops\\": %lld, \\"runs\\": %lld, "
+                "\\"refills\\": %lld, \\"hits\\": %lld, \\"pairs\\": %lld, \\"cells\\": %lld, "
+                "\\"gpu_s\\": %.4f, \\"seed_s\\": %.4f, \\"walk_s\\": %.4f, \\"align_s\\": %.4f, "
+                "\\"stats_s\\": %.4f, \\"total_s\\": %.4f}\\n", tpz_batches,
+                (long long) tpz_total.offsets, (long long) tpz_total.streams,
+                (long long) tpz_total.elements, (long long) tpz_total.rounds,
+                (long long) tpz_total.pops, (long long) tpz_total.runs,
+                (long long) tpz_total.refills, (long long) tpz_total.hits,
+                (long long) tpz_total.pairs, (long long) tpz_total.cells, tpz_total.gpu_s,
+                tpz_total.seed_s, tpz_total.walk_s, tpz_total.align_s, tpz_total.stats_s,
+                tpz_total.total_s);
+        pthread_mutex_unlock(&tpz_db_mutex);
     }
 
     subject = seq_alloc();

@@ -328,3 +328,128 @@ def gapless_scan_bytes(seeds, q_len, text, nl, rank=None):
     Lw = np.minimum(np.minimum(idx, o), 128)
     left = np.maximum(np.minimum(Lw - 1, idx - prv), 0)
     return right, left
+
+
+_LETTERS = np.frombuffer(MATRIX_ORDER.encode(), np.uint8)
+
+
+def _write_fasta_flat(f, res, lens, names_from, prefix, chunk=1 << 26):
+    """FASTA (60 letters per line) of sequences stored back to back in `res`
+    with lengths `lens`, written in chunks of about `chunk` residues with
+    numpy: sequence k is named prefix + str(names_from + k)."""
+    lens = np.asarray(lens, np.int64)
+    ends = np.cumsum(lens)
+    k0 = 0
+    while k0 < len(lens):
+        r0 = int(ends[k0 - 1]) if k0 else 0
+        k1 = max(k0 + 1, int(np.searchsorted(ends, r0 + chunk, side="right")))
+        k1 = min(k1, len(lens))
+        L = lens[k0:k1]
+        n_res = int(L.sum())
+        heads = "".join(">%s%d\n" % (prefix, names_from + k) for k in range(k0, k1)).encode()
+        hl = np.array([len(">%s%d\n" % (prefix, names_from + k)) for k in range(k0, k1)],
+                      np.int64)
+        nlines = (L + 59) // 60
+        rec = hl + L + nlines
+        base = np.zeros(len(L), np.int64)
+        base[1:] = np.cumsum(rec[:-1])
+        out = np.full(int(rec.sum()), ord("\n"), np.uint8)
+        # residue g of the chunk goes to g + (bytes of headers and line ends
+        # before it): a prefix sum over a sparse increment array
+        start = np.zeros(len(L), np.int64)
+        start[1:] = np.cumsum(L[:-1])
+        inc = np.zeros(n_res + 1, np.int32)
+        inc[start] = ((base - start) + hl - np.concatenate(
+            [[0], base[:-1] - start[:-1] + hl[:-1] + (L[:-1] - 1) // 60])).astype(np.int32)
+        # one byte more at every line start after the first of a sequence
+        extra_lines = nlines - 1
+        line_seq = np.repeat(np.arange(len(L)), extra_lines)
+        t = np.arange(int(extra_lines.sum())) - np.repeat(np.cumsum(extra_lines) - extra_lines,
+                                                          extra_lines)
+        inc[start[line_seq] + 60 * (t + 1)] = 1       # (line starts never hit a sequence start)
+        dest = np.cumsum(inc[:n_res], dtype=np.int64)
+        dest += np.arange(n_res, dtype=np.int64)
+        out[dest] = _LETTERS[res[r0:r0 + n_res].astype(np.int64)]
+        # headers
+        hpos = np.repeat(base, hl) + (np.arange(int(hl.sum())) -
+                                       np.repeat(np.cumsum(hl) - hl, hl))
+        out[hpos] = np.frombuffer(heads, np.uint8)
+        f.write(out.tobytes())
+        k0 = k1
+
+
+def write_search_fasta(db_path, q_path, rng, n_db, n_queries, hom_frac=0.1, max_hom=5,
+                       ident=(0.3, 0.9), mean=350.0, sd=250.0, lo=30, hi=2000):
+    """The end-to-end search workload written straight to FASTA, vectorised
+    for databases of millions of proteins: n_queries i.i.d. queries, a
+    fraction hom_frac of them with U{1..max_hom} planted homologs (identity
+    uniform in `ident`, 2 % indels) placed at random database positions, the
+    other database proteins i.i.d.  Returns (database residues, queries
+    residues) as (flat, offsets, lengths) triples."""
+    ql = lognormal_lengths(rng, n_queries, mean, sd, lo, hi)
+    qr, qo, qn = concat_random(rng, ql)
+    bearing = np.nonzero(rng.random(n_queries) < hom_frac)[0]
+    reps = rng.integers(1, max_hom + 1, len(bearing))
+    src = np.repeat(bearing, reps)
+    hr, ho, hn = mutate(rng, qr, qo[src], qn[src], rng.uniform(ident[0], ident[1], len(src)))
+    n = max(n_db, len(src))
+    lens = lognormal_lengths(rng, n, mean, sd, lo, hi).astype(np.int64)
+    slots = rng.permutation(n)[:len(src)]
+    lens[slots] = hn
+    off = np.zeros(n, np.int64)
+    off[1:] = np.cumsum(lens[:-1])
+    res = random_residues(rng, int(lens.sum()))
+    # the homologs' residues into their slots
+    seg = np.repeat(np.arange(len(src)), hn)
+    j = np.arange(int(hn.sum())) - np.repeat(np.cumsum(hn) - hn, hn)
+    res[np.repeat(off[slots], hn) + j] = hr[np.repeat(ho, hn) + j]
+    del seg
+    with open(db_path, "wb") as f:
+        _write_fasta_flat(f, res, lens, 0, "s")
+    with open(q_path, "wb") as f:
+        _write_fasta_flat(f, qr, qn, 0, "q")
+    return (res, off, lens), (qr, qo, qn)
+
+
+def search_text(rng, n_db, n_queries, hom_frac=0.1, max_hom=5, ident=(0.3, 0.9), mean=350.0,
+                sd=250.0, lo=30, hi=2000):
+    """The end-to-end search workload as the index's text directly (for
+    databases of millions of proteins): the .psq text of n_db proteins
+    (internal letters, '\n' after each; i.i.d. background residues, with
+    U{1..max_hom} planted homologs of a fraction hom_frac of the queries at
+    random positions, identity uniform in `ident`, 2 % indels), their
+    lengths, and the queries as (flat residues, offsets, lengths)."""
+    from .index import INTERNAL_OF_INDEX
+    global _BG_TABLE
+    ql = lognormal_lengths(rng, n_queries, mean, sd, lo, hi)
+    qr, qo, qn = concat_random(rng, ql)
+    bearing = np.nonzero(rng.random(n_queries) < hom_frac)[0]
+    reps = rng.integers(1, max_hom + 1, len(bearing))
+    src = np.repeat(bearing, reps)
+    hr, ho, hn = mutate(rng, qr, qo[src], qn[src], rng.uniform(ident[0], ident[1], len(src)))
+    n = max(n_db, len(src))
+    lens = lognormal_lengths(rng, n, mean, sd, lo, hi).astype(np.int64)
+    slots = rng.permutation(n)[:len(src)]
+    lens[slots] = hn
+    starts = np.cumsum(lens + 1) - (lens + 1)
+    if _BG_TABLE is None:
+        _BG_TABLE = _quantile_table(np.cumsum(background()))
+    letters = INTERNAL_OF_INDEX[_BG_TABLE.astype(np.int64)]
+    total = int(lens.sum()) + n
+    text = np.empty(total, np.uint8)
+    step = 1 << 28
+    for a in range(0, total, step):
+        b = min(total, a + step)
+        text[a:b] = letters[rng.integers(0, 65536, b - a, dtype=np.uint16)]
+    text[starts + lens] = ord("\n")
+    j = np.arange(int(hn.sum())) - np.repeat(np.cumsum(hn) - hn, hn)
+    text[np.repeat(starts[slots], hn) + j] = INTERNAL_OF_INDEX[hr[np.repeat(ho, hn) + j].astype(np.int64)]
+    return text, lens, (qr, qo, qn)
+
+
+def write_queries_fasta(path, queries, prefix="q"):
+    """Queries given as (flat residues, offsets, lengths), stored back to
+    back, to FASTA."""
+    qr, qo, qn = queries
+    with open(path, "wb") as f:
+        _write_fasta_flat(f, qr, qn, 0, prefix)
