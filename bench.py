@@ -571,21 +571,28 @@ def search_leg(args):
         cores = _cores()
         batch = str(max(1, min(2048, (n_q + 3) // 4)))
         gpu_env = {"TOPAZ_AMD_SEARCH_STATS": "1", "TOPAZ_AMD_SEARCH_BATCH": batch}
-        for mode, flags in (("default", []), ("fast", ["-F"])):
+        plain_n = min(args.plain_sample, sample)
+        synth.write_queries_fasta(work / "qp.fa", (q[0], q[1][:plain_n], q[2][:plain_n]))
+        modes = [m for m in (("default", []), ("fast", ["-F"])) if m[0] in args.search_modes]
+        for mode, flags in modes:
             f = ["-H", "100", "-E", "0.001"] + flags
             ref_txt, ref_i = _run_search(REF / "topaz", work, work / "DB", work / "qs.fa", f, cores,
                                          {}, "%s reference_cpu x%d" % (mode, sample))
             gpu_txt, gpu_i = _run_search(exe, work, work / "DB", work / "qs.fa", f, 4, gpu_env,
                                          "%s gpu x%d" % (mode, sample))
-            pl_txt, pl_i = _run_search(exe, work, work / "DB", work / "qs.fa", f, 1,
+            pl_txt, pl_i = _run_search(exe, work, work / "DB", work / "qp.fa", f, 1,
                                        dict(gpu_env, TOPAZ_AMD_SEARCH_PLAIN="1"),
-                                       "%s gpu plain x%d" % (mode, sample))
+                                       "%s gpu plain x%d" % (mode, plain_n))
             ref_b = _per_query(ref_txt)
+            pl_b = _per_query(pl_txt)
             row = {"reference_cpu": dict(ref_i, cores=cores,
                                          queries_per_s=round(sample / ref_i["wall_s"], 2)),
                    "gpu_sample": dict(gpu_i, queries_per_s=round(sample / gpu_i["wall_s"], 2)),
-                   "gpu_plain_sample": dict(pl_i, queries_per_s=round(sample / pl_i["wall_s"], 2)),
-                   "identical": _per_query(gpu_txt) == ref_b and _per_query(pl_txt) == ref_b,
+                   "gpu_plain_sample": dict(pl_i, queries=plain_n,
+                                            queries_per_s=round(plain_n / pl_i["wall_s"], 2)),
+                   "identical": _per_query(gpu_txt) == ref_b and
+                                all(ref_b.get(k) == v for k, v in pl_b.items()) and
+                                all(k in pl_b for k in ref_b if int(k[1:]) < plain_n),
                    "hit_lines": sum(len(v) for v in ref_b.values())}
             if n_q > sample:
                 _, all_i = _run_search(exe, work, work / "DB", work / "q.fa", f, 4, gpu_env,
@@ -604,6 +611,111 @@ def search_leg(args):
         if not args.keep:
             shutil.rmtree(work, ignore_errors=True)
     return out
+
+
+def serve_search_leg(args, rates=(1, 2, 5, 10, 20, 50, 100, 200)):
+    """Serving searches: the search leg's database resident on the device
+    (tpz_search_server), requests of U{1..500} queries arriving as a Poisson
+    process; for each rate, at least --serve-requests requests: p50 / p99
+    latency (submit to hits on the host) and the sustained query rate.
+    Every served request's hits are compared with the same queries searched
+    offline (one tpz_search batch of all the pool's queries)."""
+    import shutil
+    import tempfile
+    import threading
+    from pathlib import Path
+    from topaz_amd import index, matrix, synth
+    from topaz_amd.search import Database, SearchServer
+    n_db, n_q = args.search_db, args.search_queries
+    hi = 2000 if n_db <= 100000 else 5000
+    hom = 0.2 if n_db <= 100000 else 0.1
+    work = Path(args.workdir) / "serve_db" if args.workdir else Path(tempfile.mkdtemp(prefix="tpz_serve_"))
+    work.mkdir(parents=True, exist_ok=True)
+    rng = np.random.default_rng(2040)
+    t0 = time.perf_counter()
+    text, lens, q = synth.search_text(rng, n_db, n_q, hom_frac=hom, hi=hi)
+    st = index.write_index(work / "DB", text, lens, ["s%d" % k for k in range(len(lens))])
+    del text
+    print("[bench] serve-search: index of %d proteins written in %.1f s"
+          % (n_db, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    queries = [bytes(index.INTERNAL_OF_INDEX[q[0][o:o + m].astype(np.int64)])
+               for o, m in zip(q[1], q[2])]
+    db = Database.from_index(work / "DB", matrix())
+    try:
+        prng = np.random.default_rng(2041)
+        pool = []
+        for _ in range(64):
+            k = int(prng.integers(1, 501))
+            pool.append([int(i) for i in prng.integers(0, len(queries), k)])
+        # offline: the pool's distinct queries, in batches of 2048
+        used = sorted({i for r in pool for i in r})
+        offline = {}
+        t1 = time.perf_counter()
+        for c in range(0, len(used), 2048):
+            part = used[c:c + 2048]
+            hits, _ = db.search([queries[i] for i in part])
+            offline.update(zip(part, hits))
+        t_off = time.perf_counter() - t1
+        print("[bench] serve-search: %d queries offline in %.1f s" % (len(used), t_off),
+              file=sys.stderr, flush=True)
+        srv = SearchServer(db, lanes=args.serve_lanes)
+        rows = []
+        identical = True
+        for rate in rates:
+            n_req = args.serve_requests
+            arrivals = np.cumsum(prng.exponential(1.0 / rate, n_req))
+            picks = prng.integers(0, len(pool), n_req)
+            tickets, done = [], []
+            st0 = srv.stats()
+
+            def waiter():
+                k = 0
+                while k < n_req:
+                    if k >= len(tickets):
+                        time.sleep(0.0005)
+                        continue
+                    hits, _, lat = srv.wait(tickets[k])
+                    done.append((picks[k], lat, hits, time.perf_counter()))
+                    k += 1
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            start = time.perf_counter()
+            for a, p in zip(arrivals, picks):
+                delay = start + a - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append(srv.submit([queries[i] for i in pool[p]]))
+            th.join()
+            wall = max(d[3] for d in done) - start
+            st1 = srv.stats()
+            for p, _, hits, _ in done:
+                if hits != [offline[i] for i in pool[p]]:
+                    identical = False
+            lats = np.array([d[1] for d in done])
+            n_queries = int(sum(len(pool[p]) for p in picks))
+            waves = st1["waves"] - st0["waves"]
+            rows.append({"offered_requests_per_s": rate, "requests": n_req,
+                         "offered_queries_per_s": round(n_queries / float(arrivals[-1]), 1),
+                         "queries_per_s": round(n_queries / wall, 1),
+                         "p50_ms": round(float(np.percentile(lats, 50)), 1),
+                         "p99_ms": round(float(np.percentile(lats, 99)), 1),
+                         "waves": waves, "requests_per_wave": round(n_req / max(waves, 1), 2)})
+            print("[bench] serve-search %d/s: %s" % (rate, rows[-1]), file=sys.stderr, flush=True)
+        sst = srv.stats()
+        srv.close()
+    finally:
+        db.close()
+        if not args.workdir and not args.keep:
+            shutil.rmtree(work, ignore_errors=True)
+    return {"workload": "serving searches: %d resident proteins (lognormal mean 350 sd 250 clipped "
+                        "[30,%d]); requests of U{1..500} queries from %d queries, Poisson arrivals, "
+                        "%d requests per rate, %d lanes, budget 16 GiB with the fixed workspace"
+                        % (n_db, hi, n_q, args.serve_requests, args.serve_lanes),
+            "offline_queries_per_s": round(len(used) / t_off, 1),
+            "rates": rows, "identical_to_offline": identical,
+            "peak_bytes": sst["peak_bytes"], "fixed_bytes": sst["fixed_bytes"],
+            "max_requests_per_wave": sst["max_requests_per_wave"]}
 
 
 def copy_leg(args):
@@ -636,14 +748,20 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
     ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "stats", "sa", "copy",
-                                      "search"], default="all")
+                                      "search", "serve-search"], default="all")
     ap.add_argument("--search-db", type=int, default=100000,
                     help="database proteins of the search leg")
     ap.add_argument("--search-queries", type=int, default=1000)
     ap.add_argument("--search-sample", type=int, default=1000,
                     help="queries the reference CPU binary (and the plain schedule) search")
+    ap.add_argument("--plain-sample", type=int, default=1000,
+                    help="queries the plain schedule searches")
+    ap.add_argument("--search-modes", default="default,fast",
+                    help="search modes to run: default and/or fast (-F)")
     ap.add_argument("--workdir", default=None, help="where the search leg writes its index")
     ap.add_argument("--keep", action="store_true", help="keep the search leg's files")
+    ap.add_argument("--serve-requests", type=int, default=100, help="requests per arrival rate")
+    ap.add_argument("--serve-lanes", type=int, default=2, help="batches the search server runs at once")
     ap.add_argument("--serve-waves", type=int, default=8, choices=[4, 8],
                     help="wavefronts per workgroup of the serving loop's launches")
     ap.add_argument("--serve-scale", type=float, default=1.0,
@@ -660,6 +778,10 @@ def main():
     if args.leg == "search":
         print(json.dumps({"metric": "topaz search queries/s, GPU vs reference CPU",
                           "search": search_leg(args)}), flush=True)
+        return
+    if args.leg == "serve-search":
+        print(json.dumps({"metric": "serving searches: p50/p99 latency vs sustained queries/s",
+                          "serving_search": serve_search_leg(args)}), flush=True)
         return
     if args.leg == "copy":
         print(json.dumps(copy_leg(args)), flush=True)

@@ -324,6 +324,44 @@ const tpz_hit* tpz_search_result_hits(const tpz_search_result* r, int32_t query)
 int tpz_search_result_stats(const tpz_search_result* r, tpz_search_stats* st);
 void tpz_search_result_destroy(tpz_search_result* r);
 
+/* Serving searches: continuous batching of search requests over a resident
+ * database.  A request is a set of queries (as for tpz_search); requests
+ * arrive from any thread.  Lanes (threads with their own streams) each take
+ * the waiting requests from the head of the queue while their estimated
+ * device memory fits the budget -- the fixed workspace of the Smith-Waterman
+ * launch slots counted in it -- merge them into one batch and search it;
+ * requests arriving meanwhile go to the next free lane, so batches overlap on
+ * the device and on the host, and the lanes' Smith-Waterman requests share
+ * launches.  Every request's hits are those tpz_search gives for its queries
+ * alone.  No reference counterpart: TOPAZ runs one thread per query over a
+ * length-keyed work queue (work_queue.c:317-352, engine.c:438-468). */
+typedef struct tpz_search_server tpz_search_server;
+typedef struct {
+    uint64_t memory_budget;     /* device bytes for requests in flight plus the fixed
+                                   workspace; 0: 16 GiB                                */
+    int32_t lanes;              /* batches in flight at once; 0: 2                     */
+    int32_t max_wave_queries;   /* queries merged into one batch at most; 0: 4096      */
+} tpz_search_server_config;
+typedef struct {
+    uint64_t requests_done, waves, max_requests_per_wave, queries_done;
+    uint64_t peak_bytes;        /* largest admitted bytes in flight, fixed bytes included */
+    uint64_t fixed_bytes;       /* the fixed workspace counted in the budget             */
+} tpz_search_server_stats;
+int tpz_search_server_create(tpz_search_server** out, tpz_db* db,
+                             const tpz_search_options* opt,
+                             const tpz_search_server_config* cfg);
+/* Queue a request (its queries are copied); fails if it alone needs more
+ * device memory than the budget leaves beside the fixed workspace. */
+int tpz_search_server_submit(tpz_search_server* s, int32_t n_queries, const char* q_chars,
+                             const int64_t* q_off, const int32_t* q_len, uint64_t* ticket);
+/* Block until the request is searched; *out as tpz_search gives it (destroy
+ * with tpz_search_result_destroy), latency from submit to results. */
+int tpz_search_server_wait(tpz_search_server* s, uint64_t ticket, tpz_search_result** out,
+                           double* latency_ms);
+int tpz_search_server_get_stats(tpz_search_server* s, tpz_search_server_stats* st);
+/* Searches the requests still queued, then stops. */
+void tpz_search_server_destroy(tpz_search_server* s);
+
 /* ---- the seed walk of one query (host logic) ---------------------------- */
 
 /* The walk of query_fetch_alignment_batch (query.c:134-174): pqueue_next

@@ -1,0 +1,127 @@
+"""tpz_search through the Python host layer, and the search server.
+
+* Offline: Database.search (persistent and plain schedules) prints, line for
+  line, what the reference binary prints for the same queries with SEG off
+  (-M), default and -B.
+* Serving: requests submitted from several threads at once, merged into
+  shared batches by the server, each get exactly the hits the same queries
+  get searched offline; a request served alone gets them too; the admitted
+  device bytes stay within the budget; a request over the budget is refused.
+"""
+import subprocess
+import threading
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from topaz_amd import TopazError, index, matrix, synth
+from topaz_amd.search import SCHED_PLAIN, Database, SearchServer, format_hits, header_ids
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+REF = ROOT / "oracle" / "_ref"
+
+
+@pytest.fixture(scope="module")
+def db_and_queries(tmp_path_factory):
+    tmp = tmp_path_factory.mktemp("search")
+    rng = np.random.default_rng(61)
+    text, lens, q = synth.search_text(rng, 20000, 120, hom_frac=0.5, hi=2000)
+    index.write_index(tmp / "DB", text, lens, ["s%d" % k for k in range(len(lens))])
+    synth.write_queries_fasta(tmp / "q.fa", q)
+    qs = [bytes(index.INTERNAL_OF_INDEX[q[0][o:o + n].astype(np.int64)])
+          for o, n in zip(q[1], q[2])]
+    db = Database.from_index(tmp / "DB", matrix())
+    yield tmp, db, qs
+    db.close()
+
+
+def _ref_lines(tmp, *flags):
+    if not (REF / "topaz").exists():
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    r = subprocess.run([str(REF / "topaz"), "search", "-f", str(tmp / "q.fa"), "-p",
+                        str(tmp / "DB"), "-T", "4", "-M"] + list(flags), capture_output=True,
+                       text=True, timeout=600, cwd=str(tmp))
+    assert r.returncode == 0
+    blocks = {}
+    for line in r.stdout.splitlines():
+        f = line.split("\t")
+        if len(f) >= 2:
+            blocks.setdefault(f[0], []).append(line)
+    return blocks
+
+
+@pytest.mark.parametrize("blasttab", [False, True])
+def test_offline_search_matches_reference(db_and_queries, blasttab):
+    tmp, db, qs = db_and_queries
+    ref = _ref_lines(tmp, *(["-B"] if blasttab else []))
+    ids = header_ids(tmp / "DB", np.fromfile(str(tmp / "DB.ind"), index.IND_DTYPE))
+    opt = db.default_options(superfast=0 if blasttab else 1)
+    for schedule in (1, SCHED_PLAIN):
+        hits, st = db.search(qs, opt, schedule)
+        ours = {}
+        for k, h in enumerate(hits):
+            lines = format_hits("q%d" % k, h, ids, superfast=not blasttab)
+            if lines:
+                ours["q%d" % k] = lines
+        assert ours == ref
+    assert sum(len(v) for v in ref.values()) > 100 and st["pairs"] > 0
+
+
+def test_server_results_equal_offline(db_and_queries):
+    tmp, db, qs = db_and_queries
+    offline, _ = db.search(qs)
+    rng = np.random.default_rng(7)
+    reqs = []
+    for _ in range(24):
+        n = int(rng.integers(1, 40))
+        reqs.append([int(i) for i in rng.integers(0, len(qs), n)])
+    srv = SearchServer(db, lanes=3, max_wave_queries=64)
+    try:
+        # alone first
+        hits, _, lat = srv.wait(srv.submit([qs[i] for i in reqs[0]]))
+        assert hits == [offline[i] for i in reqs[0]] and lat > 0
+        # then all at once from three threads
+        out = {}
+        errors = []
+
+        def client(part):
+            try:
+                tickets = [(r, srv.submit([qs[i] for i in reqs[r]])) for r in part]
+                for r, t in tickets:
+                    out[r] = srv.wait(t)[0]
+            except Exception as e:      # reported below
+                errors.append(e)
+
+        th = [threading.Thread(target=client, args=(list(range(k, len(reqs), 3)),))
+              for k in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert not errors, errors
+        for r, idx in enumerate(reqs):
+            assert out[r] == [offline[i] for i in idx], r
+        st = srv.stats()
+        assert st["requests_done"] == len(reqs) + 1
+        assert st["max_requests_per_wave"] > 1
+        assert st["peak_bytes"] <= 16 << 30
+    finally:
+        srv.close()
+
+
+def test_server_refuses_request_over_budget(db_and_queries):
+    tmp, db, qs = db_and_queries
+    srv = None
+    try:
+        # a budget barely above the fixed workspace: one long query is too much
+        probe = SearchServer(db, lanes=1)
+        fixed = probe.stats()["fixed_bytes"]
+        probe.close()
+        srv = SearchServer(db, memory_budget=fixed + 4096, lanes=1)
+        with pytest.raises(TopazError):
+            srv.submit([qs[0] * 4])
+    finally:
+        if srv:
+            srv.close()

@@ -102,6 +102,12 @@ struct StreamArgs {
     int16_t* out;               // per slot: the score when length >= min_len, else -1
                                 // (also for a slot outside the suffix array)
     int32_t min_len;
+    // the protein (db_index) of each slot's text position, from a table of
+    // every 64th position's protein and each protein's last position (its
+    // newline); 0xffffffff for a slot outside the suffix array
+    const uint32_t* prot_block;
+    const uint64_t* prot_end;
+    uint32_t* out_protein;
 };
 
 }  // namespace tpz

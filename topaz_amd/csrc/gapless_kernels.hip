@@ -232,13 +232,22 @@ stream_kernel(StreamArgs sa) {
         const int64_t sid = seg.out + lane;
         if (lane < seg.count) {
             const int64_t slot = seg.first + (int64_t)lane * seg.dir;
+            uint32_t prot = 0xffffffffu;
             if (slot >= 0 && slot < a.t_size) {
                 idx = a.sa[slot];
-                if (idx >= 0 && idx < a.t_size && seg.query >= 0 && seg.query < a.n_queries &&
-                    seg.offset >= 0 && seg.offset < a.q_len[seg.query])
-                    pass = 2 * (int)a.t_fwd[idx] ==
-                           (int)a.q_fwd[a.qf_off[seg.query] + seg.offset];
+                if (idx >= 0 && idx < a.t_size) {
+                    // the protein holding idx: the one holding idx & ~63 or a
+                    // later one (the last protein's newline is the text's end)
+                    uint32_t k = sa.prot_block[idx >> 6];
+                    while ((uint64_t)idx > sa.prot_end[k]) ++k;
+                    prot = k;
+                    if (seg.query >= 0 && seg.query < a.n_queries && seg.offset >= 0 &&
+                        seg.offset < a.q_len[seg.query])
+                        pass = 2 * (int)a.t_fwd[idx] ==
+                               (int)a.q_fwd[a.qf_off[seg.query] + seg.offset];
+                }
             }
+            sa.out_protein[sid] = prot;
             if (!pass) sa.out[sid] = -1;
         }
         const uint64_t mask = __ballot(pass);

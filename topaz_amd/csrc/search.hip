@@ -34,11 +34,13 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/topaz_amd.h"
@@ -210,8 +212,8 @@ struct StatsMem final : RunMem {
 struct Slot {
     hipStream_t stream = nullptr;
     DevBuf d_q, d_qoff, d_qlen, d_rq, d_ro, d_idx;
-    DevBuf d_qf, d_qr, d_qf_off, d_qr_off, d_segs, d_out;
-    HostBuf h_stage, h_out, h_idx;
+    DevBuf d_qf, d_qr, d_qf_off, d_qr_off, d_segs, d_out, d_prot;
+    HostBuf h_stage, h_out, h_prot, h_idx;
     StatsMem stats;
     ~Slot() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -234,6 +236,7 @@ struct tpz_db {
     DeviceBuffer d_raw;                  // the text's bytes, for the suffix-array search
     std::vector<int16_t> table;          // gapless scores per (text code, query code)
     DeviceBuffer d_table;
+    DeviceBuffer d_pblock, d_pend;       // the protein index on the device (stream_kernel)
     tpz_server* sw = nullptr;            // stage S over the proteins
     int8_t matrix[625];
     int32_t gap_open = 11, gap_extend = 1, max_w = 1;
@@ -390,7 +393,8 @@ struct ChunkReq {
 
 int run_chunks(tpz_db* db, Slot& sl, const std::vector<StreamDef>& defs,
                const std::vector<ChunkReq>& reqs, int64_t n_out, int32_t n_queries,
-               int32_t min_len, const int16_t** host_out, double* gpu_s) {
+               int32_t min_len, const int16_t** host_out, const uint32_t** host_prot,
+               double* gpu_s) {
     std::vector<StreamSeg> segs;
     segs.reserve((size_t)n_out / 32 + reqs.size());
     for (const ChunkReq& r : reqs) {
@@ -411,7 +415,9 @@ int run_chunks(tpz_db* db, Slot& sl, const std::vector<StreamDef>& defs,
     void* ds = sl.d_segs.get(seg_bytes + 64);
     void* dout = sl.d_out.get((size_t)n_out * 2 + 64);
     int16_t* hout = (int16_t*)sl.h_out.get((size_t)n_out * 2 + 64);
-    if (!hs || !ds || !dout || !hout) return -1;
+    void* dprot = sl.d_prot.get((size_t)n_out * 4 + 64);
+    uint32_t* hprot = (uint32_t*)sl.h_prot.get((size_t)n_out * 4 + 64);
+    if (!hs || !ds || !dout || !hout || !dprot || !hprot) return -1;
     memcpy(hs, segs.data(), seg_bytes);
     const double t0 = now_s();
     if (check(hipMemcpyAsync(ds, hs, seg_bytes, hipMemcpyHostToDevice, sl.stream), "upload"))
@@ -422,15 +428,21 @@ int run_chunks(tpz_db* db, Slot& sl, const std::vector<StreamDef>& defs,
     a.n_segs = (int64_t)segs.size();
     a.out = (int16_t*)dout;
     a.min_len = min_len;
+    a.prot_block = (const uint32_t*)db->d_pblock.ptr;
+    a.prot_end = (const uint64_t*)db->d_pend.ptr;
+    a.out_protein = (uint32_t*)dprot;
     const int grid = (int)std::max<int64_t>(
         1, std::min<int64_t>(((int64_t)segs.size() + 3) / 4, (int64_t)std::max(num_cu(), 1) * 32));
     if (check(tpz_launch_streams(&a, grid, sl.stream), "stream_kernel") ||
         check(hipMemcpyAsync(hout, dout, (size_t)n_out * 2, hipMemcpyDeviceToHost, sl.stream),
               "download") ||
+        check(hipMemcpyAsync(hprot, dprot, (size_t)n_out * 4, hipMemcpyDeviceToHost, sl.stream),
+              "download") ||
         check(hipStreamSynchronize(sl.stream), "streams"))
         return -1;
     *gpu_s += now_s() - t0;
     *host_out = hout;
+    *host_prot = hprot;
     return 0;
 }
 
@@ -446,7 +458,9 @@ struct Options {
 struct QueryWork {
     int32_t first_stream = 0, n_streams = 0;   // range in the batch's stream list
     std::vector<std::vector<int16_t>> vals;     // per stream: elements computed
+    std::vector<std::vector<uint32_t>> prot;    // per stream: their proteins
     std::vector<uint8_t> open;                  // per stream: more elements may exist
+    int32_t first_req = 0, n_req = 0;           // the round's requests of this query
     std::vector<uint64_t> hit_index;            // hit list, traversal order
     std::vector<int32_t> hit_gapless;
     WalkStats walk;
@@ -524,6 +538,7 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         }
         qw[k].n_streams = (int32_t)defs.size() - qw[k].first_stream;
         qw[k].vals.resize(qw[k].n_streams);
+        qw[k].prot.resize(qw[k].n_streams);
         qw[k].open.assign(qw[k].n_streams, 1);
     }
     st.streams += (int64_t)defs.size();
@@ -535,34 +550,41 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
     for (int round = 0;; ++round) {
         std::vector<ChunkReq> reqs;
         int64_t n_out = 0;
-        for (int32_t s = 0; s < (int32_t)defs.size(); ++s) {
-            QueryWork& w = qw[query_of(s)];
-            const int32_t ls = s - w.first_stream;
-            if (!w.open[ls]) continue;
-            const int64_t have = (int64_t)w.vals[ls].size();
-            if (have >= cap_elems) continue;       // left to the walk's refills
-            const int32_t c = (int32_t)std::min<int64_t>(chunk, cap_elems - have);
-            reqs.push_back(ChunkReq{s, defs[s].first + have * defs[s].dir, c, n_out});
-            n_out += c;
+        for (int32_t k = 0; k < b.n; ++k) {
+            QueryWork& w = qw[k];
+            w.first_req = (int32_t)reqs.size();
+            for (int32_t ls = 0; ls < w.n_streams; ++ls) {
+                if (!w.open[ls]) continue;
+                const int32_t s = w.first_stream + ls;
+                const int64_t have = (int64_t)w.vals[ls].size();
+                if (have >= cap_elems) continue;       // left to the walk's refills
+                const int32_t c = (int32_t)std::min<int64_t>(chunk, cap_elems - have);
+                reqs.push_back(ChunkReq{s, defs[s].first + have * defs[s].dir, c, n_out});
+                n_out += c;
+            }
+            w.n_req = (int32_t)reqs.size() - w.first_req;
         }
         if (reqs.empty()) break;
         const int16_t* hv = nullptr;
-        if (run_chunks(db, sl, defs, reqs, n_out, b.n, min_len, &hv, &st.gpu_s)) return -1;
+        const uint32_t* hp = nullptr;
+        if (run_chunks(db, sl, defs, reqs, n_out, b.n, min_len, &hv, &hp, &st.gpu_s)) return -1;
         st.elements += n_out;
         ++st.rounds;
-        for (const ChunkReq& r : reqs) {
-            QueryWork& w = qw[query_of(r.stream)];
-            const int32_t ls = r.stream - w.first_stream;
-            std::vector<int16_t>& v = w.vals[ls];
-            // a stream that is dead at its first slot is never created
-            for (int32_t i = 0; i < r.count; ++i) {
-                v.push_back(hv[r.out + i]);
-                if (hv[r.out + i] == kStreamEnd) {
-                    w.open[ls] = 0;
-                    break;
-                }
+        // append each request's elements up to the stream's first non-seed,
+        // one query per task
+        pool().run(b.n, [&](int64_t kk) {
+            QueryWork& w = qw[kk];
+            for (int32_t e = w.first_req; e < w.first_req + w.n_req; ++e) {
+                const ChunkReq& r = reqs[e];
+                const int32_t ls = r.stream - w.first_stream;
+                const int16_t* src = hv + r.out;
+                const int16_t* end = std::find(src, src + r.count, kStreamEnd);
+                const int32_t take = (int32_t)(end - src) + (end != src + r.count);
+                w.vals[ls].insert(w.vals[ls].end(), src, src + take);
+                w.prot[ls].insert(w.prot[ls].end(), hp + r.out, hp + r.out + take);
+                if (end != src + r.count) w.open[ls] = 0;
             }
-        }
+        });
         chunk = (int32_t)std::min<int64_t>((int64_t)chunk * 2, 1 << 20);
     }
     const double t_streams = now_s();
@@ -581,30 +603,31 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         std::vector<int32_t> local;      // walk stream -> batch stream
         for (int32_t ls = 0; ls < w.n_streams; ++ls) {
             std::vector<int16_t>& v = w.vals[ls];
-            if (v.empty() || v[0] == kStreamEnd) continue;
+            if (v.empty() || v[0] == kStreamEnd) {     // never created (query.c:119, 127)
+                std::vector<int16_t>().swap(v);
+                std::vector<uint32_t>().swap(w.prot[ls]);
+                continue;
+            }
             const StreamDef& d = defs[w.first_stream + ls];
-            walk.add_stream(d.offset, d.first, d.dir, v.data(), (int32_t)v.size(), !w.open[ls]);
+            walk.add_stream(d.offset, d.first, d.dir, std::move(v), std::move(w.prot[ls]),
+                            !w.open[ls]);
             local.push_back(w.first_stream + ls);
-            std::vector<int16_t>().swap(v);
         }
-        const int64_t* sa = db->sa;
-        const ProteinIndex& P = db->proteins;
-        auto protein_of = [&](int64_t slot) { return P.of((uint64_t)sa[slot]); };
+        auto protein_of = [&](int32_t s, int32_t i, int64_t) { return walk.protein(s, i); };
         std::vector<int32_t> want(local.size(), 0);
         while (walk.run(protein_of) == Walk::kNeed) {
             const int32_t ws = walk.waiting();
             const int32_t n = want[ws] = std::max(2 * want[ws], 4096);
-            const StreamDef& d = defs[local[ws]];
             std::vector<ChunkReq> one{ChunkReq{local[ws], walk.next_slot(ws), n, 0}};
             const int16_t* hv = nullptr;
+            const uint32_t* hp = nullptr;
             std::lock_guard<std::mutex> lk(gpu_mu);
             double dummy = 0;
-            if (run_chunks(db, sl, defs, one, n, b.n, min_len, &hv, &dummy)) {
+            if (run_chunks(db, sl, defs, one, n, b.n, min_len, &hv, &hp, &dummy)) {
                 walk_rc = -1;
                 return;
             }
-            (void)d;
-            walk.supply(ws, hv, n, false);
+            walk.supply(ws, hv, hp, n, false);
         }
         walk.hits().traversal(w.hit_index, w.hit_gapless);
         w.walk = walk.stats();
@@ -787,6 +810,9 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     db->gap_extend = gap_extend;
     for (int i = 0; i < 625; ++i) db->max_w = std::max<int32_t>(db->max_w, matrix[i]);
     if (tpz_text_create(&db->dtext, text, t_size, sa)) return -1;
+    if (db->d_pblock.upload(db->proteins.blocks().data(), db->proteins.blocks().size() * 4) ||
+        db->d_pend.upload(db->proteins.ends().data(), db->proteins.ends().size() * 8))
+        return -1;
     if (db->d_raw.upload(text, t_size)) return -1;
     db->table.assign(kTableRows * kTableCols, 0);
     gapless_table(db->dtext->codes, matrix, db->table.data());
@@ -890,5 +916,251 @@ int tpz_search_result_stats(const tpz_search_result* r, tpz_search_stats* st) {
 }
 
 void tpz_search_result_destroy(tpz_search_result* r) { delete r; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving: continuous batching of search requests over a resident database
+// ---------------------------------------------------------------------------
+namespace tpz {
+namespace {
+
+struct SearchRequest {
+    uint64_t id = 0;
+    std::vector<char> chars;
+    std::vector<int64_t> off;
+    std::vector<int32_t> len;
+    int64_t offsets = 0;            // query offsets, for the memory estimate
+    size_t bytes = 0;               // device bytes admitted for it
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+    int rc = 0;
+    std::string err;
+    std::unique_ptr<tpz_search_result> res;
+};
+
+}  // namespace
+}  // namespace tpz
+
+struct tpz_search_server {
+    tpz_db* db = nullptr;
+    Options opt{};
+    tpz_search_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<tpz::SearchRequest*> pending;
+    std::unordered_map<uint64_t, tpz::SearchRequest*> reqs;
+    size_t in_use = 0, fixed = 0;
+    double elements_per_offset = 64;      // running estimate from finished waves
+    uint64_t next_id = 1;
+    bool stop = false;
+    tpz_search_server_stats stats{};
+    std::vector<std::thread> lanes;
+};
+
+namespace tpz {
+namespace {
+
+// Device bytes a request occupies while its wave runs: query copies and
+// offset records, the walk streams' elements (estimated per offset from
+// the waves so far) and segments, and its share of the Smith-Waterman
+// request (pairs of at most a full hit list per query).
+size_t request_device_bytes(const tpz_search_server* s, int64_t residues, int64_t n_queries,
+                            int64_t offsets) {
+    const double per_offset = 16.0 + 2.5 * s->elements_per_offset;
+    const int64_t hits = (int64_t)std::max(s->opt.o.number_of_alignments,
+                                           (int32_t)(1.5 * s->opt.o.num_hits));
+    return (size_t)(4 * residues + per_offset * (double)offsets + 48.0 * hits * n_queries) + 4096;
+}
+
+void search_lane(tpz_search_server* s) {
+    Slot* sl = take_slot(s->db);
+    for (;;) {
+        std::vector<SearchRequest*> wave;
+        size_t bytes = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (s->stop && s->pending.empty()) {
+                    lk.unlock();
+                    if (sl) give_slot(s->db, sl);
+                    return;
+                }
+                // admit from the head while the memory and the wave size allow;
+                // a lane alone on the device always takes the head
+                int64_t queries = 0;
+                while (!s->pending.empty()) {
+                    SearchRequest* r = s->pending.front();
+                    const bool fits = s->in_use + bytes + r->bytes <= s->cfg.memory_budget - s->fixed;
+                    const bool room = queries + (int64_t)r->len.size() <= s->cfg.max_wave_queries;
+                    if (!wave.empty() && (!fits || !room)) break;
+                    if (wave.empty() && !fits && s->in_use > 0) break;
+                    wave.push_back(r);
+                    bytes += r->bytes;
+                    queries += (int64_t)r->len.size();
+                    s->pending.pop_front();
+                }
+                if (!wave.empty()) break;
+                s->cv_work.wait(lk);
+            }
+            s->in_use += bytes;
+            s->stats.peak_bytes = std::max<uint64_t>(s->stats.peak_bytes, s->in_use + s->fixed);
+        }
+        // one merged batch
+        std::vector<char> chars;
+        std::vector<int64_t> off;
+        std::vector<int32_t> len;
+        std::vector<size_t> first;
+        for (SearchRequest* r : wave) {
+            first.push_back(len.size());
+            const int64_t base = (int64_t)chars.size();
+            chars.insert(chars.end(), r->chars.begin(), r->chars.end());
+            for (size_t k = 0; k < r->len.size(); ++k) {
+                off.push_back(base + r->off[k]);
+                len.push_back(r->len[k]);
+            }
+        }
+        first.push_back(len.size());
+        std::vector<std::vector<tpz_hit>> hits;
+        tpz_search_stats st{};
+        int rc = sl ? search_batch(s->db, s->opt, *sl,
+                                   Batch{(int32_t)len.size(), chars.data(), off.data(), len.data()},
+                                   hits, st)
+                    : -1;
+        const std::string err = rc ? std::string(tpz_last_error()) : std::string();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->in_use -= bytes;
+            const double t = now_s() * 1e3;
+            int64_t n_off = 0;
+            for (size_t w = 0; w < wave.size(); ++w) {
+                SearchRequest* r = wave[w];
+                n_off += r->offsets;
+                r->rc = rc;
+                r->err = err;
+                if (!rc) {
+                    r->res = std::make_unique<tpz_search_result>();
+                    for (size_t k = first[w]; k < first[w + 1]; ++k)
+                        r->res->hits.push_back(std::move(hits[k]));
+                    r->res->stats = st;
+                }
+                r->t_done = t;
+                r->done = true;
+            }
+            if (!rc && n_off > 0)
+                s->elements_per_offset = 0.5 * s->elements_per_offset +
+                                         0.5 * (double)st.elements / (double)n_off;
+            s->stats.requests_done += wave.size();
+            s->stats.waves += 1;
+            s->stats.queries_done += len.size();
+            s->stats.max_requests_per_wave =
+                std::max<uint64_t>(s->stats.max_requests_per_wave, wave.size());
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+}  // namespace tpz
+
+extern "C" {
+
+int tpz_search_server_create(tpz_search_server** out, tpz_db* db,
+                             const tpz_search_options* opt,
+                             const tpz_search_server_config* cfg) {
+    if (!out || !db || !opt) return fail("tpz_search_server_create: null argument");
+    *out = nullptr;
+    if (opt->min_hit_length < 1 || opt->number_of_alignments < 0 || opt->number_of_seeds < 0 ||
+        opt->first_chunk < 1 || opt->max_stream_elements < 1)
+        return fail("tpz_search_server_create: invalid options");
+    auto s = std::make_unique<tpz_search_server>();
+    s->db = db;
+    s->opt = Options{*opt};
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.memory_budget == 0) s->cfg.memory_budget = (uint64_t)16 << 30;
+    if (s->cfg.lanes <= 0) s->cfg.lanes = 2;
+    if (s->cfg.max_wave_queries <= 0) s->cfg.max_wave_queries = 4096;
+    // fixed device memory: the Smith-Waterman launch slots' workspace
+    tpz_server_stats sws{};
+    if (tpz_server_get_stats(db->sw, &sws)) return -1;
+    s->fixed = (size_t)sws.workspace_bytes;
+    s->stats.fixed_bytes = s->fixed;
+    if (s->fixed >= s->cfg.memory_budget)
+        return fail("tpz_search_server_create: the budget of %llu bytes does not cover the %zu "
+                    "bytes of fixed workspace", (unsigned long long)s->cfg.memory_budget, s->fixed);
+    tpz_search_server* raw = s.release();
+    for (int k = 0; k < raw->cfg.lanes; ++k) raw->lanes.emplace_back(search_lane, raw);
+    *out = raw;
+    return 0;
+}
+
+int tpz_search_server_submit(tpz_search_server* s, int32_t n_queries, const char* q_chars,
+                             const int64_t* q_off, const int32_t* q_len, uint64_t* ticket) {
+    if (!s || !ticket) return fail("tpz_search_server_submit: null argument");
+    if (n_queries < 0 || (n_queries > 0 && (!q_chars || !q_off || !q_len)))
+        return fail("tpz_search_server_submit: bad queries");
+    auto r = std::make_unique<SearchRequest>();
+    const Maps& M = maps();
+    for (int32_t k = 0; k < n_queries; ++k) {
+        if (q_len[k] < 0) return fail("query %d has negative length", k);
+        r->off.push_back((int64_t)r->chars.size());
+        r->len.push_back(q_len[k]);
+        r->chars.insert(r->chars.end(), q_chars + q_off[k], q_chars + q_off[k] + q_len[k]);
+        for (int32_t i = 0; i + s->opt.o.min_hit_length < q_len[k]; ++i)
+            r->offsets += M.upper_only[(unsigned char)q_chars[q_off[k] + i]] != 'V';
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->bytes = request_device_bytes(s, (int64_t)r->chars.size(), n_queries, r->offsets);
+    if (r->bytes > s->cfg.memory_budget - s->fixed)
+        return fail("tpz_search_server_submit: the request needs about %zu device bytes, more "
+                    "than the %llu the budget leaves beside the fixed workspace", r->bytes,
+                    (unsigned long long)(s->cfg.memory_budget - s->fixed));
+    if (s->stop) return fail("tpz_search_server_submit: server stopped");
+    r->id = s->next_id++;
+    r->t_submit = now_s() * 1e3;
+    *ticket = r->id;
+    s->reqs[r->id] = r.get();
+    s->pending.push_back(r.release());
+    s->cv_work.notify_one();
+    return 0;
+}
+
+int tpz_search_server_wait(tpz_search_server* s, uint64_t ticket, tpz_search_result** out,
+                           double* latency_ms) {
+    if (!s || !out) return fail("tpz_search_server_wait: null argument");
+    *out = nullptr;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail("unknown ticket %llu", (unsigned long long)ticket);
+    SearchRequest* r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->reqs.erase(it);
+    lk.unlock();
+    std::unique_ptr<SearchRequest> own(r);
+    if (r->rc) return fail("%s", r->err.c_str());
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    *out = r->res.release();
+    return 0;
+}
+
+int tpz_search_server_get_stats(tpz_search_server* s, tpz_search_server_stats* st) {
+    if (!s || !st) return fail("null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *st = s->stats;
+    return 0;
+}
+
+void tpz_search_server_destroy(tpz_search_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->lanes) t.join();
+    for (auto& kv : s->reqs) delete kv.second;
+    delete s;
+}
 
 }  // extern "C"

@@ -94,6 +94,8 @@ public:
         return k;
     }
     uint64_t size() const { return end_.size(); }
+    const std::vector<uint32_t>& blocks() const { return block_; }
+    const std::vector<uint64_t>& ends() const { return end_; }
 
 private:
     std::vector<uint32_t> block_;
@@ -120,17 +122,28 @@ public:
     // after these exists (the suffix array ends).
     void add_stream(int32_t offset, int64_t first_slot, int32_t dir, const int16_t* vals,
                     int32_t n, bool complete);
+    // The same, taking over computed elements that end at their first
+    // kStreamEnd, if any, and each element's protein (db_index) alongside.
+    void add_stream(int32_t offset, int64_t first_slot, int32_t dir,
+                    std::vector<int16_t>&& vals, std::vector<uint32_t>&& proteins,
+                    bool complete);
     // Run until done or until a stream needs more elements.
+    // protein_of(stream, element, slot): the db_index of that element's text
+    // position, asked only when the hit list needs it.
     template <typename ProteinOf>
-    Status run(ProteinOf&& protein_of_slot);
+    Status run(ProteinOf&& protein_of);
+    // The protein of element i of stream s when the stream was given them.
+    uint32_t protein(int32_t s, int32_t i) const { return streams_[s].proteins[i]; }
     // The stream run() waits for, and the slot of its first missing element.
     int32_t waiting() const { return cur_stream_; }
     int64_t next_slot(int32_t s) const;
     int32_t offset_of(int32_t s) const { return streams_[s].offset; }
     int32_t dir_of(int32_t s) const { return streams_[s].dir; }
     int32_t computed(int32_t s) const { return (int32_t)streams_[s].vals.size(); }
-    // Append elements to stream s (those after its last computed one).
-    void supply(int32_t s, const int16_t* vals, int32_t n, bool complete);
+    // Append elements to stream s (those after its last computed one), with
+    // their proteins when the stream holds them (else proteins is null).
+    void supply(int32_t s, const int16_t* vals, const uint32_t* proteins, int32_t n,
+                bool complete);
     // Streams whose next unpopped run starts within `margin` elements of the
     // end of their computed elements, best first, at most `limit` of them
     // (the waiting stream excluded): prefetch candidates.
@@ -147,6 +160,7 @@ private:
         int64_t first;
         bool complete;
         std::vector<int16_t> vals;
+        std::vector<uint32_t> proteins;     // empty: asked of protein_of
     };
     struct Run {
         int16_t leader;
@@ -216,7 +230,7 @@ bool HitList::add(int32_t score, ProteinOf&& protein) {
 }
 
 template <typename ProteinOf>
-Walk::Status Walk::run(ProteinOf&& protein_of_slot) {
+Walk::Status Walk::run(ProteinOf&& protein_of) {
     if (done_) return kDone;
     for (;;) {
         if (cur_stream_ < 0) {
@@ -262,7 +276,8 @@ Walk::Status Walk::run(ProteinOf&& protein_of_slot) {
             ++stats_.pops;
             cur_index_ = i + 1;
             const int64_t slot = s.first + (int64_t)i * s.dir;
-            if (hits_.add(v[i], [&] { return protein_of_slot(slot); })) {
+            const int32_t st = cur_stream_;
+            if (hits_.add(v[i], [&] { return protein_of(st, i, slot); })) {
                 if (++counter_ == limit_) {
                     done_ = true;
                     return kDone;

@@ -173,14 +173,35 @@ void Walk::add_stream(int32_t offset, int64_t first_slot, int32_t dir, const int
     push(vals[0], (int32_t)streams_.size() - 1, 0);
 }
 
+void Walk::add_stream(int32_t offset, int64_t first_slot, int32_t dir,
+                      std::vector<int16_t>&& vals, std::vector<uint32_t>&& proteins,
+                      bool complete) {
+    Stream s;
+    s.offset = offset;
+    s.dir = dir;
+    s.first = first_slot;
+    s.complete = complete || (!vals.empty() && vals.back() == kStreamEnd);
+    stats_.elements += (int64_t)vals.size();
+    s.vals = std::move(vals);
+    s.proteins = std::move(proteins);
+    streams_.push_back(std::move(s));
+    push(streams_.back().vals[0], (int32_t)streams_.size() - 1, 0);
+}
+
 int64_t Walk::next_slot(int32_t s) const {
     const Stream& st = streams_[s];
     return st.first + (int64_t)st.vals.size() * st.dir;
 }
 
-void Walk::supply(int32_t s, const int16_t* vals, int32_t n, bool complete) {
+void Walk::supply(int32_t s, const int16_t* vals, const uint32_t* proteins, int32_t n,
+                  bool complete) {
     Stream& st = streams_[s];
-    if (!st.complete) st.complete = append_until_end(st.vals, vals, n) || complete;
+    if (!st.complete) {
+        const size_t before = st.vals.size();
+        st.complete = append_until_end(st.vals, vals, n) || complete;
+        if (proteins) st.proteins.insert(st.proteins.end(), proteins,
+                                          proteins + (st.vals.size() - before));
+    }
     stats_.elements += n;
 }
 
@@ -241,14 +262,16 @@ extern "C" int tpz_walk_query(const tpz_walk_input* in, tpz_extend_fn extend, vo
         }
     }
     const int64_t* sa = in->sa;
-    auto protein_of = [&](int64_t slot) { return proteins.of((uint64_t)sa[slot]); };
+    auto protein_of = [&](int32_t, int32_t, int64_t slot) {
+        return proteins.of((uint64_t)sa[slot]);
+    };
     while (w.run(protein_of) == Walk::kNeed) {
         const int32_t s = w.waiting();
         const int32_t n = want[s];
         want[s] = std::min(2 * n, 1 << 16);
         if (compute(w.offset_of(s), w.next_slot(s), w.dir_of(s), n))
             return fail("tpz_walk_query: the extension callback failed");
-        w.supply(s, buf.data(), n, false);
+        w.supply(s, buf.data(), nullptr, n, false);
     }
     std::vector<uint64_t> ix;
     std::vector<int32_t> sc;

@@ -55,10 +55,35 @@ class SearchStats(C.Structure):
 HIT_FIELDS = [f for f, _ in Hit._fields_]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("lanes", C.c_int32),
+                ("max_wave_queries", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("waves", C.c_uint64),
+                ("max_requests_per_wave", C.c_uint64), ("queries_done", C.c_uint64),
+                ("peak_bytes", C.c_uint64), ("fixed_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _declare(L):
     if getattr(L, "_search_declared", False):
         return
     VP = C.c_void_p
+    L.tpz_search_server_create.argtypes = [C.POINTER(VP), VP, C.POINTER(SearchOptions),
+                                           C.POINTER(ServerConfig)]
+    L.tpz_search_server_create.restype = C.c_int
+    L.tpz_search_server_submit.argtypes = [VP, C.c_int32, VP, VP, VP, C.POINTER(C.c_uint64)]
+    L.tpz_search_server_submit.restype = C.c_int
+    L.tpz_search_server_wait.argtypes = [VP, C.c_uint64, C.POINTER(VP), C.POINTER(C.c_double)]
+    L.tpz_search_server_wait.restype = C.c_int
+    L.tpz_search_server_get_stats.argtypes = [VP, C.POINTER(ServerStats)]
+    L.tpz_search_server_get_stats.restype = C.c_int
+    L.tpz_search_server_destroy.argtypes = [VP]
+    L.tpz_search_server_destroy.restype = None
     L.tpz_db_create.argtypes = [C.POINTER(VP), VP, C.c_uint64, VP, C.c_uint64, VP, VP, VP,
                                 C.c_int32, C.c_int32]
     L.tpz_db_create.restype = C.c_int
@@ -135,6 +160,10 @@ class Database:
         """(db_size, db_proteins) as engine.c:52-55 sets them."""
         return self.t_size - self.n_proteins, self.n_proteins
 
+    def default_options(self, **kw):
+        ds, dp = self.db_size()
+        return options(db_size=ds, db_proteins=dp, **kw)
+
     def search(self, queries, opt=None, schedule=SCHED_PERSISTENT):
         """queries: list of bytes (internal alphabet, after SEG).  Returns
         (list per query of hit dicts in printed order, stats dict)."""
@@ -142,30 +171,90 @@ class Database:
         if opt is None:
             ds, dp = self.db_size()
             opt = options(db_size=ds, db_proteins=dp)
-        qs = [bytes(q) for q in queries]
-        ql = np.array([len(q) for q in qs], np.int32)
-        qo = np.zeros(len(qs), np.int64)
-        if len(qs):
-            qo[1:] = np.cumsum(ql[:-1])
-        buf = b"".join(qs) + b"\0"
+        buf, qo, ql = _pack(queries)
         r = C.c_void_p()
-        check(L.tpz_search(self._h, C.byref(opt), len(qs), buf, qo.ctypes.data, ql.ctypes.data,
+        check(L.tpz_search(self._h, C.byref(opt), len(ql), buf, qo.ctypes.data, ql.ctypes.data,
                            schedule, C.byref(r)), "tpz_search")
-        try:
-            out = []
-            for k in range(len(qs)):
-                n = L.tpz_search_result_count(r, k)
-                hp = L.tpz_search_result_hits(r, k)
-                out.append([{f: getattr(hp[i], f) for f in HIT_FIELDS} for i in range(n)])
-            st = SearchStats()
-            check(L.tpz_search_result_stats(r, C.byref(st)))
-            return out, st.as_dict()
-        finally:
-            L.tpz_search_result_destroy(r)
+        return _take_result(L, r, len(ql))
 
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().tpz_db_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _pack(queries):
+    qs = [bytes(q) for q in queries]
+    ql = np.array([len(q) for q in qs], np.int32)
+    qo = np.zeros(len(qs), np.int64)
+    if len(qs):
+        qo[1:] = np.cumsum(ql[:-1])
+    return b"".join(qs) + b"\0", qo, ql
+
+
+def _take_result(L, r, n_queries):
+    """(hits per query as dicts, stats dict) of a tpz_search_result, which
+    is destroyed."""
+    try:
+        out = []
+        for k in range(n_queries):
+            n = L.tpz_search_result_count(r, k)
+            hp = L.tpz_search_result_hits(r, k)
+            out.append([{f: getattr(hp[i], f) for f in HIT_FIELDS} for i in range(n)])
+        st = SearchStats()
+        check(L.tpz_search_result_stats(r, C.byref(st)))
+        return out, st.as_dict()
+    finally:
+        L.tpz_search_result_destroy(r)
+
+
+class SearchServer:
+    """tpz_search_server: searches requests (lists of queries) arriving from
+    any thread in merged batches over a resident Database."""
+
+    def __init__(self, db, opt=None, memory_budget=0, lanes=0, max_wave_queries=0):
+        L = _lib.lib()
+        _declare(L)
+        self._db = db
+        self.opt = opt if opt is not None else db.default_options()
+        cfg = ServerConfig(memory_budget, lanes, max_wave_queries)
+        h = C.c_void_p()
+        check(L.tpz_search_server_create(C.byref(h), db._h, C.byref(self.opt), C.byref(cfg)),
+              "tpz_search_server_create")
+        self._h = h
+
+    def submit(self, queries):
+        buf, qo, ql = _pack(queries)
+        t = C.c_uint64()
+        check(_lib.lib().tpz_search_server_submit(self._h, len(ql), buf, qo.ctypes.data,
+                                                  ql.ctypes.data, C.byref(t)),
+              "tpz_search_server_submit")
+        return (t.value, len(ql))
+
+    def wait(self, ticket):
+        """(hits per query, stats of the batch it ran in, latency in ms)."""
+        L = _lib.lib()
+        r = C.c_void_p()
+        lat = C.c_double()
+        check(L.tpz_search_server_wait(self._h, ticket[0], C.byref(r), C.byref(lat)),
+              "tpz_search_server_wait")
+        hits, st = _take_result(L, r, ticket[1])
+        return hits, st, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        check(_lib.lib().tpz_search_server_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().tpz_search_server_destroy(self._h)
             self._h = None
 
     def __del__(self):
