@@ -268,8 +268,13 @@ def sw_leg(args):
         t_plain = time.perf_counter() - t0
         psc, pre, prd = b.results()
         assert np.array_equal(psc, sc) and np.array_equal(pre, re_) and np.array_equal(prd, rd)
-        plain = {"ms_per_step": round(t_plain * 1e3, 3), "gcups": round(cells / t_plain / 1e9, 2),
-                 "queries_per_s": round(n_queries / t_plain, 2)}
+        p_kernel_s = b.kernel_ms() / 1e3
+        plain = {"schedule": "one launch per query, one wavefront per pair (systolic over query "
+                             "rows, int32)",
+                 "ms_per_step": round(t_plain * 1e3, 3), "gcups": round(cells / t_plain / 1e9, 2),
+                 "queries_per_s": round(n_queries / t_plain, 2),
+                 "roofline_frac": round(cells * OPS_PER_CELL / p_kernel_s / 1e12 / VALU_PEAK_TOPS,
+                                        4)}
         print("[bench] plain: %.2f ms/step" % (t_plain * 1e3), file=sys.stderr, flush=True)
     b.close()
 
@@ -297,6 +302,10 @@ def sw_leg(args):
                                "queries, autotuned, replayed from a hipGraph"},
         "tuning": tuning,
         "queries_per_s": round(n_queries / t_step, 2),
+        # pairs whose byte (16-lane) pass saturates and the reference rescores
+        # with 8 word lanes (score + bias >= 255), and those it drops (32767)
+        "word_rescored_pairs": int(((sc - int(matrix().min()) >= 255) & (sc < 32767)).sum()),
+        "null_pairs": int((sc >= 32767).sum()),
         "plain": plain,
         "speedup_vs_plain": round(plain["ms_per_step"] / (t_step * 1e3), 2) if plain else None,
         "roofline": {"bound": "valu", "achieved": round(kernel_tops, 2), "peak": VALU_PEAK_TOPS,
@@ -509,16 +518,23 @@ def _run_search(exe, work, prefix, qfa, flags, threads, env, label):
     """One `topaz search`: wall seconds, TOPAZ's own elapsed time, stdout, and
     the library's totals when the binary prints them."""
     import subprocess
+    # the binary's stderr (per-batch progress lines of the GPU build) goes to
+    # a file as it is written, under gpurun_out/ when that folder exists
+    logdir = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) \
+        else str(work)
+    log = os.path.join(logdir, "search_%s.err" % label.replace(" ", "_"))
     t0 = time.perf_counter()
-    r = subprocess.run([str(exe), "search", "-f", str(qfa), "-p", str(prefix), "-T", str(threads)]
-                       + flags, capture_output=True, text=True, cwd=str(work),
-                       env=dict(os.environ, **env), timeout=1500)
+    with open(log, "w") as ferr:
+        r = subprocess.run([str(exe), "search", "-f", str(qfa), "-p", str(prefix), "-T",
+                            str(threads)] + flags, stdout=subprocess.PIPE, stderr=ferr, text=True,
+                           cwd=str(work), env=dict(os.environ, **env), timeout=1500)
     wall = time.perf_counter() - t0
+    err = open(log).read()
     if r.returncode != 0:
-        raise SystemExit("%s failed: %s" % (label, r.stderr[-3000:]))
-    el = [ln for ln in r.stderr.split("\n") if ln.startswith("elapsed time:")]
-    st = [ln for ln in r.stderr.split("\n") if ln.startswith("tpz-stats ")]
-    up = [ln for ln in r.stderr.split("\n") if ln.startswith("tpz: database resident")]
+        raise SystemExit("%s failed: %s" % (label, err[-3000:]))
+    el = [ln for ln in err.split("\n") if ln.startswith("elapsed time:")]
+    st = [ln for ln in err.split("\n") if ln.startswith("tpz-stats ")]
+    up = [ln for ln in err.split("\n") if ln.startswith("tpz: database resident")]
     info = {"wall_s": round(wall, 3),
             "topaz_elapsed_s": float(el[-1].split()[2]) if el else None,
             "db_upload_s": float(up[-1].split()[-2]) if up else None}

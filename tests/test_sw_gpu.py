@@ -58,10 +58,20 @@ def test_random_and_homolog_pairs(schedule):
     assert (sc < 251).sum() > 100 and ((sc >= 251) & (sc < 32767)).sum() > 20
 
 
-def test_long_and_tiny_queries():
+@pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
+def test_long_and_tiny_queries(schedule):
     rng = np.random.default_rng(2)
     w = synth.pair_workload(rng, 6, 30, mean=400, sd=600, lo=1, hi=1500)
-    compare(w)
+    compare(w, schedule=schedule)
+
+
+def test_plain_multi_pass_queries():
+    """Queries of 2,049 .. 5,000 residues: the plain schedule's wavefront per
+    pair takes several passes of 2,048 rows, handing each column's bottom row
+    through the workspace."""
+    rng = np.random.default_rng(12)
+    w = synth.pair_workload(rng, 4, 24, mean=3200, sd=900, lo=2049, hi=5000)
+    compare(w, schedule=SCHED_PLAIN)
 
 
 def test_tiny_sequences_all_corners():
@@ -82,7 +92,8 @@ def test_other_matrices_and_gaps(name, go, ge):
     compare(small_workload(4, nq=6, per_q=30), mat=matrix(name), go=go, ge=ge)
 
 
-def test_overflow_to_word_and_drop():
+@pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
+def test_overflow_to_word_and_drop(schedule):
     # W-rich long identical pairs: byte overflows, word may reach 32767
     rng = np.random.default_rng(5)
     q1 = np.full(3100, 17, np.int8)          # W x 3100 -> self score 34100
@@ -94,7 +105,7 @@ def test_overflow_to_word_and_drop():
     pq = np.array([0, 0, 1, 1, 1, 0], np.int32)
     ps = np.array([0, 1, 2, 3, 4, 4], np.int32)
     w = dict(q=_concat(qs), s=_concat(ss), pair_query=pq, pair_subject=ps)
-    sc = compare(w)
+    sc = compare(w, schedule=schedule)
     assert sc[0] == OVERFLOW and sc[1] == 22000
 
 

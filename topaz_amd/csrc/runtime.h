@@ -68,10 +68,6 @@ std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pai
                                                  const std::vector<int32_t>& pair_subject,
                                                  const std::vector<int32_t>& s_len,
                                                  int32_t n_queries);
-// One query per task (the plain schedule).
-void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
-                       const std::vector<int32_t>& pair_subject, const std::vector<int32_t>& q_len,
-                       const std::vector<int32_t>& s_len, int rows, Tasks& t);
 // Up to nq_max queries of equal chunk count per task (the persistent schedule).
 void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
                         const std::vector<int32_t>& pair_subject, const std::vector<int32_t>& q_len,

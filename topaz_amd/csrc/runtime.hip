@@ -18,6 +18,7 @@
 #include "gapless_kernels.h"
 #include "traceback_kernels.h"
 #include "sw_kernels.h"
+#include "sw_pair_kernels.h"
 
 namespace tpz {
 
@@ -95,24 +96,6 @@ std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pai
     return per;
 }
 
-// One query per task (the plain schedule): a query's pairs, longest subject
-// first, cut into tasks of 128, so the two halves of a lane and the lanes of
-// a wavefront walk subjects of similar length.
-void build_query_tasks(const std::vector<std::vector<int32_t>>& per,
-                              const std::vector<int32_t>& pair_subject,
-                              const std::vector<int32_t>& q_len,
-                              const std::vector<int32_t>& s_len, int rows, Tasks& t) {
-    t.clear();
-    for (int32_t q = 0; q < (int32_t)per.size(); ++q) {
-        const auto& v = per[q];
-        for (size_t i = 0; i < v.size(); i += kTaskPairs) {
-            t.open(&q, 1, chunk_rows(q_len[q], rows) * s_len[pair_subject[v[i]]]);
-            for (size_t k = i; k < v.size() && k < i + kTaskPairs; ++k) t.index.push_back(v[k]);
-        }
-    }
-    t.begin.push_back((int32_t)t.index.size());
-}
-
 // Tasks of up to nq (<= 4) queries (the persistent schedule).  Queries with
 // the same number of row chunks are packed together: repeatedly the nq
 // queries whose longest remaining subjects are longest give their longest
@@ -187,12 +170,16 @@ using namespace tpz;
 // workspace and the captured launch graphs.
 struct SwPlan {
     tpz_sw_config cfg{};
-    DeviceTasks plain_tasks;                        // one query per task
     DeviceTasks packed_tasks;                       // up to queries_per_task per task
-    // work items (4 task slots each): per query for the plain schedule, by
-    // decreasing cost for the persistent one
-    std::vector<int32_t> items_plain, items_persistent, plain_query_items;
-    DeviceBuffer d_items_plain, d_items_persistent;
+    // work items (W task slots each) of the persistent schedule, by
+    // decreasing cost
+    std::vector<int32_t> items_persistent;
+    DeviceBuffer d_items_persistent;
+    // plain schedule: the pairs of each query (a launch per query, a
+    // wavefront per pair), their workspace offsets when the query needs
+    // more than one pass, and the workspace of the largest such launch
+    std::vector<int32_t> plain_begin;               // per query, into plain_pairs
+    DeviceBuffer d_plain_pairs, d_plain_ws_off, d_plain_ws;
     int32_t big_tasks = 0;
     size_t ws_slots = 0;
     int64_t ws_words_per_slot = 0;
@@ -297,41 +284,42 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
     if (check_config(c)) return -1;
     auto plan = std::make_unique<SwPlan>();
     plan->cfg = c;
-    build_query_tasks(b->per_query, b->pair_subject, b->q_len, b->s_len, c.rows_per_chunk,
-                      plan->plain_tasks.host);
     build_packed_tasks(b->per_query, b->pair_subject, b->q_len, b->s_len, c.rows_per_chunk,
                        c.queries_per_task, plan->packed_tasks.host);
-    int rc = plan->plain_tasks.upload() | plan->packed_tasks.upload();
-    // plain: per query, its tasks one per wavefront of an item
+    int rc = plan->packed_tasks.upload();
     const int W = c.waves_per_group;
-    size_t max_items_per_query = 0;
     {
-        const Tasks& t = plan->plain_tasks.host;
-        const int n_tasks = t.count();
-        int k = 0;
-        while (k < n_tasks) {
-            int e = k;
-            while (e < n_tasks && t.query[kTaskQueries * e] == t.query[kTaskQueries * k]) ++e;
-            plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / W));
-            for (int j = k; j < e; j += W)
-                for (int w = 0; w < W; ++w) plan->items_plain.push_back(j + w < e ? j + w : -1);
-            max_items_per_query = std::max<size_t>(max_items_per_query, (size_t)(e - k + W - 1) / W);
-            k = e;
+        // plain: each query's pairs in one list; a query longer than one
+        // pass of the pair kernel needs 2 words per subject column per pair
+        std::vector<int32_t> pairs;
+        std::vector<int64_t> ws_off;
+        int64_t ws_max = 0;
+        for (int32_t q = 0; q < b->n_queries; ++q) {
+            plan->plain_begin.push_back((int32_t)pairs.size());
+            const bool passes = b->q_len[q] > 64 * tpz_sw_pair_rows(b->q_len[q]);
+            int64_t ws = 0;
+            for (int32_t p : b->per_query[q]) {
+                pairs.push_back(p);
+                ws_off.push_back(ws);
+                if (passes) ws += 2 * (int64_t)b->s_len[b->pair_subject[p]];
+            }
+            ws_max = std::max(ws_max, ws);
         }
-        plan->plain_query_items.push_back((int32_t)(plan->items_plain.size() / W));
+        plan->plain_begin.push_back((int32_t)pairs.size());
+        rc |= plan->d_plain_pairs.upload(pairs.data(), pairs.size() * 4);
+        rc |= plan->d_plain_ws_off.upload(ws_off.data(), ws_off.size() * 8);
+        rc |= plan->d_plain_ws.alloc((size_t)ws_max * 4 + 16);
     }
     const size_t persistent_slots = (size_t)std::max(num_cu(), 1) * c.groups_per_cu;
     build_items(plan->packed_tasks.host, (int)persistent_slots, c.big_fraction, W,
                 plan->items_persistent, plan->big_tasks);
-    rc |= plan->d_items_plain.upload(plan->items_plain.data(), plan->items_plain.size() * 4);
     rc |= plan->d_items_persistent.upload(plan->items_persistent.data(),
                                           plan->items_persistent.size() * 4);
     // workspace: per workgroup, one boundary buffer (bottom H and F of a
     // chunk: 2 words per column and lane) per wavefront
     plan->ws_words_per_slot = (int64_t)W * 2 * (b->max_subject + 2) * 64;
     const size_t n_items_p = plan->items_persistent.size() / W;
-    plan->ws_slots = std::max<size_t>(std::max(std::min(persistent_slots, n_items_p),
-                                               max_items_per_query), 1);
+    plan->ws_slots = std::max<size_t>(std::min(persistent_slots, n_items_p), 1);
     rc |= plan->d_workspace.alloc(plan->ws_slots * (size_t)plan->ws_words_per_slot * 4);
     if (rc) return -1;
     // the old plan's buffers may still be read by queued work
@@ -482,7 +470,7 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
     const SwPlan& pl = *b->plan;
     const int rows = pl.cfg.rows_per_chunk, W = pl.cfg.waves_per_group;
     SwKernelArgs a = make_args(b);
-    const DeviceTasks& dt = schedule == TPZ_SCHED_PERSISTENT ? pl.packed_tasks : pl.plain_tasks;
+    const DeviceTasks& dt = pl.packed_tasks;
     a.pair_index = (const int32_t*)dt.d_index.ptr;
     a.task_begin = (const int32_t*)dt.d_begin.ptr;
     a.task_query = (const int32_t*)dt.d_query.ptr;
@@ -501,13 +489,22 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
                 return -1;
         }
     } else {
-        // one launch per query, one workgroup per item of that query
-        a.task_counter = nullptr;
-        for (size_t qk = 0; qk + 1 < pl.plain_query_items.size(); ++qk) {
-            const int i0 = pl.plain_query_items[qk], i1 = pl.plain_query_items[qk + 1];
-            a.item_task = (const int32_t*)pl.d_items_plain.ptr + (size_t)W * i0;
-            a.n_items = i1 - i0;
-            if (check(tpz_launch_sw_forward(&a, i1 - i0, rows, W, stream()), "sw_forward (plain)"))
+        // one launch per query, one wavefront per pair of that query
+        PairArgs pa{};
+        pa.q_res = a.q_res; pa.q_off = a.q_off; pa.q_len = a.q_len;
+        pa.s_res = a.s_res; pa.s_off = a.s_off; pa.s_len = a.s_len;
+        pa.pair_query = a.pair_query; pa.pair_subject = a.pair_subject;
+        pa.matrix = a.matrix; pa.gap_open = a.gap_open; pa.gap_extend = a.gap_extend;
+        pa.workspace = (int32_t*)pl.d_plain_ws.ptr;
+        pa.out_score = a.out_score; pa.out_ref = a.out_ref; pa.out_read = a.out_read;
+        for (int32_t q = 0; q + 1 < (int32_t)pl.plain_begin.size(); ++q) {
+            const int32_t i0 = pl.plain_begin[q], i1 = pl.plain_begin[q + 1];
+            if (i1 == i0) continue;
+            pa.pairs = (const int32_t*)pl.d_plain_pairs.ptr + i0;
+            pa.ws_off = (const int64_t*)pl.d_plain_ws_off.ptr + i0;
+            pa.n_list = i1 - i0;
+            if (check(tpz_launch_sw_pair(&pa, tpz_sw_pair_rows(b->q_len[q]), stream()),
+                      "sw_pair (plain)"))
                 return -1;
         }
     }
