@@ -3,6 +3,7 @@
 * Offline: Database.search (persistent and plain schedules) prints, line for
   line, what the reference binary prints for the same queries with SEG off
   (-M), default and -B.
+* Streams cut short at a tiny cap: the same hits.
 * Serving: requests submitted from several threads at once, merged into
   shared batches by the server, each get exactly the hits the same queries
   get searched offline; a request served alone gets them too; the admitted
@@ -67,6 +68,18 @@ def test_offline_search_matches_reference(db_and_queries, blasttab):
                 ours["q%d" % k] = lines
         assert ours == ref
     assert sum(len(v) for v in ref.values()) > 100 and st["pairs"] > 0
+
+
+def test_streams_cut_at_the_cap_are_searched_again(db_and_queries):
+    """A walk that reaches the end of a stream cut at max_stream_elements
+    searches its query again with a higher cap: the hits are those of an
+    uncapped search."""
+    tmp, db, qs = db_and_queries
+    full, _ = db.search(qs[:40])
+    opt = db.default_options(max_stream_elements=8, first_chunk=2)
+    cut, st = db.search(qs[:40], opt)
+    assert cut == full
+    assert st["refills"] > 0
 
 
 def test_server_results_equal_offline(db_and_queries):

@@ -84,30 +84,38 @@ struct LowerBoundArgs {
     int64_t* out;               // per record: the index
 };
 
-// A segment of a walk stream: up to 64 consecutive suffix-array slots
-// first, first + dir, ... of one (query, offset), extended with sa_hsp.
-struct StreamSeg {
+// A walk stream (query.c:112-128): the suffix-array slots first,
+// first + dir, ... of one (query, offset).  24 bytes.
+struct StreamDef {
     int32_t query;
     int32_t offset;
     int64_t first;
-    int64_t out;                // element index of the first slot in StreamArgs::out
-    int32_t count;              // 1..64
     int32_t dir;                // +1 or -1
+    int32_t pad;
 };
 
-struct StreamArgs {
+// One round of stream extension: request r takes the `chunk` slots of
+// stream req_stream[r] from its element `have` on (every listed stream
+// holds `have` elements), extended with sa_hsp.  Outputs per request, at
+// r * chunk: the score where the extension is a seed (length >= min_len),
+// -1 otherwise (also for a slot outside the suffix array), and the protein
+// (db_index) of the slot's text position from a table of every 64th
+// position's protein and each protein's last position (its newline;
+// 0xffffffff outside the suffix array); and req_end[r], lowered to the
+// first element holding -1 (preset to chunk by the caller).
+struct StreamRoundArgs {
     GaplessArgs g;              // text, suffix array (required), queries, table
-    const StreamSeg* segs;
-    int64_t n_segs;
-    int16_t* out;               // per slot: the score when length >= min_len, else -1
-                                // (also for a slot outside the suffix array)
+    const StreamDef* defs;
+    const int32_t* req_stream;
+    int64_t n_req;
+    int64_t have;
+    int32_t chunk;
     int32_t min_len;
-    // the protein (db_index) of each slot's text position, from a table of
-    // every 64th position's protein and each protein's last position (its
-    // newline); 0xffffffff for a slot outside the suffix array
+    int16_t* out;
+    uint32_t* out_protein;
+    int32_t* req_end;
     const uint32_t* prot_block;
     const uint64_t* prot_end;
-    uint32_t* out_protein;
 };
 
 }  // namespace tpz
@@ -116,6 +124,7 @@ extern "C" hipError_t tpz_launch_gapless(const tpz::GaplessArgs* args, int grid,
                                          hipStream_t stream);
 extern "C" hipError_t tpz_launch_sa_search(const tpz::SaSearchArgs* args, hipStream_t stream);
 extern "C" hipError_t tpz_launch_lower_bound(const tpz::LowerBoundArgs* args, hipStream_t stream);
-extern "C" hipError_t tpz_launch_streams(const tpz::StreamArgs* args, int grid, hipStream_t stream);
+extern "C" hipError_t tpz_launch_stream_round(const tpz::StreamRoundArgs* args, int grid,
+                                              hipStream_t stream);
 
 #endif

@@ -210,28 +210,36 @@ gapless_kernel(GaplessArgs a) {
     }
 }
 
-// Walk streams: one segment (up to 64 consecutive suffix-array slots of one
-// query offset) per wavefront iteration, lane l on slot first + l * dir;
-// then the same compaction and two-lane scans as gapless_kernel.  The
-// output is one int16 per slot: the score where the extension is a seed
-// (length >= min_len), -1 otherwise.
+// One round of walk-stream extension: 64 consecutive slots of one request
+// per wavefront iteration (request g / spr, part g % spr, spr = the parts of
+// a chunk), lane l on slot first + (have + part * 64 + l) * dir; then the
+// same compaction and two-lane scans as gapless_kernel.  Each lane's slot
+// is marked in LDS when its element is -1; the wavefront lowers its
+// request's req_end to the first marked slot.
 __global__ void __launch_bounds__(kWaves * 64)
-stream_kernel(StreamArgs sa) {
+stream_round_kernel(StreamRoundArgs sa) {
     __shared__ GaplessLds lds;
+    __shared__ uint8_t is_end[kWaves][64];
     const GaplessArgs& a = sa.g;
     for (int e = threadIdx.x; e < kTableRows * kTableCols; e += blockDim.x) lds.table[e] = a.table[e];
     __syncthreads();
 
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     Slot* slots = lds.slot[wave];
+    uint8_t* ends = is_end[wave];
+    const int64_t spr = (sa.chunk + 63) / 64;
+    const int64_t n_waves = sa.n_req * spr;
     const int64_t waves_total = (int64_t)gridDim.x * kWaves;
-    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < sa.n_segs; g += waves_total) {
-        const StreamSeg seg = sa.segs[g];
+    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < n_waves; g += waves_total) {
+        const int64_t req = g / spr;
+        const int32_t part = (int32_t)(g - req * spr);
+        const StreamDef d = sa.defs[sa.req_stream[req]];
+        const int32_t count = min(64, sa.chunk - part * 64);
+        const int64_t out0 = req * sa.chunk + (int64_t)part * 64;
         bool pass = false;
         int64_t idx = 0;
-        const int64_t sid = seg.out + lane;
-        if (lane < seg.count) {
-            const int64_t slot = seg.first + (int64_t)lane * seg.dir;
+        if (lane < count) {
+            const int64_t slot = d.first + (sa.have + (int64_t)part * 64 + lane) * d.dir;
             uint32_t prot = 0xffffffffu;
             if (slot >= 0 && slot < a.t_size) {
                 idx = a.sa[slot];
@@ -241,29 +249,37 @@ stream_kernel(StreamArgs sa) {
                     uint32_t k = sa.prot_block[idx >> 6];
                     while ((uint64_t)idx > sa.prot_end[k]) ++k;
                     prot = k;
-                    if (seg.query >= 0 && seg.query < a.n_queries && seg.offset >= 0 &&
-                        seg.offset < a.q_len[seg.query])
-                        pass = 2 * (int)a.t_fwd[idx] ==
-                               (int)a.q_fwd[a.qf_off[seg.query] + seg.offset];
+                    if (d.query >= 0 && d.query < a.n_queries && d.offset >= 0 &&
+                        d.offset < a.q_len[d.query])
+                        pass = 2 * (int)a.t_fwd[idx] == (int)a.q_fwd[a.qf_off[d.query] + d.offset];
                 }
             }
-            sa.out_protein[sid] = prot;
-            if (!pass) sa.out[sid] = -1;
+            sa.out_protein[out0 + lane] = prot;
+            if (!pass) sa.out[out0 + lane] = -1;
         }
+        ends[lane] = !pass;
         const uint64_t mask = __ballot(pass);
         const int n_pass = __popcll(mask);
         if (pass) {
             const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-            slots[rank] = Slot{idx, sid, seg.offset, seg.query};
+            slots[rank] = Slot{idx, out0 + lane, d.offset, d.query};
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int min_len = sa.min_len;
         extend_passing(a, lds, slots, n_pass, lane, [&](int64_t s_id, int score, int length) {
-            sa.out[s_id] = (int16_t)(length >= min_len ? score : -1);
+            const bool seed = length >= min_len;
+            sa.out[s_id] = (int16_t)(seed ? score : -1);
+            ends[s_id - out0] = !seed;
         });
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint64_t end_mask = __ballot(lane < count && ends[lane]);
+        if (end_mask && lane == 0)
+            atomicMin(sa.req_end + req, part * 64 + (int32_t)__builtin_ctzll(end_mask));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -425,9 +441,11 @@ extern "C" hipError_t tpz_launch_lower_bound(const tpz::LowerBoundArgs* args, hi
     return hipGetLastError();
 }
 
-extern "C" hipError_t tpz_launch_streams(const tpz::StreamArgs* args, int grid,
-                                         hipStream_t stream) {
-    if (args->n_segs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tpz::stream_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream, *args);
+extern "C" hipError_t tpz_launch_stream_round(const tpz::StreamRoundArgs* args, int grid,
+                                              hipStream_t stream) {
+    if (args->n_req <= 0) return hipSuccess;
+    if (args->chunk < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tpz::stream_round_kernel, dim3(grid), dim3(tpz::kWaves * 64), 0, stream,
+                       *args);
     return hipGetLastError();
 }

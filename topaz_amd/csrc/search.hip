@@ -8,12 +8,16 @@
 //      reports (lower_bound_kernel), query.c:104-110.
 //   2. Streams: per offset the forward suffix at that index and the backward
 //      one before it (query.c:112-128).  Every stream is extended on the GPU
-//      slot by slot until its first non-seed (stream_kernel), in rounds of
-//      doubling length shared by all streams still open: the walk pops
-//      nearly every element of every stream, so they are computed up front.
-//   3. The walk and hit list per query on the host (walk.h), in parallel
-//      over queries; a stream that reaches the round cap is refilled on
-//      demand.
+//      slot by slot until its first non-seed (stream_round_kernel), in
+//      rounds of doubling length shared by all streams still open, each
+//      round's elements kept on the device (only each request's first
+//      non-seed comes back); the walk pops nearly every element of every
+//      stream, so they are computed up front.  compact_kernel then lays each
+//      stream's seeds out back to back.
+//   3. The walk and hit list of each query on the GPU, one wavefront per
+//      query (walk_core.h, walk_kernels.hip); only the hit lists come back.
+//      A query whose walk reaches the end of a stream cut at
+//      max_stream_elements is searched again with a higher cap.
 //   4. Forward Smith-Waterman of every hit whose gapless score is >= 40
 //      (ssw_wrapper.c:171-176) through the library's serving loop over the
 //      resident proteins; E-values and bitscores on the host with the
@@ -46,7 +50,9 @@
 #include "../../include/topaz_amd.h"
 #include "gapless_kernels.h"
 #include "runtime.h"
+#include "sw_kernels.h"
 #include "walk.h"
+#include "walk_kernels.h"
 
 namespace tpz {
 namespace {
@@ -212,8 +218,16 @@ struct StatsMem final : RunMem {
 struct Slot {
     hipStream_t stream = nullptr;
     DevBuf d_q, d_qoff, d_qlen, d_rq, d_ro, d_idx;
-    DevBuf d_qf, d_qr, d_qf_off, d_qr_off, d_segs, d_out, d_prot;
-    HostBuf h_stage, h_out, h_prot, h_idx;
+    DevBuf d_qf, d_qr, d_qf_off, d_qr_off;
+    // stream rounds: definitions, the round's requests and their ends, and
+    // per round its output arrays
+    DevBuf d_defs, d_req, d_req_end, d_round_ptrs, d_jobs;
+    std::deque<DevBuf> round_vals, round_prot;
+    // the walk: contiguous streams, per-query tables, walk state, hits
+    DevBuf d_vals, d_prot, d_sbase, d_slen, d_sopen;
+    DevBuf d_qstream, d_qcap, d_qmem, d_qout, d_mem;
+    DevBuf d_oidx, d_oscore, d_on, d_ostatus, d_ostats;
+    HostBuf h_stage, h_req_end, h_oidx, h_oscore;
     StatsMem stats;
     ~Slot() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -304,13 +318,6 @@ void give_slot(tpz_db* db, Slot* s) {
     db->free_slots.emplace_back(s);
 }
 
-// One stream of the walk (query.c:112-128): offset, first slot, direction.
-struct StreamDef {
-    int32_t query, offset;
-    int64_t first;
-    int32_t dir;
-};
-
 struct Batch {
     int32_t n = 0;
     const char* q_chars = nullptr;
@@ -382,70 +389,6 @@ GaplessArgs stream_gapless_args(tpz_db* db, Slot& sl, int32_t n_queries, const i
     return a;
 }
 
-// Extend `count` slots of each listed stream from slot `from[i]` on; the
-// elements land in out[i * ... ] per the offsets given (host pinned copy).
-struct ChunkReq {
-    int32_t stream;
-    int64_t from;
-    int32_t count;
-    int64_t out;        // element index in the round's output
-};
-
-int run_chunks(tpz_db* db, Slot& sl, const std::vector<StreamDef>& defs,
-               const std::vector<ChunkReq>& reqs, int64_t n_out, int32_t n_queries,
-               int32_t min_len, const int16_t** host_out, const uint32_t** host_prot,
-               double* gpu_s) {
-    std::vector<StreamSeg> segs;
-    segs.reserve((size_t)n_out / 32 + reqs.size());
-    for (const ChunkReq& r : reqs) {
-        const StreamDef& d = defs[r.stream];
-        for (int32_t c = 0; c < r.count; c += 64) {
-            StreamSeg g;
-            g.query = d.query;
-            g.offset = d.offset;
-            g.first = r.from + (int64_t)c * d.dir;
-            g.out = r.out + c;
-            g.count = std::min(64, r.count - c);
-            g.dir = d.dir;
-            segs.push_back(g);
-        }
-    }
-    const size_t seg_bytes = segs.size() * sizeof(StreamSeg);
-    char* hs = sl.h_stage.get(seg_bytes + 64);
-    void* ds = sl.d_segs.get(seg_bytes + 64);
-    void* dout = sl.d_out.get((size_t)n_out * 2 + 64);
-    int16_t* hout = (int16_t*)sl.h_out.get((size_t)n_out * 2 + 64);
-    void* dprot = sl.d_prot.get((size_t)n_out * 4 + 64);
-    uint32_t* hprot = (uint32_t*)sl.h_prot.get((size_t)n_out * 4 + 64);
-    if (!hs || !ds || !dout || !hout || !dprot || !hprot) return -1;
-    memcpy(hs, segs.data(), seg_bytes);
-    const double t0 = now_s();
-    if (check(hipMemcpyAsync(ds, hs, seg_bytes, hipMemcpyHostToDevice, sl.stream), "upload"))
-        return -1;
-    StreamArgs a{};
-    a.g = stream_gapless_args(db, sl, n_queries, (const int32_t*)sl.d_qlen.p);
-    a.segs = (const StreamSeg*)ds;
-    a.n_segs = (int64_t)segs.size();
-    a.out = (int16_t*)dout;
-    a.min_len = min_len;
-    a.prot_block = (const uint32_t*)db->d_pblock.ptr;
-    a.prot_end = (const uint64_t*)db->d_pend.ptr;
-    a.out_protein = (uint32_t*)dprot;
-    const int grid = (int)std::max<int64_t>(
-        1, std::min<int64_t>(((int64_t)segs.size() + 3) / 4, (int64_t)std::max(num_cu(), 1) * 32));
-    if (check(tpz_launch_streams(&a, grid, sl.stream), "stream_kernel") ||
-        check(hipMemcpyAsync(hout, dout, (size_t)n_out * 2, hipMemcpyDeviceToHost, sl.stream),
-              "download") ||
-        check(hipMemcpyAsync(hprot, dprot, (size_t)n_out * 4, hipMemcpyDeviceToHost, sl.stream),
-              "download") ||
-        check(hipStreamSynchronize(sl.stream), "streams"))
-        return -1;
-    *gpu_s += now_s() - t0;
-    *host_out = hout;
-    *host_prot = hprot;
-    return 0;
-}
-
 struct Options {
     tpz_search_options o;
     int32_t capacity(int32_t qlen) const {
@@ -457,15 +400,22 @@ struct Options {
 // Per-query state through the stages.
 struct QueryWork {
     int32_t first_stream = 0, n_streams = 0;   // range in the batch's stream list
-    std::vector<std::vector<int16_t>> vals;     // per stream: elements computed
-    std::vector<std::vector<uint32_t>> prot;    // per stream: their proteins
-    std::vector<uint8_t> open;                  // per stream: more elements may exist
-    int32_t first_req = 0, n_req = 0;           // the round's requests of this query
     std::vector<uint64_t> hit_index;            // hit list, traversal order
     std::vector<int32_t> hit_gapless;
-    WalkStats walk;
-    int32_t counted = 0;
+    bool need = false;                          // walk reached a stream cut at the cap
 };
+
+// Upload v into b on the stream (pageable: staged before the call returns).
+template <typename T>
+T* upload_vec(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    void* p = b.get(v.size() * sizeof(T) + 64);
+    if (!p) return nullptr;
+    if (!v.empty() &&
+        check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st),
+              "upload"))
+        return nullptr;
+    return (T*)p;
+}
 
 // The whole search of one batch of queries (all stages).
 int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
@@ -532,113 +482,192 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         for (int32_t r = rec_begin[k]; r < rec_begin[k + 1]; ++r) {
             const int64_t pos = idx[r];
             if (pos == -1) continue;
-            if (pos < (int64_t)db->t_size) defs.push_back(StreamDef{k, ro[r], pos, 1});
+            if (pos < (int64_t)db->t_size) defs.push_back(StreamDef{k, ro[r], pos, 1, 0});
             if (pos != 0 && pos - 1 < (int64_t)db->t_size)
-                defs.push_back(StreamDef{k, ro[r], pos - 1, -1});
+                defs.push_back(StreamDef{k, ro[r], pos - 1, -1, 0});
         }
         qw[k].n_streams = (int32_t)defs.size() - qw[k].first_stream;
-        qw[k].vals.resize(qw[k].n_streams);
-        qw[k].prot.resize(qw[k].n_streams);
-        qw[k].open.assign(qw[k].n_streams, 1);
     }
-    st.streams += (int64_t)defs.size();
-    auto query_of = [&](int32_t s) { return defs[s].query; };
+    const int64_t n_streams = (int64_t)defs.size();
+    if (n_streams >= INT32_MAX) return fail("tpz_search: too many walk streams in one batch");
+    st.streams += n_streams;
 
-    // rounds: every open stream gets its next `chunk` slots
-    int32_t chunk = std::max(1, opt.o.first_chunk);
+    // rounds: every open stream gets its next `c` slots, on the device; only
+    // each request's first non-seed comes back
+    std::vector<int64_t> slen((size_t)n_streams, 0);
+    std::vector<uint8_t> sopen((size_t)n_streams, 0);
+    struct Piece {
+        int32_t stream, round, n;
+        int64_t src, at;            // element in the round's output, in the stream
+    };
+    std::vector<Piece> pieces;
+    std::vector<int32_t> open_list((size_t)n_streams);
+    for (int64_t s = 0; s < n_streams; ++s) open_list[s] = (int32_t)s;
+    const StreamDef* d_defs = upload_vec(sl.d_defs, defs, sl.stream);
+    if (!d_defs) return -1;
     const int64_t cap_elems = opt.o.max_stream_elements;
-    for (int round = 0;; ++round) {
-        std::vector<ChunkReq> reqs;
-        int64_t n_out = 0;
-        for (int32_t k = 0; k < b.n; ++k) {
-            QueryWork& w = qw[k];
-            w.first_req = (int32_t)reqs.size();
-            for (int32_t ls = 0; ls < w.n_streams; ++ls) {
-                if (!w.open[ls]) continue;
-                const int32_t s = w.first_stream + ls;
-                const int64_t have = (int64_t)w.vals[ls].size();
-                if (have >= cap_elems) continue;       // left to the walk's refills
-                const int32_t c = (int32_t)std::min<int64_t>(chunk, cap_elems - have);
-                reqs.push_back(ChunkReq{s, defs[s].first + have * defs[s].dir, c, n_out});
-                n_out += c;
-            }
-            w.n_req = (int32_t)reqs.size() - w.first_req;
+    int64_t have = 0;
+    int32_t chunk = std::max(1, opt.o.first_chunk);
+    std::vector<const int16_t*> round_vals;
+    std::vector<const uint32_t*> round_prot;
+    const double g0 = now_s();
+    for (int round = 0; !open_list.empty(); ++round) {
+        const int64_t n_req = (int64_t)open_list.size();
+        const int32_t c = (int32_t)std::min<int64_t>(chunk, cap_elems - have);
+        if (n_req >= INT32_MAX) return fail("tpz_search: stream round too large");
+        if ((int)sl.round_vals.size() <= round) {
+            sl.round_vals.emplace_back();
+            sl.round_prot.emplace_back();
         }
-        if (reqs.empty()) break;
-        const int16_t* hv = nullptr;
-        const uint32_t* hp = nullptr;
-        if (run_chunks(db, sl, defs, reqs, n_out, b.n, min_len, &hv, &hp, &st.gpu_s)) return -1;
-        st.elements += n_out;
+        int16_t* rv = (int16_t*)sl.round_vals[round].get((size_t)(n_req * c) * 2 + 64);
+        uint32_t* rp = (uint32_t*)sl.round_prot[round].get((size_t)(n_req * c) * 4 + 64);
+        int32_t* d_end = (int32_t*)sl.d_req_end.get((size_t)n_req * 4 + 64);
+        int32_t* h_end = (int32_t*)sl.h_req_end.get((size_t)n_req * 4 + 64);
+        const int32_t* d_req = upload_vec(sl.d_req, open_list, sl.stream);
+        if (!rv || !rp || !d_end || !h_end || !d_req) return -1;
+        round_vals.push_back(rv);
+        round_prot.push_back(rp);
+        StreamRoundArgs a{};
+        a.g = stream_gapless_args(db, sl, b.n, (const int32_t*)sl.d_qlen.p);
+        a.defs = d_defs;
+        a.req_stream = d_req;
+        a.n_req = n_req;
+        a.have = have;
+        a.chunk = c;
+        a.min_len = min_len;
+        a.out = rv;
+        a.out_protein = rp;
+        a.req_end = d_end;
+        a.prot_block = (const uint32_t*)db->d_pblock.ptr;
+        a.prot_end = (const uint64_t*)db->d_pend.ptr;
+        const int64_t waves = n_req * ((c + 63) / 64);
+        const int grid = (int)std::max<int64_t>(
+            1, std::min<int64_t>((waves + 3) / 4, (int64_t)std::max(num_cu(), 1) * 32));
+        if (check(tpz_launch_fill_i32(d_end, c, (int32_t)n_req, sl.stream), "clear ends") ||
+            check(tpz_launch_stream_round(&a, grid, sl.stream), "stream_round_kernel") ||
+            check(hipMemcpyAsync(h_end, d_end, (size_t)n_req * 4, hipMemcpyDeviceToHost,
+                                 sl.stream), "download") ||
+            check(hipStreamSynchronize(sl.stream), "stream round"))
+            return -1;
+        st.elements += n_req * c;
         ++st.rounds;
-        // append each request's elements up to the stream's first non-seed,
-        // one query per task
-        pool().run(b.n, [&](int64_t kk) {
-            QueryWork& w = qw[kk];
-            for (int32_t e = w.first_req; e < w.first_req + w.n_req; ++e) {
-                const ChunkReq& r = reqs[e];
-                const int32_t ls = r.stream - w.first_stream;
-                const int16_t* src = hv + r.out;
-                const int16_t* end = std::find(src, src + r.count, kStreamEnd);
-                const int32_t take = (int32_t)(end - src) + (end != src + r.count);
-                w.vals[ls].insert(w.vals[ls].end(), src, src + take);
-                w.prot[ls].insert(w.prot[ls].end(), hp + r.out, hp + r.out + take);
-                if (end != src + r.count) w.open[ls] = 0;
+        std::vector<int32_t> next;
+        for (int64_t i = 0; i < n_req; ++i) {
+            const int32_t s = open_list[i], e = h_end[i];
+            if (e > 0) pieces.push_back(Piece{s, round, e, i * c, have});
+            slen[s] += e;
+            if (e == c) {
+                if (have + c < cap_elems) next.push_back(s);
+                else sopen[s] = 1;          // cut at the cap
             }
-        });
+        }
+        open_list.swap(next);
+        have += c;
         chunk = (int32_t)std::min<int64_t>((int64_t)chunk * 2, 1 << 20);
+    }
+
+    // each stream's seeds back to back, in stream order
+    std::vector<int64_t> sbase((size_t)n_streams + 1, 0);
+    for (int64_t s = 0; s < n_streams; ++s) sbase[s + 1] = sbase[s] + slen[s];
+    const int64_t n_elem = sbase[n_streams];
+    std::vector<CompactJob> jobs(pieces.size());
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        const Piece& pc = pieces[i];
+        jobs[i] = CompactJob{pc.src, sbase[pc.stream] + pc.at, pc.n, pc.round};
+    }
+    std::vector<const void*> ptrs;
+    for (auto* v : round_vals) ptrs.push_back(v);
+    for (auto* v : round_prot) ptrs.push_back(v);
+    int16_t* d_vals = (int16_t*)sl.d_vals.get((size_t)n_elem * 2 + 64);
+    uint32_t* d_prot = (uint32_t*)sl.d_prot.get((size_t)n_elem * 4 + 64);
+    const CompactJob* d_jobs = upload_vec(sl.d_jobs, jobs, sl.stream);
+    const void* const* d_ptrs = upload_vec(sl.d_round_ptrs, ptrs, sl.stream);
+    if (!d_vals || !d_prot || !d_jobs || !d_ptrs) return -1;
+    {
+        CompactArgs a{};
+        a.jobs = d_jobs;
+        a.n_jobs = (int64_t)jobs.size();
+        a.round_vals = (const int16_t* const*)d_ptrs;
+        a.round_prot = (const uint32_t* const*)(d_ptrs + round_vals.size());
+        a.vals = d_vals;
+        a.prot = d_prot;
+        if (check(tpz_launch_compact(&a, sl.stream), "compact_kernel")) return -1;
     }
     const double t_streams = now_s();
     st.seed_s += t_streams - t0;
 
-    // 3. the walk of every query (host, parallel); a stream cut at the cap
-    // is refilled on demand, one GPU call per refill
-    std::mutex gpu_mu;
-    int walk_rc = 0;
+    // 3. the walk of every query, one wavefront each
     const int32_t max_score = 255 * std::max(1, db->max_w);
-    pool().run(b.n, [&](int64_t kk) {
-        const int32_t k = (int32_t)kk;
-        QueryWork& w = qw[k];
-        Walk walk;
-        walk.reset(opt.o.number_of_seeds, opt.capacity(b.q_len[k]), max_score);
-        std::vector<int32_t> local;      // walk stream -> batch stream
-        for (int32_t ls = 0; ls < w.n_streams; ++ls) {
-            std::vector<int16_t>& v = w.vals[ls];
-            if (v.empty() || v[0] == kStreamEnd) {     // never created (query.c:119, 127)
-                std::vector<int16_t>().swap(v);
-                std::vector<uint32_t>().swap(w.prot[ls]);
-                continue;
-            }
-            const StreamDef& d = defs[w.first_stream + ls];
-            walk.add_stream(d.offset, d.first, d.dir, std::move(v), std::move(w.prot[ls]),
-                            !w.open[ls]);
-            local.push_back(w.first_stream + ls);
+    std::vector<int32_t> q_stream(b.n + 1), q_cap(b.n);
+    std::vector<int64_t> q_mem(b.n + 1, 0), q_out(b.n + 1, 0);
+    std::vector<int32_t> slen32((size_t)n_streams);
+    for (int64_t s = 0; s < n_streams; ++s) slen32[s] = (int32_t)slen[s];
+    for (int32_t k = 0; k < b.n; ++k) {
+        q_stream[k] = qw[k].first_stream;
+        q_cap[k] = opt.capacity(b.q_len[k]);
+        q_mem[k + 1] = q_mem[k] +
+                       (int64_t)walk_mem_layout(qw[k].n_streams, q_cap[k], max_score).total;
+        q_out[k + 1] = q_out[k] + q_cap[k];
+    }
+    q_stream[b.n] = (int32_t)n_streams;
+    const int64_t n_out = q_out[b.n];
+    WalkArgs wa{};
+    wa.s_base = upload_vec(sl.d_sbase, sbase, sl.stream);
+    wa.s_len = upload_vec(sl.d_slen, slen32, sl.stream);
+    wa.s_open = upload_vec(sl.d_sopen, sopen, sl.stream);
+    wa.vals = d_vals;
+    wa.prot = d_prot;
+    wa.q_stream = upload_vec(sl.d_qstream, q_stream, sl.stream);
+    wa.q_capacity = upload_vec(sl.d_qcap, q_cap, sl.stream);
+    wa.q_mem = upload_vec(sl.d_qmem, q_mem, sl.stream);
+    wa.mem = (char*)sl.d_mem.get((size_t)q_mem[b.n] + 256);
+    wa.q_out = upload_vec(sl.d_qout, q_out, sl.stream);
+    wa.out_index = (uint32_t*)sl.d_oidx.get((size_t)n_out * 4 + 64);
+    wa.out_score = (int32_t*)sl.d_oscore.get((size_t)n_out * 4 + 64);
+    wa.out_n = (int32_t*)sl.d_on.get((size_t)b.n * 4 + 64);
+    wa.out_status = (int32_t*)sl.d_ostatus.get((size_t)b.n * 4 + 64);
+    wa.out_stats = (int64_t*)sl.d_ostats.get((size_t)b.n * 24 + 64);
+    wa.n_queries = b.n;
+    wa.limit = opt.o.number_of_seeds;
+    wa.max_score = max_score;
+    uint32_t* h_idx = (uint32_t*)sl.h_oidx.get((size_t)n_out * 4 + 64);
+    int32_t* h_sc = (int32_t*)sl.h_oscore.get((size_t)n_out * 4 + 64);
+    if (!wa.s_base || !wa.s_len || !wa.s_open || !wa.q_stream || !wa.q_capacity || !wa.q_mem ||
+        !wa.mem || !wa.q_out || !wa.out_index || !wa.out_score || !wa.out_n || !wa.out_status ||
+        !wa.out_stats || !h_idx || !h_sc)
+        return -1;
+    std::vector<int32_t> h_n(b.n), h_status(b.n);
+    std::vector<int64_t> h_stats((size_t)b.n * 3);
+    if (check(tpz_launch_walk(&wa, sl.stream), "walk_kernel") ||
+        check(hipMemcpyAsync(h_n.data(), wa.out_n, (size_t)b.n * 4, hipMemcpyDeviceToHost,
+                             sl.stream), "download") ||
+        check(hipMemcpyAsync(h_status.data(), wa.out_status, (size_t)b.n * 4,
+                             hipMemcpyDeviceToHost, sl.stream), "download") ||
+        check(hipMemcpyAsync(h_stats.data(), wa.out_stats, (size_t)b.n * 24,
+                             hipMemcpyDeviceToHost, sl.stream), "download") ||
+        check(hipMemcpyAsync(h_idx, wa.out_index, (size_t)n_out * 4, hipMemcpyDeviceToHost,
+                             sl.stream), "download") ||
+        check(hipMemcpyAsync(h_sc, wa.out_score, (size_t)n_out * 4, hipMemcpyDeviceToHost,
+                             sl.stream), "download") ||
+        check(hipStreamSynchronize(sl.stream), "walk"))
+        return -1;
+    st.gpu_s += now_s() - g0;
+    std::vector<int32_t> need;
+    for (int32_t k = 0; k < b.n; ++k) {
+        if (h_status[k] == kWalkBroken)
+            return fail("tpz_search: the walk of query %d found its hit list inconsistent", k);
+        st.pops += h_stats[3 * (size_t)k];
+        st.runs += h_stats[3 * (size_t)k + 1];
+        if (h_status[k] == kWalkNeed) {
+            qw[k].need = true;
+            need.push_back(k);
+            continue;
         }
-        auto protein_of = [&](int32_t s, int32_t i, int64_t) { return walk.protein(s, i); };
-        std::vector<int32_t> want(local.size(), 0);
-        while (walk.run(protein_of) == Walk::kNeed) {
-            const int32_t ws = walk.waiting();
-            const int32_t n = want[ws] = std::max(2 * want[ws], 4096);
-            std::vector<ChunkReq> one{ChunkReq{local[ws], walk.next_slot(ws), n, 0}};
-            const int16_t* hv = nullptr;
-            const uint32_t* hp = nullptr;
-            std::lock_guard<std::mutex> lk(gpu_mu);
-            double dummy = 0;
-            if (run_chunks(db, sl, defs, one, n, b.n, min_len, &hv, &hp, &dummy)) {
-                walk_rc = -1;
-                return;
-            }
-            walk.supply(ws, hv, hp, n, false);
-        }
-        walk.hits().traversal(w.hit_index, w.hit_gapless);
-        w.walk = walk.stats();
-        w.counted = walk.counted();
-    });
-    if (walk_rc) return -1;
-    for (const QueryWork& w : qw) {
-        st.pops += w.walk.pops;
-        st.runs += w.walk.runs;
-        st.refills += w.walk.refills;
-        st.hits += (int64_t)w.hit_index.size();
+        if (h_n[k] < 0 || h_n[k] > q_cap[k])
+            return fail("tpz_search: the walk of query %d returned %d hits", k, h_n[k]);
+        qw[k].hit_index.assign(h_idx + q_out[k], h_idx + q_out[k] + h_n[k]);
+        qw[k].hit_gapless.assign(h_sc + q_out[k], h_sc + q_out[k] + h_n[k]);
+        st.hits += h_n[k];
     }
     const double t_walk = now_s();
     st.walk_s += t_walk - t_streams;
@@ -774,6 +803,26 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
             }
         }
         st.stats_s += now_s() - t_sw;
+    }
+
+    // queries whose walk ran into a stream cut at max_stream_elements: the
+    // whole search again for them with a cap 16 times higher
+    if (!need.empty()) {
+        st.refills += (int64_t)need.size();
+        Options more = opt;
+        more.o.max_stream_elements =
+            cap_elems > INT64_MAX / 16 ? INT64_MAX : cap_elems * 16;
+        std::vector<int64_t> n_off(need.size());
+        std::vector<int32_t> n_len(need.size());
+        for (size_t i = 0; i < need.size(); ++i) {
+            n_off[i] = b.q_off[need[i]];
+            n_len[i] = b.q_len[need[i]];
+        }
+        std::vector<std::vector<tpz_hit>> again;
+        if (search_batch(db, more, sl, Batch{(int32_t)need.size(), b.q_chars, n_off.data(),
+                                             n_len.data()}, again, st))
+            return -1;
+        for (size_t i = 0; i < need.size(); ++i) out[need[i]] = std::move(again[i]);
     }
     return 0;
 }
@@ -962,15 +1011,19 @@ namespace tpz {
 namespace {
 
 // Device bytes a request occupies while its wave runs: query copies and
-// offset records, the walk streams' elements (estimated per offset from
-// the waves so far) and segments, and its share of the Smith-Waterman
-// request (pairs of at most a full hit list per query).
+// offset records; per offset two walk streams with their definitions,
+// round requests, layout and copy jobs, and their elements (estimated per
+// offset from the waves so far) both in the rounds' outputs and laid out
+// for the walk; the queries' walk state and hit lists (walk_bytes); and
+// their share of the Smith-Waterman request (pairs of at most a full hit
+// list per query).
 size_t request_device_bytes(const tpz_search_server* s, int64_t residues, int64_t n_queries,
-                            int64_t offsets) {
-    const double per_offset = 16.0 + 2.5 * s->elements_per_offset;
+                            int64_t offsets, int64_t walk_bytes) {
+    const double per_offset = 250.0 + 12.0 * s->elements_per_offset;
     const int64_t hits = (int64_t)std::max(s->opt.o.number_of_alignments,
                                            (int32_t)(1.5 * s->opt.o.num_hits));
-    return (size_t)(4 * residues + per_offset * (double)offsets + 48.0 * hits * n_queries) + 4096;
+    return (size_t)(8 * residues + per_offset * (double)offsets + (double)walk_bytes +
+                    48.0 * hits * n_queries) + 4096;
 }
 
 void search_lane(tpz_search_server* s) {
@@ -1102,16 +1155,24 @@ int tpz_search_server_submit(tpz_search_server* s, int32_t n_queries, const char
         return fail("tpz_search_server_submit: bad queries");
     auto r = std::make_unique<SearchRequest>();
     const Maps& M = maps();
+    const int32_t max_score = 255 * std::max(1, s->db->max_w);
+    int64_t walk_bytes = 0;
     for (int32_t k = 0; k < n_queries; ++k) {
         if (q_len[k] < 0) return fail("query %d has negative length", k);
         r->off.push_back((int64_t)r->chars.size());
         r->len.push_back(q_len[k]);
         r->chars.insert(r->chars.end(), q_chars + q_off[k], q_chars + q_off[k] + q_len[k]);
+        int64_t offs = 0;
         for (int32_t i = 0; i + s->opt.o.min_hit_length < q_len[k]; ++i)
-            r->offsets += M.upper_only[(unsigned char)q_chars[q_off[k] + i]] != 'V';
+            offs += M.upper_only[(unsigned char)q_chars[q_off[k] + i]] != 'V';
+        r->offsets += offs;
+        const int32_t cap = s->opt.capacity(q_len[k]);
+        walk_bytes += (int64_t)walk_mem_layout((int32_t)std::min<int64_t>(2 * offs, INT32_MAX - 1),
+                                               cap, max_score).total + 8LL * cap;
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    r->bytes = request_device_bytes(s, (int64_t)r->chars.size(), n_queries, r->offsets);
+    r->bytes = request_device_bytes(s, (int64_t)r->chars.size(), n_queries, r->offsets,
+                                    walk_bytes);
     if (r->bytes > s->cfg.memory_budget - s->fixed)
         return fail("tpz_search_server_submit: the request needs about %zu device bytes, more "
                     "than the %llu the budget leaves beside the fixed workspace", r->bytes,
