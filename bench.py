@@ -555,13 +555,18 @@ def search_leg(args):
     import shutil
     import tempfile
     from pathlib import Path
-    from tests.test_dropin_gpu import REF, _link_engine
+    from tests.test_dropin_gpu import REF, _link_dropin, _link_engine
     from topaz_amd import index, synth
     n_db, n_q = args.search_db, args.search_queries
     sample = min(args.search_sample, n_q)
     hi = 2000 if n_db <= 100000 else 5000
     hom = 0.2 if n_db <= 100000 else 0.1
-    work = Path(tempfile.mkdtemp(prefix="tpz_search_", dir=args.workdir))
+    # with --workdir the files go to <workdir>/search (the profiles name them there)
+    if args.workdir:
+        work = Path(args.workdir) / "search"
+        work.mkdir(parents=True, exist_ok=True)
+    else:
+        work = Path(tempfile.mkdtemp(prefix="tpz_search_"))
     out = {"workload": "search: %d database proteins (lognormal mean 350 sd 250 clipped [30,%d]), "
                        "%d queries, %d%% with 1-5 planted homologs at 30-90%% identity; -H 100 "
                        "-E 1e-3, default and -F" % (n_db, hi, n_q, int(hom * 100)),
@@ -589,7 +594,8 @@ def search_leg(args):
         gpu_env = {"TOPAZ_AMD_SEARCH_STATS": "1", "TOPAZ_AMD_SEARCH_BATCH": batch}
         plain_n = min(args.plain_sample, sample)
         synth.write_queries_fasta(work / "qp.fa", (q[0], q[1][:plain_n], q[2][:plain_n]))
-        modes = [m for m in (("default", []), ("fast", ["-F"])) if m[0] in args.search_modes]
+        modes = [m for m in (("default", []), ("fast", ["-F"]), ("blasttab", ["-B"]))
+                 if m[0] in args.search_modes.split(",")]
         for mode, flags in modes:
             f = ["-H", "100", "-E", "0.001"] + flags
             ref_txt, ref_i = _run_search(REF / "topaz", work, work / "DB", work / "qs.fa", f, cores,
@@ -610,6 +616,20 @@ def search_leg(args):
                                 all(ref_b.get(k) == v for k, v in pl_b.items()) and
                                 all(k in pl_b for k in ref_b if int(k[1:]) < plain_n),
                    "hit_lines": sum(len(v) for v in ref_b.values())}
+            if args.dropin_sample > 0:
+                # the link-level drop-in (one library call per seed and per
+                # hit) and the batch loops (query.c's and suffix.c's loops
+                # over the batch entry points) on the first queries
+                nd = min(args.dropin_sample, sample)
+                synth.write_queries_fasta(work / "qd.fa", (q[0], q[1][:nd], q[2][:nd]))
+                ref_d = {k: v for k, v in ref_b.items() if int(k[1:]) < nd}
+                for name, batch in (("dropin_per_call", False), ("batch_loops", True)):
+                    txt, inf = _run_search(_link_dropin(work, batch=batch), work, work / "DB",
+                                           work / "qd.fa", f, cores, {},
+                                           "%s %s x%d" % (mode, name, nd))
+                    row[name] = dict(inf, queries=nd, ms_per_query=round(1e3 * inf["wall_s"] / nd, 2),
+                                     identical=_per_query(txt) == ref_d)
+                    row["identical"] = row["identical"] and row[name]["identical"]
             if n_q > sample:
                 _, all_i = _run_search(exe, work, work / "DB", work / "q.fa", f, 4, gpu_env,
                                        "%s gpu x%d" % (mode, n_q))
@@ -734,6 +754,59 @@ def serve_search_leg(args, rates=(1, 2, 5, 10, 20, 50, 100, 200)):
             "max_requests_per_wave": sst["max_requests_per_wave"]}
 
 
+def index_leg(args):
+    """`topaz index` of a synthetic database: the reference binary (its own
+    libdivsufsort, one core) against TOPAZ linked with the library's
+    divsufsort64 (suffix array built on the GPU); both must write identical
+    index files.  Lognormal lengths (mean 350, sd 250, clipped [30, 2000]),
+    1 % of the proteins repeated exactly, as real databases hold identical
+    sequences."""
+    import filecmp
+    import shutil
+    import subprocess
+    import tempfile
+    from pathlib import Path
+    from tests.test_dropin_gpu import REF, _link_index
+    from topaz_amd import synth
+    if not (REF / "topaz").exists():
+        raise SystemExit("reference checker not built (make -C oracle ref)")
+    tmp = Path(tempfile.mkdtemp(prefix="tpz_index_", dir=args.workdir))
+    try:
+        rng = np.random.default_rng(2029)
+        t0 = time.perf_counter()
+        db, _ = synth.search_database(rng, args.index_db, 1, hom_frac=0.0, hi=2000)
+        db += [db[k] for k in rng.integers(0, len(db), max(1, args.index_db // 100))]
+        synth.write_fasta(tmp / "db.fa", db, "s")
+        print("[bench] index: %d proteins written in %.1f s" % (len(db), time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
+        exe = _link_index(tmp)
+        times = {}
+        for name, binary in (("reference_cpu", REF / "topaz"), ("gpu_sa", exe)):
+            (tmp / name).mkdir(exist_ok=True)
+            t0 = time.perf_counter()
+            r = subprocess.run([str(binary), "index", "-f", str(tmp / "db.fa"), "-p",
+                                str(tmp / name / "DB")], capture_output=True, text=True,
+                               timeout=1000, cwd=str(tmp))
+            times[name] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise SystemExit("%s index failed: %s" % (name, r.stderr[-2000:]))
+            print("[bench] index %s: %.2f s" % (name, times[name]), file=sys.stderr, flush=True)
+        files = sorted(p.name for p in (tmp / "reference_cpu").iterdir())
+        identical = files == sorted(p.name for p in (tmp / "gpu_sa").iterdir()) and all(
+            filecmp.cmp(tmp / "reference_cpu" / f, tmp / "gpu_sa" / f, shallow=False)
+            for f in files)
+        text_bytes = os.path.getsize(tmp / "reference_cpu" / "DB.psq")
+    finally:
+        if not args.keep:
+            shutil.rmtree(tmp, ignore_errors=True)
+    return {"workload": "topaz index of %d proteins (lognormal mean 350 sd 250 clipped [30,2000], "
+                        "1%% repeated exactly)" % len(db),
+            "proteins": len(db), "text_bytes": text_bytes,
+            "index_s": {k: round(v, 2) for k, v in times.items()},
+            "speedup": round(times["reference_cpu"] / times["gpu_sa"], 2),
+            "files_identical": identical, "files": files}
+
+
 def copy_leg(args):
     """A plain device-to-device copy of 1 GiB, three times: the calibration
     run for the HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch of
@@ -764,7 +837,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default configuration")
     ap.add_argument("--leg", choices=["all", "sw", "gapless", "serve", "stats", "sa", "copy",
-                                      "search", "serve-search"], default="all")
+                                      "search", "serve-search", "index"], default="all")
     ap.add_argument("--search-db", type=int, default=100000,
                     help="database proteins of the search leg")
     ap.add_argument("--search-queries", type=int, default=1000)
@@ -773,7 +846,10 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=1000,
                     help="queries the plain schedule searches")
     ap.add_argument("--search-modes", default="default,fast",
-                    help="search modes to run: default and/or fast (-F)")
+                    help="search modes to run, comma-separated: default, fast (-F), blasttab (-B)")
+    ap.add_argument("--dropin-sample", type=int, default=0,
+                    help="queries the link-level drop-in and the batch loops search (0: neither)")
+    ap.add_argument("--index-db", type=int, default=1000000, help="proteins of the index leg")
     ap.add_argument("--workdir", default=None, help="where the search leg writes its index")
     ap.add_argument("--keep", action="store_true", help="keep the search leg's files")
     ap.add_argument("--serve-requests", type=int, default=100, help="requests per arrival rate")
@@ -801,6 +877,10 @@ def main():
         return
     if args.leg == "copy":
         print(json.dumps(copy_leg(args)), flush=True)
+        return
+    if args.leg == "index":
+        print(json.dumps({"metric": "topaz index wall seconds, GPU suffix array vs reference CPU",
+                          "index": index_leg(args)}), flush=True)
         return
     if args.leg == "sa":
         print(json.dumps({"metric": "suffix array of the database text", "suffix_array": sa_leg(args)}),

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <utility>
+#include <functional>
 #include <vector>
 
 namespace tpz {
@@ -63,15 +64,23 @@ struct DeviceTasks {
     }
 };
 
-// Pairs of each query, longest subject first (ties: lower pair number).
-std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pair_query,
-                                                 const std::vector<int32_t>& pair_subject,
-                                                 const std::vector<int32_t>& s_len,
-                                                 int32_t n_queries);
+// Pairs of each query, longest subject first (ties: lower pair number):
+// query q's pairs are pair[begin[q] .. begin[q + 1]), and len[] holds the
+// length of each one's subject.
+struct QueryPairs {
+    std::vector<int64_t> begin;
+    std::vector<int32_t> pair, len;
+    int64_t count(int32_t q) const { return begin[q + 1] - begin[q]; }
+};
+QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
+                          const std::vector<int32_t>& pair_subject,
+                          const std::vector<int32_t>& s_len, int32_t n_queries);
 // Up to nq_max queries of equal chunk count per task (the persistent schedule).
-void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
-                        const std::vector<int32_t>& pair_subject, const std::vector<int32_t>& q_len,
-                        const std::vector<int32_t>& s_len, int rows, int nq_max, Tasks& t);
+void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len, int rows,
+                        int nq_max, Tasks& t);
+// fn(i) for i in [0, n) on up to `threads` host threads (the caller's
+// included); contiguous index ranges per thread.
+void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& range_fn);
 // Work items of the persistent schedule (`waves` entries each), most
 // expensive first.
 void build_items(const Tasks& t, int n_slots, double big_fraction, int waves,

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/topaz_amd.h"
@@ -83,17 +84,61 @@ static int64_t chunk_rows(int32_t m, int rows) {
 }
 
 // Pairs of each query, longest subject first (ties: lower pair number).
-std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pair_query,
-                                                        const std::vector<int32_t>& pair_subject,
-                                                        const std::vector<int32_t>& s_len,
-                                                        int32_t n_queries) {
-    std::vector<std::vector<int32_t>> per(n_queries);
-    for (int32_t p = 0; p < (int32_t)pair_query.size(); ++p) per[pair_query[p]].push_back(p);
-    for (auto& v : per)
-        std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
-            return s_len[pair_subject[x]] > s_len[pair_subject[y]];
-        });
-    return per;
+void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& range_fn) {
+    if (n <= 0) return;
+    const int t = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
+    std::vector<std::thread> th;
+    for (int k = 1; k < t; ++k)
+        th.emplace_back([&, k] { range_fn(n * k / t, n * (k + 1) / t); });
+    range_fn(0, n / t);
+    for (auto& x : th) x.join();
+}
+
+static int host_threads() {
+    static const int n = [] {
+        int v = (int)std::thread::hardware_concurrency();
+        if (const char* e = getenv("TOPAZ_AMD_HOST_THREADS")) v = atoi(e);
+        return std::max(1, std::min(v, 64));
+    }();
+    return n;
+}
+
+// A counting sort by query (pairs stay in pair order), then per query a sort
+// of (subject length descending, pair number) keys; each subject length is
+// read once.  Large lists are split over host threads.
+QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
+                          const std::vector<int32_t>& pair_subject,
+                          const std::vector<int32_t>& s_len, int32_t n_queries) {
+    QueryPairs r;
+    const int64_t n = (int64_t)pair_query.size();
+    const int threads = n >= (1 << 18) ? host_threads() : 1;
+    r.begin.assign((size_t)n_queries + 1, 0);
+    for (int64_t p = 0; p < n; ++p) ++r.begin[pair_query[p] + 1];
+    for (int32_t q = 0; q < n_queries; ++q) r.begin[q + 1] += r.begin[q];
+    r.pair.resize((size_t)n);
+    r.len.resize((size_t)n);
+    {
+        std::vector<int64_t> at(r.begin.begin(), r.begin.end() - 1);
+        for (int64_t p = 0; p < n; ++p) r.pair[at[pair_query[p]]++] = (int32_t)p;
+    }
+    parallel_for(n_queries, threads, [&](int64_t q0, int64_t q1) {
+        std::vector<uint64_t> key;
+        for (int64_t q = q0; q < q1; ++q) {
+            const int64_t b = r.begin[q], e = r.begin[q + 1];
+            key.resize((size_t)(e - b));
+            for (int64_t i = b; i < e; ++i) {
+                const int32_t p = r.pair[i];
+                key[i - b] = ((uint64_t)(uint32_t)(INT32_MAX - s_len[pair_subject[p]]) << 32) |
+                             (uint32_t)p;
+            }
+            std::sort(key.begin(), key.end());
+            for (int64_t i = b; i < e; ++i) {
+                r.pair[i] = (int32_t)(uint32_t)key[i - b];
+                r.len[i] = INT32_MAX - (int32_t)(key[i - b] >> 32);
+            }
+        }
+    });
+    return r;
 }
 
 // Tasks of up to nq (<= 4) queries (the persistent schedule).  Queries with
@@ -103,21 +148,18 @@ std::vector<std::vector<int32_t>> pairs_by_query(const std::vector<int32_t>& pai
 // subjects then share a wavefront with other queries' long subjects instead
 // of its own short ones, which cuts the columns a wavefront walks past the
 // end of a lane's subject.
-void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
-                               const std::vector<int32_t>& pair_subject,
-                               const std::vector<int32_t>& q_len,
-                               const std::vector<int32_t>& s_len, int rows, int nq_max,
-                               Tasks& t) {
+void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len, int rows,
+                        int nq_max, Tasks& t) {
     t.clear();
-    const int32_t nq = (int32_t)per.size();
+    const int32_t nq = (int32_t)per.begin.size() - 1;
     std::vector<int32_t> order;
     for (int32_t q = 0; q < nq; ++q)
-        if (!per[q].empty()) order.push_back(q);
+        if (per.count(q) > 0) order.push_back(q);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         return chunk_rows(q_len[x], rows) < chunk_rows(q_len[y], rows);
     });
-    std::vector<size_t> head(nq, 0);
-    auto head_len = [&](int32_t q) { return s_len[pair_subject[per[q][head[q]]]]; };
+    std::vector<int64_t> head(per.begin.begin(), per.begin.end() - 1);
+    auto head_len = [&](int32_t q) { return per.len[head[q]]; };
     // max-heap on (longest remaining subject, then lower query number)
     auto below = [&](int32_t x, int32_t y) {
         const int32_t lx = head_len(x), ly = head_len(y);
@@ -143,14 +185,14 @@ void build_packed_tasks(const std::vector<std::vector<int32_t>>& per,
                 int best = -1;
                 for (int k = 0; k < ns; ++k) {
                     const int32_t q = sel[k];
-                    if (head[q] < per[q].size() && (best < 0 || below(sel[best], q))) best = k;
+                    if (head[q] < per.begin[q + 1] && (best < 0 || below(sel[best], q))) best = k;
                 }
                 if (best < 0) break;
                 const int32_t q = sel[best];
-                t.index.push_back(per[q][head[q]++]);
+                t.index.push_back(per.pair[head[q]++]);
             }
             for (int k = 0; k < ns; ++k) {
-                if (head[sel[k]] < per[sel[k]].size()) {
+                if (head[sel[k]] < per.begin[sel[k] + 1]) {
                     heap.push_back(sel[k]);
                     std::push_heap(heap.begin(), heap.end(), below);
                 }
@@ -202,7 +244,7 @@ struct tpz_sw_batch {
     int32_t gap_open = 11, gap_extend = 1, bias = 0;
     int32_t max_subject = 0;
     std::vector<int32_t> q_len, s_len, pair_query, pair_subject;
-    std::vector<std::vector<int32_t>> per_query;    // pairs of each query, longest subject first
+    QueryPairs per_query;                           // pairs of each query, longest subject first
     DeviceBuffer d_q_res, d_q_off, d_q_len, d_s_res, d_s_off, d_s_len;
     DeviceBuffer d_pair_query, d_pair_subject, d_matrix;
     DeviceBuffer d_score, d_ref, d_read;            // results
@@ -284,7 +326,7 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
     if (check_config(c)) return -1;
     auto plan = std::make_unique<SwPlan>();
     plan->cfg = c;
-    build_packed_tasks(b->per_query, b->pair_subject, b->q_len, b->s_len, c.rows_per_chunk,
+    build_packed_tasks(b->per_query, b->q_len, c.rows_per_chunk,
                        c.queries_per_task, plan->packed_tasks.host);
     int rc = plan->packed_tasks.upload();
     const int W = c.waves_per_group;
@@ -298,7 +340,8 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
             plan->plain_begin.push_back((int32_t)pairs.size());
             const bool passes = b->q_len[q] > 64 * tpz_sw_pair_rows(b->q_len[q]);
             int64_t ws = 0;
-            for (int32_t p : b->per_query[q]) {
+            for (int64_t i = b->per_query.begin[q]; i < b->per_query.begin[q + 1]; ++i) {
+                const int32_t p = b->per_query.pair[i];
                 pairs.push_back(p);
                 ws_off.push_back(ws);
                 if (passes) ws += 2 * (int64_t)b->s_len[b->pair_subject[p]];

@@ -706,15 +706,16 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
             return -1;
     }
     st.pairs += n_pairs;
-    for (int64_t p = 0; p < n_pairs; ++p)
-        st.cells += (int64_t)b.q_len[pq[p]] * db->len[ps[p]];
     const double t_sw = now_s();
     st.align_s += t_sw - t_walk;
 
-    // 5. hits in output order (engine.c:193-234), E-values (ssw_wrapper.c:171-245)
+    // 5. hits in output order (engine.c:193-234), E-values (ssw_wrapper.c:171-245),
+    // in parallel over queries
     const tpz_search_options& o = opt.o;
     std::vector<std::vector<int32_t>> stat_hits(b.n);   // reported hits needing statistics
-    for (int32_t k = 0; k < b.n; ++k) {
+    std::vector<int64_t> q_cells(b.n, 0);
+    pool().run(b.n, [&](int64_t kk) {
+        const int32_t k = (int32_t)kk;
         const QueryWork& w = qw[k];
         const int32_t m = b.q_len[k];
         const double correction = log(o.k * m * (double)o.db_size) / o.H;
@@ -728,6 +729,7 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
             x.qstart = x.qend = x.sstart = x.send = -1;
             x.length = x.mismatch = x.gapopen = -1;
             const int64_t p = pair_of[hit_base[k] + h];
+            if (p >= 0) q_cells[k] += (int64_t)b.q_len[k] * db->len[ps[p]];
             if (p < 0) {
                 // not aligned (gapless score < 40): hit_alloc's fields
                 // (hit.c:15-38) but for bitscore and E-value
@@ -760,7 +762,8 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
             if (x.evalue <= o.evalue) res.push_back(x);
         if (!o.superfast)
             for (size_t i = 0; i < res.size(); ++i) stat_hits[k].push_back((int32_t)i);
-    }
+    });
+    for (int32_t k = 0; k < b.n; ++k) st.cells += q_cells[k];
 
     // 6. traceback statistics of the reported hits (query.c:250-299)
     if (!o.superfast) {

@@ -470,8 +470,8 @@ int tpz_server_submit(tpz_server* s, int32_t n_queries, const int8_t* q_res,
         return fail("request needs %zu device bytes, more than the budget of %llu", r->bytes,
                     (unsigned long long)s->cfg.memory_budget);
     {
-        const auto per = pairs_by_query(r->pq, r->ps, s->s_len, n_queries);
-        build_packed_tasks(per, r->ps, r->q_len, s->s_len, 32, kTaskQueries, r->tasks);
+        const QueryPairs per = pairs_by_query(r->pq, r->ps, s->s_len, n_queries);
+        build_packed_tasks(per, r->q_len, 32, kTaskQueries, r->tasks);
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return fail("server stopped");
