@@ -19,10 +19,17 @@
 // pops a stream in runs: a run starts at an element smaller than every
 // earlier element of its stream (a strict prefix minimum, the run's leader)
 // and takes every following element >= the leader.  Runs of one stream have
-// strictly decreasing leaders.  Popping whole runs from a max-heap keyed by
+// strictly decreasing leaders.  Popping whole runs from a queue ordered by
 // (leader, insertion number) -- a run inserted when the run before it in its
 // stream ends -- gives the reference's pop sequence element for element, with
-// one heap operation per run instead of per element.
+// one queue operation per run instead of per element.
+//
+// Run queue.  A run pushed while popping a run with leader L has a leader
+// below L, so the leaders popped never increase; and among equal leaders the
+// newest run pops first.  The queue is therefore a stack per leader score
+// (top[score]) with a pointer that only moves down, and since a stream has
+// at most one run waiting, the stacks are threaded through one node per
+// stream (its run's first element and the stream below it on its stack).
 //
 // Hit list.  At most `capacity` proteins with their best gapless score, in
 // the priority tree's order: (score, insertion number), a re-put counting as
@@ -50,11 +57,9 @@ constexpr int16_t kStreamEnd = -1;
 
 #define TPZ_HD __host__ __device__ __forceinline__
 
-struct WalkRun {
-    int32_t leader;
-    uint32_t seq;
-    int32_t stream;
-    int32_t start;
+struct WalkNode {
+    int32_t start;               // first element of the stream's waiting run
+    int32_t next;                // the stream below it on its stack, -1: none
 };
 
 struct WalkEntry {
@@ -74,7 +79,8 @@ struct WalkStreams {
 };
 
 struct WalkMem {
-    WalkRun* heap;               // n_streams entries
+    int32_t* top;                // max_score + 1, -1 initially: stream on top per leader
+    WalkNode* node;              // n_streams entries
     WalkEntry* entry;            // capacity + 1
     int32_t* free_stack;         // capacity + 1
     int32_t* head;               // max_score + 1, -1 initially
@@ -111,37 +117,6 @@ TPZ_HD uint32_t walk_mix(uint32_t k) {
     k *= 0x846ca68bU;
     k ^= k >> 16;
     return k;
-}
-
-TPZ_HD bool run_below(const WalkRun& a, const WalkRun& b) {
-    return a.leader != b.leader ? a.leader < b.leader : a.seq < b.seq;
-}
-
-TPZ_HD void heap_push(WalkRun* h, int32_t& n, WalkRun r) {
-    int32_t i = n++;
-    while (i > 0) {
-        const int32_t p = (i - 1) >> 1;
-        if (!run_below(h[p], r)) break;
-        h[i] = h[p];
-        i = p;
-    }
-    h[i] = r;
-}
-
-TPZ_HD WalkRun heap_pop(WalkRun* h, int32_t& n) {
-    const WalkRun top = h[0];
-    const WalkRun last = h[--n];
-    int32_t i = 0;
-    for (;;) {
-        int32_t c = 2 * i + 1;
-        if (c >= n) break;
-        if (c + 1 < n && run_below(h[c], h[c + 1])) ++c;
-        if (!run_below(last, h[c])) break;
-        h[i] = h[c];
-        i = c;
-    }
-    if (n > 0) h[i] = last;
-    return top;
 }
 
 // The table is at most half full, so every probe sequence meets an empty
@@ -276,19 +251,34 @@ TPZ_HD WalkResult walk_query(const WalkStreams& st, const WalkMem& m, int32_t ca
                              int32_t limit, int32_t max_score, const Scan& scan,
                              uint32_t* out_index, int32_t* out_score) {
     WalkResult res{kWalkDone, 0, 0, 0, 0};
-    int32_t heap_n = 0;
-    uint32_t seq = 0;
-    for (int32_t s = 0; s < st.n; ++s)
-        if (st.len[s] > 0) heap_push(m.heap, heap_n, WalkRun{st.vals[st.base[s]], seq++, s, 0});
+    // the streams in creation order, each with its first run (element 0)
+    int32_t queued = 0;
+    for (int32_t s = 0; s < st.n; ++s) {
+        if (st.len[s] <= 0) continue;
+        const int32_t x = st.vals[st.base[s]];
+        if (x < 0 || x > max_score) {
+            res.status = kWalkBroken;
+            return res;
+        }
+        m.node[s].start = 0;
+        m.node[s].next = m.top[x];
+        m.top[x] = s;
+        ++queued;
+    }
+    int32_t cur = max_score;           // no run waits above it
     int32_t size = 0, n_entries = 0, n_free = 0, min_score = max_score + 1;
     bool stop = false, broken = false;
-    while (heap_n > 0 && !stop) {
-        const WalkRun r = heap_pop(m.heap, heap_n);
+    while (queued > 0 && !stop) {
+        while (m.top[cur] < 0) --cur;
+        const int32_t stream = m.top[cur];
+        const int32_t leader = cur, start = m.node[stream].start;
+        m.top[cur] = m.node[stream].next;
+        --queued;
         ++res.runs;
-        const int16_t* v = st.vals + st.base[r.stream];
-        const uint32_t* pr = st.prot + st.base[r.stream];
-        const int32_t len = st.len[r.stream];
-        int32_t i = r.start;
+        const int16_t* v = st.vals + st.base[stream];
+        const uint32_t* pr = st.prot + st.base[stream];
+        const int32_t len = st.len[stream];
+        int32_t i = start;
         for (;;) {
             const bool full = size >= capacity;
             if (full)
@@ -297,7 +287,7 @@ TPZ_HD WalkResult walk_query(const WalkStreams& st, const WalkMem& m, int32_t ca
             // hit scores >= it (a list of capacity 0 has no worst hit and
             // counts everything: the reference dereferences NULL there)
             const int32_t worst = size == 0 ? 0x7fffffff : min_score;
-            const int32_t j = scan(v, i, len, r.start, r.leader, !full, worst);
+            const int32_t j = scan(v, i, len, start, leader, !full, worst);
             const int32_t n_fast = j - i;
             if (n_fast > 0) {
                 if (res.counted < limit && res.counted + n_fast >= limit) {
@@ -309,10 +299,20 @@ TPZ_HD WalkResult walk_query(const WalkStreams& st, const WalkMem& m, int32_t ca
                 res.counted += n_fast;
                 res.pops += n_fast;
             }
-            if (run_ends_at(v, j, len, r.start, r.leader)) {
-                if (j < len && v[j] != kStreamEnd)
-                    heap_push(m.heap, heap_n, WalkRun{v[j], seq++, r.stream, j});
-                else if (j >= len && st.open && st.open[r.stream]) {
+            if (run_ends_at(v, j, len, start, leader)) {
+                if (j < len && v[j] != kStreamEnd) {
+                    // the stream's next run, on top of its leader's stack
+                    // (v[j] < leader)
+                    if (v[j] < 0) {
+                        res.status = kWalkBroken;
+                        stop = true;
+                        break;
+                    }
+                    m.node[stream].start = j;
+                    m.node[stream].next = m.top[v[j]];
+                    m.top[v[j]] = stream;
+                    ++queued;
+                } else if (j >= len && st.open && st.open[stream]) {
                     res.status = kWalkNeed;
                     stop = true;
                 }
@@ -347,7 +347,7 @@ TPZ_HD WalkResult walk_query(const WalkStreams& st, const WalkMem& m, int32_t ca
 // Bytes of one query's WalkMem for n_streams streams, and where each array
 // starts (256-byte aligned), from `base`.
 struct WalkMemLayout {
-    size_t heap, entry, free_stack, head, tail, key, val, total;
+    size_t top, node, entry, free_stack, head, tail, key, val, total;
     uint32_t table;
 };
 
@@ -357,12 +357,13 @@ TPZ_HD WalkMemLayout walk_mem_layout(int32_t n_streams, int32_t capacity, int32_
     WalkMemLayout L;
     L.table = walk_table_size(capacity);
     size_t o = 0;
+    L.top = o; o = walk_align(o + 4 * (size_t)(max_score + 1));
     L.head = o; o = walk_align(o + 4 * (size_t)(max_score + 1));
     L.tail = o; o = walk_align(o + 4 * (size_t)(max_score + 1));
     L.val = o; o = walk_align(o + 4 * (size_t)L.table);
     // the arrays above start as -1; those below need no initial value
     L.key = o; o = walk_align(o + 4 * (size_t)L.table);
-    L.heap = o; o = walk_align(o + sizeof(WalkRun) * (size_t)(n_streams + 1));
+    L.node = o; o = walk_align(o + sizeof(WalkNode) * (size_t)(n_streams + 1));
     L.entry = o; o = walk_align(o + sizeof(WalkEntry) * (size_t)(capacity + 1));
     L.free_stack = o; o = walk_align(o + 4 * (size_t)(capacity + 1));
     L.total = o;
@@ -374,7 +375,8 @@ TPZ_HD size_t walk_mem_init_bytes(const WalkMemLayout& L) { return L.key; }
 
 TPZ_HD WalkMem walk_mem_at(char* base, const WalkMemLayout& L) {
     WalkMem m;
-    m.heap = reinterpret_cast<WalkRun*>(base + L.heap);
+    m.top = reinterpret_cast<int32_t*>(base + L.top);
+    m.node = reinterpret_cast<WalkNode*>(base + L.node);
     m.entry = reinterpret_cast<WalkEntry*>(base + L.entry);
     m.free_stack = reinterpret_cast<int32_t*>(base + L.free_stack);
     m.head = reinterpret_cast<int32_t*>(base + L.head);

@@ -2,7 +2,7 @@
 // copy that gathers each stream's rounds into one contiguous run of
 // elements before it.
 //
-// The walk is a chain of dependent steps (heap, hit list), so a query gets
+// The walk is a chain of dependent steps (run queue, hit list), so a query gets
 // one wavefront whose 64 lanes all execute the same steps on the same data:
 // their loads and stores hit the same addresses and coalesce, and every
 // lane sees its own stores.  Only the scan of a run's elements is split
