@@ -207,6 +207,19 @@ def gapless_leg(args):
         if name == "matched" and not args.no_cpu:
             out["cpu_baseline"] = gapless_cpu_baseline(text, queries, seeds, per_seed)
     dev_text.close()
+    # the memory system's rate for this access pattern: the same number of
+    # random window pairs read with no compute (topaz_amd/probe_window_gather)
+    import subprocess
+    probe = os.path.join(ROOT, "topaz_amd", "probe_window_gather")
+    if os.path.exists(probe):
+        r = subprocess.run([probe, str(len(text)), str(n_seeds)], capture_output=True, text=True,
+                           timeout=300)
+        if r.returncode == 0:
+            g = json.loads(r.stdout)
+            g["matched_kernel_vs_gather"] = round(g["ms_two_windows"] / out["matched"]["kernel_ms"], 3)
+            out["window_gather"] = g
+            print("[bench] gapless: window-pair gather alone %.3f ms (matched kernel at %.2f of it)"
+                  % (g["ms_two_windows"], g["matched_kernel_vs_gather"]), file=sys.stderr, flush=True)
     out["workload"] = ("gapless HSP extension: %d-byte text (lognormal proteins, mean 350, "
                        "clipped [30,5000]), 10000 queries, %d seeds per set grouped by query; "
                        "bytes = text bytes the rule reads + 16 B seed + 8 B result"
@@ -902,6 +915,17 @@ def main():
             out["serving"] = serving_leg(args)
             out["stats"] = stats_leg(args)
             out["suffix_array"] = sa_leg(args)
+            # end-to-end `topaz search` on the small configuration: 100,000
+            # proteins, 1,000 queries, default and -F, reference CPU vs GPU
+            sargs = argparse.Namespace(**vars(args))
+            sargs.search_db, sargs.search_queries, sargs.search_sample = 100000, 1000, 1000
+            sargs.plain_sample, sargs.search_modes, sargs.dropin_sample = 200, "default,fast", 0
+            out["search"] = search_leg(sargs)
+            d = out["search"]["default"]
+            out["search_queries_per_s"] = d["gpu_sample"]["queries_per_s"]
+            out["search_vs_reference_cpu"] = d["speedup_vs_reference"]
+            out["search_batched_vs_plain"] = d["batched_vs_plain"]
+            out["search_identical"] = all(out["search"][m]["identical"] for m in ("default", "fast"))
             top = [r for r in out["serving"]["rates"] if r["offered_requests_per_s"] == 100]
             if top:
                 out["latency_p50_ms"] = top[0]["p50_ms"]

@@ -53,6 +53,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stdout))
     os.replace(tmp, LIB)
+    # the gapless leg's calibration program (random window reads, no compute)
+    probe = PKG / "probe_window_gather"
+    cmd = [hipcc()] + [f for f in flags if f not in ("-fPIC",)] + \
+        [str(CSRC / "probe_window_gather.hip"), "-o", str(probe) + ".tmp"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s" % (" ".join(cmd), r.stdout))
+    os.replace(str(probe) + ".tmp", probe)
     return LIB
 
 
