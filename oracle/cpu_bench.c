@@ -13,9 +13,10 @@
  *   cells pairs queries seconds
  *
  * gapless: FILE holds a text, queries and seeds (query, offset, text
- * position); threads take chunks of 65,536 seeds in order and run the
- * reference's sa_hsp (search.c:99-190) on each, until SECONDS have passed;
- * the chunks taken are complete, so the seeds done are a prefix.  Prints
+ * position); threads take chunks of 65,536 seeds in order, going round the
+ * list again at its end, and run the reference's sa_hsp (search.c:99-190)
+ * on each, until SECONDS have passed; the chunks taken are complete, so the
+ * seeds done are seeds 0 .. done-1 taken modulo the list.  Prints
  *   seeds seconds
  *
  * File layouts (little endian) are written by bench.py.  Built by
@@ -157,15 +158,17 @@ static void* g_worker(void* arg) {
         int score, length;
         pthread_mutex_lock(&j->mu);
         k0 = j->next;
-        if (k0 < j->n_seeds && now() <= j->deadline) j->next += G_CHUNK;
+        if (now() <= j->deadline) j->next += G_CHUNK;
         else k0 = -1;
         pthread_mutex_unlock(&j->mu);
         if (k0 < 0) break;
-        for (k = k0; k < k0 + G_CHUNK && k < j->n_seeds; ++k)
-            /* seed k's text position is db.SA[k] */
-            sa_hsp(&j->db, j->queries[j->sq[k]], j->so[k], k, &j->opt, &score, &length);
+        for (k = k0; k < k0 + G_CHUNK; ++k) {
+            /* seed i's text position is db.SA[i] */
+            const int64_t i = k % j->n_seeds;
+            sa_hsp(&j->db, j->queries[j->sq[i]], j->so[i], i, &j->opt, &score, &length);
+        }
         pthread_mutex_lock(&j->mu);
-        j->done += (k0 + G_CHUNK < j->n_seeds ? G_CHUNK : j->n_seeds - k0);
+        j->done += G_CHUNK;
         pthread_mutex_unlock(&j->mu);
     }
     return NULL;
