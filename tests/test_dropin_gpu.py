@@ -87,7 +87,8 @@ def _per_query(text):
     return head, blocks
 
 
-@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"]])
+@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"], ["-o", "2", "-e", "4"],
+                                   ["-B", "-o", "3", "-e", "3"]])
 def test_hit_lists_identical(workspace, flags):
     tmp, dropin, _ = workspace
     ref = _search(REF / "topaz", tmp, *flags)
@@ -96,7 +97,8 @@ def test_hit_lists_identical(workspace, flags):
     assert _per_query(ours) == _per_query(ref)
 
 
-@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"], ["-B", "-T", "4"]])
+@pytest.mark.parametrize("flags", [[], ["-B"], ["-F"], ["-B", "-T", "4"],
+                                   ["-B", "-o", "3", "-e", "3"]])
 def test_batch_loops_hit_lists_identical(workspace, flags):
     """query.c and suffix.c with their per-seed and per-hit loops replaced by
     sa_hsp_batch (seeds of a query, and the seed walk prefetched in batches),
@@ -176,6 +178,8 @@ def _run(exe, tmp, args, env=None):
     ([], {}), (["-B"], {}), (["-F"], {}),
     (["-T", "3"], {"TOPAZ_AMD_SEARCH_BATCH": "5"}),      # several batches at once
     (["-B"], {"TOPAZ_AMD_SEARCH_PLAIN": "1"}),           # one query at a time
+    (["-B", "-o", "3", "-e", "3"], {}),                  # gap_open <= gap_extend: the lanes
+    (["-o", "1", "-e", "2"], {}),
 ])
 def test_search_engine_hit_lists_identical(workspace, flags, env):
     """TOPAZ with its workers searching whole batches through tpz_search
@@ -183,7 +187,12 @@ def test_search_engine_hit_lists_identical(workspace, flags, env):
     stock binary's hit lists."""
     tmp = workspace[0]
     exe = _link_engine(tmp)
-    ref = _search(REF / "topaz", tmp, *[f for f in flags if f.startswith("-B") or f == "-F"])
+    ref_flags = []
+    for k, f in enumerate(flags):
+        if f == "-T" or (k > 0 and flags[k - 1] == "-T"):
+            continue
+        ref_flags.append(f)
+    ref = _search(REF / "topaz", tmp, *ref_flags)
     ours = _run(exe, tmp, flags, env)
     assert ref.count("\n") > 20
     assert _per_query(ours) == _per_query(ref)

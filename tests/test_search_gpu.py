@@ -70,6 +70,33 @@ def test_offline_search_matches_reference(db_and_queries, blasttab):
     assert sum(len(v) for v in ref.values()) > 100 and st["pairs"] > 0
 
 
+@pytest.mark.parametrize("go,ge,modes", [(3, 3, (False, True)), (6, 6, (True,)),
+                                         (1, 4, (False,))])
+def test_offline_search_lane_gaps(db_and_queries, go, ge, modes):
+    """gap_open <= gap_extend (-o/-e): the alignments and the statistics'
+    reverse pass follow the reference's lanes; the printed lines equal the
+    reference binary's, default and -B.  (With -B only settings whose
+    alignments here stay inside banded_sw's band: for gap-heavy ones the
+    reference's traceback reads direction cells its last band pass did not
+    write, and its own statistics then vary from process to process.)"""
+    tmp, _, qs = db_and_queries
+    ids = header_ids(tmp / "DB", np.fromfile(str(tmp / "DB.ind"), index.IND_DTYPE))
+    db = Database.from_index(tmp / "DB", matrix(), gap_open=go, gap_extend=ge)
+    try:
+        for blasttab in modes:
+            ref = _ref_lines(tmp, "-o", str(go), "-e", str(ge), *(["-B"] if blasttab else []))
+            hits, st = db.search(qs, db.default_options(superfast=0 if blasttab else 1))
+            ours = {}
+            for k, h in enumerate(hits):
+                lines = format_hits("q%d" % k, h, ids, superfast=not blasttab)
+                if lines:
+                    ours["q%d" % k] = lines
+            assert ours == ref, (go, ge, blasttab)
+            assert st["pairs"] > 0
+    finally:
+        db.close()
+
+
 def test_streams_cut_at_the_cap_are_searched_again(db_and_queries):
     """A walk that reaches the end of a stream cut at max_stream_elements
     searches its query again with a higher cap: the hits are those of an

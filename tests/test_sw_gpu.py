@@ -109,6 +109,69 @@ def test_overflow_to_word_and_drop(schedule):
     assert sc[0] == OVERFLOW and sc[1] == 22000
 
 
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+@pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
+def test_golden_forward_every_gap_setting(schedule):
+    """The golden fixture (the reference's own ssw_align, compiled from its
+    sources): every (matrix, gap_open, gap_extend) group, gap_open <=
+    gap_extend included, where the kernels follow the reference's lanes."""
+    from topaz_amd.matrices import NAMES
+    d = np.load(GOLD / "sw_pairs.npz")
+    q = (d["q_res"], d["q_off"], d["q_len"])
+    s = (d["s_res"], d["s_off"], d["s_len"])
+    lanes = 0
+    for mi, go, ge in sorted(set(zip(d["matrix"], d["gap_open"], d["gap_extend"]))):
+        sel = np.nonzero((d["matrix"] == mi) & (d["gap_open"] == go) & (d["gap_extend"] == ge))[0]
+        b = SwBatch(q, s, sel.astype(np.int32), sel.astype(np.int32), matrix(NAMES[mi]),
+                    int(go), int(ge))
+        sc, fe, re_ = b.run(schedule).results()
+        b.close()
+        null = d["is_null"][sel]
+        assert np.array_equal(sc == OVERFLOW, null), (mi, go, ge)
+        ok = ~null
+        assert np.array_equal(sc[ok], d["score"][sel][ok]), (mi, go, ge)
+        assert np.array_equal(fe[ok], d["ref_end"][sel][ok]), (mi, go, ge)
+        assert np.array_equal(re_[ok], d["read_end"][sel][ok]), (mi, go, ge)
+        lanes += len(sel) if go <= ge else 0
+    assert lanes >= 50
+
+
+@pytest.mark.parametrize("go,ge", [(2, 2), (6, 6), (1, 3), (0, 0), (5, 9), (0, 4)])
+@pytest.mark.parametrize("schedule", [SCHED_PERSISTENT, SCHED_PLAIN])
+def test_lane_gap_settings_against_oracle(go, ge, schedule):
+    """gap_open <= gap_extend: the reference's lanes lose deletion-insertion
+    corners and differ from textbook alignment; the lane kernel follows
+    them (the oracle's lane emulation, pinned by the golden cases)."""
+    from topaz_amd.sw import _concat
+    rng = np.random.default_rng(100 + 10 * go + ge)
+    w = synth.pair_workload(rng, 6, 25, mean=200, sd=180, lo=1, hi=900)
+    compare(w, go=go, ge=ge, schedule=schedule)
+    # tiny queries: every row a lane's first row
+    qs = [rng.integers(0, 25, int(rng.integers(1, 20))).astype(np.int8) for _ in range(10)]
+    ss = [rng.integers(0, 25, int(rng.integers(1, 60))).astype(np.int8) for _ in range(20)]
+    w = dict(q=_concat(qs), s=_concat(ss), pair_query=np.repeat(np.arange(10), 20).astype(np.int32),
+             pair_subject=np.tile(np.arange(20), 10).astype(np.int32))
+    compare(w, go=go, ge=ge, schedule=schedule)
+
+
+def test_lane_gaps_word_pass_and_drop():
+    """gap_open <= gap_extend with scores past the byte pass (word pass) and
+    past 32767 (dropped)."""
+    from topaz_amd.sw import _concat
+    rng = np.random.default_rng(7)
+    q1 = np.full(3100, 17, np.int8)
+    q2 = rng.integers(0, 25, 800).astype(np.int8)
+    ss = [q1.copy(), q1[:2000].copy(), q2.copy(), q2[100:700].copy(),
+          rng.integers(0, 25, 300).astype(np.int8)]
+    pq = np.array([0, 0, 1, 1, 1, 0], np.int32)
+    ps = np.array([0, 1, 2, 3, 4, 4], np.int32)
+    w = dict(q=_concat([q1, q2]), s=_concat(ss), pair_query=pq, pair_subject=ps)
+    sc = compare(w, go=3, ge=3)
+    assert sc[0] == OVERFLOW and 255 < sc[1] < OVERFLOW
+
+
 def test_empty_batch_and_empty_subjects():
     from topaz_amd.sw import _concat
     qs = [np.array([1, 2, 3], np.int8)]

@@ -36,6 +36,7 @@
 #include "gapless_kernels.h"
 #include "runtime.h"
 #include "sw_kernels.h"
+#include "sw_lane_kernels.h"
 #include "traceback_kernels.h"
 
 namespace tpz {
@@ -436,7 +437,9 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     meta32[14] = -1;
     char* d = (char*)c.in.get(bytes);
     int32_t* dout = (int32_t*)c.out.get(64);
-    const size_t ws_words = (size_t)4 * 2 * (n + 2) * 64;     // four wavefront buffers
+    // four wavefront buffers (textbook gaps) or the lane kernel's three rows
+    const bool lanes = !textbook_gaps(go, ge);
+    const size_t ws_words = lanes ? (size_t)3 * lane_rows(m) + 16 : (size_t)4 * 2 * (n + 2) * 64;
     uint32_t* dws = (uint32_t*)c.ws.get(ws_words * 4);
     hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
     SwKernelArgs a{};
@@ -463,7 +466,13 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.out_score = dout;
     a.out_ref = dout + 1;
     a.out_read = dout + 2;
-    hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, c.st), "sw_forward");
+    if (lanes) {
+        LaneArgs la = lane_args(a, matrix_bias(mat, 25), (int32_t*)dws, lane_rows(m));
+        la.n_list = 1;
+        hip_or_die(tpz_launch_sw_lane(&la, 1, c.st), "sw_lane");
+    } else {
+        hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, c.st), "sw_forward");
+    }
     int32_t* ho = (int32_t*)(h + o_out);
     hip_or_die(hipMemcpyAsync(ho, dout, 12, hipMemcpyDeviceToHost, c.st), "readback");
     c.finish("sw_forward");
@@ -471,9 +480,9 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
 }
 
 void check_gaps(int go, int ge) {
-    if (!(ge >= 0 && go > ge && go <= 255)) {
-        fprintf(stderr, "topaz_amd: gap penalties %d/%d are not supported (need "
-                        "0 <= gap_extend < gap_open <= 255)\n", go, ge);
+    if (!gaps_valid(go, ge)) {
+        fprintf(stderr, "topaz_amd: gap penalties %d/%d out of range (the reference takes "
+                        "them as uint8_t, 0..255)\n", go, ge);
         _exit(EXIT_FAILURE);
     }
 }
@@ -954,7 +963,7 @@ void prefetch_server(const db_t* db, const options_t* opt) {
     static std::vector<const db_t*> claimed;
     if (last.load(std::memory_order_acquire) == db) return;
     const int go = opt->gap_open, ge = opt->gap_extend;
-    if (!(ge >= 0 && go > ge && go <= 255)) return;       // the alignment call reports it
+    if (!gaps_valid(go, ge)) return;                       // the alignment call reports it
     if (!db->index || db->Psize == 0) return;
     {
         // one thread builds; the others do not wait for it here

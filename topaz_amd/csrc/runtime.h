@@ -64,6 +64,13 @@ struct DeviceTasks {
     }
 };
 
+// Gap penalties the reference's ssw takes (uint8_t, ssw.h:107-118).
+inline bool gaps_valid(int32_t go, int32_t ge) { return go >= 0 && go <= 255 && ge >= 0 && ge <= 255; }
+// Whether the reference's lane recurrence is textbook affine-gap local
+// alignment (gap_open > gap_extend), which the forward, pair and reverse
+// kernels compute; otherwise the lane kernel emulates the lanes.
+inline bool textbook_gaps(int32_t go, int32_t ge) { return go > ge; }
+
 // Pairs of each query, longest subject first (ties: lower pair number):
 // query q's pairs are pair[begin[q] .. begin[q + 1]), and len[] holds the
 // length of each one's subject.

@@ -20,6 +20,7 @@
 #include "traceback_kernels.h"
 #include "sw_kernels.h"
 #include "sw_pair_kernels.h"
+#include "sw_lane_kernels.h"
 
 namespace tpz {
 
@@ -252,6 +253,12 @@ struct tpz_sw_batch {
     std::unique_ptr<SwPlan> plan;
     hipEvent_t ev[2] = {nullptr, nullptr};
     double ms = 0;
+    // gap_open <= gap_extend: the reference's lane recurrence (sw_lane_kernel)
+    // in place of the textbook one, with its own workspace
+    bool lanes = false;
+    DeviceBuffer d_lane_ws;
+    int lane_blocks = 0;
+    int64_t lane_ws_rows = 0;
 };
 
 namespace tpz {
@@ -403,9 +410,8 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     *out = nullptr;
     if (ensure_init()) return -1;
     if (n_queries < 0 || n_subjects < 0 || n_pairs < 0) return fail("negative counts");
-    if (!(gap_extend >= 0 && gap_open > gap_extend && gap_open <= 255))
-        return fail("gap penalties must satisfy 0 <= gap_extend < gap_open <= 255 "
-                    "(got %d/%d)", gap_open, gap_extend);
+    if (!gaps_valid(gap_open, gap_extend))
+        return fail("gap penalties must lie in [0, 255] (got %d/%d)", gap_open, gap_extend);
     if (!matrix) return fail("null matrix");
     int32_t minw = 0;
     for (int i = 0; i < 625; ++i) {
@@ -476,6 +482,16 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     rc |= b->d_error.alloc(64);
     if (!rc) rc |= check(hipMemsetAsync(b->d_error.ptr, 0, 64, stream()), "memset");
     b->per_query = pairs_by_query(b->pair_query, b->pair_subject, b->s_len, n_queries);
+    if (!textbook_gaps(gap_open, gap_extend) && n_pairs > 0) {
+        b->lanes = true;
+        int32_t max_q = 0;
+        for (int32_t k = 0; k < n_queries; ++k) max_q = std::max(max_q, q_len[k]);
+        b->lane_ws_rows = lane_rows(max_q);
+        b->lane_blocks = (int)std::min<int64_t>((n_pairs + kLaneGroupsPerBlock - 1) / kLaneGroupsPerBlock,
+                                                (int64_t)std::max(num_cu(), 1) * 16);
+        rc |= b->d_lane_ws.alloc((size_t)b->lane_blocks * kLaneGroupsPerBlock * 3 *
+                                 (size_t)b->lane_ws_rows * 4 + 64);
+    }
     for (auto& e : b->ev) rc |= check(hipEventCreate(&e), "hipEventCreate");
     tpz_sw_config cfg = default_config();
     if (!rc) rc |= build_plan(b, cfg);
@@ -520,6 +536,25 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
     // the timeout flag of the staircase waits is read after every run: clear
     // it with a kernel too (a graph replays it)
     if (check(tpz_launch_fill_i32(a.error, 0, 1, stream()), "clear error flag")) return -1;
+    if (b->lanes) {
+        // the reference's lanes: all pairs in one launch (persistent) or
+        // each query's pairs in one (plain)
+        LaneArgs la = lane_args(a, b->bias, (int32_t*)b->d_lane_ws.ptr, b->lane_ws_rows);
+        if (schedule == TPZ_SCHED_PERSISTENT) {
+            la.n_list = b->n_pairs;
+            return check(tpz_launch_sw_lane(&la, b->lane_blocks, stream()), "sw_lane");
+        }
+        for (int32_t q = 0; q + 1 < (int32_t)pl.plain_begin.size(); ++q) {
+            const int32_t i0 = pl.plain_begin[q], i1 = pl.plain_begin[q + 1];
+            if (i1 == i0) continue;
+            la.pairs = (const int32_t*)pl.d_plain_pairs.ptr + i0;
+            la.n_list = i1 - i0;
+            const int blocks = std::min(b->lane_blocks, (i1 - i0 + kLaneGroupsPerBlock - 1) /
+                                                            kLaneGroupsPerBlock);
+            if (check(tpz_launch_sw_lane(&la, blocks, stream()), "sw_lane (plain)")) return -1;
+        }
+        return 0;
+    }
     if (schedule == TPZ_SCHED_PERSISTENT) {
         const int n_items = (int)(pl.items_persistent.size() / W);
         if (n_items > 0) {
@@ -962,8 +997,8 @@ char* BlockList::take(size_t bytes, void* (*grow)(size_t, void*), void* ctx) {
 
 int stats_check(const StatsInput& in) {
     if (in.n_queries < 0 || in.n_subjects < 0 || in.n_hits < 0) return fail("negative counts");
-    if (!(in.gap_extend >= 0 && in.gap_open > in.gap_extend && in.gap_open <= 255))
-        return fail("gap penalties must satisfy 0 <= gap_extend < gap_open <= 255");
+    if (!gaps_valid(in.gap_open, in.gap_extend))
+        return fail("gap penalties must lie in [0, 255] (got %d/%d)", in.gap_open, in.gap_extend);
     if (!in.matrix) return fail("null matrix");
     for (int32_t h = 0; h < in.n_hits; ++h) {
         const int32_t qi = in.hit_query[h], si = in.hit_subject[h];
@@ -1054,8 +1089,28 @@ int stats_compute(const StatsInput& in, RunMem& m, StatsOutput& out) {
         return -1;
     r.n_pairs = n;
     r.gap_open = in.gap_open; r.gap_extend = in.gap_extend; r.bias = -minw;
-    // one launch per rows-per-lane class of the hits' lane-padded prefixes
-    {
+    if (!textbook_gaps(in.gap_open, in.gap_extend)) {
+        // the reference's lanes over the reversed prefixes, all hits at once
+        LaneArgs la{};
+        la.q_res = r.q_res; la.q_off = r.q_off; la.s_res = r.s_res; la.s_off = r.s_off;
+        la.pair_query = r.pair_query; la.pair_subject = r.pair_subject;
+        la.n_list = n;
+        la.matrix = r.matrix;
+        la.gap_open = in.gap_open; la.gap_extend = in.gap_extend; la.bias = -minw;
+        la.read_end = r.read_end; la.ref_end = r.ref_end; la.score = r.score;
+        int32_t max_prefix = 0;
+        for (int32_t h = 0; h < n; ++h) max_prefix = std::max(max_prefix, in.read_end1[h] + 1);
+        la.ws_rows = lane_rows(max_prefix);
+        const int blocks = (int)std::min<int64_t>((n + kLaneGroupsPerBlock - 1) / kLaneGroupsPerBlock,
+                                                  (int64_t)std::max(num_cu(), 1) * 16);
+        la.workspace = (int32_t*)m.dev((size_t)blocks * kLaneGroupsPerBlock * 3 *
+                                       (size_t)la.ws_rows * 4 + 64);
+        la.out_ref = r.out_ref_begin;
+        la.out_read = r.out_read_begin;
+        if (!la.workspace || check(tpz_launch_sw_lane(&la, blocks, m.stream()), "sw_lane (reverse)"))
+            return -1;
+    } else {
+        // one launch per rows-per-lane class of the hits' lane-padded prefixes
         std::vector<int32_t> cls[3];
         for (int32_t h = 0; h < n; ++h) {
             const int32_t L = in.read_end1[h] + 1;

@@ -843,9 +843,9 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     if (ensure_init()) return -1;
     if (!text || !sa || !protein_start || !protein_len || !matrix || t_size == 0)
         return fail("tpz_db_create: null or empty input");
-    if (!(gap_extend >= 0 && gap_open > gap_extend && gap_open <= 255))
-        return fail("tpz_db_create: gap penalties must satisfy 0 <= gap_extend < gap_open <= 255 "
-                    "(got %d/%d)", gap_open, gap_extend);
+    if (!gaps_valid(gap_open, gap_extend))
+        return fail("tpz_db_create: gap penalties must lie in [0, 255] (got %d/%d)", gap_open,
+                    gap_extend);
     if (n_proteins > (uint64_t)INT32_MAX) return fail("tpz_db_create: too many proteins");
     auto db = std::make_unique<tpz_db>();
     db->t_size = t_size;

@@ -32,6 +32,7 @@
 #include "../../include/topaz_amd.h"
 #include "runtime.h"
 #include "sw_kernels.h"
+#include "sw_lane_kernels.h"
 
 namespace tpz {
 namespace {
@@ -103,7 +104,7 @@ struct Slot {
     std::vector<size_t> pair_base;      // first merged pair of each request
     size_t n_pairs = 0;
     size_t bytes = 0;                   // admitted request bytes
-    GrowBuffer d_in, d_out;
+    GrowBuffer d_in, d_out, d_lane_ws;
     DeviceBuffer d_ws;
     PinnedBuffer h_in, h_out;
     int error_word = 0;                 // offset (words) of the error flag in d_out
@@ -124,7 +125,8 @@ struct tpz_server {
     int32_t n_subjects = 0, max_subject = 0;
     std::vector<int32_t> s_len;
     DeviceBuffer d_s_res, d_s_off, d_s_len, d_matrix;
-    int32_t gap_open = 11, gap_extend = 1;
+    int32_t gap_open = 11, gap_extend = 1, bias = 0;
+    bool lanes = false;                  // gap_open <= gap_extend: the lane kernel
     tpz_server_config cfg{};
     int device = 0;
     int grid = 0;                        // workgroups of one launch at most
@@ -256,8 +258,22 @@ int launch_slot(tpz_server* s, Slot& sl) {
         a.out_score = dout;
         a.out_ref = dout + np;
         a.out_read = dout + 2 * np;
-        const int grid = std::min(n_items, s->grid);
-        if (check(tpz_launch_sw_forward(&a, grid, 32, W, sl.stream), "sw_forward")) return -1;
+        if (s->lanes) {
+            // the reference's lanes, every pair of the launch
+            int32_t max_q = 0;
+            for (int32_t v : q_len) max_q = std::max(max_q, v);
+            const int64_t rows = lane_rows(max_q);
+            const int blocks = (int)std::min<int64_t>((np + kLaneGroupsPerBlock - 1) / kLaneGroupsPerBlock,
+                                                      (int64_t)std::max(num_cu(), 1) * 16);
+            if (sl.d_lane_ws.reserve((size_t)blocks * kLaneGroupsPerBlock * 3 * rows * 4 + 64))
+                return -1;
+            LaneArgs la = lane_args(a, s->bias, (int32_t*)sl.d_lane_ws.ptr, rows);
+            la.n_list = (int64_t)np;
+            if (check(tpz_launch_sw_lane(&la, blocks, sl.stream), "sw_lane")) return -1;
+        } else {
+            const int grid = std::min(n_items, s->grid);
+            if (check(tpz_launch_sw_forward(&a, grid, 32, W, sl.stream), "sw_forward")) return -1;
+        }
     }
     if (check(hipMemcpyAsync(sl.h_out.ptr, dout, out_words * 4, hipMemcpyDeviceToHost, sl.stream),
               "readback") ||
@@ -373,14 +389,15 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
     *out = nullptr;
     if (ensure_init()) return -1;
     if (n_subjects < 0) return fail("negative subject count");
-    if (!(gap_extend >= 0 && gap_open > gap_extend && gap_open <= 255))
-        return fail("gap penalties must satisfy 0 <= gap_extend < gap_open <= 255 "
-                    "(got %d/%d)", gap_open, gap_extend);
+    if (!gaps_valid(gap_open, gap_extend))
+        return fail("gap penalties must lie in [0, 255] (got %d/%d)", gap_open, gap_extend);
     if (!matrix) return fail("null matrix");
     auto s = std::make_unique<tpz_server>();
     s->n_subjects = n_subjects;
     s->gap_open = gap_open;
     s->gap_extend = gap_extend;
+    s->lanes = !textbook_gaps(gap_open, gap_extend);
+    for (int i = 0; i < 625; ++i) s->bias = std::max<int32_t>(s->bias, -matrix[i]);
     s->device = device_id();
     if (cfg) s->cfg = *cfg;
     if (s->cfg.memory_budget == 0) s->cfg.memory_budget = (uint64_t)1 << 30;
@@ -411,7 +428,9 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
              s->d_s_len.upload(s_len, (size_t)n_subjects * 4) | s->d_matrix.upload(matrix, 625);
     s->grid = std::max(num_cu(), 1) *
               std::max(1, std::min(kGroupsPerCu, tpz_sw_forward_groups_per_cu(32, s->cfg.waves_per_group)));
-    s->ws_words_per_slot = (int64_t)s->cfg.waves_per_group * 2 * (s->max_subject + 2) * 64;
+    // the forward kernel's workspace (the lane kernel's grows with the
+    // launch's longest query, in the slot)
+    s->ws_words_per_slot = s->lanes ? 0 : (int64_t)s->cfg.waves_per_group * 2 * (s->max_subject + 2) * 64;
     for (int k = 0; k < s->cfg.max_inflight && !rc; ++k) {
         auto sl = std::make_unique<Slot>();
         rc |= check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");

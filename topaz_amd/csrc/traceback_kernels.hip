@@ -444,11 +444,13 @@ banded_trace_kernel(BandArgs a) {
 // The same cells, values and direction codes as band_pass, with the lanes
 // across the band and the three band rows in LDS.  Within a row, E and the
 // diagonal come from the row above; the horizontal gap F is a running max
-// along the row, F(j) = max(G(j-1) - gapO, F(j-1) - gapE), where G is the
-// cell's value without F: with gapO > gapE a cell whose value came from F
-// never opens a better F than F's own extension, so F over a chunk of 64
-// cells is a prefix max of G(j-1) - gapO + l*gapE, carried from chunk to
-// chunk.
+// along the row.  With G the cell's value without F (max(E, 0) and the
+// diagonal; G >= 0), the previous cell's value is H = max(G, F, 0), so
+// F(j) = max(H(j-1) - gapO, F(j-1) - gapE)
+//      = max(G(j-1) - gapO, F(j-1) - min(gapO, gapE))
+// (the -gapO of a zero H is below G - gapO).  F over a chunk of 64 cells is
+// then a prefix max of G(j-1) - gapO + l*c, c = min(gapO, gapE), carried
+// from chunk to chunk.
 struct BandLds {
     int32_t hb[kBandSlots], eb[kBandSlots], hc[kBandSlots];
 };
@@ -493,13 +495,14 @@ __device__ int band_pass_wave(const int8_t* ref, const int8_t* read, int refLen,
             const int g = imax(e1, sub);
             int g_prev = __shfl_up(g, 1);
             if (lane == 0) g_prev = carry_g;
-            int b = g_prev - go + lane * ge;
-            if (lane == 0) b = imax(b, carry_f - ge);
+            const int c = imin(go, ge);
+            int b = g_prev - go + lane * c;
+            if (lane == 0) b = imax(b, carry_f - c);
             for (int d = 1; d < 64; d <<= 1) {
                 const int o = __shfl_up(b, d);
                 if (lane >= d) b = imax(b, o);
             }
-            const int f = b - lane * ge;
+            const int f = b - lane * c;
             const int f1 = imax(f, 0);
             const int gap = e1 > f1 ? e1 : f1;
             const int h = gap > sub ? gap : sub;
