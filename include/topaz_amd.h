@@ -42,7 +42,8 @@ typedef struct tpz_sw_batch tpz_sw_batch;
  *                       q_off[k] + q_len[k])
  *   s_res/s_off/s_len   subjects, same layout
  *   pair_query/pair_subject  n_pairs pair definitions
- *   matrix              25 x 25 scores; gap_open > gap_extend >= 0, both <= 255
+ *   matrix              25 x 25 scores; gap penalties 0..255 (with gap_open <=
+ *                       gap_extend the reference's lanes run literally, slower)
  * Per pair the result is what ssw_align (ssw.c:835-897, score_size 2) returns:
  * the byte (16-lane) result if its score + bias < 255, else the word (8-lane)
  * result, else "overflow" (the reference returns NULL). */
@@ -254,8 +255,8 @@ int tpz_gapless_run_noalloc(const tpz_text* text, int32_t n_queries, const char*
  * -- the '\n'-separated internal-alphabet text (db->T, Tsize bytes), its
  * suffix array (db->SA, Tsize entries) and the protein index
  * (db->index[k].protein_start / protein_len) -- plus the scoring the
- * searches use (the substitution matrix, 25 x 25, and gap penalties with
- * gap_open > gap_extend).  The text and suffix array are read on the host as
+ * searches use (the substitution matrix, 25 x 25, and gap penalties
+ * 0..255).  The text and suffix array are read on the host as
  * well (the hit list maps suffix-array slots to proteins), so the caller
  * keeps them alive until tpz_db_destroy.  The proteins must tile the text as
  * TOPAZ's index writes it: each protein's letters and its newline, back to
