@@ -363,6 +363,17 @@ int tpz_search_server_get_stats(tpz_search_server* s, tpz_search_server_stats* s
 /* Searches the requests still queued, then stops. */
 void tpz_search_server_destroy(tpz_search_server* s);
 
+/* ---- SEG: low-complexity masking of a query (host logic) ---------------- */
+
+/* run_seg (seg_wrapper.c:15-49; seg/seg.c, seg/genwin.c with TOPAZ's settings:
+ * window 12, locut 2.2, hicut 2.5, maxtrim 100): the low-complexity segments
+ * of seq[0..len), letters as read from FASTA (the 20 standard amino acids
+ * counted in either case, any other byte not), overwritten with 'X' in
+ * place, as engine.c:254-263 does before converting a query to the internal
+ * alphabet.  Returns the number of residues masked, or -1 for a null
+ * sequence or a negative length.  Needs no device. */
+int32_t tpz_seg(char* seq, int32_t len);
+
 /* ---- the seed walk of one query (host logic) ---------------------------- */
 
 /* The walk of query_fetch_alignment_batch (query.c:134-174): pqueue_next

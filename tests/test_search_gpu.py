@@ -38,12 +38,12 @@ def db_and_queries(tmp_path_factory):
     db.close()
 
 
-def _ref_lines(tmp, *flags):
+def _ref_lines(tmp, *flags, queries="q.fa", seg_off=True):
     if not (REF / "topaz").exists():
         pytest.skip("reference checker not built (make -C oracle ref)")
-    r = subprocess.run([str(REF / "topaz"), "search", "-f", str(tmp / "q.fa"), "-p",
-                        str(tmp / "DB"), "-T", "4", "-M"] + list(flags), capture_output=True,
-                       text=True, timeout=600, cwd=str(tmp))
+    r = subprocess.run([str(REF / "topaz"), "search", "-f", str(tmp / queries), "-p",
+                        str(tmp / "DB"), "-T", "4"] + (["-M"] if seg_off else []) + list(flags),
+                       capture_output=True, text=True, timeout=600, cwd=str(tmp))
     assert r.returncode == 0
     blocks = {}
     for line in r.stdout.splitlines():
@@ -95,6 +95,41 @@ def test_offline_search_lane_gaps(db_and_queries, go, ge, modes):
             assert st["pairs"] > 0
     finally:
         db.close()
+
+
+def test_offline_search_with_seg(db_and_queries):
+    """SEG on (TOPAZ's default, no -M): queries read from FASTA and masked by
+    the library's SEG (topaz_amd.search.read_queries) give the reference
+    binary's lines, default and -B.  Half the queries carry planted
+    low-complexity stretches."""
+    tmp, db, _ = db_and_queries
+    from topaz_amd.search import read_queries
+    rng = np.random.default_rng(77)
+    recs = []
+    for k, (qid, q) in enumerate(read_queries(tmp / "q.fa", use_seg=False)):
+        letters = bytes(index.INTERNAL_OF_INDEX.tolist().index(c) for c in q)
+        text = synth.to_letters(np.frombuffer(letters, np.uint8)).encode()
+        if k % 2 == 0:
+            at = int(rng.integers(0, len(text) + 1))
+            junk = [b"Q" * 25, b"SG" * 15, b"PAPAPPAAPAPPA" * 2, b"EEEKKEEEKKEKE" * 2][k % 4]
+            text = text[:at] + junk + text[at:]
+        recs.append((qid, text))
+    with open(tmp / "q_seg.fa", "w") as f:
+        for qid, text in recs:
+            f.write(">%s\n%s\n" % (qid, text.decode()))
+    qs = read_queries(tmp / "q_seg.fa")
+    masked = sum(q.count(b"V") for _, q in qs)
+    assert masked > 500
+    ids = header_ids(tmp / "DB", np.fromfile(str(tmp / "DB.ind"), index.IND_DTYPE))
+    for blasttab in (False, True):
+        ref = _ref_lines(tmp, *(["-B"] if blasttab else []), queries="q_seg.fa", seg_off=False)
+        hits, _ = db.search([q for _, q in qs], db.default_options(superfast=0 if blasttab else 1))
+        ours = {}
+        for (qid, _), h in zip(qs, hits):
+            lines = format_hits(qid, h, ids, superfast=not blasttab)
+            if lines:
+                ours[qid] = lines
+        assert ours == ref, blasttab
 
 
 def test_streams_cut_at_the_cap_are_searched_again(db_and_queries):

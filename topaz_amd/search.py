@@ -264,6 +264,56 @@ class SearchServer:
             pass
 
 
+# upper_aa (lexicographical.h:35-58): A..Y, '*' and '\n' kept, a..y upper-cased,
+# anything else 'X'
+_UPPER_AA = bytes(c if (65 <= c <= 89 or c in (42, 10)) else c - 32 if 97 <= c <= 121 else 88
+                  for c in range(256))
+# aa2internal (lexicographical.h:135-190) of upper-case letters; anything else 'V'
+_AA2INTERNAL = bytearray(b"V" * 256)
+for _a, _i in zip(b"ACMLJIVTSKRQZEDBNHYFWXP*G\n", b"ABCDEFGHIJKLMNOPQRSTUVWXY\n"):
+    _AA2INTERNAL[_a] = _i
+_AA2INTERNAL = bytes(_AA2INTERNAL)
+
+
+def seg(letters):
+    """SEG as TOPAZ runs it on a query (run_seg, seg_wrapper.c:15-49):
+    low-complexity stretches of `letters` (bytes) replaced by 'X'."""
+    L = _lib.lib()
+    if not getattr(L, "_seg_declared", False):
+        L.tpz_seg.argtypes = [C.c_char_p, C.c_int32]
+        L.tpz_seg.restype = C.c_int32
+        L._seg_declared = True
+    buf = C.create_string_buffer(bytes(letters), len(letters) + 1)
+    if L.tpz_seg(buf, len(letters)) < 0:
+        raise _lib.TopazError("tpz_seg failed")
+    return buf.raw[:len(letters)]
+
+
+def prepare_query(letters, use_seg=True):
+    """A query's FASTA letters (newlines removed) as TOPAZ searches them:
+    upper_aa (seq_seq, seq.c:80-104), SEG unless -M (engine.c:254-263), then
+    the internal alphabet (seq_2internal, seq.c:106-117)."""
+    s = bytes(letters).translate(_UPPER_AA)
+    if use_seg:
+        s = seg(s)
+    return s.translate(_AA2INTERNAL)
+
+
+def read_queries(path, use_seg=True):
+    """The queries of a FASTA file as TOPAZ reads them (fasta.c:60-76): a
+    list of (id, internal-alphabet bytes); the id is the header up to its
+    end of line, without the '>'."""
+    out = []
+    data = Path(path).read_bytes()
+    for rec in data.split(b"\n>"):
+        rec = rec[1:] if rec.startswith(b">") else rec
+        if not rec.strip():
+            continue
+        head, _, body = rec.partition(b"\n")
+        out.append((head.decode(), prepare_query(body.replace(b"\n", b""), use_seg)))
+    return out
+
+
 def format_hits(qid, hits, subject_ids, superfast=True):
     """The lines hit_print (engine.c:139-167) writes for one query."""
     lines = []
