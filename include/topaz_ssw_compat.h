@@ -3,8 +3,10 @@
  *
  * libtopaz_amd.so exports these functions with the reference's names,
  * signatures and struct layouts, so TOPAZ's own query.c / suffix.c / engine.c
- * link against it in place of ssw.o, ssw_wrapper.o and search.c's sa_hsp.
- * Every call computes on the GPU.  Errors end the process with a message,
+ * link against it in place of ssw.o, ssw_wrapper.o, search.o and
+ * alignment.o.  Every call on TOPAZ's search path computes on the GPU; the
+ * functions TOPAZ never calls (the last block below) run on the host.
+ * Errors end the process with a message,
  * as the reference's own fatal paths do (query.c:311-314): these calls have
  * no error return.
  *
@@ -16,6 +18,7 @@
 #ifndef TOPAZ_SSW_COMPAT_H
 #define TOPAZ_SSW_COMPAT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -134,7 +137,8 @@ typedef struct {
 /* ssw.h:87-210 -- ssw_init (ssw.c:799), init_destroy / init_destroy2
  * (ssw.c:822-833), ssw_align (ssw.c:835), align_destroy (ssw.c:980),
  * ssw_align_stats (ssw.c:900).  score_size must be 2 (what TOPAZ passes,
- * ssw_wrapper.c:127); gap penalties must satisfy gapE < gapO. */
+ * ssw_wrapper.c:127); gap penalties 0..255 (uint8_t there), gapO <= gapE
+ * included. */
 s_profile* ssw_init(const int8_t* read, const int32_t readLen, const int8_t* mat,
                     const int32_t n, const int8_t score_size);
 void init_destroy(s_profile* p);
@@ -189,6 +193,15 @@ void align_using_ssw(seq_t* s1, seq_t* s2, hit_t* hit, options_t* opt);
 int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
            int* gapless_score, int* gapless_length);
 
+/* search.h:15 -- search.c:237-361: the suffix-array interval of the pattern
+ * q[offset .. offset + Psize), Psize = length > 0 ? length : seq_len -
+ * offset, searched on the device over the resident db->T and db->SA.
+ * Returns -1 with *idx = -1 (the other outputs untouched) for an empty
+ * database or Psize <= 0; a pattern outside the query's buffer (which the
+ * reference reads anyway) ends the process. */
+int sa_search(db_t* db, seq_t* q, int64_t offset, int64_t length, int64_t* idx,
+              int64_t* num_matches, int64_t* last_position);
+
 /* ---- batching extension (no reference counterpart) --------------------
  * One call per query instead of one call per hit, with the reference's own
  * types.  Subjects are read from the database (db_seq, db.c:233-247: the
@@ -231,6 +244,45 @@ int db_index(db_t* db, uint64_t pos, int64_t* index);
  * last_position may be null. */
 int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
                     int64_t* num_matches, int64_t* last_position);
+
+/* ---- exported by the reference, never called by TOPAZ ----------------
+ * Single scalar computations on the caller's host arrays, run on the host
+ * with the reference's results (tie rules and fatal paths included), so a
+ * program links with neither search.o nor alignment.o. */
+
+/* search.h:12 -- search.c:32-46: letters of suffix SA[position] and the
+ * query from qoffset that agree (at most seq_len - qoffset once the first
+ * agrees). */
+int sa_lcp(db_t* db, seq_t* q, int qoffset, int64_t position);
+
+/* search.h:13 -- search.c:48-97: right-only gapless extension of a seed;
+ * *gapless_length = -1 when fewer than 3 letters remain or the first
+ * letters differ. */
+int sa_gapless(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
+               int* gapless_score, int* gapless_length);
+
+/* alignment.h:32 -- alignment.c:266-287: the score of s1[0 .. length)
+ * against itself (s2 unused, as in the reference). */
+int ungapped_rawscore(char* s1, char* s2, uint32_t length, options_t* opt);
+
+/* alignment.c:289-315: returns max_length (the reference's scan never
+ * runs). */
+int ungapped_length(char* s1, char* s2, uint32_t max_length, options_t* opt);
+
+/* alignment.h:31 -- alignment.c:317-396: full-matrix Gotoh of s1 (columns)
+ * against s2 (rows) with the trace codes of alignment.h:12-21, the hit's
+ * coordinates, counts and E-value; aborts, as the reference does, when no
+ * cell scores above 0. */
+int gotoh(seq_t* s1, seq_t* s2, hit_t* hit, options_t* p);
+
+/* alignment.h:24-30 and alignment.c:27-264: gotoh's steps. */
+int _forward_pass(int8_t* s1, int8_t* s2, size_t s1_len, size_t s2_len, int* E, uint16_t* trE,
+                  int* F, uint16_t* trF, int* H, uint16_t* trH, hit_t* hit, options_t* opt);
+void _backtrack(uint16_t* trE, uint16_t* trF, uint16_t* trH, int pos, int max_x, hit_t* hit);
+void _initialise(seq_t* s1, seq_t* s2, int* E, int* F, int* H, uint16_t* trE, uint16_t* trF,
+                 uint16_t* trH, options_t* par);
+void printdp32(int* m, size_t dim1, size_t dim2);
+void printdp8(uint8_t* m, size_t dim1, size_t dim2);
 
 #ifdef __cplusplus
 }

@@ -335,11 +335,12 @@ def test_batch_entry_points_equal_per_hit_calls():
 
 
 def test_sa_search_batch_equals_reference():
-    """sa_search_batch gives, offset by offset, what the reference's own
-    sa_search (search.c:237-361, compiled from its sources) writes: the
-    interval start, the number of matches and the last match length, for
-    suffixes that occur (query parts copied from the text), suffixes that do
-    not, repeats, and an offset past the query's end (return -1)."""
+    """sa_search_batch, and the library's single sa_search with and without
+    a pattern length, give what the reference's own sa_search
+    (search.c:237-361, compiled from its sources) writes: the interval
+    start, the number of matches and the last match length, for suffixes
+    that occur (query parts copied from the text), suffixes that do not,
+    repeats, and an offset past the query's end (return -1)."""
     from tests.refcheck import DbT
     if not REF_SO.exists():
         pytest.skip("reference checker not built (make -C oracle ref)")
@@ -364,6 +365,8 @@ def test_sa_search_batch_equals_reference():
     L.sa_search_batch.argtypes = [C.POINTER(DbT), C.POINTER(SeqT), C.c_int, C.POINTER(C.c_int),
                                   PI64, PI64, PI64]
     L.sa_search_batch.restype = C.c_int
+    L.sa_search.argtypes = R.sa_search.argtypes
+    L.sa_search.restype = C.c_int
     queries = [prots[3][10:200] + _internal(synth.random_residues(rng, 60)) + prots[11][:50],
                _internal(synth.random_residues(rng, 300)),
                b"A" * 45 + prots[7][40:90],
@@ -385,6 +388,22 @@ def test_sa_search_batch_equals_reference():
                                C.byref(rl)) == 0
             assert (idx[k], nm[k], lp[k]) == (ri.value, rn.value, rl.value), (k, qb[k:k + 20])
             checked += 1
+        # the single call, with the pattern's length given (search.c:268-270)
+        # or taken to the query's end
+        for k in [int(x) for x in rng.integers(0, n, 25)]:
+            for length in [x for x in (0, 1, 3, 12, len(qb) - k) if x <= len(qb) - k]:
+                got, exp = [], []
+                for fn, out in ((L.sa_search, got), (R.sa_search, exp)):
+                    vi, vn, vl = C.c_int64(-9), C.c_int64(-9), C.c_int64(-9)
+                    rc = fn(C.byref(db), C.byref(q), k, length, C.byref(vi), C.byref(vn),
+                            C.byref(vl))
+                    out.append((rc, vi.value, vn.value, vl.value))
+                assert got == exp, (k, length, got, exp)
+                checked += 1
+        for fn in (L.sa_search, R.sa_search):
+            vi, vn = C.c_int64(-9), C.c_int64(-9)
+            assert fn(C.byref(db), C.byref(q), len(qb), 0, C.byref(vi), C.byref(vn), None) == -1
+            assert (vi.value, vn.value) == (-1, -9)
         # past the end: both return -1 with idx -1
         end = np.array([len(qb)], np.int32)
         one = np.full(1, -9, np.int64)

@@ -1,9 +1,10 @@
 """End-to-end drop-in check: TOPAZ's own search driver linked against
-libtopaz_amd.so (in place of ssw.o, ssw_wrapper.o and search.c's sa_hsp)
+libtopaz_amd.so in place of ssw.o, ssw_wrapper.o, search.o and alignment.o
 prints exactly the hit lists of the stock reference binary.
 
 Both binaries come from the reference sources compiled by oracle/Makefile
-into oracle/_ref/ (the checker); only the alignment boundary differs.
+into oracle/_ref/ (the checker); only the alignment and suffix-search
+boundary differs.
 """
 import os
 import subprocess
@@ -18,23 +19,22 @@ pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parent.parent
 REF = ROOT / "oracle" / "_ref"
 OBJ = REF / "obj"
+# what the library replaces: every symbol these objects define
+REPLACED = {"ssw.o", "ssw_wrapper.o", "search.o", "alignment.o"}
 
 
 def _link_dropin(tmp, batch=False):
-    """TOPAZ's objects minus ssw.o, ssw_wrapper.o and search.o's sa_hsp,
-    linked against the library.  batch: query.o and suffix.o replaced by
-    query_batch.o and suffix_batch.o, whose per-seed and per-hit loops call
-    the batch entry points."""
-    drop = {"ssw.o", "ssw_wrapper.o", "search.o"} | ({"query.o", "suffix.o"} if batch else set())
+    """TOPAZ's objects minus ssw.o, ssw_wrapper.o, search.o and
+    alignment.o, linked against the library (sa_search, sa_hsp and the
+    alignment calls then run on the GPU).  batch: query.o and suffix.o
+    replaced by query_batch.o and suffix_batch.o, whose per-seed and per-hit
+    loops call the batch entry points."""
+    drop = REPLACED | ({"query.o", "suffix.o"} if batch else set())
     objs = sorted(str(p) for p in OBJ.rglob("*.o"))
     keep = [o for o in objs if Path(o).name not in drop]
+    extra = []
     if batch:
         keep += [str(REF / "batch" / "query_batch.o"), str(REF / "batch" / "suffix_batch.o")]
-    search_local = str(tmp / "search_local.o")
-    # keep the reference's sa_search & co., hide its sa_hsp
-    subprocess.run(["objcopy", "-L", "sa_hsp", str(OBJ / "search.o"), search_local], check=True)
-    extra = [search_local]
-    if batch:
         # the batch build also takes db_index from the library (db.o keeps the rest)
         db_local = str(tmp / "db_local.o")
         subprocess.run(["objcopy", "-L", "db_index", str(OBJ / "db.o"), db_local], check=True)

@@ -13,7 +13,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libtopaz_amd.so"
 SOURCES = ["runtime.hip", "sw_kernels.hip", "gapless_kernels.hip", "traceback_kernels.hip", "compat.hip",
            "server.hip", "suffix_array.hip", "gapless_noalloc.hip", "walk.hip", "search.hip", "sw_pair_kernels.hip",
-           "walk_kernels.hip", "sw_lane_kernels.hip", "seg.hip"]
+           "walk_kernels.hip", "sw_lane_kernels.hip", "seg.hip", "compat_scalar.hip"]
 ARCH = os.environ.get("TOPAZ_AMD_ARCH", "gfx950")
 
 

@@ -33,6 +33,7 @@
 
 #include "../../include/topaz_amd.h"
 #include "../../include/topaz_ssw_compat.h"
+#include "alphabet.h"
 #include "gapless_kernels.h"
 #include "runtime.h"
 #include "sw_kernels.h"
@@ -485,33 +486,6 @@ void check_gaps(int go, int ge) {
                         "them as uint8_t, 0..255)\n", go, ge);
         _exit(EXIT_FAILURE);
     }
-}
-
-// lexicographical.h:19-74 tables, built once.
-struct Alphabet {
-    int8_t table2[128];       // aa_table2
-    char upper[256];          // upper_aa
-    Alphabet() {
-        static const char internal_order[] = "ACMLJIVTSKRQZEDBNHYFWXP*G";
-        static const char matrix_order[] = "ARNDCQEGHILKMFPSTWYVBJZX*";
-        for (int c = 0; c < 128; ++c) table2[c] = 23;
-        for (int k = 0; k < 25; ++k) {
-            const int8_t idx = (int8_t)(strchr(matrix_order, internal_order[k]) - matrix_order);
-            table2['A' + k] = idx;
-            table2['a' + k] = idx;
-        }
-        for (int c = 0; c < 256; ++c) {
-            const char ch = (char)c;
-            if ((ch >= 'A' && ch <= 'Y') || ch == '*' || ch == '\n') upper[c] = ch;
-            else if (ch >= 'a' && ch <= 'y') upper[c] = (char)(ch - 32);
-            else upper[c] = 'X';
-        }
-    }
-    int8_t idx(char c) const { return table2[(int)(unsigned char)c & 127]; }
-};
-const Alphabet& alphabet() {
-    static const Alphabet a;
-    return a;
 }
 
 }  // namespace
@@ -1176,21 +1150,29 @@ int db_index(db_t* db, uint64_t pos, int64_t* index) {
     return 0;
 }
 
-int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
-                    int64_t* num_matches, int64_t* last_position) {
-    if (n <= 0) return 0;
-    if (!db->T || !db->SA || db->Tsize <= 0) {                     // search.c:274-276
-        for (int k = 0; k < n; ++k) idx[k] = -1;
-        return -1;
-    }
+}  // extern "C"
+
+// sa_search (search.c:237-361) for n offsets of one query on the GPU: each
+// offset's index, match count and last position, and sa_search's return
+// value in status[k].  The pattern at offset o is bytes o .. view_len - 1 of
+// the query's buffer.
+static void search_offsets(db_t* db, seq_t* q, int64_t view_len, int n, const int* offsets,
+                           int64_t* idx, int64_t* num_matches, int64_t* last_position,
+                           int32_t* status) {
+    for (int k = 0; k < n; ++k)
+        if (offsets[k] < 0) {
+            // the reference reads before the query's first byte
+            fprintf(stderr, "topaz_amd: sa_search: negative query offset %d\n", offsets[k]);
+            _exit(EXIT_FAILURE);
+        }
     const ResidentText& rt = resident_text(db);
     const char* raw = resident_raw(db);
     ThreadCtx& c = ctx();
     // pinned block: [query bytes | offsets | idx, num_matches, last_position | status]
-    const size_t qb = ((size_t)q->seq_len + 16) & ~(size_t)15;
+    const size_t qb = ((size_t)view_len + 16) & ~(size_t)15;
     const size_t ob = ((size_t)n * 4 + 15) & ~(size_t)15;
     char* h = c.pin.get(qb + ob + (size_t)n * 24 + (size_t)n * 4);
-    memcpy(h, q->seq_buf.buf, (size_t)q->seq_len);
+    if (view_len > 0) memcpy(h, q->seq_buf.buf, (size_t)view_len);
     int32_t* ho = reinterpret_cast<int32_t*>(h + qb);
     for (int k = 0; k < n; ++k) ho[k] = offsets[k];
     int64_t* hr = reinterpret_cast<int64_t*>(h + qb + ob);
@@ -1200,24 +1182,62 @@ int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
     a.t_size = (int64_t)db->Tsize;
     a.sa = (const int64_t*)rt.text->d_sa.ptr;
     a.query = h;
-    a.q_len = q->seq_len;
+    a.q_len = view_len;
     a.offsets = ho;
     a.n = n;
     a.out = hr;
     a.status = hs;
     hip_or_die(tpz_launch_sa_search(&a, c.st), "sa_search");
     c.finish("sa_search");
-    int rc = 0;
     for (int k = 0; k < n; ++k) {
         idx[k] = hr[3 * k];
-        if (hs[k] != 0) {
-            rc = -1;
-            continue;
-        }
+        status[k] = hs[k];
+        if (hs[k] != 0) continue;
         if (num_matches) num_matches[k] = hr[3 * k + 1];
         if (last_position) last_position[k] = hr[3 * k + 2];
     }
+}
+
+extern "C" {
+
+int sa_search_batch(db_t* db, seq_t* q, int n, const int* offsets, int64_t* idx,
+                    int64_t* num_matches, int64_t* last_position) {
+    if (n <= 0) return 0;
+    if (!db->T || !db->SA || db->Tsize <= 0) {                     // search.c:274-276
+        for (int k = 0; k < n; ++k) idx[k] = -1;
+        return -1;
+    }
+    std::vector<int32_t> status((size_t)n);
+    search_offsets(db, q, q->seq_len, n, offsets, idx, num_matches, last_position,
+                   status.data());
+    int rc = 0;
+    for (int k = 0; k < n; ++k)
+        if (status[k] != 0) rc = -1;
     return rc;
+}
+
+int sa_search(db_t* db, seq_t* q, int64_t offset, int64_t length, int64_t* idx,
+              int64_t* num_matches, int64_t* last_position) {
+    if (idx) *idx = -1;
+    // Psize = length > 0 ? length : seq_len - offset (search.c:268-270)
+    const int64_t psize = length > 0 ? length : (int64_t)q->seq_len - offset;
+    if (!db->T || !db->SA || db->Tsize <= 0 || !q->seq_buf.buf || psize <= 0) return -1;
+    if (offset < 0 || offset > INT32_MAX || offset + psize > (int64_t)q->seq_buf.len) {
+        // the reference reads outside the query's buffer
+        fprintf(stderr, "topaz_amd: sa_search: pattern [%lld, %lld) outside the query buffer "
+                        "(%d bytes)\n", (long long)offset, (long long)(offset + psize),
+                q->seq_buf.len);
+        _exit(EXIT_FAILURE);
+    }
+    const int off = (int)offset;
+    int64_t i = -1, nm = 0, lp = 0;
+    int32_t st = 0;
+    search_offsets(db, q, offset + psize, 1, &off, &i, &nm, &lp, &st);
+    if (st != 0) return -1;
+    if (idx) *idx = i;
+    if (num_matches) *num_matches = nm;
+    if (last_position) *last_position = lp;
+    return 0;
 }
 
 void sa_hsp_batch(db_t* db, seq_t* q, int n, const int* qoffsets, const int64_t* positions,
