@@ -402,7 +402,8 @@ int8_t matrix_bias(const int8_t* mat, int32_t n) {
 void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
                  const int8_t* mat, int go, int ge, int32_t out[3]) {
     ThreadCtx& c = ctx();
-    const size_t m_pad = (size_t)m + 16, n_pad = (((size_t)n + 3) & ~(size_t)3) + 16;
+    // the subject starts on a 16-byte boundary and is padded with residue 25
+    const size_t m_pad = (((size_t)m + 15) & ~(size_t)15) + 16, n_pad = (((size_t)n + 15) & ~(size_t)15) + 16;
     // layout of the input block
     const size_t o_q = 0, o_s = o_q + m_pad, o_mat = o_s + n_pad, o_meta = (o_mat + 625 + 15) & ~(size_t)15;
     const size_t bytes = o_meta + 128;
@@ -440,7 +441,7 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     int32_t* dout = (int32_t*)c.out.get(64);
     // four wavefront buffers (textbook gaps) or the lane kernel's three rows
     const bool lanes = !textbook_gaps(go, ge);
-    const size_t ws_words = lanes ? (size_t)3 * lane_rows(m) + 16 : (size_t)4 * 2 * (n + 2) * 64;
+    const size_t ws_words = lanes ? (size_t)3 * lane_rows(m) + 16 : (size_t)4 * 2 * (n + 4) * 64;
     uint32_t* dws = (uint32_t*)c.ws.get(ws_words * 4);
     hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
     SwKernelArgs a{};
@@ -827,7 +828,7 @@ int sa_hsp(db_t* db, seq_t* q, int qoffset, int64_t position, options_t* opt,
 // ---------------------------------------------------------------------------
 namespace {
 // Subjects of the hits as matrix indices (db_seq then aa_table2 of upper_aa,
-// db.c:233-247, ssw_wrapper.c:188-201), 4-byte aligned for the batch.
+// db.c:233-247, ssw_wrapper.c:188-201), back to back; the batch re-packs them.
 struct HitSubjects {
     std::vector<int8_t> res;
     std::vector<int64_t> off;

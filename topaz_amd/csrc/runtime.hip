@@ -367,7 +367,7 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
                                           plan->items_persistent.size() * 4);
     // workspace: per workgroup, one boundary buffer (bottom H and F of a
     // chunk: 2 words per column and lane) per wavefront
-    plan->ws_words_per_slot = (int64_t)W * 2 * (b->max_subject + 2) * 64;
+    plan->ws_words_per_slot = (int64_t)W * 2 * (b->max_subject + 4) * 64;
     const size_t n_items_p = plan->items_persistent.size() / W;
     plan->ws_slots = std::max<size_t>(std::min(persistent_slots, n_items_p), 1);
     rc |= plan->d_workspace.alloc(plan->ws_slots * (size_t)plan->ws_words_per_slot * 4);
@@ -446,13 +446,13 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
             qr[qo[k] + j] = r;
         }
     }
-    // subjects: each starts on a 4-byte boundary
+    // subjects: each starts on a 16-byte boundary, padded with residue 25
     std::vector<int64_t> so(n_subjects);
     int64_t stot = 0;
     for (int32_t k = 0; k < n_subjects; ++k) {
         if (s_len[k] < 0) { delete b; return fail("subject %d has negative length", k); }
         so[k] = stot;
-        stot += (s_len[k] + 3) & ~3;
+        stot += (s_len[k] + 15) & ~15;
         b->max_subject = std::max(b->max_subject, s_len[k]);
     }
     std::vector<int8_t> sr((size_t)stot + 64, 25);

@@ -408,13 +408,13 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
     if (s->cfg.waves_per_group != 4 && s->cfg.waves_per_group != 8)
         return fail("waves_per_group must be 4 or 8 (got %d)", s->cfg.waves_per_group);
     s->s_len.assign(s_len, s_len + n_subjects);
-    // subjects: each starts on a 4-byte boundary, padded with residue 25
+    // subjects: each starts on a 16-byte boundary, padded with residue 25
     std::vector<int64_t> so(n_subjects);
     int64_t stot = 0;
     for (int32_t k = 0; k < n_subjects; ++k) {
         if (s_len[k] < 0) return fail("subject %d has negative length", k);
         so[k] = stot;
-        stot += (s_len[k] + 3) & ~3;
+        stot += (s_len[k] + 15) & ~15;
         s->max_subject = std::max(s->max_subject, s_len[k]);
     }
     std::vector<int8_t> sr((size_t)stot + 64, 25);
@@ -430,7 +430,7 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
               std::max(1, std::min(kGroupsPerCu, tpz_sw_forward_groups_per_cu(32, s->cfg.waves_per_group)));
     // the forward kernel's workspace (the lane kernel's grows with the
     // launch's longest query, in the slot)
-    s->ws_words_per_slot = s->lanes ? 0 : (int64_t)s->cfg.waves_per_group * 2 * (s->max_subject + 2) * 64;
+    s->ws_words_per_slot = s->lanes ? 0 : (int64_t)s->cfg.waves_per_group * 2 * (s->max_subject + 4) * 64;
     for (int k = 0; k < s->cfg.max_inflight && !rc; ++k) {
         auto sl = std::make_unique<Slot>();
         rc |= check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");

@@ -8,8 +8,9 @@
 namespace tpz {
 
 // Device pointers of one forward pass over a batch of (query, subject) pairs.
-// Residues are substitution-matrix indices 0..24; subjects start on 4-byte
-// boundaries of s_res and are padded with residue 25.  A task is a range of
+// Residues are substitution-matrix indices 0..24; subjects start on 16-byte
+// boundaries of s_res and are padded with residue 25 to the next one, and at
+// least 16 bytes follow the last subject.  A task is a range of
 // at most 128 entries of pair_index (or of the pair numbers themselves when
 // pair_index is null) whose pairs belong to at most four queries, listed in
 // task_query (4 per task, -1 past the last).  A work item is either one

@@ -94,6 +94,13 @@ __device__ __forceinline__ void load_col(ProfCol<R>& c, const int16_t* p) {
     }
 }
 
+// Rows 8q .. 8q + 7 of a profile column (one 16-byte load).
+template <int R>
+__device__ __forceinline__ void load_quarter(ProfCol<R>& c, const int16_t* p, int q) {
+    const uint4 v = reinterpret_cast<const uint4*>(p)[q];
+    c.d[4 * q] = v.x; c.d[4 * q + 1] = v.y; c.d[4 * q + 2] = v.z; c.d[4 * q + 3] = v.w;
+}
+
 // Row r of the low-half and high-half columns as one packed pair.
 template <int R>
 __device__ __forceinline__ s2 row_w(const ProfCol<R>& lo, const ProfCol<R>& hi, int r) {
@@ -231,15 +238,27 @@ __device__ __forceinline__ uint32_t load_res4(const int8_t* s, int c, int n) {
 // every step advances two independent dependency chains.  Returns false if
 // a wait on the producer of the top boundary gave up.
 //
+// The loop body covers four columns (two column pairs).  A column pair
+// reads its profile columns in quarters of 8 rows, each loaded one quarter
+// ahead of its rows: quarter 0 during the previous pair (as soon as that
+// pair's rows 0-7 are done, into the same registers), quarter 1 when the
+// pair starts, quarter k + 2 when the pair's rows of quarter k are done.  So
+// no column pair starts by waiting for LDS, and at most three quarters of
+// profile columns are live at a time.  The next pair's residues come from
+// the residue dword of the current four columns, or of the next four,
+// loaded at the top of the iteration.
+//
 // Each iteration reloads the prefetched top boundary and the residue dword
 // right after their last use, into the same variables: no loaded value is
-// copied at the back edge, so no iteration waits for the loads or stores it
-// has just issued.
+// copied at the back edge before its load was waited for, so no iteration
+// waits for the loads or stores it has just issued.
 template <int R>
 __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds, const ChunkIO& io,
                                           int ncols, const int8_t* s_lo, int n_lo,
                                           const int8_t* s_hi, int n_hi, int off_lo, int off_hi,
                                           int j0, int chunk, LaneBest& best) {
+    constexpr int kS = WaveLds<R>::kStride;
+    constexpr int kQuarters = R / 8;
     const int lane = threadIdx.x % 64;
     const s2 go = s2{(short)a.gap_open, (short)a.gap_open};
     const s2 ge = s2{(short)a.gap_extend, (short)a.gap_extend};
@@ -271,16 +290,36 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
     // iteration.
     __builtin_amdgcn_s_waitcnt(kWaitVmem);
 
+    // profile columns of the current column pair (a = even, b = odd column),
+    // starting with quarter 0 of columns 0 and 1
+    ProfCol<R> pa_lo, pa_hi, pb_lo, pb_hi;
+    load_quarter<R>(pa_lo, lds.prof + (off_lo + (res_lo & 0xff)) * kS, 0);
+    load_quarter<R>(pa_hi, lds.prof + (off_hi + (res_hi & 0xff)) * kS, 0);
+    load_quarter<R>(pb_lo, lds.prof + (off_lo + ((res_lo >> 8) & 0xff)) * kS, 0);
+    load_quarter<R>(pb_hi, lds.prof + (off_hi + ((res_hi >> 8) & 0xff)) * kS, 0);
+    uint32_t cur_lo = res_lo, cur_hi = res_hi;   // residues of the current pair
+
     // Columns i and i + 1 from the prefetched boundary T, which is then
-    // reloaded with columns i + 2 and i + 3.
-    auto column_pair = [&](int i, int ra_lo, int ra_hi, int rb_lo, int rb_hi,
+    // reloaded with columns i + 2 and i + 3; nx_lo / nx_hi hold the residues
+    // of columns i + 2 (bits 0-7) and i + 3 (bits 8-15).
+    auto column_pair = [&](int i, uint32_t nx_lo, uint32_t nx_hi,
                            TopPair& T) __attribute__((always_inline)) {
-        constexpr int kS = WaveLds<R>::kStride;
-        ProfCol<R> pa_lo, pa_hi, pb_lo, pb_hi;
-        load_col<R>(pa_lo, lds.prof + (off_lo + ra_lo) * kS);
-        load_col<R>(pa_hi, lds.prof + (off_hi + ra_hi) * kS);
-        load_col<R>(pb_lo, lds.prof + (off_lo + rb_lo) * kS);
-        load_col<R>(pb_hi, lds.prof + (off_hi + rb_hi) * kS);
+        const int16_t* ca_lo = lds.prof + (off_lo + (cur_lo & 0xff)) * kS;
+        const int16_t* ca_hi = lds.prof + (off_hi + (cur_hi & 0xff)) * kS;
+        const int16_t* cb_lo = lds.prof + (off_lo + ((cur_lo >> 8) & 0xff)) * kS;
+        const int16_t* cb_hi = lds.prof + (off_hi + ((cur_hi >> 8) & 0xff)) * kS;
+        const int16_t* na_lo = lds.prof + (off_lo + (nx_lo & 0xff)) * kS;
+        const int16_t* na_hi = lds.prof + (off_hi + (nx_hi & 0xff)) * kS;
+        const int16_t* nb_lo = lds.prof + (off_lo + ((nx_lo >> 8) & 0xff)) * kS;
+        const int16_t* nb_hi = lds.prof + (off_hi + ((nx_hi >> 8) & 0xff)) * kS;
+        cur_lo = nx_lo;
+        cur_hi = nx_hi;
+        if (kQuarters > 1) {
+            load_quarter<R>(pa_lo, ca_lo, 1);
+            load_quarter<R>(pa_hi, ca_hi, 1);
+            load_quarter<R>(pb_lo, cb_lo, 1);
+            load_quarter<R>(pb_hi, cb_hi, 1);
+        }
 
         s2 fa = from_bits(T.f0 & keep), fb = from_bits(T.f1 & keep);
         s2 da = hdiag;                  // H(i-1, j0-1)
@@ -321,6 +360,23 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
                 E[q] = vmax(usub(E[q], ge), hg);
                 fb = vmax(usub(fb, ge), hg);
             }
+            if (r % 8 == 0) {
+                // rows of quarter k = r / 8 - 1 done in both columns
+                const int k = r / 8 - 1;
+                if (k == 0) {
+                    // quarter 0 of the next pair
+                    load_quarter<R>(pa_lo, na_lo, 0);
+                    load_quarter<R>(pa_hi, na_hi, 0);
+                    load_quarter<R>(pb_lo, nb_lo, 0);
+                    load_quarter<R>(pb_hi, nb_hi, 0);
+                }
+                if (k + 2 < kQuarters) {
+                    load_quarter<R>(pa_lo, ca_lo, k + 2);
+                    load_quarter<R>(pa_hi, ca_hi, k + 2);
+                    load_quarter<R>(pb_lo, cb_lo, k + 2);
+                    load_quarter<R>(pb_hi, cb_hi, k + 2);
+                }
+            }
         }
         // column b, last row
         {
@@ -346,8 +402,9 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             publish(io, io.pub_base + i + 2);
     };
 
-    // ncols is even (the host pads subjects with residue 25 columns)
-    for (int i = 0; i < ncols; i += 2) {
+    // ncols is a multiple of 4 (the host pads subjects with residue 25
+    // columns); res_lo / res_hi hold the residues of columns i .. i + 3
+    for (int i = 0; i < ncols; i += 4) {
         if (io.wait_ctr && (i % kSyncCols) == 0 && waited < ncols) {
             // columns up to i + kAhead must be final before they are prefetched
             const int need = i + kAhead < ncols ? i + kAhead : ncols;
@@ -356,12 +413,12 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
                 waited = need;
             }
         }
-        const int sh = (i & 2) * 8;
-        const int ra_lo = (res_lo >> sh) & 0xff, ra_hi = (res_hi >> sh) & 0xff;
-        const int rb_lo = (res_lo >> (sh + 8)) & 0xff, rb_hi = (res_hi >> (sh + 8)) & 0xff;
-        res_lo = load_res4(s_lo, (i + 2) & ~3, n_lo);
-        res_hi = load_res4(s_hi, (i + 2) & ~3, n_hi);
-        column_pair(i, ra_lo, ra_hi, rb_lo, rb_hi, T);
+        const uint32_t rn_lo = load_res4(s_lo, i + 4, n_lo);
+        const uint32_t rn_hi = load_res4(s_hi, i + 4, n_hi);
+        column_pair(i, res_lo >> 16, res_hi >> 16, T);
+        column_pair(i + 2, rn_lo, rn_hi, T);
+        res_lo = rn_lo;
+        res_hi = rn_hi;
     }
 
     // Lowest row of the saved column equal to the chunk best, then merge:
@@ -476,7 +533,7 @@ __device__ __forceinline__ TaskView task_view(const SwKernelArgs& a, int task) {
         ncols = o > ncols ? o : ncols;
     }
     ncols = __builtin_amdgcn_readfirstlane(ncols);
-    t.ncols = ncols + (ncols & 1);     // columns go in pairs; the pad column scores kPadScore
+    t.ncols = (ncols + 3) & ~3;        // columns go in fours; pad columns score kPadScore
     return t;
 }
 
