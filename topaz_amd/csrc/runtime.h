@@ -30,7 +30,18 @@ struct DeviceBuffer {
     void reset();
     int alloc(size_t n);
     int upload(const void* src, size_t n);
+    // alloc(n), then the bytes through pinned staging (stage_to_device)
+    int upload_large(const void* src, size_t n);
 };
+
+// Bytes [0, n) of device memory dst, produced on the host by fill(out,
+// begin, end) (which writes source bytes [begin, end) to out), through pinned
+// staging buffers: host threads fill one buffer while the copy engine drains
+// the previous ones.  For the multi-gigabyte database arrays, whose pageable
+// copies and single-threaded conversions would otherwise dominate the
+// upload.  Returns when every byte is on the device.
+int stage_to_device(void* dst, size_t n,
+                    const std::function<void(uint8_t* out, size_t begin, size_t end)>& fill);
 
 // ---- batched Smith-Waterman planning (shared by batches and the server) ----
 constexpr int kGroupsPerCu = 2;     // default resident sw_forward workgroups per CU

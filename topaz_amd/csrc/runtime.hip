@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -102,6 +103,65 @@ static int host_threads() {
         return std::max(1, std::min(v, 64));
     }();
     return n;
+}
+
+int DeviceBuffer::upload_large(const void* src, size_t n) {
+    if (alloc(n)) return -1;
+    if (!n || !src) return 0;
+    const uint8_t* s = static_cast<const uint8_t*>(src);
+    return stage_to_device(ptr, n, [s](uint8_t* out, size_t b, size_t e) { memcpy(out, s + b, e - b); });
+}
+
+namespace {
+constexpr int kStageBuffers = 3;
+constexpr size_t kStageBytes = (size_t)128 << 20;
+struct Staging {
+    std::mutex mu;
+    uint8_t* buf[kStageBuffers] = {};
+    hipEvent_t ev[kStageBuffers] = {};
+    bool ready = false;
+};
+Staging& staging() {
+    static Staging* s = new Staging();   // lives until the process ends
+    return *s;
+}
+}  // namespace
+
+int stage_to_device(void* dst, size_t n,
+                    const std::function<void(uint8_t* out, size_t begin, size_t end)>& fill) {
+    if (n == 0) return 0;
+    Staging& st = staging();
+    std::lock_guard<std::mutex> lk(st.mu);
+    if (!st.ready) {
+        for (int k = 0; k < kStageBuffers; ++k) {
+            if (check(hipHostMalloc((void**)&st.buf[k], kStageBytes, hipHostMallocDefault),
+                      "hipHostMalloc (staging)") ||
+                check(hipEventCreateWithFlags(&st.ev[k], hipEventDisableTiming), "hipEventCreate"))
+                return -1;
+        }
+        st.ready = true;
+    }
+    const int threads = host_threads();
+    uint8_t* d = static_cast<uint8_t*>(dst);
+    size_t k = 0;
+    for (size_t off = 0; off < n; off += kStageBytes, ++k) {
+        const int b = (int)(k % kStageBuffers);
+        // the copy that last read this buffer must have finished
+        if (k >= (size_t)kStageBuffers && check(hipEventSynchronize(st.ev[b]), "staging wait"))
+            return -1;
+        const size_t len = std::min(kStageBytes, n - off);
+        uint8_t* out = st.buf[b];
+        // pieces of 1 MB over the host threads
+        const int64_t pieces = (int64_t)((len + (1 << 20) - 1) >> 20);
+        parallel_for(pieces, threads, [&](int64_t p0, int64_t p1) {
+            const size_t a = (size_t)p0 << 20, e = std::min(len, (size_t)p1 << 20);
+            if (a < e) fill(out + a, off + a, off + e);
+        });
+        if (check(hipMemcpyAsync(d + off, out, len, hipMemcpyHostToDevice, g_stream), "staged upload") ||
+            check(hipEventRecord(st.ev[b], g_stream), "hipEventRecord"))
+            return -1;
+    }
+    return check(hipStreamSynchronize(g_stream), "staged upload");
 }
 
 // A counting sort by query (pairs stay in pair order), then per query a sort
@@ -768,8 +828,20 @@ static int query_row(uint8_t c) { return maps().qmap[c & 127]; }
 // the less frequent of the first sixteen).  With codes in byte order the
 // twenty amino acids fell into ten banks.
 static int text_codes(const uint8_t* text, uint64_t n, TextCodes& tc) {
+    // byte counts, per host thread over 1 MB pieces, then summed
+    const int threads = n >= ((uint64_t)1 << 24) ? host_threads() : 1;
+    const int64_t pieces = (int64_t)((n + (1 << 20) - 1) >> 20);
+    std::vector<std::array<uint64_t, 256>> part((size_t)std::max(1, threads));
+    for (auto& c : part) c.fill(0);
+    std::atomic<int> slot{0};
+    parallel_for(pieces, threads, [&](int64_t p0, int64_t p1) {
+        std::array<uint64_t, 256>& c = part[(size_t)slot++];
+        const uint64_t e = std::min<uint64_t>(n, (uint64_t)p1 << 20);
+        for (uint64_t i = (uint64_t)p0 << 20; i < e; ++i) ++c[text[i]];
+    });
     uint64_t count[256] = {0};
-    for (uint64_t i = 0; i < n; ++i) ++count[text[i]];
+    for (const auto& c : part)
+        for (int b = 0; b < 256; ++b) count[b] += c[b];
     int letters[256], n_letters = 0;
     for (int c = 0; c < 256; ++c)
         if (count[c] && c != '\n') letters[n_letters++] = c;
@@ -850,16 +922,16 @@ int tpz_text_create(tpz_text** out, const char* text, uint64_t t_size, const int
     if (text_codes(T, t_size, t->codes)) { delete t; return -1; }
     // the code copy followed by terminators, and the head window: the left
     // scan stops before T[0] (i < idx), so there T[0] is a terminator
-    std::vector<uint8_t> buf(t_size + kTextPad);
     const uint8_t* code = t->codes.code;
-    int rc = 0;
-    for (uint64_t i = 0; i < t_size; ++i) buf[i] = code[T[i]];
-    memset(buf.data() + t_size, kTextTerm, kTextPad);
-    rc |= t->d_fwd.upload(buf.data(), buf.size());
+    int rc = t->d_fwd.alloc(t_size + kTextPad);
+    if (!rc)
+        rc |= stage_to_device(t->d_fwd.ptr, t_size + kTextPad, [&](uint8_t* out, size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) out[i - b] = i < t_size ? code[T[i]] : (uint8_t)kTextTerm;
+        });
     std::vector<uint8_t> head(kHeadBytes, kTextTerm);
-    for (uint64_t i = 1; i < t_size && i < (uint64_t)kWindow; ++i) head[kWindow + i] = buf[i];
+    for (uint64_t i = 1; i < t_size && i < (uint64_t)kWindow; ++i) head[kWindow + i] = code[T[i]];
     rc |= t->d_head.upload(head.data(), head.size());
-    if (!rc && sa) rc |= t->d_sa.upload(sa, t_size * 8);
+    if (!rc && sa) rc |= t->d_sa.upload_large(sa, t_size * 8);
     if (!rc) rc |= check(hipStreamSynchronize(stream()), "text upload");
     if (rc) { delete t; return -1; }
     *out = t;

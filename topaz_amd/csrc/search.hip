@@ -865,13 +865,14 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     if (db->d_pblock.upload(db->proteins.blocks().data(), db->proteins.blocks().size() * 4) ||
         db->d_pend.upload(db->proteins.ends().data(), db->proteins.ends().size() * 8))
         return -1;
-    if (db->d_raw.upload(text, t_size)) return -1;
+    if (db->d_raw.upload_large(text, t_size)) return -1;
     db->table.assign(kTableRows * kTableCols, 0);
     gapless_table(db->dtext->codes, matrix, db->table.data());
     if (db->d_table.upload(db->table.data(), db->table.size() * 2)) return -1;
     // the proteins as matrix rows (aa_table2 of the text, ssw_wrapper.c:193-201)
     const Maps& M = maps();
-    std::vector<int8_t> res(t_size);
+    // written whole by the loop below: no zero fill of gigabytes first
+    std::unique_ptr<int8_t[]> res(new int8_t[t_size]);
     std::vector<int64_t> off(n_proteins);
     for (uint64_t k = 0; k < n_proteins; ++k) off[k] = (int64_t)protein_start[k];
     pool().run((int64_t)((t_size + (1 << 20) - 1) >> 20), [&](int64_t c) {
@@ -884,7 +885,7 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     cfg.max_launch_pairs = 64 << 20;
     cfg.waves_per_group = 4;
     if (const char* e = getenv("TOPAZ_AMD_SEARCH_WAVES")) cfg.waves_per_group = atoi(e);
-    if (tpz_server_create(&db->sw, (int32_t)n_proteins, res.data(), off.data(), protein_len,
+    if (tpz_server_create(&db->sw, (int32_t)n_proteins, res.get(), off.data(), protein_len,
                           matrix, gap_open, gap_extend, &cfg))
         return -1;
     if (check(hipStreamSynchronize(stream()), "database upload")) return -1;

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -417,15 +418,33 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
         stot += (s_len[k] + 15) & ~15;
         s->max_subject = std::max(s->max_subject, s_len[k]);
     }
-    std::vector<int8_t> sr((size_t)stot + 64, 25);
-    for (int32_t k = 0; k < n_subjects; ++k)
-        for (int32_t j = 0; j < s_len[k]; ++j) {
-            const int8_t r = s_res[s_off[k] + j];
-            if (r < 0 || r > 24) return fail("subject %d residue %d out of range", k, r);
-            sr[so[k] + j] = r;
-        }
-    int rc = s->d_s_res.upload(sr.data(), sr.size()) | s->d_s_off.upload(so.data(), so.size() * 8) |
-             s->d_s_len.upload(s_len, (size_t)n_subjects * 4) | s->d_matrix.upload(matrix, 625);
+    // the packed residues go to the device as host threads write them (a
+    // database's proteins are gigabytes): bytes [b, e) of the packed array
+    // start inside subject k = the last with so[k] <= b
+    const size_t packed = (size_t)stot + 64;
+    std::atomic<int64_t> bad{-1};             // a subject with a residue out of range
+    int rc = s->d_s_res.alloc(packed);
+    if (!rc)
+        rc = stage_to_device(s->d_s_res.ptr, packed, [&](uint8_t* o, size_t b, size_t e) {
+            int64_t k = (int64_t)(std::upper_bound(so.begin(), so.end(), (int64_t)b) - so.begin()) - 1;
+            for (size_t pos = b; pos < e;) {
+                while (k + 1 < n_subjects && (size_t)so[k + 1] <= pos) ++k;
+                const size_t stop = k + 1 < n_subjects ? std::min(e, (size_t)so[k + 1]) : e;
+                const int64_t j0 = k >= 0 ? (int64_t)pos - so[k] : 0;
+                const int64_t len = k >= 0 ? s_len[k] : 0;
+                for (int64_t j = j0; pos < stop; ++pos, ++j) {
+                    int8_t r = 25;
+                    if (j < len) {
+                        r = s_res[s_off[k] + j];
+                        if (r < 0 || r > 24) { int64_t none = -1; bad.compare_exchange_strong(none, k); r = 25; }
+                    }
+                    o[pos - b] = (uint8_t)r;
+                }
+            }
+        });
+    if (!rc && bad.load() >= 0) return fail("subject %lld has a residue out of range", (long long)bad.load());
+    rc |= s->d_s_off.upload(so.data(), so.size() * 8) |
+          s->d_s_len.upload(s_len, (size_t)n_subjects * 4) | s->d_matrix.upload(matrix, 625);
     s->grid = std::max(num_cu(), 1) *
               std::max(1, std::min(kGroupsPerCu, tpz_sw_forward_groups_per_cu(32, s->cfg.waves_per_group)));
     // the forward kernel's workspace (the lane kernel's grows with the
