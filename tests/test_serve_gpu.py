@@ -101,7 +101,7 @@ def test_memory_budget_admission():
     reqs = requests(rng, db, 10, max_q=20, max_pairs=80)
     sizes = []
     for qs, pq, ps in reqs:
-        sizes.append(sum(len(q) for q in qs) + 12 * len(qs) + 40 * len(pq) + 256)
+        sizes.append(sum(len(q) for q in qs) + 12 * len(qs) + 72 * len(pq) + 256)
     budget = 2 * max(sizes) + 64
     srv = Server(db, matrix(), 11, 1, memory_budget=budget, max_inflight=3)
     tickets = [srv.submit(*r) for r in reqs]

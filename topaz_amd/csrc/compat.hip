@@ -412,6 +412,12 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     memset(h, 0, bytes);
     memcpy(h + o_q, read, (size_t)m);
     memset(h + o_s, 25, n_pad);
+    if (m > kMaxSwLength || n > kMaxSwLength) {
+        fprintf(stderr, "topaz_amd: sequences of more than %d residues are not supported\n",
+                kMaxSwLength);
+        fflush(stderr);
+        _exit(EXIT_FAILURE);
+    }
     for (int32_t i = 0; i < m; ++i)
         if (read[i] < 0 || read[i] > 24) residue_out_of_range("query", i, read[i]);
     for (int32_t i = 0; i < n; ++i)
@@ -437,11 +443,17 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     meta32[12] = -1;
     meta32[13] = -1;
     meta32[14] = -1;
+    meta32[15] = item_columns(n);  // task_cols of task 0
+    int32_t* rec32 = reinterpret_cast<int32_t*>(h + o_meta + 80);
+    rec32[0] = 0;                  // entry 0: the subject at s_res + 0
+    rec32[1] = n;
+    rec32[2] = -1;                 // no further pair in its run
+    rec32[3] = 0;                  // query slot 0
     char* d = (char*)c.in.get(bytes);
     int32_t* dout = (int32_t*)c.out.get(64);
     // four wavefront buffers (textbook gaps) or the lane kernel's three rows
     const bool lanes = !textbook_gaps(go, ge);
-    const size_t ws_words = lanes ? (size_t)3 * lane_rows(m) + 16 : (size_t)4 * 2 * (n + 4) * 64;
+    const size_t ws_words = lanes ? (size_t)3 * lane_rows(m) + 16 : (size_t)4 * 2 * ((size_t)item_columns(n) + 4) * 64;
     uint32_t* dws = (uint32_t*)c.ws.get(ws_words * 4);
     hip_or_die(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.st), "upload");
     SwKernelArgs a{};
@@ -454,6 +466,10 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
     a.pair_query = (const int32_t*)(d + o_meta + 24);
     a.pair_subject = (const int32_t*)(d + o_meta + 28);
     a.pair_index = nullptr;
+    a.entry_rec = (const int4*)(d + o_meta + 80);
+    a.task_cols = (const int32_t*)(d + o_meta + 60 + 16);
+    a.out_key = (unsigned long long*)(dout + 4);
+    a.s_pad = (int64_t)(n_pad - 16);   // 16 residues 25 after the subject
     a.task_begin = (const int32_t*)(d + o_meta + 32);
     a.task_query = (const int32_t*)(d + o_meta + 60);
     a.item_task = (const int32_t*)(d + o_meta + 40);
@@ -473,7 +489,9 @@ void forward_one(const int8_t* read, int32_t m, const int8_t* ref, int32_t n,
         la.n_list = 1;
         hip_or_die(tpz_launch_sw_lane(&la, 1, c.st), "sw_lane");
     } else {
+        hip_or_die(tpz_launch_fill_i32(dout + 4, 0, 2, c.st), "clear key");
         hip_or_die(tpz_launch_sw_forward(&a, 1, 32, 4, c.st), "sw_forward");
+        hip_or_die(tpz_launch_sw_decode(&a, 1, c.st), "sw_decode");
     }
     int32_t* ho = (int32_t*)(h + o_out);
     hip_or_die(hipMemcpyAsync(ho, dout, 12, hipMemcpyDeviceToHost, c.st), "readback");

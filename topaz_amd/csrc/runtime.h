@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <utility>
 #include <functional>
 #include <vector>
@@ -49,29 +50,50 @@ constexpr int kGroupWaves = 4;      // default wavefronts per sw_forward workgro
 constexpr int kTaskPairs = 128;     // two pairs per lane of a wavefront
 constexpr int kTaskQueries = 4;     // queries whose profiles a wavefront can hold
 
-// Tasks of the kernel: ranges of at most 128 pairs of up to 4 queries.
+// Longest query or subject of the forward kernel: its keys hold columns and
+// rows below 2^24 (with room for a run's padding columns).
+constexpr int32_t kMaxSwLength = (1 << 24) - 64;
+
+// Columns a pair with an n-residue subject takes in a lane half's run (the
+// forward kernel's item_cols): n rounded up to 4, at least 8.
+inline int32_t item_columns(int32_t n) { return std::max<int32_t>(8, (n + 3) & ~3); }
+
+// Tasks of the kernel: pairs of up to 4 queries.  A task's first (up to 128)
+// entries start the runs of its wavefront's lane halves; next[] chains the
+// further entries of a run (-1 ends it).
 struct Tasks {
-    std::vector<int32_t> index;   // pair numbers, grouped per task
+    std::vector<int32_t> index;   // pair numbers, grouped per task (one entry each)
+    std::vector<int32_t> next;    // per entry: the next entry of its run, -1 = none
     std::vector<int32_t> begin;   // task boundaries into index
     std::vector<int32_t> query;   // kTaskQueries query numbers per task, -1 = none
-    std::vector<int64_t> cost;    // rows (whole chunks) x widest subject
-    void clear() { index.clear(); begin.clear(); query.clear(); cost.clear(); }
-    void open(const int32_t* qs, int nq, int64_t c) {
+    std::vector<int32_t> cols;    // per task: columns of its longest run
+    std::vector<int64_t> cost;    // rows (whole chunks) x cols
+    void clear() { index.clear(); next.clear(); begin.clear(); query.clear(); cols.clear(); cost.clear(); }
+    void open(const int32_t* qs, int nq) {
         begin.push_back((int32_t)index.size());
         for (int k = 0; k < kTaskQueries; ++k) query.push_back(k < nq ? qs[k] : -1);
-        cost.push_back(c);
+        cols.push_back(0);
+        cost.push_back(0);
     }
     int32_t count() const { return (int32_t)cost.size(); }
 };
 
+// Per entry of t: the forward kernel's record (SwKernelArgs::entry_rec):
+// subject offset / 16 (s_off 16-byte aligned), subject length, next entry,
+// query slot within the entry's task.
+void build_entry_records(const Tasks& t, const int32_t* pair_query, const int32_t* pair_subject,
+                         const int64_t* s_off, const int32_t* s_len, std::vector<int4>& rec);
+
 // A task set on the device.
 struct DeviceTasks {
     Tasks host;
-    DeviceBuffer d_index, d_begin, d_query;
-    int upload() {
+    DeviceBuffer d_index, d_begin, d_query, d_cols, d_rec;
+    int upload(const std::vector<int4>& rec) {
         return d_index.upload(host.index.data(), host.index.size() * 4) |
                d_begin.upload(host.begin.data(), host.begin.size() * 4) |
-               d_query.upload(host.query.data(), host.query.size() * 4);
+               d_query.upload(host.query.data(), host.query.size() * 4) |
+               d_cols.upload(host.cols.data(), host.cols.size() * 4) |
+               d_rec.upload(rec.data(), rec.size() * sizeof(int4));
     }
 };
 
@@ -93,7 +115,9 @@ struct QueryPairs {
 QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
                           const std::vector<int32_t>& pair_subject,
                           const std::vector<int32_t>& s_len, int32_t n_queries);
-// Up to nq_max queries of equal chunk count per task (the persistent schedule).
+// Up to nq_max queries of equal chunk count per task (the persistent
+// schedule), the runs of the lane halves filled as described at the
+// definition.
 void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len, int rows,
                         int nq_max, Tasks& t);
 // fn(i) for i in [0, n) on up to `threads` host threads (the caller's

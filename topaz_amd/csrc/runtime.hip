@@ -205,10 +205,13 @@ QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
 // Tasks of up to nq (<= 4) queries (the persistent schedule).  Queries with
 // the same number of row chunks are packed together: repeatedly the nq
 // queries whose longest remaining subjects are longest give their longest
-// remaining pairs, longest first, until the task holds 128.  A query's long
-// subjects then share a wavefront with other queries' long subjects instead
-// of its own short ones, which cuts the columns a wavefront walks past the
-// end of a lane's subject.
+// remaining pairs, longest first, until the task holds 128: one per lane
+// half, the first of its run.  A query's long subjects then share a
+// wavefront with other queries' long subjects instead of its own short ones.
+// The task's width is its longest pair; then every run, from the shortest up,
+// takes the longest remaining pair of the same queries that fits in what is
+// left of that width, while one does, so the lane halves walk pairs instead
+// of padding columns.
 void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len, int rows,
                         int nq_max, Tasks& t) {
     t.clear();
@@ -219,13 +222,26 @@ void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         return chunk_rows(q_len[x], rows) < chunk_rows(q_len[y], rows);
     });
+    // positions of per (each query's pairs, longest first) not taken yet:
+    // untaken(i) is the first untaken position >= i (path halving)
+    const int64_t n = per.begin.back();
+    std::vector<int64_t> nx((size_t)n + 1);
+    std::iota(nx.begin(), nx.end(), (int64_t)0);
+    auto untaken = [&](int64_t i) {
+        while (nx[i] != i) { nx[i] = nx[nx[i]]; i = nx[i]; }
+        return i;
+    };
+    auto take = [&](int64_t i) { nx[i] = i + 1; };
     std::vector<int64_t> head(per.begin.begin(), per.begin.end() - 1);
+    auto has = [&](int32_t q) { return (head[q] = untaken(head[q])) < per.begin[q + 1]; };
     auto head_len = [&](int32_t q) { return per.len[head[q]]; };
     // max-heap on (longest remaining subject, then lower query number)
     auto below = [&](int32_t x, int32_t y) {
         const int32_t lx = head_len(x), ly = head_len(y);
         return lx != ly ? lx < ly : x > y;
     };
+    std::vector<int32_t> load;     // columns of each run of the open task
+    std::vector<int32_t> last;     // last entry of each run
     size_t g0 = 0;
     while (g0 < order.size()) {
         const int64_t task_rows = chunk_rows(q_len[order[g0]], rows);
@@ -241,19 +257,55 @@ void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len
                 sel[ns++] = heap.back();
                 heap.pop_back();
             }
-            t.open(sel, ns, task_rows * head_len(sel[0]));
+            t.open(sel, ns);
+            load.clear();
+            last.clear();
             for (int taken = 0; taken < kTaskPairs; ++taken) {
                 int best = -1;
                 for (int k = 0; k < ns; ++k) {
                     const int32_t q = sel[k];
-                    if (head[q] < per.begin[q + 1] && (best < 0 || below(sel[best], q))) best = k;
+                    if (has(q) && (best < 0 || below(sel[best], q))) best = k;
                 }
                 if (best < 0) break;
                 const int32_t q = sel[best];
-                t.index.push_back(per.pair[head[q]++]);
+                load.push_back(item_columns(per.len[head[q]]));
+                last.push_back((int32_t)t.index.size());
+                t.index.push_back(per.pair[head[q]]);
+                t.next.push_back(-1);
+                take(head[q]);
             }
+            const int32_t width = load.empty() ? 0 : load[0];
+            // runs from the shortest (the last seeded) up take the longest
+            // remaining pair that fits (a pair of length <= gap takes
+            // item_columns <= gap columns: gaps are multiples of 4)
+            for (int lane = (int)load.size() - 1; lane >= 0; --lane) {
+                for (;;) {
+                    const int32_t gap = width - load[lane];
+                    if (gap < 8) break;
+                    int64_t pick = -1;
+                    for (int k = 0; k < ns; ++k) {
+                        const int32_t q = sel[k];
+                        const int64_t b = per.begin[q], e = per.begin[q + 1];
+                        // first position of q whose length is <= gap
+                        const int64_t f = std::lower_bound(per.len.begin() + b, per.len.begin() + e, gap,
+                                                           std::greater<int32_t>()) - per.len.begin();
+                        const int64_t i = f < e ? untaken(f) : e;
+                        if (i < e && (pick < 0 || per.len[i] > per.len[pick])) pick = i;
+                    }
+                    if (pick < 0) break;
+                    const int32_t e = (int32_t)t.index.size();
+                    t.index.push_back(per.pair[pick]);
+                    t.next.push_back(-1);
+                    t.next[last[lane]] = e;
+                    last[lane] = e;
+                    load[lane] += item_columns(per.len[pick]);
+                    take(pick);
+                }
+            }
+            t.cols.back() = width;
+            t.cost.back() = task_rows * width;
             for (int k = 0; k < ns; ++k) {
-                if (head[sel[k]] < per.begin[sel[k] + 1]) {
+                if (has(sel[k])) {
                     heap.push_back(sel[k]);
                     std::push_heap(heap.begin(), heap.end(), below);
                 }
@@ -264,6 +316,30 @@ void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len
     t.begin.push_back((int32_t)t.index.size());
 }
 
+void build_entry_records(const Tasks& t, const int32_t* pair_query, const int32_t* pair_subject,
+                         const int64_t* s_off, const int32_t* s_len, std::vector<int4>& rec) {
+    rec.resize(t.index.size());
+    const int32_t n_tasks = t.count();
+    const int threads = t.index.size() >= ((size_t)1 << 20) ? host_threads() : 1;
+    parallel_for(n_tasks, threads, [&](int64_t k0, int64_t k1) {
+        for (int64_t k = k0; k < k1; ++k) {
+            const int32_t* tq = t.query.data() + (size_t)kTaskQueries * k;
+            for (int32_t e = t.begin[k]; e < t.begin[k + 1]; ++e) {
+                const int32_t p = t.index[e];
+                const int32_t sid = pair_subject[p], q = pair_query[p];
+                int slot = 0;
+                for (int j = 1; j < kTaskQueries; ++j)
+                    if (tq[j] == q) slot = j;
+                int4 r;
+                r.x = (int32_t)(uint32_t)(s_off[sid] >> 4);
+                r.y = s_len[sid];
+                r.z = t.next[e];
+                r.w = slot;
+                rec[e] = r;
+            }
+        }
+    });
+}
 
 }  // namespace tpz
 
@@ -305,10 +381,13 @@ struct tpz_sw_batch {
     int32_t gap_open = 11, gap_extend = 1, bias = 0;
     int32_t max_subject = 0;
     std::vector<int32_t> q_len, s_len, pair_query, pair_subject;
+    std::vector<int64_t> s_off;                     // packed subject offsets (16-byte aligned)
+    int64_t s_pad = 0;                              // offset of the padding after the last subject
     QueryPairs per_query;                           // pairs of each query, longest subject first
     DeviceBuffer d_q_res, d_q_off, d_q_len, d_s_res, d_s_off, d_s_len;
     DeviceBuffer d_pair_query, d_pair_subject, d_matrix;
     DeviceBuffer d_score, d_ref, d_read;            // results
+    DeviceBuffer d_key;                             // per entry: best key of the forward kernel
     DeviceBuffer d_counter, d_error;
     std::unique_ptr<SwPlan> plan;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -395,7 +474,10 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
     plan->cfg = c;
     build_packed_tasks(b->per_query, b->q_len, c.rows_per_chunk,
                        c.queries_per_task, plan->packed_tasks.host);
-    int rc = plan->packed_tasks.upload();
+    std::vector<int4> rec;
+    build_entry_records(plan->packed_tasks.host, b->pair_query.data(), b->pair_subject.data(),
+                        b->s_off.data(), b->s_len.data(), rec);
+    int rc = plan->packed_tasks.upload(rec);
     const int W = c.waves_per_group;
     {
         // plain: each query's pairs in one list; a query longer than one
@@ -427,7 +509,7 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
                                           plan->items_persistent.size() * 4);
     // workspace: per workgroup, one boundary buffer (bottom H and F of a
     // chunk: 2 words per column and lane) per wavefront
-    plan->ws_words_per_slot = (int64_t)W * 2 * (b->max_subject + 4) * 64;
+    plan->ws_words_per_slot = (int64_t)W * 2 * (item_columns(b->max_subject) + 4) * 64;
     const size_t n_items_p = plan->items_persistent.size() / W;
     plan->ws_slots = std::max<size_t>(std::min(persistent_slots, n_items_p), 1);
     rc |= plan->d_workspace.alloc(plan->ws_slots * (size_t)plan->ws_words_per_slot * 4);
@@ -484,8 +566,13 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     b->s_len.assign(s_len, s_len + n_subjects);
     b->pair_query.assign(pair_query, pair_query + n_pairs);
     b->pair_subject.assign(pair_subject, pair_subject + n_pairs);
-    for (int32_t k = 0; k < n_queries; ++k)
+    for (int32_t k = 0; k < n_queries; ++k) {
         if (q_len[k] < 0) { delete b; return fail("query %d has negative length", k); }
+        if (q_len[k] > kMaxSwLength) {
+            delete b;
+            return fail("query %d is longer than %d residues", k, kMaxSwLength);
+        }
+    }
     for (int32_t p = 0; p < n_pairs; ++p) {
         if (pair_query[p] < 0 || pair_query[p] >= n_queries ||
             pair_subject[p] < 0 || pair_subject[p] >= n_subjects) {
@@ -511,10 +598,15 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     int64_t stot = 0;
     for (int32_t k = 0; k < n_subjects; ++k) {
         if (s_len[k] < 0) { delete b; return fail("subject %d has negative length", k); }
+        if (s_len[k] > kMaxSwLength) {
+            delete b;
+            return fail("subject %d is longer than %d residues", k, kMaxSwLength);
+        }
         so[k] = stot;
         stot += (s_len[k] + 15) & ~15;
         b->max_subject = std::max(b->max_subject, s_len[k]);
     }
+    b->s_pad = stot;
     std::vector<int8_t> sr((size_t)stot + 64, 25);
     for (int32_t k = 0; k < n_subjects; ++k) {
         for (int32_t j = 0; j < s_len[k]; ++j) {
@@ -533,6 +625,8 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
     rc |= b->d_pair_query.upload(pair_query, (size_t)n_pairs * 4);
     rc |= b->d_pair_subject.upload(pair_subject, (size_t)n_pairs * 4);
     rc |= b->d_matrix.upload(matrix, 625);
+    rc |= b->d_key.alloc((size_t)n_pairs * 8 + 8);
+    b->s_off = std::move(so);
     rc |= b->d_score.alloc((size_t)n_pairs * 4 + 4);
     rc |= b->d_ref.alloc((size_t)n_pairs * 4 + 4);
     rc |= b->d_read.alloc((size_t)n_pairs * 4 + 4);
@@ -567,10 +661,12 @@ static SwKernelArgs make_args(tpz_sw_batch* b) {
     a.q_off = (const int64_t*)b->d_q_off.ptr;
     a.q_len = (const int32_t*)b->d_q_len.ptr;
     a.s_res = (const int8_t*)b->d_s_res.ptr;
+    a.s_pad = b->s_pad;
     a.s_off = (const int64_t*)b->d_s_off.ptr;
     a.s_len = (const int32_t*)b->d_s_len.ptr;
     a.pair_query = (const int32_t*)b->d_pair_query.ptr;
     a.pair_subject = (const int32_t*)b->d_pair_subject.ptr;
+    a.out_key = (unsigned long long*)b->d_key.ptr;
     a.matrix = (const int8_t*)b->d_matrix.ptr;
     a.gap_open = b->gap_open;
     a.gap_extend = b->gap_extend;
@@ -591,6 +687,8 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
     SwKernelArgs a = make_args(b);
     const DeviceTasks& dt = pl.packed_tasks;
     a.pair_index = (const int32_t*)dt.d_index.ptr;
+    a.entry_rec = (const int4*)dt.d_rec.ptr;
+    a.task_cols = (const int32_t*)dt.d_cols.ptr;
     a.task_begin = (const int32_t*)dt.d_begin.ptr;
     a.task_query = (const int32_t*)dt.d_query.ptr;
     // the timeout flag of the staircase waits is read after every run: clear
@@ -621,9 +719,13 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
             a.item_task = (const int32_t*)pl.d_items_persistent.ptr;
             a.n_items = n_items;
             a.task_counter = (int32_t*)b->d_counter.ptr;
-            if (check(tpz_launch_fill_i32(a.task_counter, 0, 1, stream()), "clear counter")) return -1;
+            if (check(tpz_launch_fill_i32(a.task_counter, 0, 1, stream()), "clear counter") ||
+                check(tpz_launch_fill_i32((int32_t*)a.out_key, 0, 2 * b->n_pairs, stream()),
+                      "clear keys"))
+                return -1;
             const int grid = (int)std::min<size_t>((size_t)n_items, pl.ws_slots);
-            if (check(tpz_launch_sw_forward(&a, grid, rows, W, stream()), "sw_forward (persistent)"))
+            if (check(tpz_launch_sw_forward(&a, grid, rows, W, stream()), "sw_forward (persistent)") ||
+                check(tpz_launch_sw_decode(&a, b->n_pairs, stream()), "sw_decode"))
                 return -1;
         }
     } else {

@@ -114,7 +114,7 @@ struct Slot {
 // Bytes a request holds on the device while in flight: its queries and pairs,
 // its share of the task and item lists, and its results.
 size_t request_bytes(size_t q_residues, size_t n_queries, size_t n_pairs) {
-    return align16(q_residues) + 12 * n_queries + 40 * n_pairs + 256;
+    return align16(q_residues) + 12 * n_queries + 72 * n_pairs + 256;
 }
 
 }  // namespace
@@ -125,6 +125,8 @@ using namespace tpz;
 struct tpz_server {
     int32_t n_subjects = 0, max_subject = 0;
     std::vector<int32_t> s_len;
+    std::vector<int64_t> so;             // packed subject offsets (16-byte aligned)
+    int64_t s_pad = 0;                   // offset of the padding after the last subject
     DeviceBuffer d_s_res, d_s_off, d_s_len, d_matrix;
     int32_t gap_open = 11, gap_extend = 1, bias = 0;
     bool lanes = false;                  // gap_open <= gap_extend: the lane kernel
@@ -184,9 +186,14 @@ int launch_slot(tpz_server* s, Slot& sl) {
                 for (int j = 0; j < kTaskQueries; ++j)
                     if (rt.query[kTaskQueries * k + j] >= 0)
                         qs[nq_k++] = (int32_t)qb + rt.query[kTaskQueries * k + j];
-                t.open(qs, nq_k, rt.cost[k]);
-                for (int32_t e = rt.begin[k]; e < rt.begin[k + 1]; ++e)
+                t.open(qs, nq_k);
+                t.cols.back() = rt.cols[k];
+                t.cost.back() = rt.cost[k];
+                const int32_t eb = (int32_t)t.index.size() - rt.begin[k];   // entry rebase
+                for (int32_t e = rt.begin[k]; e < rt.begin[k + 1]; ++e) {
                     t.index.push_back((int32_t)pb + rt.index[e]);
+                    t.next.push_back(rt.next[e] >= 0 ? rt.next[e] + eb : -1);
+                }
             }
             qb += r->q_len.size();
             pb += r->pq.size();
@@ -198,8 +205,11 @@ int launch_slot(tpz_server* s, Slot& sl) {
     const int W = s->cfg.waves_per_group;
     build_items(t, s->grid, 1.0, W, items, n_big);
     const int n_items = (int)(items.size() / W);
+    std::vector<int4> rec;
+    if (!s->lanes) build_entry_records(t, pq.data(), ps.data(), s->so.data(), s->s_len.data(), rec);
 
     // one host blob, one copy: residues | q_off | q_len | pq | ps | index | begin | tquery | items
+    // | cols | records
     const size_t o_res = 0;
     const size_t o_qoff = align16(o_res + nres + 16);
     const size_t o_qlen = align16(o_qoff + 8 * nq);
@@ -209,10 +219,15 @@ int launch_slot(tpz_server* s, Slot& sl) {
     const size_t o_begin = align16(o_index + 4 * t.index.size());
     const size_t o_tq = align16(o_begin + 4 * t.begin.size());
     const size_t o_items = align16(o_tq + 4 * t.query.size());
-    const size_t in_bytes = align16(o_items + 4 * items.size()) + 16;
-    const size_t out_words = 3 * np + 2;          // score | ref | read | error | counter
+    const size_t o_cols = align16(o_items + 4 * items.size());
+    const size_t o_rec = align16(o_cols + 4 * t.cols.size());
+    const size_t in_bytes = align16(o_rec + sizeof(int4) * rec.size()) + 16;
+    // score | ref | read | error | counter | (to 8 bytes) keys
+    const size_t o_key_words = (3 * np + 2 + 1) & ~(size_t)1;
+    const size_t out_words = 3 * np + 2;
+    const size_t dev_out_words = o_key_words + 2 * np;
     if (sl.h_in.reserve(in_bytes) || sl.d_in.reserve(in_bytes) ||
-        sl.h_out.reserve(out_words * 4) || sl.d_out.reserve(out_words * 4))
+        sl.h_out.reserve(out_words * 4) || sl.d_out.reserve(dev_out_words * 4))
         return -1;
     char* h = sl.h_in.ptr;
     size_t rb = 0;
@@ -228,11 +243,14 @@ int launch_slot(tpz_server* s, Slot& sl) {
     memcpy(h + o_begin, t.begin.data(), 4 * t.begin.size());
     memcpy(h + o_tq, t.query.data(), 4 * t.query.size());
     memcpy(h + o_items, items.data(), 4 * items.size());
+    memcpy(h + o_cols, t.cols.data(), 4 * t.cols.size());
+    memcpy(h + o_rec, rec.data(), sizeof(int4) * rec.size());
     char* d = (char*)sl.d_in.ptr;
     int32_t* dout = (int32_t*)sl.d_out.ptr;
     sl.error_word = (int)(3 * np);
     if (check(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, sl.stream), "upload") ||
-        check(tpz_launch_fill_i32(dout + 3 * np, 0, 2, sl.stream), "clear flags"))
+        check(tpz_launch_fill_i32(dout + 3 * np, 0, 2, sl.stream), "clear flags") ||
+        check(tpz_launch_fill_i32(dout + o_key_words, 0, (int32_t)(2 * np), sl.stream), "clear keys"))
         return -1;
     if (n_items > 0) {
         SwKernelArgs a{};
@@ -240,11 +258,15 @@ int launch_slot(tpz_server* s, Slot& sl) {
         a.q_off = (const int64_t*)(d + o_qoff);
         a.q_len = (const int32_t*)(d + o_qlen);
         a.s_res = (const int8_t*)s->d_s_res.ptr;
+        a.s_pad = s->s_pad;
         a.s_off = (const int64_t*)s->d_s_off.ptr;
         a.s_len = (const int32_t*)s->d_s_len.ptr;
         a.pair_query = (const int32_t*)(d + o_pq);
         a.pair_subject = (const int32_t*)(d + o_ps);
         a.pair_index = (const int32_t*)(d + o_index);
+        a.entry_rec = (const int4*)(d + o_rec);
+        a.task_cols = (const int32_t*)(d + o_cols);
+        a.out_key = (unsigned long long*)(dout + o_key_words);
         a.task_begin = (const int32_t*)(d + o_begin);
         a.task_query = (const int32_t*)(d + o_tq);
         a.item_task = (const int32_t*)(d + o_items);
@@ -273,7 +295,9 @@ int launch_slot(tpz_server* s, Slot& sl) {
             if (check(tpz_launch_sw_lane(&la, blocks, sl.stream), "sw_lane")) return -1;
         } else {
             const int grid = std::min(n_items, s->grid);
-            if (check(tpz_launch_sw_forward(&a, grid, 32, W, sl.stream), "sw_forward")) return -1;
+            if (check(tpz_launch_sw_forward(&a, grid, 32, W, sl.stream), "sw_forward") ||
+                check(tpz_launch_sw_decode(&a, (int32_t)np, sl.stream), "sw_decode"))
+                return -1;
         }
     }
     if (check(hipMemcpyAsync(sl.h_out.ptr, dout, out_words * 4, hipMemcpyDeviceToHost, sl.stream),
@@ -414,6 +438,7 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
     int64_t stot = 0;
     for (int32_t k = 0; k < n_subjects; ++k) {
         if (s_len[k] < 0) return fail("subject %d has negative length", k);
+        if (s_len[k] > kMaxSwLength) return fail("subject %d is longer than %d residues", k, kMaxSwLength);
         so[k] = stot;
         stot += (s_len[k] + 15) & ~15;
         s->max_subject = std::max(s->max_subject, s_len[k]);
@@ -443,13 +468,14 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
             }
         });
     if (!rc && bad.load() >= 0) return fail("subject %lld has a residue out of range", (long long)bad.load());
+    s->s_pad = stot;
     rc |= s->d_s_off.upload(so.data(), so.size() * 8) |
           s->d_s_len.upload(s_len, (size_t)n_subjects * 4) | s->d_matrix.upload(matrix, 625);
     s->grid = std::max(num_cu(), 1) *
               std::max(1, std::min(kGroupsPerCu, tpz_sw_forward_groups_per_cu(32, s->cfg.waves_per_group)));
     // the forward kernel's workspace (the lane kernel's grows with the
     // launch's longest query, in the slot)
-    s->ws_words_per_slot = s->lanes ? 0 : (int64_t)s->cfg.waves_per_group * 2 * (s->max_subject + 4) * 64;
+    s->ws_words_per_slot = s->lanes ? 0 : (int64_t)s->cfg.waves_per_group * 2 * (item_columns(s->max_subject) + 4) * 64;
     for (int k = 0; k < s->cfg.max_inflight && !rc; ++k) {
         auto sl = std::make_unique<Slot>();
         rc |= check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -459,6 +485,7 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
         s->slots.push_back(std::move(sl));
     }
     rc |= check(hipStreamSynchronize(stream()), "upload sync");
+    s->so = std::move(so);
     if (rc) {
         for (auto& sl : s->slots) {
             if (sl->ev_done) (void)hipEventDestroy(sl->ev_done);
@@ -481,6 +508,7 @@ int tpz_server_submit(tpz_server* s, int32_t n_queries, const int8_t* q_res,
     size_t tot = 0;
     for (int32_t k = 0; k < n_queries; ++k) {
         if (q_len[k] < 0) return fail("query %d has negative length", k);
+        if (q_len[k] > kMaxSwLength) return fail("query %d is longer than %d residues", k, kMaxSwLength);
         tot += (size_t)q_len[k];
     }
     r->q_res.resize(tot);

@@ -21,9 +21,12 @@
 // gives the byte result when the score is below 255 - bias, the word result
 // below 32767, and the reference's NULL at 32767.
 //
-// Mapping (inter-task): one wavefront = up to 128 pairs of up to four
-// queries; lane l holds two pairs, one in each 16-bit half of every VGPR, so
-// each VOP3P instruction advances two DP cells.  Query rows are processed in
+// Mapping (inter-task): one wavefront = a task of pairs of up to four
+// queries; lane l holds two runs of pairs, one in each 16-bit half of every
+// VGPR, so each VOP3P instruction advances two DP cells.  A run is one pair
+// or several back to back along the columns: a long pair sets the task's
+// width and the other halves fill it with more pairs, instead of walking
+// padding columns past a short subject's end.  Query rows are processed in
 // chunks of 32 consecutive rows held in registers (H, E) while the lane walks
 // the subject; a chunk's bottom row (H, F) is written per column to a
 // per-wave workspace that the next chunk reads back in place.  The profile
@@ -36,10 +39,12 @@
 // chunk already counted in the running maximum, so it is never a maximum.
 //
 // End positions: within a chunk, a column whose max rises strictly above the
-// chunk's best so far is an "event"; on an event the lane's 32 H values of
-// that half are saved to LDS.  At the chunk end the saved column gives the
-// lowest row; chunks combine by (larger score, then smaller column, then the
-// earlier chunk), which is the reference's first-column / lowest-row rule.
+// best so far of the half's pair is an "event"; on an event the lane's 32 H
+// values of that half are saved to LDS.  Where the pair ends (or the chunk),
+// the saved column gives the lowest row, and (score, column, row) goes into
+// the pair's 64-bit key with an atomic max; keys order by larger score, then
+// smaller column, then smaller row, which is the reference's first-column /
+// lowest-row rule, whichever chunk or wavefront finishes first.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -108,14 +113,42 @@ __device__ __forceinline__ s2 row_w(const ProfCol<R>& lo, const ProfCol<R>& hi, 
                                            (r & 1) ? 0x07060302u : 0x05040100u));
 }
 
-// Best of the chunks finished so far, per half of the lane; chunk is the
-// chunk index of that best (ties on score and column go to the earlier one).
-struct LaneBest {
-    int score[2];
-    int ref[2];
-    int read[2];
-    int chunk[2];
+// Lane runs.  A lane half works a run of pairs, one after another along the
+// columns: each pair takes item_cols(n) columns (its subject, then padding
+// residues), and the next pair of the run starts where they end.  Entry
+// records (SwKernelArgs::entry_rec) chain a run: subject offset / 16,
+// subject length, the next entry of the run (-1 at its end) and the query
+// slot.  The boundaries are the same in every chunk of rows.
+__device__ __forceinline__ int item_cols(int n) {
+    const int c = (n + 3) & ~3;
+    return c < 8 ? 8 : c;
+}
+constexpr int kNoBoundary = 0x7fffffff;
+
+// The run of one lane half, as a chunk walks its columns.  The residue
+// stream switches to the next pair one step of four columns before the pair
+// starts (its residues are loaded a step ahead); the rest switches when it
+// starts.  nrec is the record of entry en, loaded when the pair before it
+// starts, so it has landed long before the boundary needs it (a pair takes
+// at least 8 columns).
+struct LaneRun {
+    int e;          // entry of the pair being computed (-1: none)
+    int c0;         // its first column
+    int nb;         // column where the next pair starts (kNoBoundary: none)
+    int off;        // profile offset of its query slot
+    int en;         // entry of the next pair (-1: none)
+    int64_t lb;     // residue stream: column c reads s_res[lb + c] while c < le
+    int le;
+    int4 nrec;
 };
+
+// Best (score, first column, lowest row) of a pair as one 64-bit key whose
+// maximum is the reference's choice: the larger score, then the smaller
+// column, then the smaller row (ssw.c:264-297).  Columns and rows below 2^24.
+__device__ __forceinline__ unsigned long long best_key(int score, int col, int row) {
+    return ((unsigned long long)score << 48) | ((unsigned long long)(0xffffff - col) << 24) |
+           (unsigned long long)(0xffffff - row);
+}
 
 // Wavefronts per workgroup (W, 4 or 8) is a template parameter of the kernel.
 constexpr int kSyncCols = 8;       // progress is published every 8 columns
@@ -222,41 +255,57 @@ __device__ __forceinline__ TopPair load_top(const uint32_t* tH, const uint32_t* 
     return p;
 }
 
-// Residues c .. c + 3 of a subject (c a multiple of 4), residue 25 past its
-// end.  The load itself is unconditional: its address is clamped to the
-// subject's last dword (subjects start on 4-byte boundaries, are padded with
-// residue 25 to the next one, and the residue array has at least 4 bytes
-// after the last subject), so no exec-masked load feeds a phi.
-__device__ __forceinline__ uint32_t load_res4(const int8_t* s, int c, int n) {
-    const int last = n > 0 ? ((n - 1) & ~3) : 0;
-    const uint32_t w = *reinterpret_cast<const uint32_t*>(s + (c < last ? c : last));
-    return c < n ? w : 0x19191919u;
+// Residues of columns c .. c + 3 (c a multiple of 4) of a lane's run: the
+// stream's subject while c < le, padding after it (s_res + s_pad holds 16
+// residues 25).  Subjects start on 16-byte boundaries and are padded with
+// residue 25 to the next one.  The load is unconditional and stays in the
+// global address space, and no select follows it, so the compiler keeps it
+// where it is issued (a load under an exec mask feeding a phi, or a flat
+// load, makes every iteration wait for all memory).
+__device__ __forceinline__ uint32_t load_res4(const SwKernelArgs& a, int64_t lb, int c, int le) {
+    return *reinterpret_cast<const uint32_t*>(a.s_res + (c < le ? lb + c : a.s_pad));
 }
 
-// One chunk of R rows [j0, j0 + R) over all subject columns, two
-// columns at a time: the second column runs one row behind the first, so
-// every step advances two independent dependency chains.  Returns false if
-// a wait on the producer of the top boundary gave up.
+// The start of a run at column 0 of a chunk: its first pair's record.
+__device__ __forceinline__ void run_start(const SwKernelArgs& a, LaneRun& L, int e0) {
+    L.e = e0;
+    L.c0 = 0;
+    const int4 r = a.entry_rec[e0 >= 0 ? e0 : 0];
+    if (e0 >= 0) {
+        L.off = (r.w & 3) * kNRes;
+        L.en = r.z;
+        L.nb = r.z >= 0 ? item_cols(r.y) : kNoBoundary;
+        L.lb = (int64_t)(uint32_t)r.x * 16;
+        L.le = r.y;
+    } else {
+        L.off = 0; L.en = -1; L.nb = kNoBoundary; L.lb = 0; L.le = 0;
+    }
+    L.nrec = a.entry_rec[L.en >= 0 ? L.en : (e0 >= 0 ? e0 : 0)];
+}
+
+// One chunk of R rows [j0, j0 + R) over all columns of a task, two columns
+// at a time: the second column runs one row behind the first, so every step
+// advances two independent dependency chains.  Returns false if a wait on
+// the producer of the top boundary gave up.
 //
 // The loop body covers four columns (two column pairs).  A column pair
 // reads its profile columns in quarters of 8 rows, each loaded one quarter
-// ahead of its rows: quarter 0 during the previous pair (as soon as that
-// pair's rows 0-7 are done, into the same registers), quarter 1 when the
-// pair starts, quarter k + 2 when the pair's rows of quarter k are done.  So
-// no column pair starts by waiting for LDS, and at most three quarters of
-// profile columns are live at a time.  The next pair's residues come from
-// the residue dword of the current four columns, or of the next four,
-// loaded at the top of the iteration.
+// ahead of its rows: quarter 0 during the previous column pair (as soon as
+// that pair's rows 0-7 are done, into the same registers; with the next
+// run pair's query slot where a run's pair ends between them), quarter 1
+// when the column pair starts, quarter k + 2 when its rows of quarter k are
+// done.  Where a pair of a lane half's run ends (a multiple of four
+// columns), the chunk's best of that pair goes to its key and the half's H,
+// E and diagonal restart at zero.
 //
-// Each iteration reloads the prefetched top boundary and the residue dword
-// right after their last use, into the same variables: no loaded value is
-// copied at the back edge before its load was waited for, so no iteration
-// waits for the loads or stores it has just issued.
+// Each iteration reloads the prefetched top boundary, the residue dword and
+// the next pairs' records right after their last use, into the same
+// variables: no loaded value is copied at the back edge before its load was
+// waited for, so no iteration waits for the loads or stores it has just
+// issued.
 template <int R>
 __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds, const ChunkIO& io,
-                                          int ncols, const int8_t* s_lo, int n_lo,
-                                          const int8_t* s_hi, int n_hi, int off_lo, int off_hi,
-                                          int j0, int chunk, LaneBest& best) {
+                                          int ncols, int e0_lo, int e0_hi, int j0) {
     constexpr int kS = WaveLds<R>::kStride;
     constexpr int kQuarters = R / 8;
     const int lane = threadIdx.x % 64;
@@ -266,8 +315,8 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
     s2 E[R], H[R], Ha[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) { E[r] = zero; H[r] = zero; }
-    s2 bc = zero;                 // chunk best per half
-    int ci_lo = -1, ci_hi = -1;
+    s2 bc = zero;                 // best of the current pair in this chunk, per half
+    int ci_lo = -1, ci_hi = -1;   // the column where it was first reached
     s2 hdiag = zero;              // H(i-1, j0-1)
 
     // all ones when the chunk has a top boundary, zero for the first chunk
@@ -281,39 +330,63 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
     // but land in one cached line instead of going to memory
     const int bstep = __builtin_amdgcn_readfirstlane(io.has_bottom ? 1 : 0);
 
+    LaneRun Llo, Lhi;
+    run_start(a, Llo, e0_lo);
+    run_start(a, Lhi, e0_hi);
     int waited = ncols < kAhead ? ncols : kAhead;       // columns known final
     if (io.wait_ctr && !wait_for(io, io.wait_base + waited)) return false;
-    uint32_t res_lo = load_res4(s_lo, 0, n_lo), res_hi = load_res4(s_hi, 0, n_hi);
+    uint32_t res_lo = load_res4(a, Llo.lb, 0, Llo.le);
+    uint32_t res_hi = load_res4(a, Lhi.lb, 0, Lhi.le);
     TopPair T = load_top(tH, tF, 0, ncols, keep, lane);
     // Let these first loads land here: a load still pending at the loop
     // entry makes the compiler wait for all vector memory at the top of every
     // iteration.
     __builtin_amdgcn_s_waitcnt(kWaitVmem);
 
-    // profile columns of the current column pair (a = even, b = odd column),
-    // starting with quarter 0 of columns 0 and 1
-    ProfCol<R> pa_lo, pa_hi, pb_lo, pb_hi;
-    load_quarter<R>(pa_lo, lds.prof + (off_lo + (res_lo & 0xff)) * kS, 0);
-    load_quarter<R>(pa_hi, lds.prof + (off_hi + (res_hi & 0xff)) * kS, 0);
-    load_quarter<R>(pb_lo, lds.prof + (off_lo + ((res_lo >> 8) & 0xff)) * kS, 0);
-    load_quarter<R>(pb_hi, lds.prof + (off_hi + ((res_hi >> 8) & 0xff)) * kS, 0);
-    uint32_t cur_lo = res_lo, cur_hi = res_hi;   // residues of the current pair
+    // The chunk's best of a half's current pair into the pair's key: the
+    // lowest row of the saved column holding it.
+    auto finalize = [&](int half, const LaneRun& L, int b, int ci) __attribute__((always_inline)) {
+        if (L.e < 0 || b <= 0) return;
+        int row = R;
+#pragma unroll
+        for (int r = R - 1; r >= 0; --r)
+            if ((int)(int16_t)lds.saved[half][r][lane] == b) row = r;
+        atomicMax(a.out_key + L.e, best_key(b, ci - L.c0, j0 + row));
+    };
+    // The pair after the current one starts at column i; the record of the
+    // one after it is loaded now, read a step or more later.
+    auto advance = [&](LaneRun& L, int i) __attribute__((always_inline)) {
+        const int4 r = L.nrec;
+        L.e = L.en;
+        L.c0 = i;
+        L.off = (r.w & 3) * kNRes;
+        L.en = r.z;
+        L.nb = r.z >= 0 ? i + item_cols(r.y) : kNoBoundary;
+        if (r.z >= 0) L.nrec = a.entry_rec[r.z];
+    };
 
-    // Columns i and i + 1 from the prefetched boundary T, which is then
-    // reloaded with columns i + 2 and i + 3; nx_lo / nx_hi hold the residues
-    // of columns i + 2 (bits 0-7) and i + 3 (bits 8-15).
-    auto column_pair = [&](int i, uint32_t nx_lo, uint32_t nx_hi,
-                           TopPair& T) __attribute__((always_inline)) {
-        const int16_t* ca_lo = lds.prof + (off_lo + (cur_lo & 0xff)) * kS;
-        const int16_t* ca_hi = lds.prof + (off_hi + (cur_hi & 0xff)) * kS;
-        const int16_t* cb_lo = lds.prof + (off_lo + ((cur_lo >> 8) & 0xff)) * kS;
-        const int16_t* cb_hi = lds.prof + (off_hi + ((cur_hi >> 8) & 0xff)) * kS;
-        const int16_t* na_lo = lds.prof + (off_lo + (nx_lo & 0xff)) * kS;
-        const int16_t* na_hi = lds.prof + (off_hi + (nx_hi & 0xff)) * kS;
-        const int16_t* nb_lo = lds.prof + (off_lo + ((nx_lo >> 8) & 0xff)) * kS;
-        const int16_t* nb_hi = lds.prof + (off_hi + ((nx_hi >> 8) & 0xff)) * kS;
-        cur_lo = nx_lo;
-        cur_hi = nx_hi;
+    // profile columns of the current column pair (a = even, b = odd
+    // column), starting with quarter 0 of columns 0 and 1
+    ProfCol<R> pa_lo, pa_hi, pb_lo, pb_hi;
+    load_quarter<R>(pa_lo, lds.prof + (Llo.off + (res_lo & 0xff)) * kS, 0);
+    load_quarter<R>(pa_hi, lds.prof + (Lhi.off + (res_hi & 0xff)) * kS, 0);
+    load_quarter<R>(pb_lo, lds.prof + (Llo.off + ((res_lo >> 8) & 0xff)) * kS, 0);
+    load_quarter<R>(pb_hi, lds.prof + (Lhi.off + ((res_hi >> 8) & 0xff)) * kS, 0);
+
+    // Columns i and i + 1 (residues in the low 16 bits of cur_lo / cur_hi)
+    // from the prefetched boundary T, which is then reloaded with columns
+    // i + 2 and i + 3; nx_lo / nx_hi hold the residues of columns i + 2 and
+    // i + 3, and noff_lo / noff_hi their profile offsets.
+    auto column_pair = [&](int i, uint32_t cur_lo, uint32_t cur_hi, uint32_t nx_lo, uint32_t nx_hi,
+                           int noff_lo, int noff_hi, TopPair& T) __attribute__((always_inline)) {
+        const int16_t* ca_lo = lds.prof + (Llo.off + (cur_lo & 0xff)) * kS;
+        const int16_t* ca_hi = lds.prof + (Lhi.off + (cur_hi & 0xff)) * kS;
+        const int16_t* cb_lo = lds.prof + (Llo.off + ((cur_lo >> 8) & 0xff)) * kS;
+        const int16_t* cb_hi = lds.prof + (Lhi.off + ((cur_hi >> 8) & 0xff)) * kS;
+        const int16_t* na_lo = lds.prof + (noff_lo + (nx_lo & 0xff)) * kS;
+        const int16_t* na_hi = lds.prof + (noff_hi + (nx_hi & 0xff)) * kS;
+        const int16_t* nb_lo = lds.prof + (noff_lo + ((nx_lo >> 8) & 0xff)) * kS;
+        const int16_t* nb_hi = lds.prof + (noff_hi + ((nx_hi >> 8) & 0xff)) * kS;
         if (kQuarters > 1) {
             load_quarter<R>(pa_lo, ca_lo, 1);
             load_quarter<R>(pa_hi, ca_hi, 1);
@@ -364,7 +437,7 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
                 // rows of quarter k = r / 8 - 1 done in both columns
                 const int k = r / 8 - 1;
                 if (k == 0) {
-                    // quarter 0 of the next pair
+                    // quarter 0 of the next column pair
                     load_quarter<R>(pa_lo, na_lo, 0);
                     load_quarter<R>(pa_hi, na_hi, 0);
                     load_quarter<R>(pb_lo, nb_lo, 0);
@@ -402,8 +475,8 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
             publish(io, io.pub_base + i + 2);
     };
 
-    // ncols is a multiple of 4 (the host pads subjects with residue 25
-    // columns); res_lo / res_hi hold the residues of columns i .. i + 3
+    // ncols is a multiple of 4, as are the pairs' boundaries; res_lo / res_hi
+    // hold the residues of columns i .. i + 3
     for (int i = 0; i < ncols; i += 4) {
         if (io.wait_ctr && (i % kSyncCols) == 0 && waited < ncols) {
             // columns up to i + kAhead must be final before they are prefetched
@@ -413,49 +486,59 @@ __device__ __forceinline__ bool run_chunk(const SwKernelArgs& a, WaveLds<R>& lds
                 waited = need;
             }
         }
-        const uint32_t rn_lo = load_res4(s_lo, i + 4, n_lo);
-        const uint32_t rn_hi = load_res4(s_hi, i + 4, n_hi);
-        column_pair(i, res_lo >> 16, res_hi >> 16, T);
-        column_pair(i + 2, rn_lo, rn_hi, T);
+        // a pair starts at column i in some lane half: the previous one's
+        // best to its key, and the half restarts at zero
+        {
+            const bool st_lo = i == Llo.nb, st_hi = i == Lhi.nb;
+            if (__builtin_amdgcn_ballot_w64(st_lo || st_hi)) {
+                if (st_lo) finalize(0, Llo, bc.x, ci_lo);
+                if (st_hi) finalize(1, Lhi, bc.y, ci_hi);
+                const uint32_t m = (st_lo ? 0xffff0000u : 0xffffffffu) & (st_hi ? 0x0000ffffu : 0xffffffffu);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    E[r] = from_bits(bits(E[r]) & m);
+                    H[r] = from_bits(bits(H[r]) & m);
+                }
+                hdiag = from_bits(bits(hdiag) & m);
+                bc = from_bits(bits(bc) & m);
+                if (st_lo) { advance(Llo, i); ci_lo = -1; }
+                if (st_hi) { advance(Lhi, i); ci_hi = -1; }
+            }
+        }
+        // the residue stream moves to the next pair a step before it starts
+        if (i + 4 == Llo.nb) {
+            Llo.lb = (int64_t)(uint32_t)Llo.nrec.x * 16 - (i + 4);
+            Llo.le = i + 4 + Llo.nrec.y;
+        }
+        if (i + 4 == Lhi.nb) {
+            Lhi.lb = (int64_t)(uint32_t)Lhi.nrec.x * 16 - (i + 4);
+            Lhi.le = i + 4 + Lhi.nrec.y;
+        }
+        const uint32_t rn_lo = load_res4(a, Llo.lb, i + 4, Llo.le);
+        const uint32_t rn_hi = load_res4(a, Lhi.lb, i + 4, Lhi.le);
+        // the column pair after this step's second one starts a run's next
+        // pair where that pair begins at column i + 4
+        const int noff_lo = i + 4 == Llo.nb ? (Llo.nrec.w & 3) * kNRes : Llo.off;
+        const int noff_hi = i + 4 == Lhi.nb ? (Lhi.nrec.w & 3) * kNRes : Lhi.off;
+        column_pair(i, res_lo, res_hi, res_lo >> 16, res_hi >> 16, Llo.off, Lhi.off, T);
+        column_pair(i + 2, res_lo >> 16, res_hi >> 16, rn_lo, rn_hi, noff_lo, noff_hi, T);
         res_lo = rn_lo;
         res_hi = rn_hi;
     }
-
-    // Lowest row of the saved column equal to the chunk best, then merge:
-    // larger score wins; on an equal score the smaller column; on an equal
-    // column the earlier chunk (already held).
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-        const int b = half ? bc.y : bc.x;
-        const int ci = half ? ci_hi : ci_lo;
-        if (b <= 0) continue;
-        int row = R;
-#pragma unroll
-        for (int r = R - 1; r >= 0; --r)
-            if ((int)(int16_t)lds.saved[half][r][lane] == b) row = r;
-        if (b > best.score[half] || (b == best.score[half] && ci < best.ref[half])) {
-            best.score[half] = b;
-            best.ref[half] = ci;
-            best.read[half] = j0 + row;
-            best.chunk[half] = chunk;
-        }
-    }
+    // the pairs still open at the chunk's last column
+    finalize(0, Llo, bc.x, ci_lo);
+    finalize(1, Lhi, bc.y, ci_hi);
     return true;
 }
 
-// The pairs of one task as seen by one lane.
+// A task as seen by one lane: the first entries of its two runs and the
+// task's queries.
 struct TaskView {
-    int pair_lo, pair_hi;
-    bool has_lo, has_hi;
+    int e_lo, e_hi;          // entries starting the lane's runs (-1: none)
     const int8_t* qs[kQ];    // the task's queries (wave-uniform); null past the last
     int ms[kQ];
     int m_max;
-    int off_lo, off_hi;      // profile offset of each half's query
-    int m_lo, m_hi;
-    const int8_t* s_lo;
-    const int8_t* s_hi;
-    int n_lo, n_hi;
-    int ncols;               // even, wave-uniform
+    int ncols;               // columns of the task, a multiple of 4, wave-uniform
 };
 
 // Substitution scores of rows j0 .. j0+R-1 of every query slot of the task
@@ -495,69 +578,21 @@ __device__ __forceinline__ TaskView task_view(const SwKernelArgs& a, int task) {
     task = __builtin_amdgcn_readfirstlane(task);
     TaskView t;
     const int p0 = a.task_begin[task], p1 = a.task_begin[task + 1];
-    const int np = p1 - p0;
-    t.has_lo = lane < np;
-    t.has_hi = lane + 64 < np;
-    const int k_lo = p0 + (t.has_lo ? lane : 0);
-    const int k_hi = p0 + (t.has_hi ? lane + 64 : 0);
-    t.pair_lo = a.pair_index ? a.pair_index[k_lo] : k_lo;
-    t.pair_hi = a.pair_index ? a.pair_index[k_hi] : k_hi;
-    int tq[kQ];
+    // the first 128 entries of a task start its runs (entry p0 + k: lane
+    // k % 64, half k / 64)
+    const int np = p1 - p0 < 128 ? p1 - p0 : 128;
+    t.e_lo = lane < np ? p0 + lane : -1;
+    t.e_hi = lane + 64 < np ? p0 + 64 + lane : -1;
     t.m_max = 0;
 #pragma unroll
     for (int k = 0; k < kQ; ++k) {
-        tq[k] = a.task_query[kQ * task + k];
-        t.qs[k] = tq[k] >= 0 ? a.q_res + a.q_off[tq[k]] : nullptr;
-        t.ms[k] = tq[k] >= 0 ? a.q_len[tq[k]] : 0;
+        const int q = a.task_query[kQ * task + k];
+        t.qs[k] = q >= 0 ? a.q_res + a.q_off[q] : nullptr;
+        t.ms[k] = q >= 0 ? a.q_len[q] : 0;
         t.m_max = t.ms[k] > t.m_max ? t.ms[k] : t.m_max;
     }
-    const int qid_lo = a.pair_query[t.pair_lo], qid_hi = a.pair_query[t.pair_hi];
-    int slot_lo = 0, slot_hi = 0;
-#pragma unroll
-    for (int k = 1; k < kQ; ++k) {
-        if (qid_lo == tq[k]) slot_lo = k;
-        if (qid_hi == tq[k]) slot_hi = k;
-    }
-    t.off_lo = slot_lo * kNRes;
-    t.off_hi = slot_hi * kNRes;
-    t.m_lo = t.has_lo ? a.q_len[qid_lo] : 0;
-    t.m_hi = t.has_hi ? a.q_len[qid_hi] : 0;
-    const int sid_lo = a.pair_subject[t.pair_lo], sid_hi = a.pair_subject[t.pair_hi];
-    t.s_lo = a.s_res + a.s_off[sid_lo];
-    t.s_hi = a.s_res + a.s_off[sid_hi];
-    t.n_lo = t.has_lo ? a.s_len[sid_lo] : 0;
-    t.n_hi = t.has_hi ? a.s_len[sid_hi] : 0;
-    int ncols = t.n_lo > t.n_hi ? t.n_lo : t.n_hi;
-    for (int off = 32; off >= 1; off >>= 1) {
-        const int o = __shfl_xor(ncols, off);
-        ncols = o > ncols ? o : ncols;
-    }
-    ncols = __builtin_amdgcn_readfirstlane(ncols);
-    t.ncols = (ncols + 3) & ~3;        // columns go in fours; pad columns score kPadScore
+    t.ncols = __builtin_amdgcn_readfirstlane(a.task_cols[task]);
     return t;
-}
-
-__device__ __forceinline__ void init_best(LaneBest& best, const TaskView& t) {
-    // score 0: ref_end1 -1 and read_end1 0, or -1 for an empty query
-    // (ssw.c:148-149, 288-297)
-    for (int h = 0; h < 2; ++h) {
-        best.score[h] = 0; best.ref[h] = -1; best.chunk[h] = 0;
-        best.read[h] = (h ? t.m_hi : t.m_lo) > 0 ? 0 : -1;
-    }
-}
-
-__device__ __forceinline__ void write_best(const SwKernelArgs& a, const TaskView& t,
-                                           const LaneBest& best) {
-    if (t.has_lo) {
-        a.out_score[t.pair_lo] = best.score[0] >= 32767 ? 32767 : best.score[0];
-        a.out_ref[t.pair_lo] = best.ref[0];
-        a.out_read[t.pair_lo] = best.read[0];
-    }
-    if (t.has_hi) {
-        a.out_score[t.pair_hi] = best.score[1] >= 32767 ? 32767 : best.score[1];
-        a.out_ref[t.pair_hi] = best.ref[1];
-        a.out_read[t.pair_hi] = best.read[1];
-    }
 }
 
 // LDS of one workgroup of W wavefronts.
@@ -568,7 +603,6 @@ struct GroupLds {
     int item;
     int error;
     int progress[W];
-    int bscore[W][2][64], bref[W][2][64], bread[W][2][64], bchunk[W][2][64];
 };
 
 // A task worked by one wavefront: its chunks in order, boundary in place.
@@ -576,10 +610,7 @@ template <int R, int W>
 __device__ void run_task_wave(const SwKernelArgs& a, WaveLds<R>& lds, uint32_t* buf, int task,
                               GroupLds<R, W>& g) {
     const TaskView t = task_view(a, task);
-    LaneBest best;
-    init_best(best, t);
-    int chunk = 0;
-    for (int j0 = 0; j0 < t.m_max; j0 += R, ++chunk) {
+    for (int j0 = 0; j0 < t.m_max; j0 += R) {
         build_profile(g.mat, lds, t, j0);
         ChunkIO io{};
         io.top = buf;              // in place: column c is read before it is written
@@ -588,22 +619,19 @@ __device__ void run_task_wave(const SwKernelArgs& a, WaveLds<R>& lds, uint32_t* 
         io.has_bottom = j0 + R < t.m_max;
         io.error = &g.error;
         io.g_error = a.error;
-        run_chunk(a, lds, io, t.ncols, t.s_lo, t.n_lo, t.s_hi, t.n_hi, t.off_lo, t.off_hi, j0,
-                  chunk, best);
+        run_chunk(a, lds, io, t.ncols, t.e_lo, t.e_hi, j0);
     }
-    write_best(a, t, best);
 }
 
 // A task worked by all W wavefronts of the workgroup: chunk c by wavefront
 // c % W, each one trailing the wavefront of the chunk above by at least
-// kAhead columns; chunk c's bottom boundary lives in buffer c % W.
+// kAhead columns; chunk c's bottom boundary lives in buffer c % W.  Every
+// chunk merges its bests into the pairs' keys itself.
 template <int R, int W>
 __device__ void run_task_group(const SwKernelArgs& a, GroupLds<R, W>& g, uint32_t* bufs,
                                size_t buf_words, int task) {
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int wave = threadIdx.x / 64;
     const TaskView t = task_view(a, task);
-    LaneBest best;
-    init_best(best, t);
     const int n_chunks = (t.m_max + R - 1) / R;
     const int nc = t.ncols;
     for (int c = wave; c < n_chunks; c += W) {
@@ -624,38 +652,8 @@ __device__ void run_task_group(const SwKernelArgs& a, GroupLds<R, W>& g, uint32_
         io.pub_base = (c / W) * nc;
         io.error = &g.error;
         io.g_error = a.error;
-        if (!run_chunk(a, g.wave[wave], io, nc, t.s_lo, t.n_lo, t.s_hi, t.n_hi, t.off_lo,
-                       t.off_hi, j0, c, best))
-            break;
+        if (!run_chunk(a, g.wave[wave], io, nc, t.e_lo, t.e_hi, j0)) break;
         if (nc == 0) publish(io, io.pub_base);
-    }
-    // merge the wavefronts' bests: larger score, then smaller column, then
-    // the earlier chunk
-    for (int h = 0; h < 2; ++h) {
-        g.bscore[wave][h][lane] = best.score[h];
-        g.bref[wave][h][lane] = best.ref[h];
-        g.bread[wave][h][lane] = best.read[h];
-        g.bchunk[wave][h][lane] = best.chunk[h];
-    }
-    __syncthreads();
-    if (wave == 0) {
-        for (int h = 0; h < 2; ++h) {
-            for (int w = 1; w < W; ++w) {
-                const int sc = g.bscore[w][h][lane], rf = g.bref[w][h][lane];
-                const int ck = g.bchunk[w][h][lane];
-                if (sc <= 0) continue;
-                if (sc > best.score[h] ||
-                    (sc == best.score[h] &&
-                     (rf < best.ref[h] || (rf == best.ref[h] && ck < best.chunk[h])))) {
-                    best.score[h] = sc;
-                    best.ref[h] = rf;
-                    best.read[h] = g.bread[w][h][lane];
-                    best.chunk[h] = ck;
-                }
-            }
-        }
-        if (!__hip_atomic_load(&g.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-            write_best(a, t, best);
     }
 }
 
@@ -693,6 +691,25 @@ sw_forward_kernel(SwKernelArgs a) {
     }
 }
 
+// Results of every entry's pair from its key: score, ref_end1 and
+// read_end1; key 0 (no cell above zero) is score 0, ref_end1 -1 and
+// read_end1 0, or -1 for an empty query (ssw.c:148-149, 288-297).
+__global__ void sw_decode_kernel(SwKernelArgs a, int32_t n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int p = a.pair_index ? a.pair_index[e] : e;
+    const unsigned long long k = a.out_key[e];
+    if (k == 0) {
+        a.out_score[p] = 0;
+        a.out_ref[p] = -1;
+        a.out_read[p] = a.q_len[a.pair_query[p]] > 0 ? 0 : -1;
+    } else {
+        a.out_score[p] = (int)(k >> 48);
+        a.out_ref[p] = 0xffffff - (int)((k >> 24) & 0xffffff);
+        a.out_read[p] = 0xffffff - (int)(k & 0xffffff);
+    }
+}
+
 // Sets n words to v (graph-capturable, unlike a memset node).
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int32_t n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -721,6 +738,14 @@ hipError_t launch(const tpz::SwKernelArgs* args, int grid, hipStream_t stream) {
     return hipGetLastError();
 }
 }  // namespace
+
+extern "C" hipError_t tpz_launch_sw_decode(const tpz::SwKernelArgs* args, int32_t n_entries,
+                                           hipStream_t stream) {
+    if (n_entries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tpz::sw_decode_kernel, dim3((n_entries + 255) / 256), dim3(256), 0, stream,
+                       *args, n_entries);
+    return hipGetLastError();
+}
 
 extern "C" int tpz_sw_forward_groups_per_cu(int rows, int waves) {
     if (rows == 16 && waves == 4) return groups_per_cu<16, 4>();
