@@ -41,13 +41,13 @@ OPS_PER_CELL = 10          # integer ops per DP cell in the reference inner loop
 def measured_traffic(key):
     """HBM bytes per launch of a kernel at its bench workload, as measured
     with rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy of
-    known size) and recorded in profiles/hbm_traffic.md; None if absent."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    known size) and recorded in profiles/hbm_traffic.json (the runs are
+    described in profiles/hbm_traffic.md); None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
-        text = open(path).read()
-        block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block).get(key)
-    except (OSError, IndexError, ValueError):
+        with open(path) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
         return None
 
 
