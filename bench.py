@@ -17,7 +17,13 @@ the reference generates them query by query.  A step is one pass of the gapless 
 seeds; the rate is the bytes the reference rule reads per seed plus the
 seed record and the result, over the kernel time.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--leg all|sw|gapless]
+The default run (--leg all) also carries the other legs in the same line:
+serving alignment requests, traceback statistics, the suffix array, and
+end-to-end `topaz search` at 100,000 proteins x 1,000 queries and at
+10,000,000 proteins x 100,000 queries (default and -F, the reference CPU
+binary on all host cores over a 1,000-query sample, hit lists compared).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--leg all|sw|gapless|...]
 
 Prints one JSON line.
 """
@@ -874,6 +880,8 @@ def main():
                     help="wavefronts per workgroup of the serving loop's launches")
     ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="fraction of the 100,000-subject serving database")
+    ap.add_argument("--no-search-10m", action="store_true",
+                    help="leave the 10M-protein search out of the default run")
     ap.add_argument("--gapless-scale", type=float, default=1.0,
                     help="fraction of the 1e9-residue, 5e7-seed gapless workload")
     args = ap.parse_args()
@@ -933,6 +941,25 @@ def main():
             if top:
                 out["latency_p50_ms"] = top[0]["p50_ms"]
                 out["latency_p99_ms"] = top[0]["p99_ms"]
+            if not args.no_search_10m:
+                # end-to-end `topaz search` at scale: 10,000,000 proteins
+                # (3.5 G residues), 100,000 queries, default and -F; the
+                # reference on all host cores over the first 1,000 queries,
+                # the plain schedule over the first 200
+                sargs.search_db, sargs.search_queries, sargs.search_sample = 10000000, 100000, 1000
+                sargs.plain_sample, sargs.search_modes = 200, "default,fast"
+                try:
+                    out["search_10M"] = s10 = search_leg(sargs)
+                    for mode in ("default", "fast"):
+                        r = s10[mode]
+                        out["search_10M_%s" % mode] = {
+                            "queries_per_s": r["gpu_all"]["queries_per_s"],
+                            "vs_reference_cpu": r["speedup_vs_reference"],
+                            "batched_vs_plain": r["batched_vs_plain"],
+                            "identical": r["identical"]}
+                except (Exception, SystemExit) as e:      # the headline line still prints
+                    out["search_10M"] = {"error": str(e)[-2000:]}
+                    print("[bench] search 10M failed: %s" % e, file=sys.stderr, flush=True)
     else:
         g = gapless_leg(args)
         m = g["matched"]

@@ -119,10 +119,13 @@ QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
 // schedule), the runs of the lane halves filled as described at the
 // definition.
 void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len, int rows,
-                        int nq_max, Tasks& t);
+                        int nq_max, Tasks& t, bool fill = true);
 // fn(i) for i in [0, n) on up to `threads` host threads (the caller's
 // included); contiguous index ranges per thread.
 void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& range_fn);
+// Host threads the library uses for its parallel loops (TOPAZ_AMD_HOST_THREADS,
+// else the machine's, at most 64).
+int host_threads();
 // Work items of the persistent schedule (`waves` entries each), most
 // expensive first.
 void build_items(const Tasks& t, int n_slots, double big_fraction, int waves,

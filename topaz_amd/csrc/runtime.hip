@@ -96,7 +96,7 @@ void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int6
     for (auto& x : th) x.join();
 }
 
-static int host_threads() {
+int host_threads() {
     static const int n = [] {
         int v = (int)std::thread::hardware_concurrency();
         if (const char* e = getenv("TOPAZ_AMD_HOST_THREADS")) v = atoi(e);
@@ -202,94 +202,105 @@ QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
     return r;
 }
 
-// Tasks of up to nq (<= 4) queries (the persistent schedule).  Queries with
-// the same number of row chunks are packed together: repeatedly the nq
-// queries whose longest remaining subjects are longest give their longest
-// remaining pairs, longest first, until the task holds 128: one per lane
-// half, the first of its run.  A query's long subjects then share a
-// wavefront with other queries' long subjects instead of its own short ones.
-// The task's width is its longest pair; then every run, from the shortest up,
-// takes the longest remaining pair of the same queries that fits in what is
-// left of that width, while one does, so the lane halves walk pairs instead
-// of padding columns.
-void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len, int rows,
-                        int nq_max, Tasks& t) {
-    t.clear();
-    const int32_t nq = (int32_t)per.begin.size() - 1;
-    std::vector<int32_t> order;
-    for (int32_t q = 0; q < nq; ++q)
-        if (per.count(q) > 0) order.push_back(q);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        return chunk_rows(q_len[x], rows) < chunk_rows(q_len[y], rows);
-    });
-    // positions of per (each query's pairs, longest first) not taken yet:
-    // untaken(i) is the first untaken position >= i (path halving)
-    const int64_t n = per.begin.back();
-    std::vector<int64_t> nx((size_t)n + 1);
-    std::iota(nx.begin(), nx.end(), (int64_t)0);
-    auto untaken = [&](int64_t i) {
+// Positions of a QueryPairs (each query's pairs, longest first) not taken
+// yet: untaken(i) is the first untaken position >= i, untaken_back(i) the
+// last one <= i (-1: none).  Path halving; pv is shifted by one.  Queries own
+// disjoint ranges, so groups of queries can be planned at once.
+struct Untaken {
+    std::vector<int64_t> nx, pv;
+    Untaken(int64_t n, int threads) : nx((size_t)n + 1), pv((size_t)n + 1) {
+        parallel_for(n + 1, threads, [&](int64_t a, int64_t b) {
+            for (int64_t i = a; i < b; ++i) nx[i] = pv[i] = i;
+        });
+    }
+    int64_t next(int64_t i) {
         while (nx[i] != i) { nx[i] = nx[nx[i]]; i = nx[i]; }
         return i;
+    }
+    int64_t back(int64_t i) {
+        int64_t k = i + 1;
+        while (pv[k] != k) { pv[k] = pv[pv[k]]; k = pv[k]; }
+        return k - 1;
+    }
+    void take(int64_t i) { nx[i] = i + 1; pv[i + 1] = i; }
+};
+
+// The tasks of one group of queries with the same chunk count (see
+// build_packed_tasks), appended to t.
+static void plan_group(const QueryPairs& per, const int32_t* qs, size_t nqs, int64_t task_rows,
+                       int nq_max, bool fill, Untaken& U, Tasks& t) {
+    std::vector<int64_t> head(nqs);
+    std::vector<int32_t> id(qs, qs + nqs);
+    for (size_t k = 0; k < nqs; ++k) head[k] = per.begin[id[k]];
+    // local index k: is there a remaining pair, and the longest's length
+    auto has = [&](size_t k) { return (head[k] = U.next(head[k])) < per.begin[id[k] + 1]; };
+    auto shortest = [&](size_t k) {
+        const int64_t i = U.back(per.begin[id[k] + 1] - 1);
+        return i >= per.begin[id[k]] ? per.len[i] : INT32_MAX;
     };
-    auto take = [&](int64_t i) { nx[i] = i + 1; };
-    std::vector<int64_t> head(per.begin.begin(), per.begin.end() - 1);
-    auto has = [&](int32_t q) { return (head[q] = untaken(head[q])) < per.begin[q + 1]; };
-    auto head_len = [&](int32_t q) { return per.len[head[q]]; };
     // max-heap on (longest remaining subject, then lower query number)
-    auto below = [&](int32_t x, int32_t y) {
-        const int32_t lx = head_len(x), ly = head_len(y);
-        return lx != ly ? lx < ly : x > y;
+    auto below = [&](size_t x, size_t y) {
+        const int32_t lx = per.len[head[x]], ly = per.len[head[y]];
+        return lx != ly ? lx < ly : id[x] > id[y];
     };
-    std::vector<int32_t> load;     // columns of each run of the open task
-    std::vector<int32_t> last;     // last entry of each run
-    size_t g0 = 0;
-    while (g0 < order.size()) {
-        const int64_t task_rows = chunk_rows(q_len[order[g0]], rows);
-        size_t g1 = g0;
-        while (g1 < order.size() && chunk_rows(q_len[order[g1]], rows) == task_rows) ++g1;
-        std::vector<int32_t> heap(order.begin() + g0, order.begin() + g1);
-        std::make_heap(heap.begin(), heap.end(), below);
-        while (!heap.empty()) {
-            int32_t sel[kTaskQueries];
-            int ns = 0;
-            while (ns < nq_max && !heap.empty()) {
-                std::pop_heap(heap.begin(), heap.end(), below);
-                sel[ns++] = heap.back();
-                heap.pop_back();
-            }
-            t.open(sel, ns);
-            load.clear();
-            last.clear();
-            for (int taken = 0; taken < kTaskPairs; ++taken) {
-                int best = -1;
-                for (int k = 0; k < ns; ++k) {
-                    const int32_t q = sel[k];
-                    if (has(q) && (best < 0 || below(sel[best], q))) best = k;
-                }
-                if (best < 0) break;
-                const int32_t q = sel[best];
-                load.push_back(item_columns(per.len[head[q]]));
-                last.push_back((int32_t)t.index.size());
-                t.index.push_back(per.pair[head[q]]);
-                t.next.push_back(-1);
-                take(head[q]);
-            }
-            const int32_t width = load.empty() ? 0 : load[0];
+    std::vector<size_t> heap(nqs);
+    std::iota(heap.begin(), heap.end(), (size_t)0);
+    std::make_heap(heap.begin(), heap.end(), below);
+    std::vector<int32_t> load, last;
+    while (!heap.empty()) {
+        size_t sel[kTaskQueries];
+        int32_t selq[kTaskQueries];
+        int ns = 0;
+        while (ns < nq_max && !heap.empty()) {
+            std::pop_heap(heap.begin(), heap.end(), below);
+            sel[ns] = heap.back();
+            selq[ns] = id[heap.back()];
+            ++ns;
+            heap.pop_back();
+        }
+        t.open(selq, ns);
+        load.clear();
+        last.clear();
+        // the seeds: a merge of the selected queries' longest remaining
+        // pairs; only the query just taken from moves on
+        int32_t hl[kTaskQueries];
+        for (int k = 0; k < ns; ++k) hl[k] = has(sel[k]) ? per.len[head[sel[k]]] : -1;
+        for (int taken = 0; taken < kTaskPairs; ++taken) {
+            int best = -1;
+            for (int k = 0; k < ns; ++k)
+                if (hl[k] >= 0 && (best < 0 || hl[k] > hl[best] ||
+                                   (hl[k] == hl[best] && selq[k] < selq[best])))
+                    best = k;
+            if (best < 0) break;
+            const size_t b = sel[best];
+            load.push_back(item_columns(per.len[head[b]]));
+            last.push_back((int32_t)t.index.size());
+            t.index.push_back(per.pair[head[b]]);
+            t.next.push_back(-1);
+            U.take(head[b]);
+            hl[best] = has(b) ? per.len[head[b]] : -1;
+        }
+        const int32_t width = load.empty() ? 0 : load[0];
+        if (fill) {
             // runs from the shortest (the last seeded) up take the longest
             // remaining pair that fits (a pair of length <= gap takes
-            // item_columns <= gap columns: gaps are multiples of 4)
+            // item_columns <= gap columns: gaps are multiples of 4); a run
+            // whose gap is below every query's shortest remaining pair takes
+            // none, and neither do the runs after it (smaller gaps)
+            int32_t least = INT32_MAX;
+            for (int k = 0; k < ns; ++k) least = std::min(least, shortest(sel[k]));
             for (int lane = (int)load.size() - 1; lane >= 0; --lane) {
+                if (width - load[lane] < 8 || width - load[lane] < least) break;
                 for (;;) {
                     const int32_t gap = width - load[lane];
-                    if (gap < 8) break;
+                    if (gap < 8 || gap < least) break;
                     int64_t pick = -1;
                     for (int k = 0; k < ns; ++k) {
-                        const int32_t q = sel[k];
-                        const int64_t b = per.begin[q], e = per.begin[q + 1];
-                        // first position of q whose length is <= gap
+                        const int64_t b = per.begin[selq[k]], e = per.begin[selq[k] + 1];
+                        // first position of the query whose length is <= gap
                         const int64_t f = std::lower_bound(per.len.begin() + b, per.len.begin() + e, gap,
                                                            std::greater<int32_t>()) - per.len.begin();
-                        const int64_t i = f < e ? untaken(f) : e;
+                        const int64_t i = f < e ? U.next(f) : e;
                         if (i < e && (pick < 0 || per.len[i] > per.len[pick])) pick = i;
                     }
                     if (pick < 0) break;
@@ -299,21 +310,95 @@ void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len
                     t.next[last[lane]] = e;
                     last[lane] = e;
                     load[lane] += item_columns(per.len[pick]);
-                    take(pick);
-                }
-            }
-            t.cols.back() = width;
-            t.cost.back() = task_rows * width;
-            for (int k = 0; k < ns; ++k) {
-                if (has(sel[k])) {
-                    heap.push_back(sel[k]);
-                    std::push_heap(heap.begin(), heap.end(), below);
+                    U.take(pick);
+                    least = INT32_MAX;
+                    for (int k = 0; k < ns; ++k) least = std::min(least, shortest(sel[k]));
                 }
             }
         }
+        t.cols.back() = width;
+        t.cost.back() = task_rows * width;
+        for (int k = 0; k < ns; ++k) {
+            if (has(sel[k])) {
+                heap.push_back(sel[k]);
+                std::push_heap(heap.begin(), heap.end(), below);
+            }
+        }
+    }
+}
+
+// Tasks of up to nq (<= 4) queries (the persistent schedule).  Queries with
+// the same number of row chunks are packed together: repeatedly the nq
+// queries whose longest remaining subjects are longest give their longest
+// remaining pairs, longest first, until the task holds 128: one per lane
+// half, the first of its run.  A query's long subjects then share a
+// wavefront with other queries' long subjects instead of its own short ones.
+// The task's width is its longest pair; with fill, every run, from the
+// shortest up, then takes the longest remaining pair of the same queries
+// that fits in what is left of that width, while one does, so the lane
+// halves walk pairs instead of padding columns.  Groups of queries are
+// planned on host threads at once (groups of more than 64 queries in parts
+// of 64) and their tasks concatenated.
+void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len, int rows,
+                        int nq_max, Tasks& t, bool fill) {
+    t.clear();
+    const int32_t nq = (int32_t)per.begin.size() - 1;
+    std::vector<int32_t> order;
+    for (int32_t q = 0; q < nq; ++q)
+        if (per.count(q) > 0) order.push_back(q);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return chunk_rows(q_len[x], rows) < chunk_rows(q_len[y], rows);
+    });
+    // parts: runs of order with one chunk count, at most 64 queries each
+    constexpr size_t kPart = 64;
+    std::vector<size_t> part;
+    for (size_t g0 = 0; g0 < order.size();) {
+        const int64_t r = chunk_rows(q_len[order[g0]], rows);
+        size_t g1 = g0;
+        while (g1 < order.size() && g1 - g0 < kPart && chunk_rows(q_len[order[g1]], rows) == r) ++g1;
+        part.push_back(g0);
         g0 = g1;
     }
-    t.begin.push_back((int32_t)t.index.size());
+    part.push_back(order.size());
+    const int64_t n_parts = (int64_t)part.size() - 1;
+    const int threads = per.begin.back() >= (1 << 18) ? host_threads() : 1;
+    Untaken U(per.begin.back(), threads);
+    std::vector<Tasks> sub((size_t)n_parts);
+    parallel_for(n_parts, threads, [&](int64_t k0, int64_t k1) {
+        for (int64_t k = k0; k < k1; ++k)
+            plan_group(per, order.data() + part[k], part[k + 1] - part[k],
+                       chunk_rows(q_len[order[part[k]]], rows), nq_max, fill, U, sub[k]);
+    });
+    // concatenate: each part's tasks and entries at its offsets
+    std::vector<int64_t> e_at((size_t)n_parts + 1, 0), t_at((size_t)n_parts + 1, 0);
+    for (int64_t k = 0; k < n_parts; ++k) {
+        e_at[k + 1] = e_at[k] + (int64_t)sub[k].index.size();
+        t_at[k + 1] = t_at[k] + sub[k].count();
+    }
+    const size_t n_tasks = (size_t)t_at[n_parts], n_entries = (size_t)e_at[n_parts];
+    t.begin.resize(n_tasks + 1);
+    t.cols.resize(n_tasks);
+    t.cost.resize(n_tasks);
+    t.query.resize(n_tasks * kTaskQueries);
+    t.index.resize(n_entries);
+    t.next.resize(n_entries);
+    parallel_for(n_parts, threads, [&](int64_t k0, int64_t k1) {
+        for (int64_t k = k0; k < k1; ++k) {
+            const Tasks& g = sub[k];
+            const int32_t eb = (int32_t)e_at[k];
+            const size_t tb = (size_t)t_at[k];
+            for (int32_t j = 0; j < g.count(); ++j) {
+                t.begin[tb + j] = eb + g.begin[j];
+                t.cols[tb + j] = g.cols[j];
+                t.cost[tb + j] = g.cost[j];
+            }
+            std::copy(g.query.begin(), g.query.end(), t.query.begin() + tb * kTaskQueries);
+            std::copy(g.index.begin(), g.index.end(), t.index.begin() + eb);
+            for (size_t x = 0; x < g.next.size(); ++x)
+                t.next[eb + x] = g.next[x] >= 0 ? g.next[x] + eb : -1;
+        }
+    });
+    t.begin[n_tasks] = (int32_t)n_entries;
 }
 
 void build_entry_records(const Tasks& t, const int32_t* pair_query, const int32_t* pair_subject,

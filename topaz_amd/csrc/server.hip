@@ -89,6 +89,7 @@ struct Request {
     std::vector<int32_t> q_len;
     std::vector<int32_t> pq, ps;
     Tasks tasks;                        // planned at submit
+    std::vector<int4> rec;              // the forward kernel's entry records (request-local entries)
     std::vector<int32_t> score, ref, read;
     size_t bytes = 0;
     double t_submit = 0, t_done = 0;
@@ -149,64 +150,35 @@ struct tpz_server {
 namespace tpz {
 namespace {
 
-// Merge the slot's requests, upload, launch, and queue the result copy.
+// Merge the slot's requests, upload, launch, and queue the result copy.  The
+// requests' task lists and entry records were built at submit (in the
+// submitting threads); here host threads copy them into the pinned upload
+// blob, their query, pair, entry and task numbers moved by each request's
+// bases.
 int launch_slot(tpz_server* s, Slot& sl) {
-    size_t nq = 0, nres = 0, np = 0;
-    for (Request* r : sl.reqs) { nq += r->q_len.size(); nres += r->q_res.size(); np += r->pq.size(); }
-    std::vector<int32_t> q_len(nq), pq(np), ps(np);
-    std::vector<int64_t> q_off(nq);
-    sl.pair_base.clear();
-    {
-        size_t qb = 0, pb = 0, rb = 0;
-        for (Request* r : sl.reqs) {
-            sl.pair_base.push_back(pb);
-            for (size_t k = 0; k < r->q_len.size(); ++k) {
-                q_len[qb + k] = r->q_len[k];
-                q_off[qb + k] = (int64_t)rb + r->q_off[k];
-            }
-            for (size_t p = 0; p < r->pq.size(); ++p) {
-                pq[pb + p] = (int32_t)qb + r->pq[p];
-                ps[pb + p] = r->ps[p];
-            }
-            qb += r->q_len.size();
-            pb += r->pq.size();
-            rb += r->q_res.size();
-        }
+    const size_t n_req = sl.reqs.size();
+    // bases of each request: queries, residues, pairs (= entries), tasks
+    std::vector<size_t> qb(n_req + 1, 0), rb(n_req + 1, 0), pb(n_req + 1, 0), tb(n_req + 1, 0);
+    for (size_t k = 0; k < n_req; ++k) {
+        const Request* r = sl.reqs[k];
+        qb[k + 1] = qb[k] + r->q_len.size();
+        rb[k + 1] = rb[k] + r->q_res.size();
+        pb[k + 1] = pb[k] + r->pq.size();
+        tb[k + 1] = tb[k] + (size_t)r->tasks.count();
     }
+    const size_t nq = qb[n_req], nres = rb[n_req], np = pb[n_req], nt = tb[n_req];
+    sl.pair_base.assign(pb.begin(), pb.end() - 1);
     sl.n_pairs = np;
-    // concatenate the requests' task lists
-    Tasks t;
-    {
-        size_t qb = 0, pb = 0;
-        for (Request* r : sl.reqs) {
-            const Tasks& rt = r->tasks;
-            for (int32_t k = 0; k < rt.count(); ++k) {
-                int32_t qs[kTaskQueries];
-                int nq_k = 0;
-                for (int j = 0; j < kTaskQueries; ++j)
-                    if (rt.query[kTaskQueries * k + j] >= 0)
-                        qs[nq_k++] = (int32_t)qb + rt.query[kTaskQueries * k + j];
-                t.open(qs, nq_k);
-                t.cols.back() = rt.cols[k];
-                t.cost.back() = rt.cost[k];
-                const int32_t eb = (int32_t)t.index.size() - rt.begin[k];   // entry rebase
-                for (int32_t e = rt.begin[k]; e < rt.begin[k + 1]; ++e) {
-                    t.index.push_back((int32_t)pb + rt.index[e]);
-                    t.next.push_back(rt.next[e] >= 0 ? rt.next[e] + eb : -1);
-                }
-            }
-            qb += r->q_len.size();
-            pb += r->pq.size();
-        }
-        t.begin.push_back((int32_t)t.index.size());
-    }
+    // work items from the merged task costs
+    Tasks costs;
+    costs.cost.reserve(nt);
+    for (const Request* r : sl.reqs) costs.cost.insert(costs.cost.end(), r->tasks.cost.begin(), r->tasks.cost.end());
     std::vector<int32_t> items;
     int32_t n_big = 0;
     const int W = s->cfg.waves_per_group;
-    build_items(t, s->grid, 1.0, W, items, n_big);
+    build_items(costs, s->grid, 1.0, W, items, n_big);
     const int n_items = (int)(items.size() / W);
-    std::vector<int4> rec;
-    if (!s->lanes) build_entry_records(t, pq.data(), ps.data(), s->so.data(), s->s_len.data(), rec);
+    const size_t n_rec = s->lanes ? 0 : np;
 
     // one host blob, one copy: residues | q_off | q_len | pq | ps | index | begin | tquery | items
     // | cols | records
@@ -216,12 +188,12 @@ int launch_slot(tpz_server* s, Slot& sl) {
     const size_t o_pq = align16(o_qlen + 4 * nq);
     const size_t o_ps = align16(o_pq + 4 * np);
     const size_t o_index = align16(o_ps + 4 * np);
-    const size_t o_begin = align16(o_index + 4 * t.index.size());
-    const size_t o_tq = align16(o_begin + 4 * t.begin.size());
-    const size_t o_items = align16(o_tq + 4 * t.query.size());
+    const size_t o_begin = align16(o_index + 4 * np);
+    const size_t o_tq = align16(o_begin + 4 * (nt + 1));
+    const size_t o_items = align16(o_tq + 4 * kTaskQueries * nt);
     const size_t o_cols = align16(o_items + 4 * items.size());
-    const size_t o_rec = align16(o_cols + 4 * t.cols.size());
-    const size_t in_bytes = align16(o_rec + sizeof(int4) * rec.size()) + 16;
+    const size_t o_rec = align16(o_cols + 4 * nt);
+    const size_t in_bytes = align16(o_rec + sizeof(int4) * n_rec) + 16;
     // score | ref | read | error | counter | (to 8 bytes) keys
     const size_t o_key_words = (3 * np + 2 + 1) & ~(size_t)1;
     const size_t out_words = 3 * np + 2;
@@ -230,21 +202,53 @@ int launch_slot(tpz_server* s, Slot& sl) {
         sl.h_out.reserve(out_words * 4) || sl.d_out.reserve(dev_out_words * 4))
         return -1;
     char* h = sl.h_in.ptr;
-    size_t rb = 0;
-    for (Request* r : sl.reqs) {
-        memcpy(h + o_res + rb, r->q_res.data(), r->q_res.size());
-        rb += r->q_res.size();
+    int64_t* h_qoff = (int64_t*)(h + o_qoff);
+    int32_t* h_qlen = (int32_t*)(h + o_qlen);
+    int32_t* h_pq = (int32_t*)(h + o_pq);
+    int32_t* h_ps = (int32_t*)(h + o_ps);
+    int32_t* h_index = (int32_t*)(h + o_index);
+    int32_t* h_begin = (int32_t*)(h + o_begin);
+    int32_t* h_tq = (int32_t*)(h + o_tq);
+    int32_t* h_cols = (int32_t*)(h + o_cols);
+    int4* h_rec = (int4*)(h + o_rec);
+    std::vector<int32_t> q_len(nq);
+    for (size_t k = 0; k < n_req; ++k) {
+        const Request* r = sl.reqs[k];
+        memcpy(h + o_res + rb[k], r->q_res.data(), r->q_res.size());
+        for (size_t j = 0; j < r->q_len.size(); ++j) {
+            h_qlen[qb[k] + j] = q_len[qb[k] + j] = r->q_len[j];
+            h_qoff[qb[k] + j] = (int64_t)rb[k] + r->q_off[j];
+        }
+        const Tasks& rt = r->tasks;
+        for (int32_t j = 0; j < rt.count(); ++j) {
+            h_begin[tb[k] + j] = (int32_t)pb[k] + rt.begin[j];
+            h_cols[tb[k] + j] = rt.cols[j];
+            for (int x = 0; x < kTaskQueries; ++x) {
+                const int32_t q = rt.query[(size_t)kTaskQueries * j + x];
+                h_tq[(tb[k] + j) * kTaskQueries + x] = q >= 0 ? (int32_t)qb[k] + q : -1;
+            }
+        }
     }
-    memcpy(h + o_qoff, q_off.data(), 8 * nq);
-    memcpy(h + o_qlen, q_len.data(), 4 * nq);
-    memcpy(h + o_pq, pq.data(), 4 * np);
-    memcpy(h + o_ps, ps.data(), 4 * np);
-    memcpy(h + o_index, t.index.data(), 4 * t.index.size());
-    memcpy(h + o_begin, t.begin.data(), 4 * t.begin.size());
-    memcpy(h + o_tq, t.query.data(), 4 * t.query.size());
+    h_begin[nt] = (int32_t)np;
     memcpy(h + o_items, items.data(), 4 * items.size());
-    memcpy(h + o_cols, t.cols.data(), 4 * t.cols.size());
-    memcpy(h + o_rec, rec.data(), sizeof(int4) * rec.size());
+    // the per-pair arrays, in pieces of 64 K pairs over host threads
+    const int64_t n_pieces = (int64_t)((np + 65535) >> 16);
+    parallel_for(n_pieces, np >= ((size_t)1 << 20) ? host_threads() : 1, [&](int64_t c0, int64_t c1) {
+        size_t k = std::upper_bound(pb.begin(), pb.end(), (size_t)c0 << 16) - pb.begin() - 1;
+        for (size_t e = (size_t)c0 << 16, stop = std::min(np, (size_t)c1 << 16); e < stop; ++e) {
+            while (e >= pb[k + 1]) ++k;
+            const Request* r = sl.reqs[k];
+            const size_t x = e - pb[k];
+            h_pq[e] = (int32_t)qb[k] + r->pq[x];
+            h_ps[e] = r->ps[x];
+            h_index[e] = (int32_t)pb[k] + r->tasks.index[x];
+            if (n_rec) {
+                int4 v = r->rec[x];
+                if (v.z >= 0) v.z += (int32_t)pb[k];
+                h_rec[e] = v;
+            }
+        }
+    });
     char* d = (char*)sl.d_in.ptr;
     int32_t* dout = (int32_t*)sl.d_out.ptr;
     sl.error_word = (int)(3 * np);
@@ -538,6 +542,9 @@ int tpz_server_submit(tpz_server* s, int32_t n_queries, const int8_t* q_res,
     {
         const QueryPairs per = pairs_by_query(r->pq, r->ps, s->s_len, n_queries);
         build_packed_tasks(per, r->q_len, 32, kTaskQueries, r->tasks);
+        if (!s->lanes)
+            build_entry_records(r->tasks, r->pq.data(), r->ps.data(), s->so.data(), s->s_len.data(),
+                                r->rec);
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return fail("server stopped");
