@@ -78,11 +78,16 @@ struct Tasks {
     int32_t count() const { return (int32_t)cost.size(); }
 };
 
+// Per pair: its subject's offset / 16 (s_off 16-byte aligned) and length,
+// gathered once so that planning reads the subject tables at random once per
+// pair.
+std::vector<int2> pair_subjects(const int32_t* pair_subject, int64_t n_pairs, const int64_t* s_off,
+                                const int32_t* s_len);
 // Per entry of t: the forward kernel's record (SwKernelArgs::entry_rec):
-// subject offset / 16 (s_off 16-byte aligned), subject length, next entry,
+// subject offset / 16, subject length (both from pair_sub), next entry,
 // query slot within the entry's task.
-void build_entry_records(const Tasks& t, const int32_t* pair_query, const int32_t* pair_subject,
-                         const int64_t* s_off, const int32_t* s_len, std::vector<int4>& rec);
+void build_entry_records(const Tasks& t, const int32_t* pair_query, const int2* pair_sub,
+                         std::vector<int4>& rec);
 
 // A task set on the device.
 struct DeviceTasks {
@@ -112,9 +117,8 @@ struct QueryPairs {
     std::vector<int32_t> pair, len;
     int64_t count(int32_t q) const { return begin[q + 1] - begin[q]; }
 };
-QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
-                          const std::vector<int32_t>& pair_subject,
-                          const std::vector<int32_t>& s_len, int32_t n_queries);
+QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query, const int2* pair_sub,
+                          int32_t n_queries);
 // Up to nq_max queries of equal chunk count per task (the persistent
 // schedule), the runs of the lane halves filled as described at the
 // definition.
@@ -123,6 +127,9 @@ void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len
 // fn(i) for i in [0, n) on up to `threads` host threads (the caller's
 // included); contiguous index ranges per thread.
 void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& range_fn);
+// fn(i) for every i in [0, n) on up to `threads` host threads (the caller's
+// included), each taking the next index from a shared counter.
+void parallel_for_each(int64_t n, int threads, const std::function<void(int64_t)>& fn);
 // Host threads the library uses for its parallel loops (TOPAZ_AMD_HOST_THREADS,
 // else the machine's, at most 64).
 int host_threads();

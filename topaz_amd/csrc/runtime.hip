@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -85,17 +86,11 @@ static int64_t chunk_rows(int32_t m, int rows) {
     return (int64_t)(m + rows - 1) / rows * rows;
 }
 
-// Pairs of each query, longest subject first (ties: lower pair number).
-void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& range_fn) {
-    if (n <= 0) return;
-    const int t = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
-    std::vector<std::thread> th;
-    for (int k = 1; k < t; ++k)
-        th.emplace_back([&, k] { range_fn(n * k / t, n * (k + 1) / t); });
-    range_fn(0, n / t);
-    for (auto& x : th) x.join();
-}
-
+// ---------------------------------------------------------------------------
+// host threads: one pool per process for every parallel loop of the library,
+// so that concurrent calls (a search's workers, the server's scheduler) share
+// the machine's cores instead of each starting threads of its own
+// ---------------------------------------------------------------------------
 int host_threads() {
     static const int n = [] {
         int v = (int)std::thread::hardware_concurrency();
@@ -103,6 +98,96 @@ int host_threads() {
         return std::max(1, std::min(v, 64));
     }();
     return n;
+}
+
+namespace {
+class Pool {
+public:
+    explicit Pool(int n) {
+        for (int k = 0; k < n; ++k) th_.emplace_back([this] { worker(); });
+    }
+    // fn(i) for i in [0, n), indices taken from a shared counter; the caller
+    // works too (so a loop inside a loop cannot wait for itself).  Returns
+    // when all are done.
+    void run(int64_t n, const std::function<void(int64_t)>& fn) {
+        if (n <= 0) return;
+        auto job = std::make_shared<Job>();
+        job->n = n;
+        job->fn = &fn;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_.push_back(job);
+        }
+        cv_.notify_all();
+        work(*job);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return job->done.load() == job->n; });
+        jobs_.erase(std::remove(jobs_.begin(), jobs_.end(), job), jobs_.end());
+    }
+
+private:
+    struct Job {
+        int64_t n = 0;
+        const std::function<void(int64_t)>* fn = nullptr;
+        std::atomic<int64_t> next{0}, done{0};
+    };
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::vector<std::shared_ptr<Job>> jobs_;
+
+    void work(Job& j) {
+        int64_t i;
+        while ((i = j.next.fetch_add(1)) < j.n) {
+            (*j.fn)(i);
+            if (j.done.fetch_add(1) + 1 == j.n) {
+                std::lock_guard<std::mutex> lk(mu_);
+                done_cv_.notify_all();
+            }
+        }
+    }
+    // the threads live until the process ends
+    void worker() {
+        for (;;) {
+            std::shared_ptr<Job> j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] {
+                    for (auto& x : jobs_)
+                        if (x->next.load() < x->n) return true;
+                    return false;
+                });
+                for (auto& x : jobs_)
+                    if (x->next.load() < x->n) { j = x; break; }
+            }
+            if (j) work(*j);
+        }
+    }
+};
+
+Pool& pool() {
+    static Pool* p = new Pool(host_threads() - 1);
+    return *p;
+}
+}  // namespace
+
+void parallel_for_each(int64_t n, int threads, const std::function<void(int64_t)>& fn) {
+    if (n <= 0) return;
+    if (threads <= 1 || n == 1) {
+        for (int64_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    pool().run(n, fn);
+}
+
+void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& range_fn) {
+    if (n <= 0) return;
+    const int t = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
+    if (t == 1) {
+        range_fn(0, n);
+        return;
+    }
+    pool().run(t, [&](int64_t k) { range_fn(n * k / t, n * (k + 1) / t); });
 }
 
 int DeviceBuffer::upload_large(const void* src, size_t n) {
@@ -164,39 +249,101 @@ int stage_to_device(void* dst, size_t n,
     return check(hipStreamSynchronize(g_stream), "staged upload");
 }
 
-// A counting sort by query (pairs stay in pair order), then per query a sort
-// of (subject length descending, pair number) keys; each subject length is
-// read once.  Large lists are split over host threads.
-QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
-                          const std::vector<int32_t>& pair_subject,
-                          const std::vector<int32_t>& s_len, int32_t n_queries) {
+
+// A counting sort by query (pairs stay in pair order; per-thread counts over
+// contiguous pieces of the pairs), then per query a stable LSD radix sort of
+// its pairs by (longest subject - subject length): longest first, equal
+// lengths in pair order.  Each subject length is read once.
+std::vector<int2> pair_subjects(const int32_t* pair_subject, int64_t n_pairs, const int64_t* s_off,
+                                const int32_t* s_len) {
+    std::vector<int2> r((size_t)n_pairs);
+    parallel_for(n_pairs, n_pairs >= (1 << 18) ? host_threads() : 1, [&](int64_t a, int64_t b) {
+        for (int64_t p = a; p < b; ++p) {
+            const int32_t k = pair_subject[p];
+            r[(size_t)p] = make_int2((int32_t)(uint32_t)(s_off[k] >> 4), s_len[k]);
+        }
+    });
+    return r;
+}
+
+QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query, const int2* pair_sub,
+                          int32_t n_queries) {
     QueryPairs r;
     const int64_t n = (int64_t)pair_query.size();
     const int threads = n >= (1 << 18) ? host_threads() : 1;
+    // pieces of at least 64 K pairs, and per-piece counts of at most ~4 words per pair
+    const int pieces = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)threads, n >> 16, 4 * n / std::max<int64_t>(1, n_queries)}));
     r.begin.assign((size_t)n_queries + 1, 0);
-    for (int64_t p = 0; p < n; ++p) ++r.begin[pair_query[p] + 1];
-    for (int32_t q = 0; q < n_queries; ++q) r.begin[q + 1] += r.begin[q];
     r.pair.resize((size_t)n);
     r.len.resize((size_t)n);
-    {
-        std::vector<int64_t> at(r.begin.begin(), r.begin.end() - 1);
-        for (int64_t p = 0; p < n; ++p) r.pair[at[pair_query[p]]++] = (int32_t)p;
+    // cnt[j][q]: pairs of query q in piece j, then where piece j's pairs of q go
+    std::vector<std::vector<int64_t>> cnt((size_t)pieces, std::vector<int64_t>((size_t)n_queries, 0));
+    parallel_for(pieces, pieces, [&](int64_t j0, int64_t j1) {
+        for (int64_t j = j0; j < j1; ++j)
+            for (int64_t p = n * j / pieces, e = n * (j + 1) / pieces; p < e; ++p)
+                ++cnt[(size_t)j][(size_t)pair_query[p]];
+    });
+    int64_t at = 0;
+    for (int32_t q = 0; q < n_queries; ++q) {
+        r.begin[q] = at;
+        for (int j = 0; j < pieces; ++j) {
+            const int64_t c = cnt[(size_t)j][(size_t)q];
+            cnt[(size_t)j][(size_t)q] = at;
+            at += c;
+        }
     }
-    parallel_for(n_queries, threads, [&](int64_t q0, int64_t q1) {
-        std::vector<uint64_t> key;
-        for (int64_t q = q0; q < q1; ++q) {
-            const int64_t b = r.begin[q], e = r.begin[q + 1];
-            key.resize((size_t)(e - b));
-            for (int64_t i = b; i < e; ++i) {
-                const int32_t p = r.pair[i];
-                key[i - b] = ((uint64_t)(uint32_t)(INT32_MAX - s_len[pair_subject[p]]) << 32) |
-                             (uint32_t)p;
+    r.begin[n_queries] = at;
+    parallel_for(pieces, pieces, [&](int64_t j0, int64_t j1) {
+        for (int64_t j = j0; j < j1; ++j) {
+            std::vector<int64_t>& w = cnt[(size_t)j];
+            for (int64_t p = n * j / pieces, e = n * (j + 1) / pieces; p < e; ++p)
+                r.pair[w[(size_t)pair_query[p]]++] = (int32_t)p;
+        }
+    });
+    parallel_for_each(n_queries, threads, [&](int64_t q) {
+        const int64_t b = r.begin[q], m = r.begin[q + 1] - b;
+        if (m <= 1) {
+            if (m == 1) r.len[b] = pair_sub[r.pair[b]].y;
+            return;
+        }
+        int32_t* pr = r.pair.data() + b;
+        int32_t* ln = r.len.data() + b;
+        int32_t longest = 0;
+        for (int64_t i = 0; i < m; ++i) longest = std::max(longest, ln[i] = pair_sub[pr[i]].y);
+        // digits of (longest - length), least significant first, at most 12 bits each
+        int bits = 0;
+        while (bits < 31 && ((int64_t)longest >> bits) != 0) ++bits;
+        if (bits == 0) return;                 // every subject empty: pair order
+        const int passes = (bits + 11) / 12, width = (bits + passes - 1) / passes;
+        thread_local std::vector<int32_t> tp, tl;
+        thread_local std::vector<int64_t> count;
+        tp.resize((size_t)m);
+        tl.resize((size_t)m);
+        count.resize((size_t)1 << width);
+        int32_t *sp = pr, *sl = ln, *dp = tp.data(), *dl = tl.data();
+        for (int pass = 0; pass < passes; ++pass) {
+            const int shift = pass * width;
+            const int32_t mask = (1 << width) - 1;
+            std::fill(count.begin(), count.end(), 0);
+            for (int64_t i = 0; i < m; ++i) ++count[(size_t)(((longest - sl[i]) >> shift) & mask)];
+            int64_t sum = 0;
+            for (auto& c : count) {
+                const int64_t v = c;
+                c = sum;
+                sum += v;
             }
-            std::sort(key.begin(), key.end());
-            for (int64_t i = b; i < e; ++i) {
-                r.pair[i] = (int32_t)(uint32_t)key[i - b];
-                r.len[i] = INT32_MAX - (int32_t)(key[i - b] >> 32);
+            for (int64_t i = 0; i < m; ++i) {
+                const int64_t o = count[(size_t)(((longest - sl[i]) >> shift) & mask)]++;
+                dp[o] = sp[i];
+                dl[o] = sl[i];
             }
+            std::swap(sp, dp);
+            std::swap(sl, dl);
+        }
+        if (sp != pr) {
+            std::copy(sp, sp + m, pr);
+            std::copy(sl, sl + m, ln);
         }
     });
     return r;
@@ -207,22 +354,24 @@ QueryPairs pairs_by_query(const std::vector<int32_t>& pair_query,
 // last one <= i (-1: none).  Path halving; pv is shifted by one.  Queries own
 // disjoint ranges, so groups of queries can be planned at once.
 struct Untaken {
-    std::vector<int64_t> nx, pv;
-    Untaken(int64_t n, int threads) : nx((size_t)n + 1), pv((size_t)n + 1) {
+    std::unique_ptr<int32_t[]> nx, pv;   // positions fit int32 (pair counts are int32)
+    Untaken(int64_t n, int threads) : nx(new int32_t[(size_t)n + 1]), pv(new int32_t[(size_t)n + 1]) {
         parallel_for(n + 1, threads, [&](int64_t a, int64_t b) {
-            for (int64_t i = a; i < b; ++i) nx[i] = pv[i] = i;
+            for (int64_t i = a; i < b; ++i) nx[i] = pv[i] = (int32_t)i;
         });
     }
     int64_t next(int64_t i) {
-        while (nx[i] != i) { nx[i] = nx[nx[i]]; i = nx[i]; }
+        int32_t* x = nx.get();
+        while (x[i] != i) { x[i] = x[x[i]]; i = x[i]; }
         return i;
     }
     int64_t back(int64_t i) {
+        int32_t* x = pv.get();
         int64_t k = i + 1;
-        while (pv[k] != k) { pv[k] = pv[pv[k]]; k = pv[k]; }
+        while (x[k] != k) { x[k] = x[x[k]]; k = x[k]; }
         return k - 1;
     }
-    void take(int64_t i) { nx[i] = i + 1; pv[i + 1] = i; }
+    void take(int64_t i) { nx[i] = (int32_t)(i + 1); pv[i + 1] = (int32_t)i; }
 };
 
 // The tasks of one group of queries with the same chunk count (see
@@ -231,7 +380,18 @@ static void plan_group(const QueryPairs& per, const int32_t* qs, size_t nqs, int
                        int nq_max, bool fill, Untaken& U, Tasks& t) {
     std::vector<int64_t> head(nqs);
     std::vector<int32_t> id(qs, qs + nqs);
-    for (size_t k = 0; k < nqs; ++k) head[k] = per.begin[id[k]];
+    int64_t n_pairs = 0;
+    for (size_t k = 0; k < nqs; ++k) {
+        head[k] = per.begin[id[k]];
+        n_pairs += per.count(id[k]);
+    }
+    const size_t max_tasks = (size_t)(n_pairs / kTaskPairs) + nqs;
+    t.index.reserve(t.index.size() + (size_t)n_pairs);
+    t.next.reserve(t.next.size() + (size_t)n_pairs);
+    t.begin.reserve(t.begin.size() + max_tasks);
+    t.query.reserve(t.query.size() + max_tasks * kTaskQueries);
+    t.cols.reserve(t.cols.size() + max_tasks);
+    t.cost.reserve(t.cost.size() + max_tasks);
     // local index k: is there a remaining pair, and the longest's length
     auto has = [&](size_t k) { return (head[k] = U.next(head[k])) < per.begin[id[k] + 1]; };
     auto shortest = [&](size_t k) {
@@ -247,6 +407,8 @@ static void plan_group(const QueryPairs& per, const int32_t* qs, size_t nqs, int
     std::iota(heap.begin(), heap.end(), (size_t)0);
     std::make_heap(heap.begin(), heap.end(), below);
     std::vector<int32_t> load, last;
+    load.reserve(kTaskPairs);
+    last.reserve(kTaskPairs);
     while (!heap.empty()) {
         size_t sel[kTaskQueries];
         int32_t selq[kTaskQueries];
@@ -364,10 +526,17 @@ void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len
     const int threads = per.begin.back() >= (1 << 18) ? host_threads() : 1;
     Untaken U(per.begin.back(), threads);
     std::vector<Tasks> sub((size_t)n_parts);
-    parallel_for(n_parts, threads, [&](int64_t k0, int64_t k1) {
-        for (int64_t k = k0; k < k1; ++k)
-            plan_group(per, order.data() + part[k], part[k + 1] - part[k],
-                       chunk_rows(q_len[order[part[k]]], rows), nq_max, fill, U, sub[k]);
+    // parts with the most pairs first, each to the next free thread
+    std::vector<int64_t> part_pairs((size_t)n_parts, 0), by_size((size_t)n_parts);
+    for (int64_t k = 0; k < n_parts; ++k)
+        for (size_t g = part[k]; g < part[k + 1]; ++g) part_pairs[k] += per.count(order[g]);
+    std::iota(by_size.begin(), by_size.end(), (int64_t)0);
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](int64_t x, int64_t y) { return part_pairs[x] > part_pairs[y]; });
+    parallel_for_each(n_parts, threads, [&](int64_t i) {
+        const int64_t k = by_size[i];
+        plan_group(per, order.data() + part[k], part[k + 1] - part[k],
+                   chunk_rows(q_len[order[part[k]]], rows), nq_max, fill, U, sub[k]);
     });
     // concatenate: each part's tasks and entries at its offsets
     std::vector<int64_t> e_at((size_t)n_parts + 1, 0), t_at((size_t)n_parts + 1, 0);
@@ -401,8 +570,8 @@ void build_packed_tasks(const QueryPairs& per, const std::vector<int32_t>& q_len
     t.begin[n_tasks] = (int32_t)n_entries;
 }
 
-void build_entry_records(const Tasks& t, const int32_t* pair_query, const int32_t* pair_subject,
-                         const int64_t* s_off, const int32_t* s_len, std::vector<int4>& rec) {
+void build_entry_records(const Tasks& t, const int32_t* pair_query, const int2* pair_sub,
+                         std::vector<int4>& rec) {
     rec.resize(t.index.size());
     const int32_t n_tasks = t.count();
     const int threads = t.index.size() >= ((size_t)1 << 20) ? host_threads() : 1;
@@ -411,13 +580,13 @@ void build_entry_records(const Tasks& t, const int32_t* pair_query, const int32_
             const int32_t* tq = t.query.data() + (size_t)kTaskQueries * k;
             for (int32_t e = t.begin[k]; e < t.begin[k + 1]; ++e) {
                 const int32_t p = t.index[e];
-                const int32_t sid = pair_subject[p], q = pair_query[p];
+                const int32_t q = pair_query[p];
                 int slot = 0;
                 for (int j = 1; j < kTaskQueries; ++j)
                     if (tq[j] == q) slot = j;
                 int4 r;
-                r.x = (int32_t)(uint32_t)(s_off[sid] >> 4);
-                r.y = s_len[sid];
+                r.x = pair_sub[p].x;
+                r.y = pair_sub[p].y;
                 r.z = t.next[e];
                 r.w = slot;
                 rec[e] = r;
@@ -469,6 +638,7 @@ struct tpz_sw_batch {
     std::vector<int64_t> s_off;                     // packed subject offsets (16-byte aligned)
     int64_t s_pad = 0;                              // offset of the padding after the last subject
     QueryPairs per_query;                           // pairs of each query, longest subject first
+    std::vector<int2> pair_sub;                     // per pair: subject offset / 16, length
     DeviceBuffer d_q_res, d_q_off, d_q_len, d_s_res, d_s_off, d_s_len;
     DeviceBuffer d_pair_query, d_pair_subject, d_matrix;
     DeviceBuffer d_score, d_ref, d_read;            // results
@@ -560,8 +730,7 @@ static int build_plan(tpz_sw_batch* b, const tpz_sw_config& c) {
     build_packed_tasks(b->per_query, b->q_len, c.rows_per_chunk,
                        c.queries_per_task, plan->packed_tasks.host);
     std::vector<int4> rec;
-    build_entry_records(plan->packed_tasks.host, b->pair_query.data(), b->pair_subject.data(),
-                        b->s_off.data(), b->s_len.data(), rec);
+    build_entry_records(plan->packed_tasks.host, b->pair_query.data(), b->pair_sub.data(), rec);
     int rc = plan->packed_tasks.upload(rec);
     const int W = c.waves_per_group;
     {
@@ -720,7 +889,8 @@ int tpz_sw_batch_create(tpz_sw_batch** out, int32_t n_queries, const int8_t* q_r
 
     rc |= b->d_error.alloc(64);
     if (!rc) rc |= check(hipMemsetAsync(b->d_error.ptr, 0, 64, stream()), "memset");
-    b->per_query = pairs_by_query(b->pair_query, b->pair_subject, b->s_len, n_queries);
+    b->pair_sub = pair_subjects(b->pair_subject.data(), n_pairs, b->s_off.data(), b->s_len.data());
+    b->per_query = pairs_by_query(b->pair_query, b->pair_sub.data(), n_queries);
     if (!textbook_gaps(gap_open, gap_extend) && n_pairs > 0) {
         b->lanes = true;
         int32_t max_q = 0;

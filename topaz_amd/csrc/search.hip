@@ -63,124 +63,52 @@ double now_s() {
 }
 
 // ---------------------------------------------------------------------------
-// host thread pool: parallel loops over queries
-// ---------------------------------------------------------------------------
-class Pool {
-public:
-    explicit Pool(int n) {
-        for (int k = 0; k < n; ++k) th_.emplace_back([this] { worker(); });
-    }
-    ~Pool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : th_) t.join();
-    }
-    // fn(i) for i in [0, n); the caller works too.  Returns when all are done.
-    void run(int64_t n, const std::function<void(int64_t)>& fn) {
-        if (n <= 0) return;
-        auto job = std::make_shared<Job>();
-        job->n = n;
-        job->fn = &fn;
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            jobs_.push_back(job);
-        }
-        cv_.notify_all();
-        work(*job);
-        std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [&] { return job->done.load() == job->n; });
-        jobs_.erase(std::remove(jobs_.begin(), jobs_.end(), job), jobs_.end());
-    }
-
-private:
-    struct Job {
-        int64_t n = 0;
-        const std::function<void(int64_t)>* fn = nullptr;
-        std::atomic<int64_t> next{0}, done{0};
-    };
-    std::vector<std::thread> th_;
-    std::mutex mu_;
-    std::condition_variable cv_, done_cv_;
-    std::vector<std::shared_ptr<Job>> jobs_;
-    bool stop_ = false;
-
-    void work(Job& j) {
-        int64_t i;
-        while ((i = j.next.fetch_add(1)) < j.n) {
-            (*j.fn)(i);
-            if (j.done.fetch_add(1) + 1 == j.n) {
-                std::lock_guard<std::mutex> lk(mu_);
-                done_cv_.notify_all();
-            }
-        }
-    }
-    void worker() {
-        for (;;) {
-            std::shared_ptr<Job> j;
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] {
-                    if (stop_) return true;
-                    for (auto& x : jobs_)
-                        if (x->next.load() < x->n) return true;
-                    return false;
-                });
-                if (stop_) return;
-                for (auto& x : jobs_)
-                    if (x->next.load() < x->n) { j = x; break; }
-            }
-            if (j) work(*j);
-        }
-    }
-};
-
-Pool& pool() {
-    static Pool* p = [] {
-        int n = (int)std::thread::hardware_concurrency();
-        if (const char* e = getenv("TOPAZ_AMD_HOST_THREADS")) n = atoi(e);
-        return new Pool(std::max(1, std::min(n, 256) - 1));
-    }();
-    return *p;
-}
-
-// ---------------------------------------------------------------------------
 // grow-only buffers of one search call
 // ---------------------------------------------------------------------------
+// Device buffers grow in the order of their slot's stream (hipMallocAsync /
+// hipFreeAsync): a plain hipFree waits for the whole device, so a search
+// growing a buffer would stall until another search's Smith-Waterman launch
+// ends.  Growth is by half again the size asked for, so a run of batches of
+// similar size settles after a few.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    hipStream_t st = nullptr;     // the owning slot's stream
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { if (p) (void)hipFree(p); }
     void* get(size_t n) {
         if (n > bytes) {
-            if (p) (void)hipFree(p);
+            if (p && check(hipFreeAsync(p, st), "hipFreeAsync")) return nullptr;
             p = nullptr;
             bytes = 0;
-            const size_t want = std::max(n + n / 4, (size_t)1 << 20);
-            if (check(hipMalloc(&p, want), "hipMalloc")) return nullptr;
+            const size_t want = std::max(n + n / 2, (size_t)1 << 20);
+            if (check(hipMallocAsync(&p, want, st), "hipMallocAsync")) return nullptr;
             bytes = want;
         }
         return p;
     }
 };
+// Pinned host buffers: no stream-ordered free exists, and hipHostFree also
+// waits for the device, so a grown-out buffer is kept until the slot goes.
 struct HostBuf {
     char* p = nullptr;
     size_t bytes = 0;
+    std::vector<char*> old;
     HostBuf() = default;
     HostBuf(const HostBuf&) = delete;
     HostBuf& operator=(const HostBuf&) = delete;
-    ~HostBuf() { if (p) (void)hipHostFree(p); }
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+        for (char* q : old) (void)hipHostFree(q);
+    }
     char* get(size_t n) {
         if (n > bytes) {
-            if (p) (void)hipHostFree(p);
+            if (p) old.push_back(p);
             p = nullptr;
             bytes = 0;
-            const size_t want = std::max(n + n / 4, (size_t)1 << 20);
+            const size_t want = std::max(n + n / 2, (size_t)1 << 20);
             if (check(hipHostMalloc((void**)&p, want, hipHostMallocDefault), "hipHostMalloc"))
                 return nullptr;
             bytes = want;
@@ -229,6 +157,14 @@ struct Slot {
     DevBuf d_oidx, d_oscore, d_on, d_ostatus, d_ostats;
     HostBuf h_stage, h_req_end, h_oidx, h_oscore;
     StatsMem stats;
+    // the stream every device buffer of the slot grows in
+    void bind() {
+        for (DevBuf* b : {&d_q, &d_qoff, &d_qlen, &d_rq, &d_ro, &d_idx, &d_qf, &d_qr, &d_qf_off,
+                          &d_qr_off, &d_defs, &d_req, &d_req_end, &d_round_ptrs, &d_jobs, &d_vals,
+                          &d_prot, &d_sbase, &d_slen, &d_sopen, &d_qstream, &d_qcap, &d_qmem,
+                          &d_qout, &d_mem, &d_oidx, &d_oscore, &d_on, &d_ostatus, &d_ostats})
+            b->st = stream;
+    }
     ~Slot() {
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -310,6 +246,7 @@ Slot* take_slot(tpz_db* db) {
     if (check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate"))
         return nullptr;
     s->stats.st = s->stream;
+    s->bind();
     return s.release();
 }
 
@@ -518,6 +455,7 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         if ((int)sl.round_vals.size() <= round) {
             sl.round_vals.emplace_back();
             sl.round_prot.emplace_back();
+            sl.round_vals.back().st = sl.round_prot.back().st = sl.stream;
         }
         int16_t* rv = (int16_t*)sl.round_vals[round].get((size_t)(n_req * c) * 2 + 64);
         uint32_t* rp = (uint32_t*)sl.round_prot[round].get((size_t)(n_req * c) * 4 + 64);
@@ -665,37 +603,54 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         }
         if (h_n[k] < 0 || h_n[k] > q_cap[k])
             return fail("tpz_search: the walk of query %d returned %d hits", k, h_n[k]);
-        qw[k].hit_index.assign(h_idx + q_out[k], h_idx + q_out[k] + h_n[k]);
-        qw[k].hit_gapless.assign(h_sc + q_out[k], h_sc + q_out[k] + h_n[k]);
         st.hits += h_n[k];
     }
+    parallel_for_each(b.n, host_threads(), [&](int64_t k) {
+        if (qw[k].need) return;
+        qw[k].hit_index.assign(h_idx + q_out[k], h_idx + q_out[k] + h_n[k]);
+        qw[k].hit_gapless.assign(h_sc + q_out[k], h_sc + q_out[k] + h_n[k]);
+    });
     const double t_walk = now_s();
     st.walk_s += t_walk - t_streams;
 
     // 4. forward Smith-Waterman of the hits with gapless score >= 40
-    std::vector<int8_t> q_res;
-    std::vector<int64_t> q_off(b.n);
-    std::vector<int32_t> pq, ps;
-    std::vector<int64_t> pair_of;          // per (query, hit) -> pair number or -1
-    std::vector<int64_t> hit_base(b.n + 1, 0);
+    // queries as matrix rows, and per query its hits with gapless score >= 40
+    // as pairs in hit order, on host threads
+    std::vector<int64_t> q_off(b.n + 1, 0);
+    std::vector<int64_t> hit_base(b.n + 1, 0), pair_base(b.n + 1, 0);
+    std::vector<int64_t> q_pairs(b.n, 0);
+    parallel_for_each(b.n, host_threads(), [&](int64_t k) {
+        const QueryWork& w = qw[k];
+        int64_t c = 0;
+        for (size_t h = 0; h < w.hit_index.size(); ++h) c += w.hit_gapless[h] >= 40;
+        q_pairs[k] = c;
+    });
     for (int32_t k = 0; k < b.n; ++k) {
-        q_off[k] = (int64_t)q_res.size();
+        q_off[k + 1] = q_off[k] + b.q_len[k];
+        hit_base[k + 1] = hit_base[k] + (int64_t)qw[k].hit_index.size();
+        pair_base[k + 1] = pair_base[k] + q_pairs[k];
+    }
+    const int64_t n_pairs = pair_base[b.n];
+    std::vector<int8_t> q_res((size_t)q_off[b.n]);
+    std::vector<int32_t> pq((size_t)n_pairs), ps((size_t)n_pairs);
+    std::vector<int64_t> pair_of((size_t)hit_base[b.n]);   // per (query, hit) -> pair number or -1
+    parallel_for_each(b.n, host_threads(), [&](int64_t k) {
         const char* q = b.q_chars + b.q_off[k];
         for (int32_t j = 0; j < b.q_len[k]; ++j)                     // ssw_wrapper.c:115-121
-            q_res.push_back(M.idx((unsigned char)M.upper[(unsigned char)q[j]]));
-        hit_base[k + 1] = hit_base[k] + (int64_t)qw[k].hit_index.size();
-    }
-    pair_of.assign((size_t)hit_base[b.n], -1);
-    for (int32_t k = 0; k < b.n; ++k) {
+            q_res[q_off[k] + j] = M.idx((unsigned char)M.upper[(unsigned char)q[j]]);
         const QueryWork& w = qw[k];
+        int64_t p = pair_base[k];
         for (size_t h = 0; h < w.hit_index.size(); ++h) {
-            if (w.hit_gapless[h] < 40) continue;
-            pair_of[hit_base[k] + h] = (int64_t)pq.size();
-            pq.push_back(k);
-            ps.push_back((int32_t)w.hit_index[h]);
+            if (w.hit_gapless[h] < 40) {
+                pair_of[hit_base[k] + h] = -1;
+                continue;
+            }
+            pair_of[hit_base[k] + h] = p;
+            pq[p] = (int32_t)k;
+            ps[p] = (int32_t)w.hit_index[h];
+            ++p;
         }
-    }
-    const int64_t n_pairs = (int64_t)pq.size();
+    });
     std::vector<int32_t> sc((size_t)n_pairs), re((size_t)n_pairs), rd((size_t)n_pairs);
     if (n_pairs > 0) {
         uint64_t ticket = 0;
@@ -714,7 +669,7 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
     const tpz_search_options& o = opt.o;
     std::vector<std::vector<int32_t>> stat_hits(b.n);   // reported hits needing statistics
     std::vector<int64_t> q_cells(b.n, 0);
-    pool().run(b.n, [&](int64_t kk) {
+    parallel_for_each(b.n, host_threads(), [&](int64_t kk) {
         const int32_t k = (int32_t)kk;
         const QueryWork& w = qw[k];
         const int32_t m = b.q_len[k];
@@ -875,7 +830,7 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     std::unique_ptr<int8_t[]> res(new int8_t[t_size]);
     std::vector<int64_t> off(n_proteins);
     for (uint64_t k = 0; k < n_proteins; ++k) off[k] = (int64_t)protein_start[k];
-    pool().run((int64_t)((t_size + (1 << 20) - 1) >> 20), [&](int64_t c) {
+    parallel_for_each((int64_t)((t_size + (1 << 20) - 1) >> 20), host_threads(), [&](int64_t c) {
         const uint64_t a = (uint64_t)c << 20, e = std::min<uint64_t>(t_size, a + (1 << 20));
         for (uint64_t i = a; i < e; ++i) res[i] = M.idx((unsigned char)text[i]);
     });

@@ -45,36 +45,51 @@ double now_ms() {
 
 size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// Device allocation that only grows.
+// Device allocation that only grows, in the order of its slot's stream
+// (hipMallocAsync / hipFreeAsync): a plain hipFree waits for the whole device,
+// which would hold a launch back until the other slot's launch ends.  Growth
+// is by half again the size asked for.
 struct GrowBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
+    hipStream_t st = nullptr;           // the slot's stream
     GrowBuffer() = default;
     GrowBuffer(const GrowBuffer&) = delete;
     GrowBuffer& operator=(const GrowBuffer&) = delete;
     ~GrowBuffer() { if (ptr) (void)hipFree(ptr); }
     int reserve(size_t n) {
         if (n <= bytes) return 0;
-        if (ptr) { (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
-        const size_t want = std::max(n, (size_t)1 << 20);
-        if (check(hipMalloc(&ptr, want), "hipMalloc")) return -1;
+        if (ptr) {
+            if (check(hipFreeAsync(ptr, st), "hipFreeAsync")) return -1;
+            ptr = nullptr;
+            bytes = 0;
+        }
+        const size_t want = std::max(n + n / 2, (size_t)1 << 20);
+        if (check(hipMallocAsync(&ptr, want, st), "hipMallocAsync")) return -1;
         bytes = want;
         return 0;
     }
 };
 
-// Pinned host allocation that only grows.
+// Pinned host allocation that only grows.  hipHostFree also waits for the
+// device, so a grown-out buffer is kept until the server goes.
 struct PinnedBuffer {
     char* ptr = nullptr;
     size_t bytes = 0;
+    std::vector<char*> old;
     PinnedBuffer() = default;
     PinnedBuffer(const PinnedBuffer&) = delete;
     PinnedBuffer& operator=(const PinnedBuffer&) = delete;
-    ~PinnedBuffer() { if (ptr) (void)hipHostFree(ptr); }
+    ~PinnedBuffer() {
+        if (ptr) (void)hipHostFree(ptr);
+        for (char* q : old) (void)hipHostFree(q);
+    }
     int reserve(size_t n) {
         if (n <= bytes) return 0;
-        if (ptr) { (void)hipHostFree(ptr); ptr = nullptr; bytes = 0; }
-        const size_t want = std::max(n, (size_t)1 << 20);
+        if (ptr) old.push_back(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        const size_t want = std::max(n + n / 2, (size_t)1 << 20);
         if (check(hipHostMalloc((void**)&ptr, want, hipHostMallocDefault), "hipHostMalloc"))
             return -1;
         bytes = want;
@@ -126,7 +141,7 @@ using namespace tpz;
 struct tpz_server {
     int32_t n_subjects = 0, max_subject = 0;
     std::vector<int32_t> s_len;
-    std::vector<int64_t> so;             // packed subject offsets (16-byte aligned)
+    std::vector<int2> s_meta;            // per subject: packed offset / 16, length
     int64_t s_pad = 0;                   // offset of the padding after the last subject
     DeviceBuffer d_s_res, d_s_off, d_s_len, d_matrix;
     int32_t gap_open = 11, gap_extend = 1, bias = 0;
@@ -483,13 +498,16 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
     for (int k = 0; k < s->cfg.max_inflight && !rc; ++k) {
         auto sl = std::make_unique<Slot>();
         rc |= check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");
+        sl->d_in.st = sl->d_out.st = sl->d_lane_ws.st = sl->stream;
         rc |= check(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming), "hipEventCreate");
         rc |= sl->d_ws.alloc((size_t)s->grid * s->ws_words_per_slot * 4);
         s->stats.workspace_bytes += (size_t)s->grid * s->ws_words_per_slot * 4;
         s->slots.push_back(std::move(sl));
     }
     rc |= check(hipStreamSynchronize(stream()), "upload sync");
-    s->so = std::move(so);
+    s->s_meta.resize((size_t)n_subjects);
+    for (int32_t k = 0; k < n_subjects; ++k)
+        s->s_meta[k] = make_int2((int32_t)(uint32_t)(so[k] >> 4), s_len[k]);
     if (rc) {
         for (auto& sl : s->slots) {
             if (sl->ev_done) (void)hipEventDestroy(sl->ev_done);
@@ -528,23 +546,38 @@ int tpz_server_submit(tpz_server* s, int32_t n_queries, const int8_t* q_res,
         r->q_off[k] = (int64_t)o;
         o += (size_t)q_len[k];
     }
-    for (int32_t p = 0; p < n_pairs; ++p)
-        if (pair_query[p] < 0 || pair_query[p] >= n_queries || pair_subject[p] < 0 ||
-            pair_subject[p] >= s->n_subjects)
-            return fail("pair %d references query %d / subject %d out of range", p,
-                        pair_query[p], pair_subject[p]);
-    r->pq.assign(pair_query, pair_query + n_pairs);
-    r->ps.assign(pair_subject, pair_subject + n_pairs);
+    // the pairs, on host threads: checked, copied, and each one's subject
+    // (offset / 16, length) gathered for the planning
+    r->pq.resize((size_t)n_pairs);
+    r->ps.resize((size_t)n_pairs);
+    std::vector<int2> pair_sub((size_t)n_pairs);
+    std::atomic<int64_t> bad{INT64_MAX};          // the lowest pair out of range
+    parallel_for(n_pairs, n_pairs >= (1 << 18) ? host_threads() : 1, [&](int64_t a, int64_t b) {
+        for (int64_t p = a; p < b; ++p) {
+            const int32_t q = pair_query[p], k = pair_subject[p];
+            if (q < 0 || q >= n_queries || k < 0 || k >= s->n_subjects) {
+                int64_t cur = bad.load();
+                while (p < cur && !bad.compare_exchange_weak(cur, p)) {}
+                return;
+            }
+            r->pq[(size_t)p] = q;
+            r->ps[(size_t)p] = k;
+            pair_sub[(size_t)p] = s->s_meta[(size_t)k];
+        }
+    });
+    if (bad.load() != INT64_MAX) {
+        const int64_t p = bad.load();
+        return fail("pair %d references query %d / subject %d out of range", (int)p,
+                    pair_query[p], pair_subject[p]);
+    }
     r->bytes = request_bytes(tot, (size_t)n_queries, (size_t)n_pairs);
     if (r->bytes > s->cfg.memory_budget)
         return fail("request needs %zu device bytes, more than the budget of %llu", r->bytes,
                     (unsigned long long)s->cfg.memory_budget);
     {
-        const QueryPairs per = pairs_by_query(r->pq, r->ps, s->s_len, n_queries);
+        const QueryPairs per = pairs_by_query(r->pq, pair_sub.data(), n_queries);
         build_packed_tasks(per, r->q_len, 32, kTaskQueries, r->tasks);
-        if (!s->lanes)
-            build_entry_records(r->tasks, r->pq.data(), r->ps.data(), s->so.data(), s->s_len.data(),
-                                r->rec);
+        if (!s->lanes) build_entry_records(r->tasks, r->pq.data(), pair_sub.data(), r->rec);
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return fail("server stopped");
