@@ -676,45 +676,65 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         const double correction = log(o.k * m * (double)o.db_size) / o.H;
         const double corrected_q = fmax(m - correction, 1.0);
         const double corrected_db = fmax((double)o.db_size - (correction * (double)o.db_proteins), 1.0);
-        std::vector<tpz_hit> all;
-        for (size_t h = 0; h < w.hit_index.size(); ++h) {
+        // each printable hit's bitscore and E-value; NULL alignments are
+        // neither printed nor counted
+        const size_t nh = w.hit_index.size();
+        thread_local std::vector<int32_t> cand;
+        thread_local std::vector<double> bits, ev;
+        cand.clear();
+        bits.resize(nh);
+        ev.resize(nh);
+        int64_t cells = 0;
+        for (size_t h = 0; h < nh; ++h) {
+            const int64_t p = pair_of[hit_base[k] + h];
+            if (p < 0) {
+                bits[h] = 0;
+                ev[h] = 1000;
+                cand.push_back((int32_t)h);
+                continue;
+            }
+            cells += (int64_t)m * db->len[ps[p]];
+            if (sc[p] >= 32767) continue;
+            const double bs = (o.lambda * sc[p] - log(o.k));
+            const double e = corrected_q * corrected_db * exp(-bs);
+            ev[h] = e > 1.0e-180 ? e : 0.0;
+            bits[h] = bs / M_LN2;
+            cand.push_back((int32_t)h);
+        }
+        q_cells[k] = cells;
+        // the first num_hits by bitscore, equal bitscores in hit order: what
+        // qsort with compare_bitscore (utils.c:140-152) keeps, since glibc
+        // sorts these pointer arrays by merging, which is stable
+        const size_t keep = std::min(cand.size(), (size_t)std::max<int64_t>(o.num_hits, 0));
+        std::partial_sort(cand.begin(), cand.begin() + keep, cand.end(), [&](int32_t x, int32_t y) {
+            return bits[x] > bits[y] || (bits[x] == bits[y] && x < y);
+        });
+        std::vector<tpz_hit>& res = out[k];
+        for (size_t i = 0; i < keep; ++i) {
+            const int32_t h = cand[i];
+            if (!(ev[h] <= o.evalue)) continue;
             tpz_hit x{};
             x.db_index = w.hit_index[h];
             x.gapless_score = w.hit_gapless[h];
             x.qstart = x.qend = x.sstart = x.send = -1;
             x.length = x.mismatch = x.gapopen = -1;
             const int64_t p = pair_of[hit_base[k] + h];
-            if (p >= 0) q_cells[k] += (int64_t)b.q_len[k] * db->len[ps[p]];
             if (p < 0) {
                 // not aligned (gapless score < 40): hit_alloc's fields
                 // (hit.c:15-38) but for bitscore and E-value
                 x.rawscore = -1;
                 x.identity = INFINITY;
-                x.bitscore = 0;
-                x.evalue = 1000;
-                all.push_back(x);
-                continue;
+            } else {
+                x.rawscore = sc[p];
+                x.send = re[p];
+                x.qend = rd[p];
+                x.length = x.gapopen = x.mismatch = 0;
+                x.identity = 0.0;
             }
-            if (sc[p] >= 32767) continue;             // NULL alignment: not aligned, not printed
-            x.rawscore = sc[p];
-            x.send = re[p];
-            x.qend = rd[p];
-            x.length = x.gapopen = x.mismatch = 0;
-            x.identity = 0.0;
-            x.bitscore = (o.lambda * x.rawscore - log(o.k));
-            x.evalue = corrected_q * corrected_db * exp(-x.bitscore);
-            x.evalue = x.evalue > 1.0e-180 ? x.evalue : 0.0;
-            x.bitscore /= M_LN2;
-            all.push_back(x);
+            x.bitscore = bits[h];
+            x.evalue = ev[h];
+            res.push_back(x);
         }
-        // qsort with compare_bitscore (utils.c:140-152): glibc sorts these
-        // pointer arrays by merging, which keeps equal bitscores in order
-        std::stable_sort(all.begin(), all.end(),
-                         [](const tpz_hit& a, const tpz_hit& c) { return a.bitscore > c.bitscore; });
-        if ((int64_t)all.size() > o.num_hits) all.resize((size_t)std::max(o.num_hits, 0));
-        std::vector<tpz_hit>& res = out[k];
-        for (const tpz_hit& x : all)
-            if (x.evalue <= o.evalue) res.push_back(x);
         if (!o.superfast)
             for (size_t i = 0; i < res.size(); ++i) stat_hits[k].push_back((int32_t)i);
     });
