@@ -652,92 +652,136 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         }
     });
     std::vector<int32_t> sc((size_t)n_pairs), re((size_t)n_pairs), rd((size_t)n_pairs);
-    if (n_pairs > 0) {
-        uint64_t ticket = 0;
-        double lat = 0;
-        if (tpz_server_submit(db->sw, b.n, q_res.data(), q_off.data(), b.q_len, (int32_t)n_pairs,
-                              pq.data(), ps.data(), &ticket) ||
-            tpz_server_wait(db->sw, ticket, sc.data(), re.data(), rd.data(), &lat))
+    // Stage S in parts of whole queries with about equal pair counts,
+    // submitted one after the other: the server launches the first part
+    // while this thread plans the next, and the output stage of a part
+    // (step 5) runs while the parts after it are on the device.
+    constexpr int kSwParts = 2;
+    std::vector<int32_t> q_cut(kSwParts + 1, b.n);
+    q_cut[0] = 0;
+    for (int part = 1; part < kSwParts; ++part)
+        q_cut[part] = (int32_t)(std::lower_bound(pair_base.begin(), pair_base.end(),
+                                                 n_pairs * part / kSwParts) - pair_base.begin());
+    for (int part = 1; part <= kSwParts; ++part) q_cut[part] = std::max(q_cut[part], q_cut[part - 1]);
+    std::vector<uint64_t> tickets(kSwParts, 0);
+    std::vector<uint8_t> submitted(kSwParts, 0);
+    auto wait_rest = [&](int from) {           // on an error: collect what was submitted
+        for (int part = from; part < kSwParts; ++part) {
+            if (!submitted[part]) continue;
+            const int64_t p0 = pair_base[q_cut[part]];
+            double lat = 0;
+            (void)tpz_server_wait(db->sw, tickets[part], sc.data() + p0, re.data() + p0,
+                                  rd.data() + p0, &lat);
+        }
+    };
+    for (int part = 0; part < kSwParts; ++part) {
+        const int32_t q0 = q_cut[part], q1 = q_cut[part + 1];
+        const int64_t p0 = pair_base[q0], np = pair_base[q1] - p0;
+        if (np == 0) continue;
+        std::vector<int64_t> part_off((size_t)(q1 - q0));
+        for (int32_t k = q0; k < q1; ++k) part_off[k - q0] = q_off[k] - q_off[q0];
+        std::vector<int32_t> part_q((size_t)np);
+        for (int64_t x = 0; x < np; ++x) part_q[x] = pq[p0 + x] - q0;
+        if (tpz_server_submit(db->sw, q1 - q0, q_res.data() + q_off[q0], part_off.data(),
+                              b.q_len + q0, (int32_t)np, part_q.data(), ps.data() + p0,
+                              &tickets[part])) {
+            wait_rest(0);
             return -1;
+        }
+        submitted[part] = 1;
     }
     st.pairs += n_pairs;
-    const double t_sw = now_s();
-    st.align_s += t_sw - t_walk;
 
     // 5. hits in output order (engine.c:193-234), E-values (ssw_wrapper.c:171-245),
-    // in parallel over queries
+    // in parallel over queries, each part's queries once its stage S is back
     const tpz_search_options& o = opt.o;
     std::vector<std::vector<int32_t>> stat_hits(b.n);   // reported hits needing statistics
     std::vector<int64_t> q_cells(b.n, 0);
-    parallel_for_each(b.n, host_threads(), [&](int64_t kk) {
-        const int32_t k = (int32_t)kk;
-        const QueryWork& w = qw[k];
-        const int32_t m = b.q_len[k];
-        const double correction = log(o.k * m * (double)o.db_size) / o.H;
-        const double corrected_q = fmax(m - correction, 1.0);
-        const double corrected_db = fmax((double)o.db_size - (correction * (double)o.db_proteins), 1.0);
-        // each printable hit's bitscore and E-value; NULL alignments are
-        // neither printed nor counted
-        const size_t nh = w.hit_index.size();
-        thread_local std::vector<int32_t> cand;
-        thread_local std::vector<double> bits, ev;
-        cand.clear();
-        bits.resize(nh);
-        ev.resize(nh);
-        int64_t cells = 0;
-        for (size_t h = 0; h < nh; ++h) {
-            const int64_t p = pair_of[hit_base[k] + h];
-            if (p < 0) {
-                bits[h] = 0;
-                ev[h] = 1000;
+    for (int part = 0; part < kSwParts; ++part) {
+        const int32_t q0 = q_cut[part], q1 = q_cut[part + 1];
+        if (submitted[part]) {
+            const int64_t p0 = pair_base[q0];
+            double lat = 0;
+            submitted[part] = 0;
+            if (tpz_server_wait(db->sw, tickets[part], sc.data() + p0, re.data() + p0, rd.data() + p0,
+                                &lat)) {
+                wait_rest(part + 1);
+                return -1;
+            }
+        }
+        parallel_for_each(q1 - q0, host_threads(), [&](int64_t kk) {
+            const int32_t k = q0 + (int32_t)kk;
+            const QueryWork& w = qw[k];
+            const int32_t m = b.q_len[k];
+            const double correction = log(o.k * m * (double)o.db_size) / o.H;
+            const double corrected_q = fmax(m - correction, 1.0);
+            const double corrected_db =
+                fmax((double)o.db_size - (correction * (double)o.db_proteins), 1.0);
+            // each printable hit's bitscore and E-value; NULL alignments are
+            // neither printed nor counted
+            const size_t nh = w.hit_index.size();
+            thread_local std::vector<int32_t> cand;
+            thread_local std::vector<double> bits, ev;
+            cand.clear();
+            bits.resize(nh);
+            ev.resize(nh);
+            int64_t cells = 0;
+            for (size_t h = 0; h < nh; ++h) {
+                const int64_t p = pair_of[hit_base[k] + h];
+                if (p < 0) {
+                    bits[h] = 0;
+                    ev[h] = 1000;
+                    cand.push_back((int32_t)h);
+                    continue;
+                }
+                cells += (int64_t)m * db->len[ps[p]];
+                if (sc[p] >= 32767) continue;
+                const double bs = (o.lambda * sc[p] - log(o.k));
+                const double e = corrected_q * corrected_db * exp(-bs);
+                ev[h] = e > 1.0e-180 ? e : 0.0;
+                bits[h] = bs / M_LN2;
                 cand.push_back((int32_t)h);
-                continue;
             }
-            cells += (int64_t)m * db->len[ps[p]];
-            if (sc[p] >= 32767) continue;
-            const double bs = (o.lambda * sc[p] - log(o.k));
-            const double e = corrected_q * corrected_db * exp(-bs);
-            ev[h] = e > 1.0e-180 ? e : 0.0;
-            bits[h] = bs / M_LN2;
-            cand.push_back((int32_t)h);
-        }
-        q_cells[k] = cells;
-        // the first num_hits by bitscore, equal bitscores in hit order: what
-        // qsort with compare_bitscore (utils.c:140-152) keeps, since glibc
-        // sorts these pointer arrays by merging, which is stable
-        const size_t keep = std::min(cand.size(), (size_t)std::max<int64_t>(o.num_hits, 0));
-        std::partial_sort(cand.begin(), cand.begin() + keep, cand.end(), [&](int32_t x, int32_t y) {
-            return bits[x] > bits[y] || (bits[x] == bits[y] && x < y);
+            q_cells[k] = cells;
+            // the first num_hits by bitscore, equal bitscores in hit order: what
+            // qsort with compare_bitscore (utils.c:140-152) keeps, since glibc
+            // sorts these pointer arrays by merging, which is stable
+            const size_t keep = std::min(cand.size(), (size_t)std::max<int64_t>(o.num_hits, 0));
+            std::partial_sort(cand.begin(), cand.begin() + keep, cand.end(), [&](int32_t x, int32_t y) {
+                return bits[x] > bits[y] || (bits[x] == bits[y] && x < y);
+            });
+            std::vector<tpz_hit>& res = out[k];
+            for (size_t i = 0; i < keep; ++i) {
+                const int32_t h = cand[i];
+                if (!(ev[h] <= o.evalue)) continue;
+                tpz_hit x{};
+                x.db_index = w.hit_index[h];
+                x.gapless_score = w.hit_gapless[h];
+                x.qstart = x.qend = x.sstart = x.send = -1;
+                x.length = x.mismatch = x.gapopen = -1;
+                const int64_t p = pair_of[hit_base[k] + h];
+                if (p < 0) {
+                    // not aligned (gapless score < 40): hit_alloc's fields
+                    // (hit.c:15-38) but for bitscore and E-value
+                    x.rawscore = -1;
+                    x.identity = INFINITY;
+                } else {
+                    x.rawscore = sc[p];
+                    x.send = re[p];
+                    x.qend = rd[p];
+                    x.length = x.gapopen = x.mismatch = 0;
+                    x.identity = 0.0;
+                }
+                x.bitscore = bits[h];
+                x.evalue = ev[h];
+                res.push_back(x);
+            }
+            if (!o.superfast)
+                for (size_t i = 0; i < res.size(); ++i) stat_hits[k].push_back((int32_t)i);
         });
-        std::vector<tpz_hit>& res = out[k];
-        for (size_t i = 0; i < keep; ++i) {
-            const int32_t h = cand[i];
-            if (!(ev[h] <= o.evalue)) continue;
-            tpz_hit x{};
-            x.db_index = w.hit_index[h];
-            x.gapless_score = w.hit_gapless[h];
-            x.qstart = x.qend = x.sstart = x.send = -1;
-            x.length = x.mismatch = x.gapopen = -1;
-            const int64_t p = pair_of[hit_base[k] + h];
-            if (p < 0) {
-                // not aligned (gapless score < 40): hit_alloc's fields
-                // (hit.c:15-38) but for bitscore and E-value
-                x.rawscore = -1;
-                x.identity = INFINITY;
-            } else {
-                x.rawscore = sc[p];
-                x.send = re[p];
-                x.qend = rd[p];
-                x.length = x.gapopen = x.mismatch = 0;
-                x.identity = 0.0;
-            }
-            x.bitscore = bits[h];
-            x.evalue = ev[h];
-            res.push_back(x);
-        }
-        if (!o.superfast)
-            for (size_t i = 0; i < res.size(); ++i) stat_hits[k].push_back((int32_t)i);
-    });
+    }
+    const double t_sw = now_s();
+    st.align_s += t_sw - t_walk;
     for (int32_t k = 0; k < b.n; ++k) st.cells += q_cells[k];
 
     // 6. traceback statistics of the reported hits (query.c:250-299)
