@@ -280,9 +280,9 @@ typedef struct {
     int32_t number_of_seeds;       /* -S */
     int32_t superfast;             /* 1: no traceback statistics (no -B) */
     /* GPU stream expansion (no reference counterpart; results do not depend
-     * on them): slots of every stream in the first round (doubling in the
-     * next), and the elements a stream is expanded to before the walk asks
-     * for more on demand */
+     * on them): slots of every stream in the first round (each later round
+     * adds half the slots held, at least this many), and the elements a
+     * stream is expanded to before the walk asks for more on demand */
     int32_t first_chunk;
     int64_t max_stream_elements;
 } tpz_search_options;

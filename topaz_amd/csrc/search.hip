@@ -501,7 +501,10 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         }
         open_list.swap(next);
         have += c;
-        chunk = (int32_t)std::min<int64_t>((int64_t)chunk * 2, 1 << 20);
+        // each round adds half the slots the open streams hold: a stream that
+        // ends inside a round wastes less of it than with doubling
+        chunk = (int32_t)std::min<int64_t>(
+            std::max<int64_t>(std::max(1, opt.o.first_chunk), have / 2), 1 << 20);
     }
 
     // each stream's seeds back to back, in stream order
