@@ -502,9 +502,11 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         open_list.swap(next);
         have += c;
         // each round adds half the slots the open streams hold: a stream that
-        // ends inside a round wastes less of it than with doubling
-        chunk = (int32_t)std::min<int64_t>(
-            std::max<int64_t>(std::max(1, opt.o.first_chunk), have / 2), 1 << 20);
+        // ends inside a round wastes less of it than with doubling; above 64,
+        // whole wavefronts of slots
+        int64_t next_chunk = std::max<int64_t>(std::max(1, opt.o.first_chunk), have / 2);
+        if (next_chunk > 64) next_chunk = (next_chunk + 63) & ~(int64_t)63;
+        chunk = (int32_t)std::min<int64_t>(next_chunk, 1 << 20);
     }
 
     // each stream's seeds back to back, in stream order
