@@ -660,8 +660,10 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
     // Stage S in parts of whole queries with about equal pair counts,
     // submitted one after the other: the server launches the first part
     // while this thread plans the next, and the output stage of a part
-    // (step 5) runs while the parts after it are on the device.
-    constexpr int kSwParts = 2;
+    // (step 5) runs while the parts after it are on the device.  A batch of
+    // fewer than 1 M pairs goes in one part: its launches are short, and a
+    // second one would add its own tail.
+    const int kSwParts = n_pairs >= (1 << 20) ? 2 : 1;
     std::vector<int32_t> q_cut(kSwParts + 1, b.n);
     q_cut[0] = 0;
     for (int part = 1; part < kSwParts; ++part)
