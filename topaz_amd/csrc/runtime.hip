@@ -974,7 +974,10 @@ static int enqueue_schedule(tpz_sw_batch* b, int32_t schedule) {
             a.item_task = (const int32_t*)pl.d_items_persistent.ptr;
             a.n_items = n_items;
             a.task_counter = (int32_t*)b->d_counter.ptr;
+            // the counter, the error flag (a run reports only its own timed-out
+            // waits) and the keys start at zero in every run, replays included
             if (check(tpz_launch_fill_i32(a.task_counter, 0, 1, stream()), "clear counter") ||
+                check(tpz_launch_fill_i32(a.error, 0, 1, stream()), "clear error flag") ||
                 check(tpz_launch_fill_i32((int32_t*)a.out_key, 0, 2 * b->n_pairs, stream()),
                       "clear keys"))
                 return -1;
