@@ -210,12 +210,16 @@ gapless_kernel(GaplessArgs a) {
     }
 }
 
-// One round of walk-stream extension: 64 consecutive slots of one request
-// per wavefront iteration (request g / spr, part g % spr, spr = the parts of
-// a chunk), lane l on slot first + (have + part * 64 + l) * dir; then the
-// same compaction and two-lane scans as gapless_kernel.  Each lane's slot
-// is marked in LDS when its element is -1; the wavefront lowers its
-// request's req_end to the first marked slot.
+// One round of walk-stream extension.  A unit is up to `width` consecutive
+// slots of one request: with a chunk wider than 32 slots width is 64 and a
+// wavefront iteration serves one unit (request u / spr, part u % spr, spr =
+// the parts of a chunk); with a narrower chunk width is the chunk rounded up
+// to a power of two and a wavefront serves 64 / width units at once, so the
+// first rounds (32 slots per stream) keep every lane busy.  Lane l takes
+// slot l % width of its unit: first + (have + part * 64 + l % width) * dir;
+// then the same compaction and two-lane scans as gapless_kernel.  Each
+// lane's slot is marked in LDS when its element is -1; each unit's first
+// lane lowers its request's req_end to the unit's first marked slot.
 __global__ void __launch_bounds__(kWaves * 64)
 stream_round_kernel(StreamRoundArgs sa) {
     __shared__ GaplessLds lds;
@@ -227,19 +231,33 @@ stream_round_kernel(StreamRoundArgs sa) {
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     Slot* slots = lds.slot[wave];
     uint8_t* ends = is_end[wave];
-    const int64_t spr = (sa.chunk + 63) / 64;
-    const int64_t n_waves = sa.n_req * spr;
+    int width = 64;
+    if (sa.chunk <= 32)
+        for (width = 1; width < sa.chunk; width <<= 1) {}
+    const int units_per_wave = 64 / width;
+    const int lane_in = lane & (width - 1);
+    const uint64_t unit_mask = width == 64 ? ~0ull : (1ull << width) - 1;
+    const int64_t spr = width == 64 ? (sa.chunk + 63) / 64 : 1;
+    const int64_t n_units = sa.n_req * spr;
+    const int64_t n_iter = (n_units + units_per_wave - 1) / units_per_wave;
     const int64_t waves_total = (int64_t)gridDim.x * kWaves;
-    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < n_waves; g += waves_total) {
-        const int64_t req = g / spr;
-        const int32_t part = (int32_t)(g - req * spr);
-        const StreamDef d = sa.defs[sa.req_stream[req]];
-        const int32_t count = min(64, sa.chunk - part * 64);
-        const int64_t out0 = req * sa.chunk + (int64_t)part * 64;
+    for (int64_t g = (int64_t)blockIdx.x * kWaves + wave; g < n_iter; g += waves_total) {
+        const int64_t unit = g * units_per_wave + lane / width;
+        const bool live = unit < n_units;
+        int64_t req = 0, out0 = 0;
+        int32_t part = 0, count = 0;
+        StreamDef d{};
+        if (live) {
+            req = unit / spr;
+            part = (int32_t)(unit - req * spr);
+            d = sa.defs[sa.req_stream[req]];
+            count = min(width, sa.chunk - part * 64);
+            out0 = req * sa.chunk + (int64_t)part * 64;
+        }
         bool pass = false;
         int64_t idx = 0;
-        if (lane < count) {
-            const int64_t slot = d.first + (sa.have + (int64_t)part * 64 + lane) * d.dir;
+        if (lane_in < count) {
+            const int64_t slot = d.first + (sa.have + (int64_t)part * 64 + lane_in) * d.dir;
             uint32_t prot = 0xffffffffu;
             if (slot >= 0 && slot < a.t_size) {
                 idx = a.sa[slot];
@@ -254,8 +272,8 @@ stream_round_kernel(StreamRoundArgs sa) {
                         pass = 2 * (int)a.t_fwd[idx] == (int)a.q_fwd[a.qf_off[d.query] + d.offset];
                 }
             }
-            sa.out_protein[out0 + lane] = prot;
-            if (!pass) sa.out[out0 + lane] = -1;
+            sa.out_protein[out0 + lane_in] = prot;
+            if (!pass) sa.out[out0 + lane_in] = -1;
         }
         ends[lane] = !pass;
         const uint64_t mask = __ballot(pass);
@@ -263,7 +281,8 @@ stream_round_kernel(StreamRoundArgs sa) {
         if (pass) {
             const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-            slots[rank] = Slot{idx, out0 + lane, d.offset, d.query};
+            // the output element, with the lane that owns it in the low 6 bits
+            slots[rank] = Slot{idx, ((out0 + lane_in) << 6) | lane, d.offset, d.query};
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -271,15 +290,17 @@ stream_round_kernel(StreamRoundArgs sa) {
         const int min_len = sa.min_len;
         extend_passing(a, lds, slots, n_pass, lane, [&](int64_t s_id, int score, int length) {
             const bool seed = length >= min_len;
-            sa.out[s_id] = (int16_t)(seed ? score : -1);
-            ends[s_id - out0] = !seed;
+            sa.out[s_id >> 6] = (int16_t)(seed ? score : -1);
+            ends[s_id & 63] = !seed;
         });
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint64_t end_mask = __ballot(lane < count && ends[lane]);
-        if (end_mask && lane == 0)
-            atomicMin(sa.req_end + req, part * 64 + (int32_t)__builtin_ctzll(end_mask));
+        const uint64_t end_mask = __ballot(lane_in < count && ends[lane]);
+        if (live && lane_in == 0) {
+            const uint64_t mine = (end_mask >> lane) & unit_mask;
+            if (mine) atomicMin(sa.req_end + req, part * 64 + (int32_t)__builtin_ctzll(mine));
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
