@@ -75,6 +75,7 @@ struct LowerBoundArgs {
     const char* text;           // db->T
     int64_t t_size;
     const int64_t* sa;          // db->SA
+    const uint32_t* sa32;       // when set, db->SA as uint32, read instead of sa
     const char* q_chars;        // the queries' bytes, back to back
     const int64_t* q_off;
     const int32_t* q_len;
@@ -116,6 +117,8 @@ struct StreamRoundArgs {
     int32_t* req_end;
     const uint32_t* prot_block;
     const uint64_t* prot_end;
+    const uint32_t* sa32;       // when set, the suffix array as uint32 (a text under 4 GiB),
+                                // read instead of g.sa
 };
 
 }  // namespace tpz

@@ -260,7 +260,7 @@ stream_round_kernel(StreamRoundArgs sa) {
             const int64_t slot = d.first + (sa.have + (int64_t)part * 64 + lane_in) * d.dir;
             uint32_t prot = 0xffffffffu;
             if (slot >= 0 && slot < a.t_size) {
-                idx = a.sa[slot];
+                idx = sa.sa32 ? (int64_t)sa.sa32[slot] : a.sa[slot];
                 if (idx >= 0 && idx < a.t_size) {
                     // the protein holding idx: the one holding idx & ~63 or a
                     // later one (the last protein's newline is the text's end)
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) lower_bound_kernel(LowerBoundArgs a) {
     while (lo < hi) {
         const int64_t mid = lo + ((hi - lo) >> 1);
         int64_t j = ml < mr ? ml : mr;
-        int64_t i = a.sa[mid] + j;
+        int64_t i = (a.sa32 ? (int64_t)a.sa32[mid] : a.sa[mid]) + j;
         int r = 0;
         while (i < n && j < m) {
             r = (int)T[i] - (int)P[j];

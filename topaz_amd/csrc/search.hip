@@ -187,6 +187,7 @@ struct tpz_db {
     std::vector<int16_t> table;          // gapless scores per (text code, query code)
     DeviceBuffer d_table;
     DeviceBuffer d_pblock, d_pend;       // the protein index on the device (stream_kernel)
+    DeviceBuffer d_sa32;                 // the suffix array as uint32 (texts under 4 GiB)
     tpz_server* sw = nullptr;            // stage S over the proteins
     int8_t matrix[625];
     int32_t gap_open = 11, gap_extend = 1, max_w = 1;
@@ -315,7 +316,7 @@ GaplessArgs stream_gapless_args(tpz_db* db, Slot& sl, int32_t n_queries, const i
     a.t_fwd = (const uint8_t*)db->dtext->d_fwd.ptr;
     a.t_head = (const uint8_t*)db->dtext->d_head.ptr;
     a.t_size = (int64_t)db->t_size;
-    a.sa = (const int64_t*)db->dtext->d_sa.ptr;
+    a.sa = db->dtext->has_sa ? (const int64_t*)db->dtext->d_sa.ptr : nullptr;
     a.q_fwd = (const uint8_t*)sl.d_qf.p;
     a.qf_off = (const int64_t*)sl.d_qf_off.p;
     a.q_rev = (const uint8_t*)sl.d_qr.p;
@@ -388,7 +389,8 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         LowerBoundArgs a{};
         a.text = (const char*)db->d_raw.ptr;
         a.t_size = (int64_t)db->t_size;
-        a.sa = (const int64_t*)db->dtext->d_sa.ptr;
+        a.sa = db->dtext->has_sa ? (const int64_t*)db->dtext->d_sa.ptr : nullptr;
+        a.sa32 = (const uint32_t*)db->d_sa32.ptr;
         a.q_chars = (const char*)sl.d_q.p;
         a.q_off = (const int64_t*)sl.d_qoff.p;
         a.q_len = (const int32_t*)sl.d_qlen.p;
@@ -478,6 +480,7 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
         a.req_end = d_end;
         a.prot_block = (const uint32_t*)db->d_pblock.ptr;
         a.prot_end = (const uint64_t*)db->d_pend.ptr;
+        a.sa32 = (const uint32_t*)db->d_sa32.ptr;
         const int64_t waves = n_req * ((c + 63) / 64);
         const int grid = (int)std::max<int64_t>(
             1, std::min<int64_t>((waves + 3) / 4, (int64_t)std::max(num_cu(), 1) * 32));
@@ -887,7 +890,22 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     db->gap_open = gap_open;
     db->gap_extend = gap_extend;
     for (int i = 0; i < 625; ++i) db->max_w = std::max<int32_t>(db->max_w, matrix[i]);
-    if (tpz_text_create(&db->dtext, text, t_size, sa)) return -1;
+    // a text under 4 GiB keeps its suffix array as uint32 on the device:
+    // half the upload (the largest array) and half the device memory
+    const bool sa32 = t_size <= 0xffffffffull;
+    if (tpz_text_create(&db->dtext, text, t_size, sa32 ? nullptr : sa)) return -1;
+    if (sa32 && (db->d_sa32.alloc((size_t)t_size * 4) ||
+                 stage_to_device(db->d_sa32.ptr, (size_t)t_size * 4,
+                                 [&](uint8_t* out, size_t b, size_t e) {
+                                     uint32_t* o = reinterpret_cast<uint32_t*>(out);
+                                     for (size_t k = b / 4; k < e / 4; ++k) {
+                                         const int64_t v = sa[k];
+                                         // an entry outside the text: no suffix
+                                         o[k - b / 4] = v >= 0 && (uint64_t)v < t_size
+                                                            ? (uint32_t)v : 0xffffffffu;
+                                     }
+                                 })))
+        return -1;
     if (db->d_pblock.upload(db->proteins.blocks().data(), db->proteins.blocks().size() * 4) ||
         db->d_pend.upload(db->proteins.ends().data(), db->proteins.ends().size() * 8))
         return -1;
