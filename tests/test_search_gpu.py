@@ -132,6 +132,21 @@ def test_offline_search_with_seg(db_and_queries):
         assert ours == ref, blasttab
 
 
+def test_int64_suffix_array_gives_the_same_hits(db_and_queries, monkeypatch):
+    """A text under 4 GiB keeps its suffix array as uint32 on the device;
+    TOPAZ_AMD_SA64 keeps the int64 form that larger texts use.  Both give
+    the same hits, default and -B."""
+    tmp, db, qs = db_and_queries
+    monkeypatch.setenv("TOPAZ_AMD_SA64", "1")
+    db64 = Database.from_index(tmp / "DB", matrix())
+    try:
+        for superfast in (1, 0):
+            opt = db.default_options(superfast=superfast)
+            assert db64.search(qs[:60], opt)[0] == db.search(qs[:60], opt)[0]
+    finally:
+        db64.close()
+
+
 def test_streams_cut_at_the_cap_are_searched_again(db_and_queries):
     """A walk that reaches the end of a stream cut at max_stream_elements
     searches its query again with a higher cap: the hits are those of an

@@ -892,7 +892,8 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     for (int i = 0; i < 625; ++i) db->max_w = std::max<int32_t>(db->max_w, matrix[i]);
     // a text under 4 GiB keeps its suffix array as uint32 on the device:
     // half the upload (the largest array) and half the device memory
-    const bool sa32 = t_size <= 0xffffffffull;
+    // (TOPAZ_AMD_SA64 keeps int64 at any size, the form of texts of 4 GiB and up)
+    const bool sa32 = t_size <= 0xffffffffull && !getenv("TOPAZ_AMD_SA64");
     if (tpz_text_create(&db->dtext, text, t_size, sa32 ? nullptr : sa)) return -1;
     if (sa32 && (db->d_sa32.alloc((size_t)t_size * 4) ||
                  stage_to_device(db->d_sa32.ptr, (size_t)t_size * 4,
