@@ -617,8 +617,10 @@ def search_leg(args):
         # others are on the host (config 4: 23.7 s with 8 workers, 24.6 s
         # with 4; batches of 4,096 were slower); with fewer queries than 8
         # full batches, 4 workers, whose larger batches pay the per-batch
-        # costs fewer times (1,000 queries: 0.40 s with 4, 0.52 s with 8)
-        gpu_workers = 8 if n_q >= 8 * 2048 else 4
+        # costs fewer times (1,000 queries: 0.40 s with 4, 0.52 s with 8),
+        # and one batch on one worker when all queries fit one (1,000
+        # queries: 0.26-0.31 s)
+        gpu_workers = 8 if n_q >= 8 * 2048 else 4 if n_q > 2048 else 1
         batch = str(max(1, min(2048, (n_q + gpu_workers - 1) // gpu_workers)))
         gpu_env = {"TOPAZ_AMD_SEARCH_STATS": "1", "TOPAZ_AMD_SEARCH_BATCH": batch}
         plain_n = min(args.plain_sample, sample)
