@@ -666,17 +666,17 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
     // (step 5) runs while the parts after it are on the device.  A batch of
     // fewer than 1 M pairs goes in one part: its launches are short, and a
     // second one would add its own tail.
-    const int kSwParts = n_pairs >= (1 << 20) ? 2 : 1;
-    std::vector<int32_t> q_cut(kSwParts + 1, b.n);
+    const int sw_parts = n_pairs >= (1 << 20) ? 2 : 1;
+    std::vector<int32_t> q_cut(sw_parts + 1, b.n);
     q_cut[0] = 0;
-    for (int part = 1; part < kSwParts; ++part)
+    for (int part = 1; part < sw_parts; ++part)
         q_cut[part] = (int32_t)(std::lower_bound(pair_base.begin(), pair_base.end(),
-                                                 n_pairs * part / kSwParts) - pair_base.begin());
-    for (int part = 1; part <= kSwParts; ++part) q_cut[part] = std::max(q_cut[part], q_cut[part - 1]);
-    std::vector<uint64_t> tickets(kSwParts, 0);
-    std::vector<uint8_t> submitted(kSwParts, 0);
+                                                 n_pairs * part / sw_parts) - pair_base.begin());
+    for (int part = 1; part <= sw_parts; ++part) q_cut[part] = std::max(q_cut[part], q_cut[part - 1]);
+    std::vector<uint64_t> tickets(sw_parts, 0);
+    std::vector<uint8_t> submitted(sw_parts, 0);
     auto wait_rest = [&](int from) {           // on an error: collect what was submitted
-        for (int part = from; part < kSwParts; ++part) {
+        for (int part = from; part < sw_parts; ++part) {
             if (!submitted[part]) continue;
             const int64_t p0 = pair_base[q_cut[part]];
             double lat = 0;
@@ -684,7 +684,7 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
                                   rd.data() + p0, &lat);
         }
     };
-    for (int part = 0; part < kSwParts; ++part) {
+    for (int part = 0; part < sw_parts; ++part) {
         const int32_t q0 = q_cut[part], q1 = q_cut[part + 1];
         const int64_t p0 = pair_base[q0], np = pair_base[q1] - p0;
         if (np == 0) continue;
@@ -707,7 +707,7 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
     const tpz_search_options& o = opt.o;
     std::vector<std::vector<int32_t>> stat_hits(b.n);   // reported hits needing statistics
     std::vector<int64_t> q_cells(b.n, 0);
-    for (int part = 0; part < kSwParts; ++part) {
+    for (int part = 0; part < sw_parts; ++part) {
         const int32_t q0 = q_cut[part], q1 = q_cut[part + 1];
         if (submitted[part]) {
             const int64_t p0 = pair_base[q0];
