@@ -11,6 +11,8 @@
 #include <functional>
 #include <vector>
 
+struct tpz_server;
+
 namespace tpz {
 
 int fail(const char* fmt, ...);          // set the error string, return -1
@@ -130,6 +132,9 @@ void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int6
 // fn(i) for every i in [0, n) on up to `threads` host threads (the caller's
 // included), each taking the next index from a shared counter.
 void parallel_for_each(int64_t n, int threads, const std::function<void(int64_t)>& fn);
+// Launches of the server's forward kernel with fewer than `pairs` pairs run
+// 8 wavefronts per workgroup (0: every launch runs the configured width).
+void server_small_launches(::tpz_server* s, int64_t pairs);
 // Host threads the library uses for its parallel loops (TOPAZ_AMD_HOST_THREADS,
 // else the machine's, at most 64).
 int host_threads();

@@ -933,6 +933,10 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     if (tpz_server_create(&db->sw, (int32_t)n_proteins, res.get(), off.data(), protein_len,
                           matrix, gap_open, gap_extend, &cfg))
         return -1;
+    // a launch of under 2 M pairs (a served request alone) is as long as its
+    // longest task: 8 wavefronts per staircase halve that; merged batches
+    // keep the configured width, which gives the throughput
+    server_small_launches(db->sw, (int64_t)2 << 20);
     if (check(hipStreamSynchronize(stream()), "database upload")) return -1;
     *out = db.release();
     return 0;

@@ -148,8 +148,10 @@ struct tpz_server {
     bool lanes = false;                  // gap_open <= gap_extend: the lane kernel
     tpz_server_config cfg{};
     int device = 0;
-    int grid = 0;                        // workgroups of one launch at most
-    int64_t ws_words_per_slot = 0;
+    // launches of fewer pairs run 8 wavefronts per workgroup, whose
+    // staircases halve the longest task's time (a small launch is as long as
+    // that); 0: every launch runs cfg.waves_per_group
+    int64_t small_launch_pairs = 0;
     std::mutex mu;
     std::condition_variable cv_sched, cv_done;
     std::deque<Request*> pending;
@@ -164,6 +166,17 @@ struct tpz_server {
 
 namespace tpz {
 namespace {
+
+// Workgroups of one launch at most with W wavefronts each, and the
+// workspace words of one workgroup slot (the lane kernel's grows with the
+// launch's longest query, in the slot).
+int grid_for(int W) {
+    return std::max(num_cu(), 1) *
+           std::max(1, std::min(kGroupsPerCu, tpz_sw_forward_groups_per_cu(32, W)));
+}
+int64_t ws_words_for(const tpz_server* s, int W) {
+    return s->lanes ? 0 : (int64_t)W * 2 * (item_columns(s->max_subject) + 4) * 64;
+}
 
 // Merge the slot's requests, upload, launch, and queue the result copy.  The
 // requests' task lists and entry records were built at submit (in the
@@ -190,8 +203,10 @@ int launch_slot(tpz_server* s, Slot& sl) {
     for (const Request* r : sl.reqs) costs.cost.insert(costs.cost.end(), r->tasks.cost.begin(), r->tasks.cost.end());
     std::vector<int32_t> items;
     int32_t n_big = 0;
-    const int W = s->cfg.waves_per_group;
-    build_items(costs, s->grid, 1.0, W, items, n_big);
+    const int W = s->small_launch_pairs > 0 && (int64_t)np < s->small_launch_pairs
+                      ? 8 : s->cfg.waves_per_group;
+    const int grid_w = grid_for(W);
+    build_items(costs, grid_w, 1.0, W, items, n_big);
     const int n_items = (int)(items.size() / W);
     const size_t n_rec = s->lanes ? 0 : np;
 
@@ -296,7 +311,7 @@ int launch_slot(tpz_server* s, Slot& sl) {
         a.gap_open = s->gap_open;
         a.gap_extend = s->gap_extend;
         a.workspace = (uint32_t*)sl.d_ws.ptr;
-        a.ws_words_per_slot = s->ws_words_per_slot;
+        a.ws_words_per_slot = ws_words_for(s, W);
         a.out_score = dout;
         a.out_ref = dout + np;
         a.out_read = dout + 2 * np;
@@ -313,7 +328,7 @@ int launch_slot(tpz_server* s, Slot& sl) {
             la.n_list = (int64_t)np;
             if (check(tpz_launch_sw_lane(&la, blocks, sl.stream), "sw_lane")) return -1;
         } else {
-            const int grid = std::min(n_items, s->grid);
+            const int grid = std::min(n_items, grid_w);
             if (check(tpz_launch_sw_forward(&a, grid, 32, W, sl.stream), "sw_forward") ||
                 check(tpz_launch_sw_decode(&a, (int32_t)np, sl.stream), "sw_decode"))
                 return -1;
@@ -422,6 +437,11 @@ void scheduler(tpz_server* s) {
 }
 
 }  // namespace
+
+void server_small_launches(tpz_server* s, int64_t pairs) {
+    if (s) s->small_launch_pairs = pairs;
+}
+
 }  // namespace tpz
 
 extern "C" {
@@ -490,18 +510,17 @@ int tpz_server_create(tpz_server** out, int32_t n_subjects, const int8_t* s_res,
     s->s_pad = stot;
     rc |= s->d_s_off.upload(so.data(), so.size() * 8) |
           s->d_s_len.upload(s_len, (size_t)n_subjects * 4) | s->d_matrix.upload(matrix, 625);
-    s->grid = std::max(num_cu(), 1) *
-              std::max(1, std::min(kGroupsPerCu, tpz_sw_forward_groups_per_cu(32, s->cfg.waves_per_group)));
-    // the forward kernel's workspace (the lane kernel's grows with the
-    // launch's longest query, in the slot)
-    s->ws_words_per_slot = s->lanes ? 0 : (int64_t)s->cfg.waves_per_group * 2 * (item_columns(s->max_subject) + 4) * 64;
+    // the forward kernel's workspace, room for a launch of either width
+    const size_t ws_bytes = 4 * (size_t)std::max(
+        grid_for(s->cfg.waves_per_group) * ws_words_for(s.get(), s->cfg.waves_per_group),
+        grid_for(8) * ws_words_for(s.get(), 8));
     for (int k = 0; k < s->cfg.max_inflight && !rc; ++k) {
         auto sl = std::make_unique<Slot>();
         rc |= check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");
         sl->d_in.st = sl->d_out.st = sl->d_lane_ws.st = sl->stream;
         rc |= check(hipEventCreateWithFlags(&sl->ev_done, hipEventDisableTiming), "hipEventCreate");
-        rc |= sl->d_ws.alloc((size_t)s->grid * s->ws_words_per_slot * 4);
-        s->stats.workspace_bytes += (size_t)s->grid * s->ws_words_per_slot * 4;
+        rc |= sl->d_ws.alloc(ws_bytes);
+        s->stats.workspace_bytes += ws_bytes;
         s->slots.push_back(std::move(sl));
     }
     rc |= check(hipStreamSynchronize(stream()), "upload sync");
