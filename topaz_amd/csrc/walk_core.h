@@ -246,8 +246,10 @@ TPZ_HD bool hit_add_slow(const WalkMem& m, int32_t capacity, int32_t max_score, 
 // through out_index / out_score (room for capacity entries).  Scan finds the
 // elements that count without touching the list (SerialScan on the host, a
 // wavefront-wide scan on the GPU: both give the same j).
-template <typename Scan>
-TPZ_HD WalkResult walk_query(const WalkStreams& st, const WalkMem& m, int32_t capacity,
+// Mem is WalkMem, or a type derived from it whose `top` lives elsewhere (the
+// GPU walk keeps the stack tops in LDS).
+template <typename Scan, typename Mem>
+TPZ_HD WalkResult walk_query(const WalkStreams& st, const Mem& m, int32_t capacity,
                              int32_t limit, int32_t max_score, const Scan& scan,
                              uint32_t* out_index, int32_t* out_score) {
     WalkResult res{kWalkDone, 0, 0, 0, 0};

@@ -36,7 +36,18 @@ struct WaveScan {
     }
 };
 
+// The run queue's stack tops (one int per leader score, read and written at
+// every pop and push) in LDS when they fit kWalkTopLds bytes: dynamic shared
+// memory of the launch, addressed as LDS so that its loads wait only for
+// LDS.
+constexpr int kWalkTopLds = 32768;
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+struct WaveMem : WalkMem {
+    lds_i32* top;
+};
+
 __global__ void __launch_bounds__(64) walk_kernel(WalkArgs a) {
+    extern __shared__ int32_t lds_top[];
     const int q = blockIdx.x;
     const int lane = threadIdx.x;
     const int32_t s0 = a.q_stream[q], n_streams = a.q_stream[q + 1] - s0;
@@ -46,11 +57,23 @@ __global__ void __launch_bounds__(64) walk_kernel(WalkArgs a) {
     uint4* init = reinterpret_cast<uint4*>(base);
     const uint4 ones = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
     for (size_t w = lane; w < walk_mem_init_bytes(L) / 16; w += 64) init[w] = ones;
+    const bool top_in_lds = 4 * (a.max_score + 1) <= kWalkTopLds;
+    if (top_in_lds)
+        for (int s = lane; s <= a.max_score; s += 64) lds_top[s] = -1;
     __syncthreads();
     const WalkMem m = walk_mem_at(base, L);
     const WalkStreams st{n_streams, a.s_base + s0, a.s_len + s0, a.s_open + s0, a.vals, a.prot};
-    const WalkResult r = walk_query(st, m, capacity, a.limit, a.max_score, WaveScan{lane},
-                                    a.out_index + a.q_out[q], a.out_score + a.q_out[q]);
+    WalkResult r;
+    if (top_in_lds) {
+        WaveMem wm;
+        static_cast<WalkMem&>(wm) = m;
+        wm.top = (lds_i32*)lds_top;
+        r = walk_query(st, wm, capacity, a.limit, a.max_score, WaveScan{lane},
+                       a.out_index + a.q_out[q], a.out_score + a.q_out[q]);
+    } else {
+        r = walk_query(st, m, capacity, a.limit, a.max_score, WaveScan{lane},
+                       a.out_index + a.q_out[q], a.out_score + a.q_out[q]);
+    }
     if (lane == 0) {
         a.out_n[q] = r.n_hits;
         a.out_status[q] = r.status;
@@ -88,6 +111,8 @@ extern "C" hipError_t tpz_launch_compact(const tpz::CompactArgs* a, hipStream_t 
 
 extern "C" hipError_t tpz_launch_walk(const tpz::WalkArgs* a, hipStream_t stream) {
     if (a->n_queries <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tpz::walk_kernel, dim3((unsigned)a->n_queries), dim3(64), 0, stream, *a);
+    const size_t top_bytes = 4 * (size_t)(a->max_score + 1);
+    hipLaunchKernelGGL(tpz::walk_kernel, dim3((unsigned)a->n_queries), dim3(64),
+                       top_bytes <= (size_t)tpz::kWalkTopLds ? top_bytes : 0, stream, *a);
     return hipGetLastError();
 }
