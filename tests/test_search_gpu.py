@@ -147,6 +147,20 @@ def test_int64_suffix_array_gives_the_same_hits(db_and_queries, monkeypatch):
         db64.close()
 
 
+@pytest.mark.parametrize("first_chunk", [3, 24, 40, 100])
+def test_stream_round_widths_give_the_same_hits(db_and_queries, first_chunk):
+    """Stream rounds serve 64 / w streams per wavefront when a round's chunk
+    is at most 32 slots (w its power-of-two width), one stream per
+    wavefront part otherwise: first chunks that leave lanes of a unit idle
+    (3 of 4), fill units of 32, leave a wavefront part-full (40) or take two
+    (100) give the hits of the default rounds."""
+    tmp, db, qs = db_and_queries
+    full, _ = db.search(qs[:40])
+    got, st = db.search(qs[:40], db.default_options(first_chunk=first_chunk))
+    assert got == full
+    assert st["rounds"] > 0
+
+
 def test_streams_cut_at_the_cap_are_searched_again(db_and_queries):
     """A walk that reaches the end of a stream cut at max_stream_elements
     searches its query again with a higher cap: the hits are those of an
