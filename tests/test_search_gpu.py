@@ -147,6 +147,26 @@ def test_int64_suffix_array_gives_the_same_hits(db_and_queries, monkeypatch):
         db64.close()
 
 
+@pytest.mark.parametrize("small_pairs", ["0", "600"])
+def test_stage_s_launch_widths_give_the_same_hits(db_and_queries, monkeypatch, small_pairs):
+    """The search database's stage S launches run 8 wavefronts per
+    workgroup below a pair count and 4 above (one workspace for both): all
+    launches with 4 ("0"), and a threshold that splits this workload's
+    launches between the two, give the hits of the default (every launch
+    of this small database with 8)."""
+    tmp, db, qs = db_and_queries
+    monkeypatch.setenv("TOPAZ_AMD_SMALL_LAUNCH_PAIRS", small_pairs)
+    other = Database.from_index(tmp / "DB", matrix())
+    try:
+        for superfast in (1, 0):
+            opt = db.default_options(superfast=superfast)
+            got = [other.search(qs[k:k + 8], opt)[0] for k in range(0, 48, 8)]
+            want = [db.search(qs[k:k + 8], opt)[0] for k in range(0, 48, 8)]
+            assert got == want
+    finally:
+        other.close()
+
+
 @pytest.mark.parametrize("first_chunk", [3, 24, 40, 100])
 def test_stream_round_widths_give_the_same_hits(db_and_queries, first_chunk):
     """Stream rounds serve 64 / w streams per wavefront when a round's chunk

@@ -936,7 +936,9 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     // a launch of under 2 M pairs (a served request alone) is as long as its
     // longest task: 8 wavefronts per staircase halve that; merged batches
     // keep the configured width, which gives the throughput
-    server_small_launches(db->sw, (int64_t)2 << 20);
+    int64_t small_pairs = (int64_t)2 << 20;
+    if (const char* e = getenv("TOPAZ_AMD_SMALL_LAUNCH_PAIRS")) small_pairs = atoll(e);
+    server_small_launches(db->sw, small_pairs);
     if (check(hipStreamSynchronize(stream()), "database upload")) return -1;
     *out = db.release();
     return 0;
