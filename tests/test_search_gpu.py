@@ -147,6 +147,23 @@ def test_int64_suffix_array_gives_the_same_hits(db_and_queries, monkeypatch):
         db64.close()
 
 
+def test_stage_s_in_two_parts_gives_the_same_hits(db_and_queries, monkeypatch):
+    """A batch from TOPAZ_AMD_SW_PART_PAIRS pairs on (1 M by default, so
+    never at this size) submits stage S in two parts of whole queries, each
+    part's output stage running while the next is on the device: with the
+    threshold at 1 pair every batch splits, and the hits stay the same."""
+    tmp, db, qs = db_and_queries
+    monkeypatch.setenv("TOPAZ_AMD_SW_PART_PAIRS", "1")
+    other = Database.from_index(tmp / "DB", matrix())
+    try:
+        for superfast in (1, 0):
+            opt = db.default_options(superfast=superfast)
+            assert other.search(qs[:60], opt)[0] == db.search(qs[:60], opt)[0]
+            assert other.search(qs[:1], opt)[0] == db.search(qs[:1], opt)[0]
+    finally:
+        other.close()
+
+
 @pytest.mark.parametrize("small_pairs", ["0", "600"])
 def test_stage_s_launch_widths_give_the_same_hits(db_and_queries, monkeypatch, small_pairs):
     """The search database's stage S launches run 8 wavefronts per

@@ -188,6 +188,7 @@ struct tpz_db {
     DeviceBuffer d_table;
     DeviceBuffer d_pblock, d_pend;       // the protein index on the device (stream_kernel)
     DeviceBuffer d_sa32;                 // the suffix array as uint32 (texts under 4 GiB)
+    int64_t sw_part_pairs = 1 << 20;     // batches from this many pairs: stage S in two parts
     tpz_server* sw = nullptr;            // stage S over the proteins
     int8_t matrix[625];
     int32_t gap_open = 11, gap_extend = 1, max_w = 1;
@@ -664,9 +665,9 @@ int search_batch(tpz_db* db, const Options& opt, Slot& sl, const Batch& b,
     // submitted one after the other: the server launches the first part
     // while this thread plans the next, and the output stage of a part
     // (step 5) runs while the parts after it are on the device.  A batch of
-    // fewer than 1 M pairs goes in one part: its launches are short, and a
-    // second one would add its own tail.
-    const int sw_parts = n_pairs >= (1 << 20) ? 2 : 1;
+    // fewer than sw_part_pairs (1 M) pairs goes in one part: its launches
+    // are short, and a second one would add its own tail.
+    const int sw_parts = n_pairs >= std::max<int64_t>(1, db->sw_part_pairs) ? 2 : 1;
     std::vector<int32_t> q_cut(sw_parts + 1, b.n);
     q_cut[0] = 0;
     for (int part = 1; part < sw_parts; ++part)
@@ -936,6 +937,7 @@ int tpz_db_create(tpz_db** out, const char* text, uint64_t t_size, const int64_t
     // a launch of under 2 M pairs (a served request alone) is as long as its
     // longest task: 8 wavefronts per staircase halve that; merged batches
     // keep the configured width, which gives the throughput
+    if (const char* e = getenv("TOPAZ_AMD_SW_PART_PAIRS")) db->sw_part_pairs = atoll(e);
     int64_t small_pairs = (int64_t)2 << 20;
     if (const char* e = getenv("TOPAZ_AMD_SMALL_LAUNCH_PAIRS")) small_pairs = atoll(e);
     server_small_launches(db->sw, small_pairs);
